@@ -1,0 +1,183 @@
+/*
+ * projectracer_gpu.h -- C ABI of the MI355X (gfx950) implementation of
+ * ProjecTracer's per-timestep hot path.
+ *
+ * The reference program is Fortran 90: its hot-path subroutines take no
+ * arguments and work on module globals (`use basics; use arrays`).  This
+ * ABI keeps that shape: one opaque context owns the device mirrors of the
+ * module arrays and is driven by argument-free calls that stand in for the
+ * reference subroutines.  Everything crossing the boundary is a plain C
+ * type (int32/int64/double and pointers to host memory laid out exactly as
+ * the Fortran arrays, column-major with the reference's halo widths), so a
+ * Fortran `interface ... bind(C)` block binds it directly (the module in
+ * projectracer_amd/fortran/projectracer_gpu.f90 is that block).
+ *
+ * Every entry point returns PT_OK (0) or one of the PT_ERR_* codes below.
+ * The reference has no return codes: it prints a message and calls `stop`
+ * (the cited lines); the Fortran shim reproduces those messages from the
+ * code.
+ *
+ * Supported subset (what the hot path and its configurations use):
+ * uniform grid (gridX=gridZ=0), order=4, Database=2, iterate=1, MG=1,
+ * RKC1=1, splitting=0, non_Newtonian=0, plastic=0, comp=1, ntype=1,
+ * iC_continuous=1, shape_function=1, tracer=1, torder=4, RK_update=1,
+ * vel_update=1, local_time=0, Chebyshev=0, CFL=0, equalize=0,
+ * iheattype=0.  pt_create rejects anything else with PT_ERR_UNSUPPORTED.
+ */
+#ifndef PROJECTRACER_GPU_H
+#define PROJECTRACER_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define PT_OK                   0
+#define PT_ERR_TRACER_OUTSIDE   1  /* PS/composition.f90:1007-1010 "Tracer advected outside of domain" */
+#define PT_ERR_MAX_CYCLES       2  /* PS/iterative_solver.f90:325-334 "Too Many Multigrid Cycles" */
+#define PT_ERR_HIGH_RESIDUAL    3  /* PS/iterative_solver.f90:438-447 "Abnormally High Residual" */
+#define PT_ERR_SINGULAR_COARSE  4  /* DGBTRF INFO>0 at PS/iterative_solver.f90:153 (ignored by the reference) */
+#define PT_ERR_HIP              5  /* HIP runtime error (no reference counterpart) */
+#define PT_ERR_INVALID          6  /* bad argument */
+#define PT_ERR_UNSUPPORTED      7  /* option outside the supported subset */
+#define PT_ERR_NO_DEVICE        8  /* no gfx950 device visible */
+
+/* ---- solver precision / initial-guess modes ---------------------------- */
+#define PT_PREC_FP64   0  /* plain path: every operation in FP64, as the reference */
+#define PT_PREC_MIXED  1  /* FP32 smoothing/restriction/prolongation inside an FP64
+                             defect-correction loop; FP64 residual, coarse LU and psi */
+
+#define PT_GUESS_WARM    0  /* warm start from the previous psi (PS/iterative_solver.f90:134) */
+#define PT_GUESS_LEARNED 1  /* learned initial guess for psi (no reference counterpart) */
+
+/* ---- namelist values (PS/basics.f90:54-66, read at PS/main.f90:11-21) --- */
+typedef struct pt_params {
+  /* &inputs: grid */
+  double aspect;          /* aspect ratio lambda */
+  int32_t nx, nz;         /* cells; rewritten by optimize_grid (PS/iterative_solver.f90:1-61) */
+  int32_t grids;          /* max multigrid levels (2..6); rewritten by optimize_grid */
+  int32_t mingrid;        /* min coarsest-grid size */
+  /* &inputs: velocity solver */
+  double  tolerance;      /* L1-mean relative residual target */
+  int32_t max_cycles;
+  int32_t Nmulti;         /* smoother iterations per level visit */
+  double  eig_ratio;      /* kept for completeness; unused by the supported path */
+  int32_t vis_iter;       /* viscosity smoother iterations before restriction */
+  int32_t nstageS;        /* RKC1 stages */
+  double  dampingS;       /* RKC1 damping */
+  double  c_2, c_1;       /* RKC1 Courant range */
+  /* &inputs: time stepping */
+  double  courant, courantT, courantC, courant_stream;
+  /* &inputs: tracers */
+  int32_t tpc;            /* tracers per grid node */
+  /* &inputs: boundary conditions */
+  int32_t Tbot;           /* 0 isothermal, 1 insulating bottom */
+  int32_t Vbc;            /* 0 free-slip, 1 rigid top/bottom */
+  /* &Rayleigh_numbers */
+  double  RaT, RaC;
+  /* &heating_conductivity (iheattype=0) */
+  double  H, conduct_factor, Htype;
+  /* &viscosity */
+  int32_t ivis;           /* 0 isoviscous, 1 T/z dependent, 2 T/C/z dependent */
+  double  visP, visT, delta_visT, eig_ratio_min, visC, delta_visC;
+  /* path selection (no reference counterpart) */
+  int32_t precision;      /* PT_PREC_* */
+  int32_t guess;          /* PT_GUESS_* */
+} pt_params;
+
+/* Fill *p with the reference defaults used by this build (inputs.nml values
+ * for the supported path; tolerance 1e-8, 6 levels, RKC1 35 stages). */
+void pt_default_params(pt_params* p);
+
+/* ---- field identifiers for pt_get_field / pt_set_field ----------------- */
+/* Each field is exchanged with its Fortran bounds (column-major, i fastest):
+ *   PT_FIELD_SF, PT_FIELD_T(ratio), PT_FIELD_TVIS, PT_FIELD_TBUOY: (-3:nx+3,-3:nz+3)
+ *   PT_FIELD_U, PT_FIELD_W: (-3:nx+3,-3:nz+3)   (span_interp halo)
+ *   PT_FIELD_CNEW, PT_FIELD_CVIS, PT_FIELD_CBUOY, PT_FIELD_CONDUCT: (-5:nx+5,-5:nz+5)
+ *   PT_FIELD_VIS: level-1 vis_grid (-3:nx+3,-3:nz+3)
+ *   PT_FIELD_RESIDUAL: level-1 residual (1:nx-1,1:nz-1)
+ *   PT_FIELD_SOURCE: tracer_space_array (-3:nx+3,-3:nz+3)
+ * pt_field_extent() returns the element count so callers can size buffers. */
+enum {
+  PT_FIELD_SF = 0, PT_FIELD_T = 1, PT_FIELD_TVIS = 2, PT_FIELD_TBUOY = 3,
+  PT_FIELD_U = 4, PT_FIELD_W = 5, PT_FIELD_CNEW = 6, PT_FIELD_CVIS = 7,
+  PT_FIELD_CBUOY = 8, PT_FIELD_CONDUCT = 9, PT_FIELD_VIS = 10,
+  PT_FIELD_RESIDUAL = 11, PT_FIELD_SOURCE = 12
+};
+
+/* ---- per-solve / per-step counters ------------------------------------ */
+typedef struct pt_solve_info {
+  int32_t q;              /* cycles of the last solve (PS/iterative_solver.f90:161-335) */
+  double  residual;       /* residual_mag(1) of the last solve */
+  double  res_ref;        /* res_ref(1) of the last solve */
+  double  dt;             /* last time step (PS/energy.f90:756-810) */
+  double  time;           /* model time */
+  int64_t solves;         /* solves since pt_create */
+  int64_t cycles;         /* multigrid cycles since pt_create */
+  int32_t n_eta;          /* viscosity filter passes of the last level-1 smoother_vis */
+  int32_t n_temp;         /* T filter passes of the last projection */
+  int32_t n_comp;         /* C filter passes of the last projection */
+  double  eig_fine;       /* ratio_min of the last level-1 smoother_vis */
+} pt_solve_info;
+
+/* ---- lifecycle ---------------------------------------------------------
+ * pt_create: compute_smoothing_parameters, multigrid_setup (optimize_grid),
+ *   compute_basics, storage, Chebyshev_prep_stream, FD_coefficients,
+ *   build_grid, derivative_eigenvalues, stability_RK4_smoother,
+ *   eigenvalue_prep, stream_space_matrix_prep, stability_vis_smoother
+ *   (PS/main.f90:168-203).  *p is copied; nx/nz/grids are rewritten in the
+ *   context as optimize_grid does (read them back with pt_get_dims). */
+int pt_create(const pt_params* p, void** ctx);
+int pt_destroy(void* ctx);
+int pt_get_dims(void* ctx, int32_t* nx, int32_t* nz, int32_t* grids, int64_t* ntr);
+
+/* pt_load_state: the tail of initial_temperature + initialize_tracers
+ * (PS/basics.f90:352-359, PS/composition.f90:848) and PS/main.f90:228-229:
+ * T node values (0:nx,0:nz), the isothermal boundary values, and tracer
+ * positions/values in ID order (ntr each); projects the tracers
+ * (tracers_to_corners) and zeroes psi and the coarse-level corrections. */
+int pt_load_state(void* ctx, const double* T_nodes, double Tbot_iso, double Ttop_iso,
+                  const double* rtr, const double* str, const double* Ttr,
+                  const double* Ctr);
+
+/* ---- hot path: drop-in replacements for the reference subroutines ------ */
+int pt_multigrid(void* ctx);            /* PS/iterative_solver.f90:124-341 */
+int pt_compute_velocities(void* ctx);   /* PS/momentum.f90:655-710 + velocityBCs */
+int pt_update_velocities(void* ctx);    /* PS/momentum.f90:173-189 */
+int pt_energy_time(void* ctx);          /* PS/energy.f90:181-556 (RK4, 3 solves, 4 projections) */
+int pt_tracers_to_corners(void* ctx);   /* PS/composition.f90:923-1087 on (rtr,str,Ttr) */
+/* One iteration of the time loop at PS/main.f90:245-299 (without stats and
+ * snapshots): multigrid, compute_velocities, energy_time, time += dt. */
+int pt_time_step(void* ctx);
+/* Run n time steps back to back (device-resident; one host sync at the end). */
+int pt_run_steps(void* ctx, int32_t n);
+
+/* ---- host <-> device exchange (when a host consumer needs the data) ----- */
+int64_t pt_field_extent(void* ctx, int32_t field_id);
+int pt_get_field(void* ctx, int32_t field_id, double* host);
+int pt_set_field(void* ctx, int32_t field_id, const double* host);
+/* Tracers in reference ID order (the device keeps them cell-sorted). */
+int pt_get_tracers(void* ctx, double* rtr, double* str, double* Ttr, double* Ctr);
+/* nempty(0:nx,0:nz): tracers binned to each node's dual cell (INT32). */
+int pt_get_nempty(void* ctx, int32_t* nempty);
+int pt_get_info(void* ctx, pt_solve_info* info);
+int pt_synchronize(void* ctx);
+
+/* ---- timing hooks (no reference counterpart; PS/basics.f90:27-31 hold
+ * the reference's omp_get_wtime accumulators) -------------------------- */
+/* Enable per-kernel HIP-event timing of the dominant kernels on the
+ * context's stream.  While enabled, pt_kernel_time reports the accumulated
+ * device time (ms) and launch count of kernel `which` (PT_K_*). */
+enum { PT_K_RKC_STAGE_L1 = 0, PT_K_TRACER_STAGE = 1, PT_K_PROJECT = 2, PT_K_COUNT = 3 };
+int pt_enable_kernel_timing(void* ctx, int32_t on);
+int pt_kernel_time(void* ctx, int32_t which, double* ms, int64_t* launches);
+
+/* Last error message (thread-local, never NULL). */
+const char* pt_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PROJECTRACER_GPU_H */

@@ -1,0 +1,1709 @@
+/*
+ * pt_oracle.c -- CPU oracle for the ProjecTracer per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see pt_oracle.h).  A single-threaded FP64
+ * restatement of the reference Fortran, routine by routine, with every
+ * floating-point expression evaluated in the reference's order so that the
+ * only differences to the flang-built reference are libm/LAPACK rounding.
+ * Compile without FMA contraction (-ffp-contract=off), as the reference is
+ * built for baseline x86-64.
+ *
+ * Citations are file:line in seantrim/ProjecTracer (PS = ProjecTracer_Source).
+ *
+ * Pinning (what checks each function; see tests/test_oracle.py):
+ *  - FD_coefficients: textbook Fornberg weights (closed form) and the
+ *    flang-built reference tables.
+ *  - Chebyshev_prep_stream: RKC1_stability = -83.852981245559235 from the
+ *    shipped stdout:10 (s=35, damping 1000).
+ *  - derivative_eigenvalues/stability_RK4: dt = 2.59875e-5 at 100x100,
+ *    courant 0.99 from the shipped stats.dat row 2.
+ *  - stream_derivatives/stream_space/enforceBCs_stream: sine-mode operator
+ *    symbols (closed form) and flang-reference solves.
+ *  - multigrid (all of a1-a15), compute_velocities, energy_time, interpolate,
+ *    compute_derivatives, tracers_to_corners, filters: flang-built
+ *    reference trajectories (tests/golden) plus closed forms (bicubic
+ *    exactness of interpolate, projection partition of unity, restriction
+ *    and prolongation of constants/linears).
+ *  - band LU: scipy.linalg.lapack dgbtrf/dgbtrs golden vectors.
+ *  - mzran: integer recurrence checked against the reference's initial
+ *    tracer cloud produced by the flang build.
+ */
+#include "pt_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#include <float.h>
+
+#define PII 3.1415926535897932   /* PS/basics.f90:32 */
+#define MAXL 7
+
+/* 2-D array with Fortran lower bounds, column-major (i fastest) */
+typedef struct { double* p; int ilo, klo, ni, nk; } A2;
+#define AT(a, i, k) ((a).p[(long)((k) - (a).klo) * (a).ni + ((i) - (a).ilo)])
+
+static A2 a2_new(int ilo, int ihi, int klo, int khi) {
+  A2 a; a.ilo = ilo; a.klo = klo; a.ni = ihi - ilo + 1; a.nk = khi - klo + 1;
+  a.p = (double*)calloc((size_t)a.ni * a.nk, sizeof(double));
+  return a;
+}
+static void a2_free(A2* a) { free(a->p); a->p = NULL; }
+static void a2_copy(A2* dst, const A2* src) {   /* same-shape whole-array assignment */
+  memcpy(dst->p, src->p, sizeof(double) * (size_t)src->ni * src->nk);
+}
+
+struct pto_case {
+  pt_params P;
+  int nx, nz, grids, nsteps, multi[52];
+  int span, span1, span_interp, spanT;
+  double aspect, dr, ds, dr2, ds2, dr3, ds3, dr4, ds4, drds, drds2, dr2ds, dr2ds2;
+  int nxg[MAXL + 1], nzg[MAXL + 1];
+  double drg[MAXL + 1], dsg[MAXL + 1], dr2g[MAXL + 1], ds2g[MAXL + 1], dr3g[MAXL + 1],
+      ds3g[MAXL + 1], dr4g[MAXL + 1], ds4g[MAXL + 1], drdsg[MAXL + 1], drds2g[MAXL + 1],
+      dr2dsg[MAXL + 1], dr2ds2g[MAXL + 1];
+  double factorial[5], dr_power[5], ds_power[5];
+  /* stencils, index +3 */
+  double D1[7], D2[7], D3[7], D4[7];
+  double Drs[7][7], Drss[7][7], Drrs[7][7], Drrss[7][7];   /* [k+3][i+3] = D?(i,k) */
+  double D[5][7];                                          /* D(m,j) = D[m][j+3] */
+  double D1_so[3], D2_so[3], Drs_so[3][3];                 /* Drs_so(i,k) = [k+1][i+1] */
+  /* eigenvalue bases (PS/energy.f90:730-743) */
+  double Tr_base, Trr_base, Trrr_base, Trrrr_base, Trs_base, Trrs_base, Trrss_base;
+  double Tr_base_so, Trr_base_so;
+  /* static eigenvalue tables per level (uniform grid => constant per level) */
+  double SR1[MAXL + 1], SR2[MAXL + 1], SR4[MAXL + 1], SR5[MAXL + 1], SR6[MAXL + 1];
+  double SI1[MAXL + 1], SI2[MAXL + 1], SI4[MAXL + 1], SI5[MAXL + 1], SI6[MAXL + 1];
+  double SM1[7][7], SM2[7][7], SM4[7][7], SM5[7][7], SM6[7][7];  /* [dk+3][di+3] */
+  /* RKC1 */
+  double TChebS[64], TCheb_xS[64], bChebS[64], mewChebS[64], mewtChebS[64], vChebS[64],
+      cChebS[64], omega0S, omega1S, RKC1_stability;
+  double delta_T, delta_C;
+  /* eigenvalue maxima scaled for the fine grid (module vars Tr_max etc.) */
+  double Tr_max, Trr_max, Ts_max, Tss_max;
+  /* fields */
+  A2 T, SF, u, w, Tvis, Tbuoy, Tratio, Cvis, Cbuoy, Cnew, conduct, conduct_r, conduct_s;
+  A2 tsa, Textend, dts_array, xg, zg;
+  double *DERr_u, *DERr_w, *DERr_tracer;   /* (1:3, 0:nx, -3:nz+3) */
+  /* multigrid */
+  A2 error[MAXL + 1], vis_grid[MAXL + 1], vis_grid_static[MAXL + 1];
+  A2 vis_x[MAXL + 1], vis_z[MAXL + 1], vis_xx[MAXL + 1], vis_zz[MAXL + 1], vis_xz[MAXL + 1],
+      vis_xxpzz[MAXL + 1], vis_xxmzz[MAXL + 1], vis_smooth[MAXL + 1], dt_vis[MAXL + 1];
+  A2 RHS[MAXL + 1], residual[MAXL + 1], dt_stream[MAXL + 1], ratio[MAXL + 1];
+  A2 SFxx, SFzz, SFxxx, SFzzz, SFxxxx, SFzzzz, SFxz, SFxxz, SFxzz, SFxxzz;  /* per call */
+  A2 S0[MAXL + 1], S1[MAXL + 1], S2[MAXL + 1];   /* iterate_stream automatic arrays */
+  double res_ref[MAXL + 1], residual_mag[52];
+  int kl, ku, ngc; double *Mc, *RowB, *Bc; int32_t* ipiv;
+  /* tracers */
+  int64_t ntr;
+  double *rtr, *str, *Ttr, *Ctr, *rtr0, *rtr1, *rtr2, *str0, *str1, *str2, *Ttr0, *Ttr1, *Ttr2;
+  int32_t* nempty;   /* (0:nx,0:nz) */
+  double *TxS, *CxS, *shape_sum;  /* (-3:nx+3,-3:nz+3) work */
+  A2 C0, T0;
+  /* state */
+  double Tbot_iso, Ttop_iso;
+  double time, dt;
+  int n_eta, n_temp, n_comp; double eig_fine; int Nvis_store, nn_store;
+  int last_q; double last_res, last_res_ref; int64_t solves, cycles;
+  int err;
+};
+
+#define DER(a, c, jj, i, k) ((a)[((long)((k) + 3) * ((c)->nx + 1) + (i)) * 3 + ((jj) - 1)])
+
+/* ------------------------------------------------------------------------ */
+/* optimize_grid: PS/iterative_solver.f90:1-61 */
+static void optimize_grid(int* nx, int* nz, int* grids, int n_points_min) {
+  int n_points_max = (*nx < *nz) ? *nx : *nz;
+  int n_coarse_min = 1, n_coarse_max = *grids - 1;
+  int n_max = n_points_max * (1 << n_coarse_max);
+  int nxbig, n_in;
+  if (*nz <= *nx) { nxbig = 1; n_in = *nz; } else { nxbig = 0; n_in = *nx; }
+  int n_temp = n_max, n_coarse_temp = 0, n_points_temp = 0;
+  for (int np = n_points_min; np <= n_points_max; np++)
+    for (int nc = n_coarse_min; nc <= n_coarse_max; nc++) {
+      int n = np * (1 << nc);
+      if (n >= n_in && n < n_temp) { n_temp = n; n_coarse_temp = nc; n_points_temp = np; }
+    }
+  *grids = n_coarse_temp + 1;
+  if (nxbig == 0) *nx = n_temp; else *nz = n_temp;
+  n_in = (*nz <= *nx) ? *nx : *nz;
+  n_points_temp = (int)lround((double)n_in / (double)(1 << n_coarse_temp));
+  n_temp = n_points_temp * (1 << n_coarse_temp);
+  if (nxbig == 0) *nz = n_temp; else *nx = n_temp;
+}
+
+/* multigrid_setup: PS/multigrid_cycle.f90:1-99 (the active W cycles) */
+static void multigrid_setup(pto_case* c) {
+  static const int w2[] = {1, 2};
+  static const int w3[] = {1, 2, 3, 2, 3, 2};
+  static const int w4[] = {1, 2, 3, 4, 3, 4, 3, 2, 3, 4, 3, 4, 3, 2};
+  static const int w5[] = {1, 2, 3, 4, 5, 4, 5, 4, 3, 4, 5, 4, 3, 2, 3, 4, 5, 4, 3, 4, 5, 4, 5, 4, 3, 2};
+  static const int w6[] = {1, 2, 3, 4, 5, 6, 5, 6, 5, 4, 5, 6, 5, 4, 3, 4, 5, 6, 5, 4, 3,
+                           2, 3, 4, 5, 6, 5, 4, 3, 4, 5, 6, 5, 4, 5, 6, 5, 6, 5, 4, 3, 2};
+  optimize_grid(&c->nx, &c->nz, &c->grids, c->P.mingrid);
+  const int* m = NULL; int n = 0;
+  switch (c->grids) {
+    case 2: m = w2; n = 2; break;  case 3: m = w3; n = 6; break;
+    case 4: m = w4; n = 14; break; case 5: m = w5; n = 26; break;
+    case 6: m = w6; n = 42; break;
+    default: c->err = PT_ERR_UNSUPPORTED; return;
+  }
+  c->nsteps = n;
+  for (int p = 1; p <= n; p++) c->multi[p] = m[p - 1];
+  c->multi[0] = c->multi[n];
+  c->multi[n + 1] = c->multi[1];
+}
+
+/* compute_smoothing_parameters: PS/smoother.f90:31-73 */
+static void compute_smoothing_parameters(pto_case* c) {
+  const pt_params* P = &c->P;
+  if (P->ivis == 0 || P->RaT == 0.0) c->delta_T = 2.0;
+  else if (P->visT != 1.0) c->delta_T = log(P->delta_visT) / log(P->visT);
+  else c->delta_T = 2.0;
+  c->delta_C = 2.0;
+  if (P->conduct_factor != 1.0) c->delta_C = fabs(log(P->delta_visC) / log(1.0 / P->conduct_factor));
+  if (P->ivis >= 2 && P->visC != 1.0) c->delta_C = fabs(log(P->delta_visC) / log(P->visC));
+}
+
+/* FD_coefficients: PS/main.f90:499-642 (Fornberg recursion) */
+static void fd_coefficients(pto_case* c) {
+  const int order = 4, bigM = 4, bigN = 6;
+  double a[8], delta[6][7][7];   /* delta(m=-1..4, n, i) -> [m+1][n][i] */
+  memset(delta, 0, sizeof delta);
+  double x0 = 0.0;
+  a[0] = 0.0;
+  int k = 1;
+  for (int i = 1; i <= bigN; i += 2) { a[i] = (double)k; a[i + 1] = -(double)k; k++; }
+#define DL(m, n, i) delta[(m) + 1][n][i]
+  DL(0, 0, 0) = 1.0;
+  double c1 = 1.0;
+  for (int n = 1; n <= bigN; n++) {
+    double c2 = 1.0;
+    for (int i = 0; i <= n - 1; i++) {
+      double c3 = a[n] - a[i];
+      c2 = c2 * c3;
+      if (n <= bigM) DL(n, n - 1, i) = 0.0;
+      for (int m = 0; m <= (n < bigM ? n : bigM); m++)
+        DL(m, n, i) = ((a[n] - x0) * DL(m, n - 1, i) - (double)m * DL(m - 1, n - 1, i)) / c3;
+    }
+    for (int m = 0; m <= (n < bigM ? n : bigM); m++)
+      DL(m, n, n) = (c1 / c2) * ((double)m * DL(m - 1, n - 1, n - 1) - (a[n - 1] - x0) * DL(m, n - 1, n - 1));
+    c1 = c2;
+  }
+  memset(c->D1, 0, sizeof c->D1); memset(c->D2, 0, sizeof c->D2);
+  c->D1[3] = DL(1, order, 0); c->D2[3] = DL(2, order, 0);
+  k = 1;
+  for (int i = 1; i <= order; i += 2) {
+    c->D1[3 + k] = DL(1, order, i); c->D1[3 - k] = DL(1, order, i + 1);
+    c->D2[3 + k] = DL(2, order, i); c->D2[3 - k] = DL(2, order, i + 1);
+    k++;
+  }
+  memset(c->D, 0, sizeof c->D);
+  for (int m = 2; m <= 4; m += 2) {
+    c->D[m - 1][3] = DL(m - 1, order + m - 2, 0);
+    c->D[m][3] = DL(m, order + m - 2, 0);
+    k = 1;
+    for (int i = 1; i <= order + m - 2; i += 2) {
+      c->D[m - 1][3 + k] = DL(m - 1, order + m - 2, i);
+      c->D[m - 1][3 - k] = DL(m - 1, order + m - 2, i + 1);
+      c->D[m][3 + k] = DL(m, order + m - 2, i);
+      c->D[m][3 - k] = DL(m, order + m - 2, i + 1);
+      k++;
+    }
+  }
+  for (int m = 1; m <= 4; m += 2) {
+    c->D[m][3] = 0.0;
+    for (int i = 1; i <= 3; i++) { c->D[m][3 + i] = -c->D[m][3 - i]; c->D[m + 1][3 + i] = c->D[m + 1][3 - i]; }
+  }
+  memset(c->D3, 0, sizeof c->D3); memset(c->D4, 0, sizeof c->D4);
+  c->D3[3] = DL(3, order + 2, 0); c->D4[3] = DL(4, order + 2, 0);
+  k = 1;
+  for (int i = 1; i <= order + 2; i += 2) {
+    c->D3[3 + k] = DL(3, order + 2, i); c->D3[3 - k] = DL(3, order + 2, i + 1);
+    c->D4[3 + k] = DL(4, order + 2, i); c->D4[3 - k] = DL(4, order + 2, i + 1);
+    k++;
+  }
+  c->D1[3] = 0.0; c->D3[3] = 0.0;
+  for (int i = 1; i <= 3; i++) {
+    c->D1[3 + i] = -c->D1[3 - i]; c->D2[3 + i] = c->D2[3 - i];
+    c->D3[3 + i] = -c->D3[3 - i]; c->D4[3 + i] = c->D4[3 - i];
+  }
+#undef DL
+  memset(c->Drs, 0, sizeof c->Drs); memset(c->Drss, 0, sizeof c->Drss);
+  memset(c->Drrs, 0, sizeof c->Drrs); memset(c->Drrss, 0, sizeof c->Drrss);
+  for (int kk = -2; kk <= 2; kk++)
+    for (int i = -2; i <= 2; i++) {
+      c->Drs[kk + 3][i + 3] = c->D1[kk + 3] * c->D1[i + 3];
+      c->Drss[kk + 3][i + 3] = c->D2[kk + 3] * c->D1[i + 3];
+      c->Drrs[kk + 3][i + 3] = c->D1[kk + 3] * c->D2[i + 3];
+      c->Drrss[kk + 3][i + 3] = c->D2[kk + 3] * c->D2[i + 3];
+    }
+  c->D1_so[0] = -0.5; c->D1_so[1] = 0.0; c->D1_so[2] = 0.5;
+  c->D2_so[0] = 1.0; c->D2_so[1] = -2.0; c->D2_so[2] = 1.0;
+  /* Drs_so(:,1)=(-.25,0,.25); Drs_so(:,0)=0; Drs_so(:,-1)=(.25,0,-.25) */
+  c->Drs_so[2][0] = -0.25; c->Drs_so[2][1] = 0.0; c->Drs_so[2][2] = 0.25;
+  c->Drs_so[1][0] = 0.0;   c->Drs_so[1][1] = 0.0; c->Drs_so[1][2] = 0.0;
+  c->Drs_so[0][0] = 0.25;  c->Drs_so[0][1] = 0.0; c->Drs_so[0][2] = -0.25;
+}
+
+/* Chebyshev_prep_stream: PS/Chebyshev.f90:20-34, 82-110 */
+static void chebyshev_prep_stream(pto_case* c) {
+  int s = c->P.nstageS;
+  c->omega0S = 1.0 + c->P.dampingS / (double)(s * s);
+  double x = c->omega0S;
+  c->TChebS[0] = 1.0; c->TChebS[1] = x; c->TCheb_xS[0] = 0.0; c->TCheb_xS[1] = 1.0;
+  for (int j = 2; j <= s; j++) {
+    c->TChebS[j] = 2.0 * x * c->TChebS[j - 1] - c->TChebS[j - 2];
+    c->TCheb_xS[j] = 2.0 * c->TChebS[j - 1] + 2.0 * x * c->TCheb_xS[j - 1] - c->TCheb_xS[j - 2];
+  }
+  c->omega1S = c->TChebS[s] / c->TCheb_xS[s];
+  for (int j = 0; j <= s; j++) c->bChebS[j] = 1.0 / c->TChebS[j];
+  c->mewtChebS[1] = c->bChebS[1] * c->omega1S;
+  for (int j = 2; j <= s; j++) {
+    c->mewChebS[j] = 2.0 * c->bChebS[j] * c->omega0S / c->bChebS[j - 1];
+    c->vChebS[j] = -c->bChebS[j] / c->bChebS[j - 2];
+    c->cChebS[j] = 1.0 - c->mewChebS[j] - c->vChebS[j];
+  }
+  for (int j = 2; j <= s; j++) c->mewtChebS[j] = 2.0 * c->bChebS[j] * c->omega1S / c->bChebS[j - 1];
+  c->RKC1_stability = -(2.0 * c->omega0S / c->omega1S);
+}
+
+/* complex helpers for derivative_eigenvalues (PS/energy.f90:632-754) */
+typedef struct { double re, im; } cplx;
+/* T_test: cmplx(cos,sin)*cmplx(cos,sin) in default (single precision) complex */
+static cplx t_test_f(float a_re, float a_im, float b_re, float b_im) {
+  float re = a_re * b_re - a_im * b_im;
+  float im = a_re * b_im + a_im * b_re;
+  cplx r = {(double)re, (double)im};
+  return r;
+}
+/* complex division a/b; in this routine b = T_test(0,0) = (1,0), for which
+ * every division algorithm returns a exactly */
+static cplx cdiv(cplx a, cplx b) {
+  double den = b.re * b.re + b.im * b.im;
+  cplx r = {(a.re * b.re + a.im * b.im) / den, (a.im * b.re - a.re * b.im) / den};
+  return r;
+}
+
+static void derivative_eigenvalues(pto_case* c) {
+  const int n_angle = 2049;
+  const double d_angle = 2.0 * PII / (double)(n_angle - 1);
+  double Tr_max = 0, Trr_max = 0, Trrr_max = 0, Trrrr_max = 0, Trs_max = 0, Trrs_max = 0,
+         Trrss_max = 0, Tr_max_so = 0, Trr_max_so = 0;
+  /* single-precision samples cmplx(dcos(kr*r),dsin(kr*r)) per angle and offset */
+  float* cr = (float*)malloc(sizeof(float) * n_angle * 7);
+  float* ci = (float*)malloc(sizeof(float) * n_angle * 7);
+  for (int a = 0; a < n_angle; a++) {
+    double kr = (double)a * d_angle;
+    for (int j = -3; j <= 3; j++) {
+      double r = (double)j;
+      cr[a * 7 + j + 3] = (float)cos(kr * r);
+      ci[a * 7 + j + 3] = (float)sin(kr * r);
+    }
+  }
+  const float one_re = (float)cos(0.0), one_im = (float)sin(0.0);
+  for (int a = 0; a < n_angle; a++) {
+    cplx Tr = {0, 0}, Trr = {0, 0}, Trrr = {0, 0}, Trrrr = {0, 0}, Tr_so = {0, 0}, Trr_so = {0, 0};
+    for (int j = -2; j <= 2; j++) {
+      cplx t = t_test_f(cr[a * 7 + j + 3], ci[a * 7 + j + 3], one_re, one_im);
+      Tr.re = Tr.re + c->D1[j + 3] * t.re; Tr.im = Tr.im + c->D1[j + 3] * t.im;
+      Trr.re = Trr.re + c->D2[j + 3] * t.re; Trr.im = Trr.im + c->D2[j + 3] * t.im;
+    }
+    for (int j = -1; j <= 1; j++) {
+      cplx t = t_test_f(cr[a * 7 + j + 3], ci[a * 7 + j + 3], one_re, one_im);
+      Tr_so.re = Tr_so.re + c->D1_so[j + 1] * t.re; Tr_so.im = Tr_so.im + c->D1_so[j + 1] * t.im;
+      Trr_so.re = Trr_so.re + c->D2_so[j + 1] * t.re; Trr_so.im = Trr_so.im + c->D2_so[j + 1] * t.im;
+    }
+    for (int j = -3; j <= 3; j++) {
+      cplx t = t_test_f(cr[a * 7 + j + 3], ci[a * 7 + j + 3], one_re, one_im);
+      Trrr.re = Trrr.re + c->D3[j + 3] * t.re; Trrr.im = Trrr.im + c->D3[j + 3] * t.im;
+      Trrrr.re = Trrrr.re + c->D4[j + 3] * t.re; Trrrr.im = Trrrr.im + c->D4[j + 3] * t.im;
+    }
+    cplx t0 = t_test_f(cr[a * 7 + 3], ci[a * 7 + 3], one_re, one_im);
+    Tr = cdiv(Tr, t0); Trr = cdiv(Trr, t0); Trrr = cdiv(Trrr, t0); Trrrr = cdiv(Trrrr, t0);
+    Tr_so = cdiv(Tr_so, t0); Trr_so = cdiv(Trr_so, t0);
+    if (fabs(Tr.im) > Tr_max) Tr_max = fabs(Tr.im);
+    if (fabs(Trr.re) > Trr_max) Trr_max = fabs(Trr.re);
+    if (fabs(Trrr.im) > Trrr_max) Trrr_max = fabs(Trrr.im);
+    if (fabs(Trrrr.re) > Trrrr_max) Trrrr_max = fabs(Trrrr.re);
+    if (fabs(Tr_so.im) > Tr_max_so) Tr_max_so = fabs(Tr_so.im);
+    if (fabs(Trr_so.re) > Trr_max_so) Trr_max_so = fabs(Trr_so.re);
+    for (int a2 = 0; a2 < n_angle; a2++) {
+      cplx Trs = {0, 0}, Trrs = {0, 0}, Trrss = {0, 0};
+      for (int j = -2; j <= 2; j++)
+        for (int jj = -2; jj <= 2; jj++) {
+          cplx t = t_test_f(cr[a * 7 + j + 3], ci[a * 7 + j + 3], cr[a2 * 7 + jj + 3], ci[a2 * 7 + jj + 3]);
+          double drs = c->Drs[jj + 3][j + 3], drrs = c->Drrs[jj + 3][j + 3], drrss = c->Drrss[jj + 3][j + 3];
+          Trs.re = Trs.re + drs * t.re; Trs.im = Trs.im + drs * t.im;
+          Trrs.re = Trrs.re + drrs * t.re; Trrs.im = Trrs.im + drrs * t.im;
+          Trrss.re = Trrss.re + drrss * t.re; Trrss.im = Trrss.im + drrss * t.im;
+        }
+      if (fabs(Trs.re) > Trs_max) Trs_max = fabs(Trs.re);
+      if (fabs(Trrs.im) > Trrs_max) Trrs_max = fabs(Trrs.im);
+      if (fabs(Trrss.re) > Trrss_max) Trrss_max = fabs(Trrss.re);
+    }
+  }
+  free(cr); free(ci);
+  c->Tr_base = Tr_max; c->Trr_base = Trr_max; c->Trrr_base = Trrr_max; c->Trrrr_base = Trrrr_max;
+  c->Trs_base = Trs_max; c->Trrss_base = Trrss_max; c->Trrs_base = Trrs_max;
+  c->Tr_base_so = Tr_max_so; c->Trr_base_so = Trr_max_so;
+}
+
+/* eigenvalue_prep: PS/iterative_solver.f90:1232-1362 for a uniform grid
+ * (xr=zs=1, xrr=zss=xrrr=...=0); the metric terms are kept so the
+ * arithmetic matches the reference operation for operation. */
+static void eigenvalue_prep(pto_case* c) {
+  const double xr = 1.0, xrr = 0.0, xrrr = 0.0, xrrrr = 0.0, zs = 1.0, zss = 0.0, zsss = 0.0, zssss = 0.0;
+  for (int l = 1; l <= c->grids; l++) {
+    double Ts = c->Tr_base / c->dsg[l], Tss = -c->Trr_base / c->ds2g[l], Tsss = c->Trrr_base / c->ds3g[l],
+           Tssss = c->Trrrr_base / c->ds4g[l];
+    double Tr = c->Tr_base / c->drg[l], Trr = -c->Trr_base / c->dr2g[l], Trrr = c->Trrr_base / c->dr3g[l],
+           Trrrr = c->Trrrr_base / c->dr4g[l];
+    double Trs = -c->Trs_base / c->drdsg[l], Trrss = c->Trrss_base / c->dr2ds2g[l],
+           Trss = c->Trrs_base / c->drds2g[l], Trrs = c->Trrs_base / c->dr2dsg[l];
+    /* real parts */
+    {
+      double Sr = 0, Ss = 0, Srr = Trr, Sss = Tss, Srrr = 0, Ssss = 0, Srrrr = Trrrr, Sssss = Tssss,
+             Srs = Trs, Srrs = 0, Srss = 0, Srrss = Trrss;
+      double Sx = Sr / xr, Sz = Ss / zs;
+      double Sxx = (Srr - Sx * xrr) / pow(xr, 2.0);
+      double Szz = (Sss - Sz * zss) / pow(zs, 2.0);
+      double Sxxx = (Srrr - 3.0 * Sxx * xr * xrr - Sx * xrrr) / pow(xr, 3.0);
+      double Szzz = (Ssss - 3.0 * Szz * zs * zss - Sz * zsss) / pow(zs, 3.0);
+      double Sxxxx = (Srrrr - 6.0 * Sxxx * xrr * pow(xr, 2.0) - (3.0 * pow(xrr, 2.0) + 4.0 * xr * xrrr) * Sxx - Sx * xrrrr) / pow(xr, 4.0);
+      double Szzzz = (Sssss - 6.0 * Szzz * zss * pow(zs, 2.0) - (3.0 * pow(zss, 2.0) + 4.0 * zs * zsss) * Szz - Sz * zssss) / pow(zs, 4.0);
+      double Sxz = Srs / xr / zs;
+      double Sxxz = (Srrs - Sxz * zs * xrr) / zs / pow(xr, 2.0);
+      double Sxzz = (Srss - Sxz * xr * zss) / xr / pow(zs, 2.0);
+      double Sxxzz = (Srrss - Sxzz * xrr * pow(zs, 2.0) - Sxxz * zss * pow(xr, 2.0) - Sxz * xrr * zss) / pow(xr * zs, 2.0);
+      c->SR1[l] = -(Sxx - Szz);
+      c->SR2[l] = -((Sxxxx + Szzzz) + 2.0 * Sxxzz);
+      c->SR4[l] = -(4.0 * Sxz);
+      c->SR5[l] = -(2.0 * (Sxxx + Sxzz));
+      c->SR6[l] = -(2.0 * (Szzz + Sxxz));
+    }
+    /* imaginary parts */
+    {
+      double Sr = Tr, Ss = Ts, Srr = 0, Sss = 0, Srrr = Trrr, Ssss = Tsss, Srrrr = 0, Sssss = 0,
+             Srs = 0, Srrs = Trrs, Srss = Trss, Srrss = 0;
+      double Sx = Sr / xr;
+      double Sxx = (Srr + fabs(Sx * xrr)) / pow(xr, 2.0);
+      double Sxxx = (Srrr + fabs(3.0 * Sxx * xr * xrr) + fabs(Sx * xrrr)) / fabs(pow(xr, 3.0));
+      double Sxxxx = (Srrrr + fabs(6.0 * Sxxx * xrr * pow(xr, 2.0)) + (fabs(3.0 * pow(xrr, 2.0) + 4.0 * xr * xrrr) * fabs(Sxx)) + fabs(Sx * xrrrr)) / pow(xr, 4.0);
+      double Sz = Ss / zs;
+      double Szz = (Sss + fabs(Sz * zss)) / pow(zs, 2.0);
+      double Szzz = (Ssss + fabs(3.0 * Szz * zs * zss) + fabs(Sz * zsss)) / fabs(pow(zs, 3.0));
+      double Szzzz = (Sssss + fabs(6.0 * Szzz * zss * pow(zs, 2.0)) + (fabs(3.0 * pow(zss, 2.0) + 4.0 * zs * zsss) * fabs(Szz)) + fabs(Sz * zssss)) / pow(zs, 4.0);
+      double Sxz = Srs / xr / zs;
+      double Sxxz = (Srrs + fabs(Sxz * zs * xrr)) / fabs(zs) / pow(xr, 2.0);
+      double Sxzz = (Srss + fabs(Sxz * xr * zss)) / fabs(xr) / pow(zs, 2.0);
+      double Sxxzz = (Srrss + fabs(Sxzz * xrr) * pow(zs, 2.0) + fabs(Sxxz * zss) * pow(xr, 2.0) + fabs(Sxz * xrr * zss)) / pow(xr * zs, 2.0);
+      c->SI1[l] = (Sxx + Szz);
+      c->SI2[l] = fabs(Sxxxx + Szzzz) + fabs(2.0 * Sxxzz);
+      c->SI4[l] = 4.0 * Sxz;
+      c->SI5[l] = 2.0 * (Sxxx + Sxzz);
+      c->SI6[l] = 2.0 * (Szzz + Sxxz);
+    }
+  }
+}
+
+/* stream_space_matrix_prep: PS/iterative_solver.f90:692-743 */
+static void stream_space_matrix_prep(pto_case* c) {
+  int g = c->grids;
+  for (int dk = -3; dk <= 3; dk++)
+    for (int di = -3; di <= 3; di++) {
+      double Sxx, Sxxx, Sxxxx, Szz, Szzz, Szzzz;
+      if (dk == 0) { Sxx = c->D2[di + 3] / c->dr2g[g]; Sxxx = c->D3[di + 3] / c->dr3g[g]; Sxxxx = c->D4[di + 3] / c->dr4g[g]; }
+      else { Sxx = 0; Sxxx = 0; Sxxxx = 0; }
+      if (di == 0) { Szz = c->D2[dk + 3] / c->ds2g[g]; Szzz = c->D3[dk + 3] / c->ds3g[g]; Szzzz = c->D4[dk + 3] / c->ds4g[g]; }
+      else { Szz = 0; Szzz = 0; Szzzz = 0; }
+      double Sxz = c->Drs[dk + 3][di + 3] / c->drdsg[g];
+      double Sxzz = c->Drss[dk + 3][di + 3] / c->drds2g[g];
+      double Sxxz = c->Drrs[dk + 3][di + 3] / c->dr2dsg[g];
+      double Sxxzz = c->Drrss[dk + 3][di + 3] / c->dr2ds2g[g];
+      c->SM1[dk + 3][di + 3] = Sxx - Szz;
+      c->SM2[dk + 3][di + 3] = Sxxxx + Szzzz + 2.0 * Sxxzz;
+      c->SM4[dk + 3][di + 3] = 4.0 * Sxz;
+      c->SM5[dk + 3][di + 3] = 2.0 * (Sxxx + Sxzz);
+      c->SM6[dk + 3][di + 3] = 2.0 * (Szzz + Sxxz);
+    }
+}
+
+/* ---- boundary conditions ---------------------------------------------- */
+/* enforceBCs_stream: PS/iterative_solver.f90:1616-1664 */
+static void enforce_bcs_stream(pto_case* c, int level, A2* S) {
+  int n = c->nxg[level], m = c->nzg[level], sp = c->span;
+  for (int k = -sp; k <= m + sp; k++) { AT(*S, 0, k) = 0.0; AT(*S, n, k) = 0.0; }
+  for (int i = -sp; i <= n + sp; i++) { AT(*S, i, 0) = 0.0; AT(*S, i, m) = 0.0; }
+  for (int k = 0; k <= m; k++) for (int i = 1; i <= sp; i++) AT(*S, -i, k) = -AT(*S, i, k);
+  for (int k = 0; k <= m; k++) for (int i = 1; i <= sp; i++) AT(*S, n + i, k) = -AT(*S, n - i, k);
+  if (c->P.Vbc == 0) {
+    for (int k = 1; k <= sp; k++) for (int i = -sp; i <= n + sp; i++) AT(*S, i, -k) = -AT(*S, i, k);
+    for (int k = 1; k <= sp; k++) for (int i = -sp; i <= n + sp; i++) AT(*S, i, m + k) = -AT(*S, i, m - k);
+  } else {
+    for (int k = 1; k <= sp; k++) for (int i = -sp; i <= n + sp; i++) AT(*S, i, -k) = AT(*S, i, k);
+    for (int k = 1; k <= sp; k++) for (int i = -sp; i <= n + sp; i++) AT(*S, i, m + k) = AT(*S, i, m - k);
+  }
+}
+
+/* compute_heating for comp=1, iheattype=0: PS/energy.f90:29-33 */
+static double heating(pto_case* c, int i, int k) {
+  double h = c->P.H * (1.0 - (0.0 + AT(c->Cvis, i, k)));
+  h = c->P.Htype * AT(c->Cvis, i, k) + h;
+  return h;
+}
+
+/* enforceBCs (span halo) and extendT (spanT halo): PS/energy.f90:558-630.
+ * Both reflect T about the walls; an isothermal wall value is held with
+ * ghost = 2*T_wall + (2/D2(0))*H*ds2 - mirror. */
+static void t_wall_ghosts(pto_case* c, A2* T0, int h) {
+  int nx = c->nx, nz = c->nz;
+  double two_over_d20 = 2.0 / c->D2[3];
+  for (int i = 1; i <= h; i++)
+    for (int k = 0; k <= nz; k++) { AT(*T0, -i, k) = AT(*T0, i, k); AT(*T0, nx + i, k) = AT(*T0, nx - i, k); }
+  for (int k = 1; k <= h; k++) {
+    if (c->P.Tbot == 0) {
+      for (int i = -h; i <= nx + h; i++) {
+        double hb = heating(c, i, 0);
+        AT(*T0, i, -k) = 2.0 * c->Tbot_iso + two_over_d20 * hb * c->ds2 - AT(*T0, i, k);
+      }
+    } else {
+      for (int i = -h; i <= nx + h; i++) AT(*T0, i, -k) = AT(*T0, i, k);
+    }
+    for (int i = -h; i <= nx + h; i++) {
+      double ht = heating(c, i, nz);
+      AT(*T0, i, nz + k) = 2.0 * c->Ttop_iso + two_over_d20 * ht * c->ds2 - AT(*T0, i, nz - k);
+    }
+  }
+  if (c->P.Tbot == 0) for (int i = -h; i <= nx + h; i++) AT(*T0, i, 0) = c->Tbot_iso;
+  for (int i = -h; i <= nx + h; i++) AT(*T0, i, nz) = c->Ttop_iso;
+}
+static void enforce_bcs_T(pto_case* c, A2* T0) { t_wall_ghosts(c, T0, c->span); }
+static void extend_T(pto_case* c, A2* T0) {
+  for (int k = 0; k <= c->nz; k++)
+    for (int i = 0; i <= c->nx; i++) AT(c->Textend, i, k) = AT(*T0, i, k);
+  t_wall_ghosts(c, &c->Textend, c->spanT);
+}
+
+/* enforceBCs_comp: PS/composition.f90:1279-1292 (spanT halo, loop order kept:
+ * the corner ghosts take whatever the side ghosts held when their row was
+ * reflected, exactly as the reference) */
+static void enforce_bcs_comp(pto_case* c, A2* C0) {
+  int nx = c->nx, nz = c->nz, h = c->spanT;
+  for (int i = 1; i <= h; i++) {
+    for (int k = 0; k <= nz; k++) { AT(*C0, -i, k) = AT(*C0, i, k); AT(*C0, nx + i, k) = AT(*C0, nx - i, k); }
+    for (int j = -h; j <= nx + h; j++) { AT(*C0, j, -i) = AT(*C0, j, i); AT(*C0, j, nz + i) = AT(*C0, j, nz - i); }
+  }
+}
+
+/* velocityBCs: PS/composition.f90:1295-1325 */
+static void velocity_bcs(pto_case* c) {
+  int nx = c->nx, nz = c->nz, h = c->span_interp;
+  A2* u = &c->u; A2* w = &c->w;
+  for (int i = 1; i <= h; i++) {
+    for (int k = 0; k <= nz; k++) { AT(*w, -i, k) = AT(*w, i, k); AT(*w, nx + i, k) = AT(*w, nx - i, k); }
+    if (c->P.Vbc == 0) {
+      for (int j = 0; j <= nx; j++) { AT(*u, j, -i) = AT(*u, j, i); AT(*u, j, nz + i) = AT(*u, j, nz - i); }
+    } else {
+      for (int j = 0; j <= nx; j++) { AT(*u, j, -i) = -AT(*u, j, i); AT(*u, j, nz + i) = -AT(*u, j, nz - i); }
+    }
+  }
+  for (int k = 1; k <= h; k++) {
+    for (int j = -h; j <= nx + h; j++) { AT(*w, j, -k) = -AT(*w, j, k); AT(*w, j, nz + k) = -AT(*w, j, nz - k); }
+    for (int j = -h; j <= nz + h; j++) { AT(*u, -k, j) = -AT(*u, k, j); AT(*u, nx + k, j) = -AT(*u, nx - k, j); }
+  }
+  for (int j = -h; j <= nz + h; j++) { AT(*u, 0, j) = 0.0; AT(*u, nx, j) = 0.0; }
+  for (int j = -h; j <= nx + h; j++) { AT(*w, j, 0) = 0.0; AT(*w, j, nz) = 0.0; }
+  if (c->P.Vbc == 1)
+    for (int j = -h; j <= nx + h; j++) { AT(*u, j, 0) = 0.0; AT(*u, j, nz) = 0.0; }
+}
+
+/* viscosityBCs: PS/iterative_solver.f90:666-690 */
+static void viscosity_bcs(pto_case* c, int l) {
+  A2* v = &c->vis_grid[l]; int n = c->nxg[l], m = c->nzg[l], sp = c->span;
+  for (int k = -sp; k <= m + sp; k++) for (int i = 1; i <= sp; i++) AT(*v, -i, k) = AT(*v, i, k);
+  for (int k = -sp; k <= m + sp; k++) for (int i = 1; i <= sp; i++) AT(*v, n + i, k) = AT(*v, n - i, k);
+  for (int k = 1; k <= sp; k++)
+    for (int i = 0; i <= n; i++) { AT(*v, i, -k) = AT(*v, i, k); AT(*v, i, m + k) = AT(*v, i, m - k); }
+}
+
+/* viscosityBCs_initialize: PS/iterative_solver.f90:636-664 */
+static void viscosity_bcs_initialize(pto_case* c, int l) {
+  A2* v = &c->vis_grid[l]; A2* f = &c->vis_grid[l - 1];
+  int n = c->nxg[l], m = c->nzg[l], sp = c->span;
+  for (int k = 1; k <= m - 1; k++) { AT(*v, 0, k) = AT(*f, 0, 2 * k); AT(*v, n, k) = AT(*f, c->nxg[l - 1], 2 * k); }
+  for (int i = 0; i <= n; i++) { AT(*v, i, 0) = AT(*f, 2 * i, 0); AT(*v, i, m) = AT(*f, 2 * i, c->nzg[l - 1]); }
+  for (int i = 1; i <= sp; i++)
+    for (int k = 0; k <= m; k++) { AT(*v, -i, k) = AT(*v, i, k); AT(*v, n + i, k) = AT(*v, n - i, k); }
+  for (int k = 1; k <= sp; k++)
+    for (int i = -sp; i <= n + sp; i++) { AT(*v, i, -k) = AT(*v, i, k); AT(*v, i, m + k) = AT(*v, i, m - k); }
+}
+
+/* ---- stencils ------------------------------------------------------------ */
+static inline double dot5x(const double* Dc, const A2* S, int i, int k) {   /* dot_product(D(-2:2),S(i-2:i+2,k)) */
+  double s = 0.0;
+  for (int a = -2; a <= 2; a++) s = s + Dc[a + 3] * AT(*S, i + a, k);
+  return s;
+}
+static inline double dot5z(const double* Dc, const A2* S, int i, int k) {
+  double s = 0.0;
+  for (int a = -2; a <= 2; a++) s = s + Dc[a + 3] * AT(*S, i, k + a);
+  return s;
+}
+static inline double dot7x(const double* Dc, const A2* S, int i, int k) {
+  double s = 0.0;
+  for (int a = -3; a <= 3; a++) s = s + Dc[a + 3] * AT(*S, i + a, k);
+  return s;
+}
+static inline double dot7z(const double* Dc, const A2* S, int i, int k) {
+  double s = 0.0;
+  for (int a = -3; a <= 3; a++) s = s + Dc[a + 3] * AT(*S, i, k + a);
+  return s;
+}
+static inline double sum5x5(double M[7][7], const A2* S, int i, int k) {   /* sum(M(-2:2,-2:2)*S(...)) */
+  double s = 0.0;
+  for (int b = -2; b <= 2; b++)
+    for (int a = -2; a <= 2; a++) s = s + M[b + 3][a + 3] * AT(*S, i + a, k + b);
+  return s;
+}
+
+/* stream_derivatives (uniform grid): PS/iterative_solver.f90:985-1018 */
+static void stream_derivatives(pto_case* c, int l, const A2* S0) {
+  int n = c->nxg[l], m = c->nzg[l];
+  for (int k = 0; k <= m; k++)
+    for (int i = 0; i <= n; i++) {
+      AT(c->SFxx, i, k) = dot5x(c->D2, S0, i, k) / c->dr2g[l];
+      AT(c->SFxxx, i, k) = dot7x(c->D3, S0, i, k) / c->dr3g[l];
+      AT(c->SFxxxx, i, k) = dot7x(c->D4, S0, i, k) / c->dr4g[l];
+    }
+  for (int k = 0; k <= m; k++)
+    for (int i = 0; i <= n; i++) {
+      AT(c->SFzz, i, k) = dot5z(c->D2, S0, i, k) / c->ds2g[l];
+      AT(c->SFzzz, i, k) = dot7z(c->D3, S0, i, k) / c->ds3g[l];
+      AT(c->SFzzzz, i, k) = dot7z(c->D4, S0, i, k) / c->ds4g[l];
+    }
+  for (int k = 0; k <= m; k++)
+    for (int i = 0; i <= n; i++) {
+      AT(c->SFxz, i, k) = sum5x5(c->Drs, S0, i, k) / c->drdsg[l];
+      AT(c->SFxxz, i, k) = sum5x5(c->Drrs, S0, i, k) / c->dr2dsg[l];
+      AT(c->SFxzz, i, k) = sum5x5(c->Drss, S0, i, k) / c->drds2g[l];
+      AT(c->SFxxzz, i, k) = sum5x5(c->Drrss, S0, i, k) / c->dr2ds2g[l];
+    }
+}
+
+/* stream_space: PS/iterative_solver.f90:890-911 (RHS - L psi) */
+static inline double stream_space(pto_case* c, int l, int i, int k) {
+  double s;
+  if (c->P.ivis == 0) {
+    double t2 = AT(c->vis_grid[l], i, k) * (AT(c->SFxxxx, i, k) + AT(c->SFzzzz, i, k) + 2.0 * AT(c->SFxxzz, i, k));
+    s = t2 - AT(c->RHS[l], i, k);
+  } else {
+    double t1 = AT(c->vis_xxmzz[l], i, k) * (AT(c->SFxx, i, k) - AT(c->SFzz, i, k));
+    double t2 = AT(c->vis_grid[l], i, k) * (AT(c->SFxxxx, i, k) + AT(c->SFzzzz, i, k) + 2.0 * AT(c->SFxxzz, i, k));
+    double t4 = 4.0 * AT(c->vis_xz[l], i, k) * AT(c->SFxz, i, k);
+    double t5 = 2.0 * AT(c->vis_x[l], i, k) * (AT(c->SFxxx, i, k) + AT(c->SFxzz, i, k));
+    double t6 = 2.0 * AT(c->vis_z[l], i, k) * (AT(c->SFzzz, i, k) + AT(c->SFxxz, i, k));
+    s = t1 + t2 + t4 + t5 + t6 - AT(c->RHS[l], i, k);
+  }
+  return -s;
+}
+
+/* compute_residual: PS/iterative_solver.f90:415-448 */
+static int compute_residual(pto_case* c, int l, int p) {
+  int n = c->nxg[l], m = c->nzg[l];
+  double rpart = 0.0;
+  for (int k = 1; k <= m - 1; k++)
+    for (int i = 1; i <= n - 1; i++) {
+      double r = stream_space(c, l, i, k);
+      AT(c->residual[l], i, k) = r;
+      rpart = rpart + fabs(r);
+    }
+  c->residual_mag[p] = rpart / (double)(n - 2) / (double)(m - 2);
+  if (l == 1 && c->residual_mag[p] > 1.0e10 * c->res_ref[l]) return PT_ERR_HIGH_RESIDUAL;
+  return PT_OK;
+}
+
+/* compute_reference_residual: PS/iterative_solver.f90:450-470 */
+static void compute_reference_residual(pto_case* c, int l) {
+  int n = c->nxg[l], m = c->nzg[l];
+  double part = 0.0;
+  for (int k = 1; k <= m - 1; k++)
+    for (int i = 1; i <= n - 1; i++) part = part + fabs(AT(c->RHS[l], i, k));
+  c->res_ref[l] = part / (double)(n - 2) / (double)(m - 2);
+}
+
+/* restrict / restrict_coefficients: PS/iterative_solver.f90:1107-1145 */
+static void restrict_full_weighting(pto_case* c, int l, const A2* in, A2* out) {
+  for (int k = 1; k <= c->nzg[l] - 1; k++) {
+    int kk = 2 * k;
+    for (int i = 1; i <= c->nxg[l] - 1; i++) {
+      int ii = 2 * i;
+      double corners = (AT(*in, ii - 1, kk - 1) + AT(*in, ii - 1, kk + 1) + AT(*in, ii + 1, kk + 1) + AT(*in, ii + 1, kk - 1)) / 16.0;
+      double sides = (AT(*in, ii - 1, kk) + AT(*in, ii, kk - 1) + AT(*in, ii + 1, kk) + AT(*in, ii, kk + 1)) / 8.0;
+      double centre = AT(*in, ii, kk) / 4.0;
+      AT(*out, i, k) = corners + sides + centre;
+    }
+  }
+}
+
+/* compute_RHS: PS/iterative_solver.f90:1075-1105 */
+static void compute_rhs(pto_case* c, int l) {
+  if (l == 1) {
+    for (int k = 1; k <= c->nz - 1; k++)
+      for (int i = 1; i <= c->nx - 1; i++) {
+        double rhs = 0.0;
+        if (c->P.RaT != 0.0) {
+          double Tr = dot5x(c->D1, &c->Tbuoy, i, k) / c->dr;
+          double Tx = Tr / 1.0;
+          rhs = c->P.RaT * Tx;
+        }
+        double Cr = dot5x(c->D1, &c->Cbuoy, i, k) / c->dr;
+        double Cx = Cr / 1.0;
+        rhs = rhs - c->P.RaC * Cx;
+        AT(c->RHS[1], i, k) = rhs;
+      }
+  } else {
+    restrict_full_weighting(c, l, &c->residual[l - 1], &c->RHS[l]);
+  }
+}
+
+/* prolong: PS/iterative_solver.f90:343-396 */
+static void prolong(pto_case* c, int lc) {
+  int lf = lc - 1;
+  A2* ec = &c->error[lc]; A2* ef = &c->error[lf];
+  for (int k = 1; k <= c->nzg[lc] - 1; k++)
+    for (int i = 1; i <= c->nxg[lc] - 1; i++) AT(*ef, 2 * i, 2 * k) = AT(*ef, 2 * i, 2 * k) + AT(*ec, i, k);
+  for (int k = 1; k <= c->nzg[lf] - 1; k += 2)
+    for (int i = 2; i <= c->nxg[lf] - 2; i += 2)
+      AT(*ef, i, k) = AT(*ef, i, k) + (AT(*ec, i / 2, (k - 1) / 2) + AT(*ec, i / 2, (k + 1) / 2)) / 2.0;
+  for (int k = 2; k <= c->nzg[lf] - 2; k += 2)
+    for (int i = 1; i <= c->nxg[lf] - 1; i += 2)
+      AT(*ef, i, k) = AT(*ef, i, k) + (AT(*ec, (i - 1) / 2, k / 2) + AT(*ec, (i + 1) / 2, k / 2)) / 2.0;
+  for (int k = 1; k <= c->nzg[lf] - 1; k += 2)
+    for (int i = 1; i <= c->nxg[lf] - 1; i += 2)
+      AT(*ef, i, k) = AT(*ef, i, k) + (AT(*ec, (i - 1) / 2, (k - 1) / 2) + AT(*ec, (i + 1) / 2, (k - 1) / 2) +
+                                       AT(*ec, (i + 1) / 2, (k + 1) / 2) + AT(*ec, (i - 1) / 2, (k + 1) / 2)) / 4.0;
+  enforce_bcs_stream(c, lf, ef);
+}
+
+/* ---- viscosity hierarchy ------------------------------------------------- */
+/* visf: PS/rheology.f90:2-23 */
+static double visf(pto_case* c, int i, int k) {
+  const pt_params* P = &c->P;
+  double zg = AT(c->zg, 0, k);
+  if (P->ivis == 1) return exp(-(AT(c->Tvis, i, k) * log(P->visT)) + (1.0 - zg) * log(P->visP));
+  if (P->ivis == 2) {
+    double dp = 0.0 + AT(c->Cvis, i, k) * log(P->visC);
+    if (P->RaT != 0.0) return exp(-(AT(c->Tvis, i, k) * log(P->visT)) - dp + (1.0 - zg) * log(P->visP));
+    return exp(-dp + (1.0 - zg) * log(P->visP));
+  }
+  return 1.0;
+}
+
+/* viscosity_gradients (uniform grid): PS/iterative_solver.f90:472-520 */
+static void viscosity_gradients(pto_case* c, int l, int n_full) {
+  int n = c->nxg[l], m = c->nzg[l];
+  A2* v = &c->vis_grid[l];
+  if (c->P.ivis == 0) {
+    for (int k = 0; k <= m; k++) for (int i = 0; i <= n; i++) {
+      AT(c->vis_x[l], i, k) = 0; AT(c->vis_z[l], i, k) = 0; AT(c->vis_xx[l], i, k) = 0; AT(c->vis_zz[l], i, k) = 0;
+      AT(c->vis_xz[l], i, k) = 0; AT(c->vis_xxpzz[l], i, k) = 0; AT(c->vis_xxmzz[l], i, k) = 0;
+    }
+    return;
+  }
+  for (int k = 0; k <= m; k++)
+    for (int i = 0; i <= n; i++) {
+      if (n_full) {
+        double sxz = 0.0;
+        for (int b = -1; b <= 1; b++) for (int a = -1; a <= 1; a++) sxz = sxz + c->Drs_so[b + 1][a + 1] * AT(*v, i + a, k + b);
+        AT(c->vis_xz[l], i, k) = sxz / c->drdsg[l];
+        double sx = 0.0; for (int a = -1; a <= 1; a++) sx = sx + c->D1_so[a + 1] * AT(*v, i + a, k);
+        AT(c->vis_x[l], i, k) = sx / c->drg[l];
+      }
+      double sxx = 0.0; for (int a = -1; a <= 1; a++) sxx = sxx + c->D2_so[a + 1] * AT(*v, i + a, k);
+      AT(c->vis_xx[l], i, k) = sxx / c->dr2g[l];
+      if (n_full) {
+        double sz = 0.0; for (int a = -1; a <= 1; a++) sz = sz + c->D1_so[a + 1] * AT(*v, i, k + a);
+        AT(c->vis_z[l], i, k) = sz / c->dsg[l];
+      }
+      double szz = 0.0; for (int a = -1; a <= 1; a++) szz = szz + c->D2_so[a + 1] * AT(*v, i, k + a);
+      AT(c->vis_zz[l], i, k) = szz / c->ds2g[l];
+      AT(c->vis_xxpzz[l], i, k) = AT(c->vis_xx[l], i, k) + AT(c->vis_zz[l], i, k);
+      if (n_full) AT(c->vis_xxmzz[l], i, k) = AT(c->vis_xx[l], i, k) - AT(c->vis_zz[l], i, k);
+      AT(c->vis_smooth[l], i, k) = AT(c->dt_vis[l], i, k) * AT(c->vis_xxpzz[l], i, k);
+    }
+}
+
+/* test_eigenvalues (RKC1 branch): PS/iterative_solver.f90:1187-1208 */
+static void test_eigenvalues(pto_case* c, int l) {
+  double ae = -(c->RKC1_stability / 2.0);
+  double be = (double)c->P.nstageS;
+  double ctemp = c->P.courant_stream * (c->RKC1_stability / (ae * ae));
+  for (int k = 1; k <= c->nzg[l] - 1; k++)
+    for (int i = 1; i <= c->nxg[l] - 1; i++) {
+      double eI = fabs(AT(c->vis_xxpzz[l], i, k) * c->SI1[l]) + AT(c->vis_grid[l], i, k) * c->SI2[l] +
+                  fabs(AT(c->vis_xz[l], i, k) * c->SI4[l]) + fabs(AT(c->vis_x[l], i, k) * c->SI5[l]) +
+                  fabs(AT(c->vis_z[l], i, k) * c->SI6[l]);
+      double eR = AT(c->vis_xxmzz[l], i, k) * c->SR1[l] + AT(c->vis_grid[l], i, k) * c->SR2[l] +
+                  AT(c->vis_xz[l], i, k) * c->SR4[l] + AT(c->vis_x[l], i, k) * c->SR5[l] +
+                  AT(c->vis_z[l], i, k) * c->SR6[l];
+      AT(c->ratio[l], i, k) = fabs(eR / eI);
+      double a = eR / ae, b = eI / be;
+      AT(c->dt_stream[l], i, k) = ctemp * eR / (a * a + b * b);
+    }
+}
+
+/* smoother_vis: PS/smoother.f90:454-580 */
+static void smoother_vis(pto_case* c, int l, int interpolate) {
+  viscosity_bcs(c, l);
+  int option, Niter; double lb, la;
+  if (interpolate == 0) { option = 1; Niter = 3; lb = -3.0; la = -2.0; }
+  else { option = 0; Niter = c->P.vis_iter; lb = -2.0; la = -1.0; }
+  c->Nvis_store = 0; c->nn_store = 1;
+  int n = 0, trigger = 0;
+  double ratio_min = 1.0e100;
+  for (;;) {
+    for (int nn = c->nn_store; nn <= Niter; nn++) {
+      viscosity_gradients(c, l, c->Nvis_store);
+      if (interpolate == 0) {
+        if (c->Nvis_store >= 1 || c->P.ivis == 0) {
+          test_eigenvalues(c, l);
+          ratio_min = 1.0e100;
+          for (int k = 1; k <= c->nzg[l] - 1; k++)
+            for (int i = 1; i <= c->nxg[l] - 1; i++)
+              if (AT(c->ratio[l], i, k) < ratio_min) ratio_min = AT(c->ratio[l], i, k);
+        }
+      }
+      if (option == 1) {
+        if ((c->Nvis_store >= 1 && ratio_min > c->P.eig_ratio_min) || c->P.ivis == 0 || n > 20) {
+          c->eig_fine = ratio_min; c->n_eta = n; trigger = 1; break;
+        }
+      } else {
+        if (interpolate >= 1 && (c->Nvis_store >= 1 || c->P.ivis == 0)) { trigger = 1; break; }
+      }
+      int ntemp = Niter - nn + 1;
+      double cc = 2.0 / (-lb - la + (lb - la) * cos((double)(2 * ntemp - 1) * PII / (double)(2 * Niter)));
+      cc = 0.99 * cc;
+      for (int k = 0; k <= c->nzg[l]; k++)
+        for (int i = 0; i <= c->nxg[l]; i++)
+          AT(c->vis_grid[l], i, k) = AT(c->vis_grid[l], i, k) + cc * AT(c->vis_smooth[l], i, k);
+      viscosity_bcs(c, l);
+    }
+    if (trigger) break;
+    c->Nvis_store = c->Nvis_store + 1;
+    c->nn_store = 1;
+    if (option == 1) { c->nn_store = 1; Niter = 6; lb = -2.0; la = -1.0; }
+    n = n + 1;
+  }
+}
+
+/* compute_viscosity (Newtonian, static=1): PS/iterative_solver.f90:560-634 */
+static void compute_viscosity(pto_case* c, int l) {
+  if (l == 1) {
+    for (int k = 0; k <= c->nz; k++)
+      for (int i = 0; i <= c->nx; i++) AT(c->vis_grid_static[1], i, k) = visf(c, i, k);
+    a2_copy(&c->vis_grid[1], &c->vis_grid_static[1]);
+    smoother_vis(c, 1, 0);
+  } else {
+    int f = l - 1;
+    A2 save[9];
+    A2* src[9] = {&c->vis_grid[f], &c->vis_x[f], &c->vis_z[f], &c->vis_xx[f], &c->vis_zz[f], &c->vis_xz[f],
+                  &c->vis_xxpzz[f], &c->vis_xxmzz[f], &c->vis_smooth[f]};
+    for (int j = 0; j < 9; j++) {
+      save[j] = a2_new(src[j]->ilo, src[j]->ilo + src[j]->ni - 1, src[j]->klo, src[j]->klo + src[j]->nk - 1);
+      a2_copy(&save[j], src[j]);
+    }
+    a2_copy(&c->vis_grid[f], &c->vis_grid_static[f]);
+    smoother_vis(c, f, 1);
+    restrict_full_weighting(c, l, &c->vis_grid[f], &c->vis_grid[l]);
+    viscosity_bcs_initialize(c, l);
+    a2_copy(&c->vis_grid_static[l], &c->vis_grid[l]);
+    smoother_vis(c, l, 0);
+    for (int j = 0; j < 9; j++) { a2_copy(src[j], &save[j]); a2_free(&save[j]); }
+  }
+}
+
+/* ---- coarsest grid ----------------------------------------------------------- */
+static inline int pf_coarse(pto_case* c, int ip, int kp) { return (c->nzg[c->grids] + 1) * ip + kp + 1; }
+static inline void indices_coarse(pto_case* c, int p, int* ip, int* kp) {
+  int N = c->nzg[c->grids] + 1;
+  *kp = p - N * ((p - 1) / N) - 1;
+  *ip = (p - 1) / N;
+}
+
+/* stream_space_matrix (uniform): PS/iterative_solver.f90:745-825 */
+static double stream_space_matrix(pto_case* c, int i, int k, int iq, int kq, int Sval) {
+  int g = c->grids, di = iq - i, dk = kq - k;
+  double v;
+  if (c->P.ivis == 0) v = AT(c->vis_grid[g], i, k) * c->SM2[dk + 3][di + 3];
+  else
+    v = AT(c->vis_xxmzz[g], i, k) * c->SM1[dk + 3][di + 3] + AT(c->vis_grid[g], i, k) * c->SM2[dk + 3][di + 3] +
+        AT(c->vis_xz[g], i, k) * c->SM4[dk + 3][di + 3] + AT(c->vis_x[g], i, k) * c->SM5[dk + 3][di + 3] +
+        AT(c->vis_z[g], i, k) * c->SM6[dk + 3][di + 3];
+  if (Sval == -1) v = -v;
+  return v;
+}
+
+static const int stencil_limit[7] = {0, 2, 2, 3, 2, 2, 0};   /* limit(-3:3) */
+
+/* Compute_Matrix_coarse analysis=1 (bandwidth): PS/iterative_solver.f90:1723-1745 */
+static void coarse_bandwidth(pto_case* c) {
+  int nxc = c->nxg[c->grids], nzc = c->nzg[c->grids];
+  c->ngc = (nxc + 1) * (nzc + 1);
+  int ku = 0, kl = 0;
+  for (int p = 1; p <= c->ngc; p++) {
+    int ip, kp; indices_coarse(c, p, &ip, &kp);
+    for (int j = -3; j <= 3; j++) {
+      int iq = j + ip;
+      if (iq > 0 && iq < nxc)
+        for (int kq = -stencil_limit[j + 3] + kp; kq <= kp + stencil_limit[j + 3]; kq++)
+          if (kq > 0 && kq < nzc) {
+            int t = pf_coarse(c, iq, kq) - p;
+            if (t > ku) ku = t;
+            if (-t > kl) kl = -t;
+          }
+    }
+  }
+  c->kl = kl; c->ku = ku;
+}
+
+/* Compute_Matrix_coarse analysis=0 + DGBEQU row scaling: PS/iterative_solver.f90:1747-1839 */
+static void compute_matrix_coarse(pto_case* c) {
+  int nxc = c->nxg[c->grids], nzc = c->nzg[c->grids];
+  int kl = c->kl, ku = c->ku, ld = 2 * kl + ku + 1, kk1 = kl + ku + 1, n = c->ngc;
+#define MC(r, q) c->Mc[(long)((q) - 1) * ld + ((r) - 1)]
+  memset(c->Mc, 0, sizeof(double) * (size_t)ld * n);
+  for (int p = 1; p <= n; p++) {
+    int ip, kp; indices_coarse(c, p, &ip, &kp);
+    if (ip > 0 && ip < nxc && kp > 0 && kp < nzc) {
+      for (int j = -3; j <= 3; j++) {
+        int iq = j + ip;
+        if (iq >= 0 && iq <= nxc)
+          for (int kq = -stencil_limit[j + 3] + kp; kq <= kp + stencil_limit[j + 3]; kq++)
+            if (kq >= 0 && kq <= nzc) {
+              int q = pf_coarse(c, iq, kq);
+              MC(kk1 + p - q, q) = stream_space_matrix(c, ip, kp, iq, kq, 1);
+            }
+      }
+      for (int j = -3; j <= 3; j++) {
+        int iq = j + ip;
+        for (int kq = -stencil_limit[j + 3] + kp; kq <= kp + stencil_limit[j + 3]; kq++)
+          if (iq < 0 || iq > nxc || kq < 0 || kq > nzc) {
+            int iiq, kkq, xval, zval;
+            if (iq < 0) { iiq = -iq; xval = -1; }
+            else if (iq > nxc) { iiq = 2 * nxc - iq; xval = -1; }
+            else { iiq = iq; xval = 1; }
+            if (kq < 0) { kkq = -kq; zval = (c->P.Vbc == 0) ? -1 : 1; }
+            else if (kq > nzc) { kkq = 2 * nzc - kq; zval = (c->P.Vbc == 0) ? -1 : 1; }
+            else { kkq = kq; zval = 1; }
+            int q = pf_coarse(c, iiq, kkq);
+            MC(kk1 + p - q, q) = MC(kk1 + p - q, q) + stream_space_matrix(c, ip, kp, iq, kq, xval * zval);
+          }
+      }
+    }
+  }
+  for (int kp = 0; kp <= nzc; kp++) { MC(kk1, pf_coarse(c, 0, kp)) = 1.0; MC(kk1, pf_coarse(c, nxc, kp)) = 1.0; }
+  for (int ip = 0; ip <= nxc; ip++) { MC(kk1, pf_coarse(c, ip, 0)) = 1.0; MC(kk1, pf_coarse(c, ip, nzc)) = 1.0; }
+  /* DGBEQU row scale factors: R(i) = 1/max_j |A(i,j)| (LAPACK dgbequ) */
+  for (int i = 1; i <= n; i++) c->RowB[i - 1] = 0.0;
+  for (int q = 1; q <= n; q++) {
+    int lo = q - ku > 1 ? q - ku : 1, hi = q + kl < n ? q + kl : n;
+    for (int p = lo; p <= hi; p++) {
+      double a = fabs(MC(kk1 + p - q, q));
+      if (a > c->RowB[p - 1]) c->RowB[p - 1] = a;
+    }
+  }
+  double smlnum = DBL_MIN, bignum = 1.0 / smlnum;
+  for (int i = 1; i <= n; i++) {
+    double r = c->RowB[i - 1];
+    if (r < smlnum) r = smlnum;
+    if (r > bignum) r = bignum;
+    c->RowB[i - 1] = 1.0 / r;
+  }
+  for (int q = 1; q <= n; q++) {
+    int lo = q - kl > 1 ? q - kl : 1, hi = q + ku < n ? q + ku : n;
+    for (int p = lo; p <= hi; p++) MC(kk1 + p - q, q) = c->RowB[p - 1] * MC(kk1 + p - q, q);
+  }
+#undef MC
+}
+
+/* DGBTF2 + DGBTRS('N') restatement (LAPACK 3.x unblocked band LU with
+ * partial pivoting; AB has LDAB = 2*kl+ku+1, KV = ku+kl). */
+static int band_lu(int n, int kl, int ku, double* ab, int32_t* ipiv) {
+  int ld = 2 * kl + ku + 1, kv = ku + kl, info = 0;
+#define AB(r, q) ab[(long)((q) - 1) * ld + ((r) - 1)]
+  for (int j = ku + 2; j <= (kv < n ? kv : n); j++)
+    for (int i = kv - j + 2; i <= kl; i++) AB(i, j) = 0.0;
+  int ju = 1;
+  for (int j = 1; j <= n; j++) {
+    if (j + kv <= n) for (int i = 1; i <= kl; i++) AB(i, j + kv) = 0.0;
+    int km = kl < n - j ? kl : n - j;
+    int jp = 1; double amax = fabs(AB(kv + 1, j));
+    for (int t = 2; t <= km + 1; t++) { double a = fabs(AB(kv + t, j)); if (a > amax) { amax = a; jp = t; } }
+    ipiv[j - 1] = jp + j - 1;
+    if (AB(kv + jp, j) != 0.0) {
+      int lim = j + ku + jp - 1; if (lim > n) lim = n;
+      if (lim > ju) ju = lim;
+      if (jp != 1)
+        for (int t = 0; t <= ju - j; t++) {   /* DSWAP with stride LDAB-1 */
+          double* a = &ab[(long)(j - 1 + t) * ld + (kv + jp - 1 - t)];
+          double* b = &ab[(long)(j - 1 + t) * ld + (kv - t)];
+          double tmp = *a; *a = *b; *b = tmp;
+        }
+      if (km > 0) {
+        double rcp = 1.0 / AB(kv + 1, j);
+        for (int t = 1; t <= km; t++) AB(kv + 1 + t, j) = rcp * AB(kv + 1 + t, j);
+        if (ju > j)
+          for (int jj = 1; jj <= ju - j; jj++) {   /* DGER: A(i,jj) += x(i)*(-y(jj)) */
+            double yv = ab[(long)(j - 1 + jj) * ld + (kv - jj)];
+            double temp = -1.0 * yv;
+            for (int t = 1; t <= km; t++)
+              ab[(long)(j - 1 + jj) * ld + (kv - jj + t)] = ab[(long)(j - 1 + jj) * ld + (kv - jj + t)] + AB(kv + 1 + t, j) * temp;
+          }
+      }
+    } else if (info == 0) info = j;
+  }
+  return info;
+}
+static void band_solve(int n, int kl, int ku, const double* ab, const int32_t* ipiv, double* b) {
+  int ld = 2 * kl + ku + 1, kd = ku + kl + 1;
+  if (kl > 0)
+    for (int j = 1; j <= n - 1; j++) {
+      int lm = kl < n - j ? kl : n - j;
+      int l = ipiv[j - 1];
+      if (l != j) { double t = b[l - 1]; b[l - 1] = b[j - 1]; b[j - 1] = t; }
+      double temp = -1.0 * b[j - 1];
+      for (int t = 1; t <= lm; t++) b[j - 1 + t] = b[j - 1 + t] + AB(kd + t, j) * temp;
+    }
+  int kk = kl + ku, kplus1 = kk + 1;
+  for (int j = n; j >= 1; j--) {
+    if (b[j - 1] != 0.0) {
+      int l = kplus1 - j;
+      b[j - 1] = b[j - 1] / AB(kplus1, j);
+      double temp = b[j - 1];
+      int lo = j - kk > 1 ? j - kk : 1;
+      for (int i = j - 1; i >= lo; i--) b[i - 1] = b[i - 1] - temp * AB(l + i, j);
+    }
+  }
+#undef AB
+}
+int pto_band_lu_solve(int n, int kl, int ku, double* ab, int32_t* ipiv, double* b) {
+  int info = band_lu(n, kl, ku, ab, ipiv);
+  if (info == 0 && b) band_solve(n, kl, ku, ab, ipiv, b);
+  return info;
+}
+
+/* ---- smoother: iterate_stream (RKC1, Nmulti iterations) --------------------- */
+/* PS/iterative_solver.f90:1418-1614 */
+static int iterate_stream(pto_case* c, int l, int Niter, int p) {
+  int n = c->nxg[l], m = c->nzg[l], sp = c->span, s = c->P.nstageS;
+  for (int it = 1; it <= Niter; it++) {
+    a2_copy(&c->S0[l], &c->error[l]);
+    stream_derivatives(c, l, &c->S0[l]);
+    int ntemp = Niter - it + 1;
+    double temp = (c->P.Nmulti > 1) ? (c->P.c_2 - c->P.c_1) * (double)(ntemp - 1) / (double)(1 - Niter) : 0.0;
+    double cc = temp + c->P.c_2;
+    for (int k = 1; k <= m - 1; k++)
+      for (int i = 1; i <= n - 1; i++) {
+        AT(c->S1[l], i, k) = AT(c->S0[l], i, k) + c->mewtChebS[1] * cc * AT(c->dt_stream[l], i, k) * stream_space(c, l, i, k);
+        AT(c->S2[l], i, k) = AT(c->S0[l], i, k);
+      }
+    enforce_bcs_stream(c, l, &c->S1[l]);
+    for (int j = 2; j <= s; j++) {
+      stream_derivatives(c, l, &c->S1[l]);
+      for (int k = 1; k <= m - 1; k++)
+        for (int i = 1; i <= n - 1; i++)
+          AT(c->error[l], i, k) = c->cChebS[j] * AT(c->S0[l], i, k) + c->mewChebS[j] * AT(c->S1[l], i, k) +
+                                  c->vChebS[j] * AT(c->S2[l], i, k) +
+                                  c->mewtChebS[j] * cc * AT(c->dt_stream[l], i, k) * stream_space(c, l, i, k);
+      enforce_bcs_stream(c, l, &c->error[l]);
+      if (j < s)
+        for (int k = -sp; k <= m + sp; k++)
+          for (int i = -sp; i <= n + sp; i++) { AT(c->S2[l], i, k) = AT(c->S1[l], i, k); AT(c->S1[l], i, k) = AT(c->error[l], i, k); }
+    }
+  }
+  stream_derivatives(c, l, &c->error[l]);
+  return compute_residual(c, l, p);
+}
+
+/* multigrid: PS/iterative_solver.f90:124-341 */
+int pto_multigrid(pto_case* c) {
+  int rc;
+  a2_copy(&c->error[1], &c->SF);
+  compute_rhs(c, 1);
+  compute_reference_residual(c, 1);
+  for (int l = 1; l <= c->grids; l++) compute_viscosity(c, l);
+  if (c->time == 0.0 || c->P.ivis > 0) {
+    compute_matrix_coarse(c);
+    int info = band_lu(c->ngc, c->kl, c->ku, c->Mc, c->ipiv);
+    if (info != 0) return PT_ERR_SINGULAR_COARSE;
+  }
+  int q = 1, done = 0;
+  for (;;) {
+    for (int p = 1; p <= c->nsteps; p++) {
+      int l = c->multi[p];
+      if (c->multi[p - 1] < c->multi[p]) {
+        compute_rhs(c, l);
+        if (p <= c->grids && q == 1) compute_reference_residual(c, l);
+      }
+      if (l < c->grids) {
+        rc = iterate_stream(c, l, c->P.Nmulti, p);
+        if (rc) return rc;
+      } else {
+        int nxc = c->nxg[l], nzc = c->nzg[l];
+        for (int t = 0; t < c->ngc; t++) c->Bc[t] = 0.0;
+        for (int k = 1; k <= nzc - 1; k++)
+          for (int i = 1; i <= nxc - 1; i++) {
+            int pp = pf_coarse(c, i, k);
+            c->Bc[pp - 1] = AT(c->RHS[l], i, k) * c->RowB[pp - 1];
+          }
+        band_solve(c->ngc, c->kl, c->ku, c->Mc, c->ipiv, c->Bc);
+        for (int pp = 1; pp <= c->ngc; pp++) {
+          int ip, kp; indices_coarse(c, pp, &ip, &kp);
+          AT(c->error[l], ip, kp) = c->Bc[pp - 1];
+        }
+        enforce_bcs_stream(c, l, &c->error[l]);
+      }
+      if (l == 1 && (c->residual_mag[1] / c->res_ref[1]) <= c->P.tolerance) { done = 1; break; }
+      if (c->multi[p + 1] < c->multi[p]) prolong(c, l);
+    }
+    c->cycles++;
+    if (done || (c->residual_mag[1] / c->res_ref[1]) <= c->P.tolerance) break;
+    q = q + 1;
+    if (q > c->P.max_cycles) return PT_ERR_MAX_CYCLES;
+  }
+  a2_copy(&c->SF, &c->error[1]);
+  c->last_q = q; c->last_res = c->residual_mag[1]; c->last_res_ref = c->res_ref[1];
+  c->solves++;
+  return PT_OK;
+}
+
+/* compute_velocities (uniform): PS/momentum.f90:655-710 */
+void pto_compute_velocities(pto_case* c) {
+  for (int k = 0; k <= c->nz; k++)
+    for (int i = 0; i <= c->nx; i++) AT(c->w, i, k) = -(dot5x(c->D1, &c->SF, i, k) / c->dr);
+  for (int k = 0; k <= c->nz; k++)
+    for (int i = 0; i <= c->nx; i++) AT(c->u, i, k) = dot5z(c->D1, &c->SF, i, k) / c->ds;
+  velocity_bcs(c);
+}
+
+int pto_update_velocities(pto_case* c) {
+  int rc = pto_multigrid(c);
+  if (rc) return rc;
+  pto_compute_velocities(c);
+  return PT_OK;
+}
+
+/* ---- energy / tracers ---------------------------------------------------------- */
+/* smoothness: PS/smoother.f90:75-95 */
+static double smoothness(pto_case* c, const A2* T0) {
+  double diff = 0.0;
+  for (int i = 0; i <= c->nx - 1; i++)
+    for (int k = 1; k <= c->nz - 1; k++) {
+      double t = AT(*T0, i, k);
+      double a1 = fabs(AT(*T0, i, k - 1) - t), a2 = fabs(AT(*T0, i + 1, k - 1) - t), a3 = fabs(AT(*T0, i + 1, k) - t),
+             a4 = fabs(AT(*T0, i, k + 1) - t), a5 = fabs(AT(*T0, i + 1, k + 1) - t);
+      double a = a1;
+      if (a2 > a) a = a2;
+      if (a3 > a) a = a3;
+      if (a4 > a) a = a4;
+      if (a5 > a) a = a5;
+      if (a > diff) diff = a;
+    }
+  return diff;
+}
+
+/* smoother_space (uniform metrics): PS/smoother.f90:121-139 */
+static inline double smoother_space(pto_case* c, const A2* T0, int i, int k) {
+  double Tr = dot5x(c->D1, T0, i, k) / c->dr, Ts = dot5z(c->D1, T0, i, k) / c->ds;
+  double Trr = dot5x(c->D2, T0, i, k) / c->dr2, Tss = dot5z(c->D2, T0, i, k) / c->ds2;
+  double Tx = Tr / 1.0, Tz = Ts / 1.0;
+  double Txx = (Trr - Tx * 0.0) / 1.0, Tzz = (Tss - Tz * 0.0) / 1.0;
+  return Txx + Tzz;
+}
+
+/* smoother_time (C filter): PS/smoother.f90:582-636 */
+static void smoother_time(pto_case* c) {
+  int Niter = 10; double lb = -5.0, la = -1.0;
+  a2_copy(&c->Cvis, &c->Cnew); a2_copy(&c->Cbuoy, &c->Cnew);
+  int nn = 0, trigger = 0;
+  A2 T0 = a2_new(-c->span, c->nx + c->span, -c->span, c->nz + c->span);
+  for (;;) {
+    for (int n = 1; n <= Niter; n++) {
+      double diff = smoothness(c, &c->Cvis);
+      if (diff < c->delta_C) { trigger = 1; break; }
+      double cc;
+      if (nn == 0) {
+        int ntemp = Niter - n + 1;
+        cc = 2.0 / (-lb - la + (lb - la) * cos((double)(2 * ntemp - 1) * PII / (double)(2 * Niter)));
+      } else cc = 1.0;
+      for (int k = -c->span; k <= c->nz + c->span; k++)
+        for (int i = -c->span; i <= c->nx + c->span; i++) AT(T0, i, k) = AT(c->Cvis, i, k);
+      for (int i = 0; i <= c->nx; i++)
+        for (int k = 0; k <= c->nz; k++) {
+          double temp = smoother_space(c, &T0, i, k);
+          double dts = AT(c->dts_array, i, k) * c->P.courantC;
+          double v = AT(T0, i, k) + cc * dts * temp;
+          if (v < 0.0) v = 0.0;
+          if (v > 1.0) v = 1.0;
+          AT(c->Cvis, i, k) = v;
+        }
+      enforce_bcs_comp(c, &c->Cvis);
+    }
+    if (trigger) break;
+    Niter = 1; nn = nn + 1;
+  }
+  a2_free(&T0);
+  c->n_comp = nn;
+}
+
+/* smoother_time_T: PS/smoother.f90:638-672 (Tvis, Tbuoy <- Tin) */
+static void smoother_time_T(pto_case* c, const A2* Tin) {
+  a2_copy(&c->Tvis, Tin); a2_copy(&c->Tbuoy, Tin);
+  if (c->P.RaT == 0.0 || c->P.ivis == 0) return;
+  A2 T0 = a2_new(-c->span, c->nx + c->span, -c->span, c->nz + c->span);
+  int n = 0;
+  for (;;) {
+    double diff = smoothness(c, &c->Tvis);
+    if (diff < c->delta_T) { c->n_temp = n; break; }
+    a2_copy(&T0, &c->Tvis);
+    for (int k = 0; k <= c->nz; k++)
+      for (int i = 0; i <= c->nx; i++) {
+        double temp = smoother_space(c, &T0, i, k);
+        double dts = AT(c->dts_array, i, k) * c->P.courantT;
+        AT(c->Tvis, i, k) = AT(T0, i, k) + dts * temp;
+      }
+    enforce_bcs_T(c, &c->Tvis);
+    n = n + 1;
+  }
+  a2_free(&T0);
+}
+
+/* load_conductivity_array: PS/energy.f90:59-86 */
+static void load_conductivity(pto_case* c) {
+  int nx = c->nx, nz = c->nz, h = c->spanT;
+  for (int i = 0; i <= nx; i++)
+    for (int k = 0; k <= nz; k++) AT(c->conduct, i, k) = 1.0 + AT(c->Cvis, i, k) * (c->P.conduct_factor - 1.0);
+  for (int i = 1; i <= h; i++)
+    for (int k = 0; k <= nz; k++) { AT(c->conduct, -i, k) = AT(c->conduct, i, k); AT(c->conduct, nx + i, k) = AT(c->conduct, nx - i, k); }
+  for (int k = 1; k <= h; k++)
+    for (int i = -h; i <= nx + h; i++) { AT(c->conduct, i, -k) = AT(c->conduct, i, k); AT(c->conduct, i, nz + k) = AT(c->conduct, i, nz - k); }
+  for (int i = -3; i <= nx + 3; i++)
+    for (int k = -3; k <= nz + 3; k++) {
+      AT(c->conduct_r, i, k) = dot5x(c->D1, &c->conduct, i, k) / c->dr;
+      AT(c->conduct_s, i, k) = dot5z(c->D1, &c->conduct, i, k) / c->ds;
+    }
+}
+
+/* shape_func (bilinear): PS/composition.f90:896-921 */
+static inline double shape_func(pto_case* c, int i, int k, double r, double s) {
+  double ri = (double)i * c->dr, sk = (double)k * c->ds;
+  double a = 1.0 - fabs(r - ri) / c->dr; if (a < 0.0) a = 0.0;
+  double b = 1.0 - fabs(s - sk) / c->ds; if (b < 0.0) b = 0.0;
+  return a * b;
+}
+
+/* tracers_to_corners_iC_continuous: PS/composition.f90:942-1087 (one thread) */
+static int tracers_to_corners(pto_case* c, const double* rtr_, const double* str_, const double* Ttr_) {
+  int nx = c->nx, nz = c->nz;
+  a2_copy(&c->Tratio, &c->T0);
+  long nw = (long)(nx + 7) * (nz + 7);
+  memset(c->TxS, 0, sizeof(double) * nw); memset(c->CxS, 0, sizeof(double) * nw);
+  memset(c->shape_sum, 0, sizeof(double) * nw);
+  memset(c->nempty, 0, sizeof(int32_t) * (size_t)(nx + 1) * (nz + 1));
+#define WK(i, k) ((long)((k) + 3) * (nx + 7) + ((i) + 3))
+  for (int64_t id = 0; id < c->ntr; id++) {
+    double r = rtr_[id], s = str_[id];
+    int i = (int)lround(r / c->dr), k = (int)lround(s / c->ds);
+    if (r < 0.0 || r > c->aspect || s < 0.0 || s > 1.0) return PT_ERR_TRACER_OUTSIDE;
+    c->nempty[(long)k * (nx + 1) + i] += 1;
+    for (int a = -1; a <= 1; a++)
+      for (int b = -1; b <= 1; b++) {
+        double sf = shape_func(c, i + a, k + b, r, s);
+        c->TxS[WK(i + a, k + b)] = c->TxS[WK(i + a, k + b)] + Ttr_[id] * sf;
+      }
+    for (int a = -1; a <= 1; a++)
+      for (int b = -1; b <= 1; b++) {
+        double sf = shape_func(c, i + a, k + b, r, s);
+        c->CxS[WK(i + a, k + b)] = c->CxS[WK(i + a, k + b)] + c->Ctr[id] * sf;
+      }
+    for (int a = -1; a <= 1; a++)
+      for (int b = -1; b <= 1; b++)
+        c->shape_sum[WK(i + a, k + b)] = c->shape_sum[WK(i + a, k + b)] + shape_func(c, i + a, k + b, r, s);
+  }
+  for (int k = 0; k <= nz; k++)
+    for (int i = 0; i <= nx; i++) {
+      double ss = c->shape_sum[WK(i, k)];
+      if (ss > 0.0) {
+        AT(c->Tratio, i, k) = c->TxS[WK(i, k)] / ss;
+        AT(c->Cnew, i, k) = c->CxS[WK(i, k)] / ss;
+      }
+    }
+#undef WK
+  enforce_bcs_comp(c, &c->Cnew);
+  smoother_time(c);
+  enforce_bcs_T(c, &c->Tratio);
+  smoother_time_T(c, &c->Tratio);
+  load_conductivity(c);
+  return PT_OK;
+}
+
+int pto_tracers_to_corners(pto_case* c) {
+  a2_copy(&c->T0, &c->Tratio);
+  return tracers_to_corners(c, c->rtr, c->str, c->Ttr);
+}
+
+/* load_tracer_space_array + tracer_space_uniform: PS/energy.f90:135-179 */
+static void load_tracer_space_array(pto_case* c) {
+  extend_T(c, &c->Tratio);
+  for (int k = -3; k <= c->nz + 3; k++)
+    for (int i = -3; i <= c->nx + 3; i++) {
+      double Tx = dot5x(c->D1, &c->Textend, i, k) / c->dr, Tz = dot5z(c->D1, &c->Textend, i, k) / c->ds;
+      double Txx = dot5x(c->D2, &c->Textend, i, k) / c->dr2, Tzz = dot5z(c->D2, &c->Textend, i, k) / c->ds2;
+      double kx = AT(c->conduct_r, i, k), kz = AT(c->conduct_s, i, k);
+      double h = heating(c, i, k);
+      AT(c->tsa, i, k) = AT(c->conduct, i, k) * (Txx + Tzz) + (kx * Tx + kz * Tz) + h;
+    }
+}
+
+/* compute_derivatives: PS/composition.f90:1457-1478 */
+static void compute_derivatives(pto_case* c, const A2* f, double* DERr) {
+  for (int k = -3; k <= c->nz + 3; k++)
+    for (int i = 0; i <= c->nx; i++)
+      for (int jj = 1; jj <= 3; jj++) {
+        double s = 0.0;
+        for (int a = -3; a <= 3; a++) s = s + c->D[jj][a + 3] * AT(*f, i + a, k);
+        DER(DERr, c, jj, i, k) = s / c->dr_power[jj];
+      }
+}
+
+/* interpolate: PS/composition.f90:1359-1393 */
+static inline double interpolate(pto_case* c, double r, double s, const double* DERr, const A2* f) {
+  int i = (int)lround(r / c->dr), k = (int)lround(s / c->ds);
+  double hr = r - (double)i * c->dr;
+  double fR[7];
+  for (int jj = -3; jj <= 3; jj++) fR[jj + 3] = AT(*f, i, k + jj);
+  for (int j = 1; j <= 3; j++) {
+    double a1 = pow(hr, (double)j) / c->factorial[j];
+    for (int jj = -3; jj <= 3; jj++) fR[jj + 3] = fR[jj + 3] + DER(DERr, c, j, i, k + jj) * a1;
+  }
+  double hs = s - (double)k * c->ds;
+  double DERs[4];
+  for (int jj = 1; jj <= 3; jj++) {
+    double d = 0.0;
+    for (int a = -3; a <= 3; a++) d = d + c->D[jj][a + 3] * fR[a + 3];
+    DERs[jj] = d / c->ds_power[jj];
+  }
+  double fi = fR[3];
+  for (int j = 1; j <= 3; j++) {
+    double a1 = pow(hs, (double)j) / c->factorial[j];
+    fi = fi + DERs[j] * a1;
+  }
+  return fi;
+}
+
+/* stability_RK4 (tracer=1, CFL=0): PS/energy.f90:756-810 */
+static void stability_rk4(pto_case* c) {
+  c->Ts_max = c->Tr_base / c->ds; c->Tss_max = -c->Trr_base / c->ds2;
+  c->Tr_max = c->Tr_base / c->dr; c->Trr_max = -c->Trr_base / c->dr2;
+  double eigA_max = 0.0, eigD_max = 0.0;
+  for (int k = 0; k <= c->nz; k++)
+    for (int i = 0; i <= c->nx; i++) {
+      double cd = AT(c->conduct, i, k);
+      double eA = fabs(-(cd * 0.0 / pow(1.0, 3.0)) + AT(c->conduct_r, i, k) / pow(1.0, 2.0)) * c->Tr_max +
+                  fabs(-(cd * 0.0 / pow(1.0, 3.0)) + AT(c->conduct_s, i, k) / pow(1.0, 2.0)) * c->Ts_max;
+      double eD = cd * (fabs(c->Trr_max / pow(1.0, 2.0)) + fabs(c->Tss_max / pow(1.0, 2.0)));
+      if (eA > eigA_max) eigA_max = eA;
+      if (eD > eigD_max) eigD_max = eD;
+    }
+  double dtA = 2.8 / eigA_max, dtD = 2.8 / eigD_max, dt;
+  if (eigA_max == 0.0) dt = dtD; else dt = 1.0 / (1.0 / dtA + 1.0 / dtD);
+  c->dt = c->P.courant * dt;
+}
+
+/* energy_time: PS/energy.f90:181-556 (tracer=1, comp=1, torder=4, RK_update=1) */
+int pto_energy_time(pto_case* c) {
+  int rc;
+  int64_t n = c->ntr;
+  stability_rk4(c);
+  double dt = c->dt;
+  a2_copy(&c->C0, &c->Cnew);
+  a2_copy(&c->T0, &c->Tratio);
+  memcpy(c->Ttr0, c->Ttr, sizeof(double) * n);
+  memcpy(c->rtr0, c->rtr, sizeof(double) * n);
+  memcpy(c->str0, c->str, sizeof(double) * n);
+  /* stage 1 */
+  load_tracer_space_array(c);
+  compute_derivatives(c, &c->tsa, c->DERr_tracer);
+  compute_derivatives(c, &c->u, c->DERr_u);
+  compute_derivatives(c, &c->w, c->DERr_w);
+  double dtstage = 0.5 * dt;
+  for (int64_t id = 0; id < n; id++) {
+    double tr = interpolate(c, c->rtr0[id], c->str0[id], c->DERr_u, &c->u);
+    double ts = interpolate(c, c->rtr0[id], c->str0[id], c->DERr_w, &c->w);
+    c->rtr[id] = tr; c->str[id] = ts;
+    double tT = interpolate(c, c->rtr0[id], c->str0[id], c->DERr_tracer, &c->tsa);
+    c->Ttr[id] = tT;
+    c->rtr1[id] = c->rtr0[id] + dtstage * tr; c->str1[id] = c->str0[id] + dtstage * ts;
+    c->Ttr1[id] = c->Ttr0[id] + dtstage * tT;
+  }
+  /* stage 2 */
+  if ((rc = tracers_to_corners(c, c->rtr1, c->str1, c->Ttr1))) return rc;
+  load_tracer_space_array(c);
+  compute_derivatives(c, &c->tsa, c->DERr_tracer);
+  if ((rc = pto_update_velocities(c))) return rc;
+  compute_derivatives(c, &c->u, c->DERr_u);
+  compute_derivatives(c, &c->w, c->DERr_w);
+  dtstage = 0.5 * dt;
+  for (int64_t id = 0; id < n; id++) {
+    double tr = interpolate(c, c->rtr1[id], c->str1[id], c->DERr_u, &c->u);
+    double ts = interpolate(c, c->rtr1[id], c->str1[id], c->DERr_w, &c->w);
+    c->rtr[id] = c->rtr[id] + 2.0 * tr; c->str[id] = c->str[id] + 2.0 * ts;
+    double tT = interpolate(c, c->rtr1[id], c->str1[id], c->DERr_tracer, &c->tsa);
+    c->Ttr[id] = c->Ttr[id] + 2.0 * tT;
+    c->rtr2[id] = c->rtr0[id] + dtstage * tr; c->str2[id] = c->str0[id] + dtstage * ts;
+    c->Ttr2[id] = c->Ttr0[id] + dtstage * tT;
+  }
+  /* stage 3 */
+  a2_copy(&c->Cnew, &c->C0);
+  if ((rc = tracers_to_corners(c, c->rtr2, c->str2, c->Ttr2))) return rc;
+  load_tracer_space_array(c);
+  compute_derivatives(c, &c->tsa, c->DERr_tracer);
+  if ((rc = pto_update_velocities(c))) return rc;
+  compute_derivatives(c, &c->u, c->DERr_u);
+  compute_derivatives(c, &c->w, c->DERr_w);
+  double dtt = dt;
+  for (int64_t id = 0; id < n; id++) {
+    double tr = interpolate(c, c->rtr2[id], c->str2[id], c->DERr_u, &c->u);
+    c->rtr[id] = c->rtr[id] + 2.0 * tr; c->rtr1[id] = c->rtr0[id] + dtt * tr;
+    double ts = interpolate(c, c->rtr2[id], c->str2[id], c->DERr_w, &c->w);
+    c->str[id] = c->str[id] + 2.0 * ts; c->str1[id] = c->str0[id] + dtt * ts;
+    double tT = interpolate(c, c->rtr2[id], c->str2[id], c->DERr_tracer, &c->tsa);
+    c->Ttr[id] = c->Ttr[id] + 2.0 * tT; c->Ttr1[id] = c->Ttr0[id] + dtt * tT;
+  }
+  /* stage 4 */
+  a2_copy(&c->Cnew, &c->C0);
+  if ((rc = tracers_to_corners(c, c->rtr1, c->str1, c->Ttr1))) return rc;
+  load_tracer_space_array(c);
+  compute_derivatives(c, &c->tsa, c->DERr_tracer);
+  if ((rc = pto_update_velocities(c))) return rc;
+  compute_derivatives(c, &c->u, c->DERr_u);
+  compute_derivatives(c, &c->w, c->DERr_w);
+  double dto6 = dt / 6.0;
+  for (int64_t id = 0; id < n; id++) {
+    double tr = interpolate(c, c->rtr1[id], c->str1[id], c->DERr_u, &c->u);
+    c->rtr[id] = c->rtr0[id] + dto6 * (c->rtr[id] + tr);
+    double ts = interpolate(c, c->rtr1[id], c->str1[id], c->DERr_w, &c->w);
+    c->str[id] = c->str0[id] + dto6 * (c->str[id] + ts);
+    double tT = interpolate(c, c->rtr1[id], c->str1[id], c->DERr_tracer, &c->tsa);
+    c->Ttr[id] = c->Ttr0[id] + dto6 * (c->Ttr[id] + tT);
+  }
+  /* wrap up */
+  a2_copy(&c->Cnew, &c->C0);
+  if ((rc = tracers_to_corners(c, c->rtr, c->str, c->Ttr))) return rc;
+  a2_copy(&c->T, &c->Tratio);
+  return PT_OK;
+}
+
+/* one iteration of the time loop: PS/main.f90:246-299 without stats/output */
+int pto_time_step(pto_case* c) {
+  int rc = pto_multigrid(c);
+  if (rc) return rc;
+  pto_compute_velocities(c);
+  rc = pto_energy_time(c);
+  if (rc) return rc;
+  c->time = c->time + c->dt;
+  return PT_OK;
+}
+
+/* ---- setup ------------------------------------------------------------------ */
+static int check_supported(const pt_params* p) {
+  if (p->nx < 4 || p->nz < 4 || p->grids < 2 || p->grids > 6 || p->tpc < 1) return PT_ERR_INVALID;
+  if (p->nstageS < 2 || p->nstageS > 60 || p->Nmulti < 1) return PT_ERR_INVALID;
+  if (p->ivis < 0 || p->ivis > 2) return PT_ERR_UNSUPPORTED;
+  if (p->RaT == 0.0) return PT_ERR_UNSUPPORTED;
+  return PT_OK;
+}
+
+int pto_create(const pt_params* p, pto_case** out) {
+  int rc = check_supported(p);
+  if (rc) return rc;
+  pto_case* c = (pto_case*)calloc(1, sizeof(pto_case));
+  c->P = *p;
+  c->nx = p->nx; c->nz = p->nz; c->grids = p->grids;
+  c->span = 3; c->span1 = 2; c->span_interp = 3; c->spanT = 5;
+  c->aspect = p->aspect;
+  compute_smoothing_parameters(c);
+  multigrid_setup(c);
+  if (c->err) { rc = c->err; free(c); return rc; }
+  /* compute_basics: PS/basics.f90:204-315 */
+  c->dr = c->aspect / (double)c->nx; c->ds = 1.0 / (double)c->nz;
+  c->dr2 = pow(c->dr, 2.0); c->ds2 = pow(c->ds, 2.0); c->dr3 = pow(c->dr, 3.0); c->ds3 = pow(c->ds, 3.0);
+  c->dr4 = pow(c->dr, 4.0); c->ds4 = pow(c->ds, 4.0);
+  c->drds = c->dr * c->ds; c->drds2 = c->dr * c->ds2; c->dr2ds = c->dr2 * c->ds; c->dr2ds2 = c->dr2 * c->ds2;
+  c->factorial[0] = 1.0; c->dr_power[0] = 1.0; c->ds_power[0] = 1.0;
+  for (int j = 1; j <= 4; j++) {
+    c->factorial[j] = (double)j * c->factorial[j - 1];
+    c->dr_power[j] = pow(c->dr, (double)j); c->ds_power[j] = pow(c->ds, (double)j);
+  }
+  c->ntr = (int64_t)(c->nx + 1) * (c->nz + 1) * p->tpc;
+  for (int j = 1; j <= c->grids; j++) {
+    c->nxg[j] = c->nx / (1 << (j - 1)); c->nzg[j] = c->nz / (1 << (j - 1));
+    c->drg[j] = c->aspect / (double)c->nxg[j]; c->dsg[j] = 1.0 / (double)c->nzg[j];
+    c->dr2g[j] = pow(c->drg[j], 2.0); c->ds2g[j] = pow(c->dsg[j], 2.0);
+    c->dr3g[j] = pow(c->drg[j], 3.0); c->ds3g[j] = pow(c->dsg[j], 3.0);
+    c->dr4g[j] = pow(c->drg[j], 4.0); c->ds4g[j] = pow(c->dsg[j], 4.0);
+    c->drdsg[j] = c->drg[j] * c->dsg[j]; c->drds2g[j] = c->drg[j] * c->ds2g[j];
+    c->dr2dsg[j] = c->dr2g[j] * c->dsg[j]; c->dr2ds2g[j] = c->dr2g[j] * c->ds2g[j];
+  }
+  coarse_bandwidth(c);
+  /* storage: PS/basics.f90:70-202 */
+  int nx = c->nx, nz = c->nz;
+  c->T = a2_new(-3, nx + 3, -3, nz + 3); c->SF = a2_new(-3, nx + 3, -3, nz + 3);
+  c->u = a2_new(-3, nx + 3, -3, nz + 3); c->w = a2_new(-3, nx + 3, -3, nz + 3);
+  c->Tvis = a2_new(-3, nx + 3, -3, nz + 3); c->Tbuoy = a2_new(-3, nx + 3, -3, nz + 3);
+  c->Tratio = a2_new(-3, nx + 3, -3, nz + 3); c->T0 = a2_new(-3, nx + 3, -3, nz + 3);
+  c->Cvis = a2_new(-5, nx + 5, -5, nz + 5); c->Cbuoy = a2_new(-5, nx + 5, -5, nz + 5);
+  c->Cnew = a2_new(-5, nx + 5, -5, nz + 5); c->C0 = a2_new(-5, nx + 5, -5, nz + 5);
+  c->conduct = a2_new(-5, nx + 5, -5, nz + 5);
+  c->conduct_r = a2_new(-3, nx + 3, -3, nz + 3); c->conduct_s = a2_new(-3, nx + 3, -3, nz + 3);
+  c->tsa = a2_new(-3, nx + 3, -3, nz + 3); c->Textend = a2_new(-5, nx + 5, -5, nz + 5);
+  c->dts_array = a2_new(0, nx, 0, nz);
+  c->xg = a2_new(-6, nx + 6, 0, 0); c->zg = a2_new(0, 0, -6, nz + 6);
+  long nder = (long)3 * (nx + 1) * (nz + 7);
+  c->DERr_u = (double*)calloc(nder, sizeof(double)); c->DERr_w = (double*)calloc(nder, sizeof(double));
+  c->DERr_tracer = (double*)calloc(nder, sizeof(double));
+  for (int l = 1; l <= c->grids; l++) {
+    int n = c->nxg[l], m = c->nzg[l];
+    c->error[l] = a2_new(-3, n + 3, -3, m + 3);
+    c->vis_grid[l] = a2_new(-3, n + 3, -3, m + 3); c->vis_grid_static[l] = a2_new(-3, n + 3, -3, m + 3);
+    c->vis_x[l] = a2_new(0, n, 0, m); c->vis_z[l] = a2_new(0, n, 0, m); c->vis_xx[l] = a2_new(0, n, 0, m);
+    c->vis_zz[l] = a2_new(0, n, 0, m); c->vis_xz[l] = a2_new(0, n, 0, m); c->vis_xxpzz[l] = a2_new(0, n, 0, m);
+    c->vis_xxmzz[l] = a2_new(0, n, 0, m); c->vis_smooth[l] = a2_new(0, n, 0, m); c->dt_vis[l] = a2_new(0, n, 0, m);
+    c->RHS[l] = a2_new(1, n - 1, 1, m - 1); c->residual[l] = a2_new(1, n - 1, 1, m - 1);
+    c->dt_stream[l] = a2_new(1, n - 1, 1, m - 1); c->ratio[l] = a2_new(1, n - 1, 1, m - 1);
+  }
+  c->SFxx = a2_new(0, nx, 0, nz); c->SFzz = a2_new(0, nx, 0, nz); c->SFxxx = a2_new(0, nx, 0, nz);
+  c->SFzzz = a2_new(0, nx, 0, nz); c->SFxxxx = a2_new(0, nx, 0, nz); c->SFzzzz = a2_new(0, nx, 0, nz);
+  c->SFxz = a2_new(0, nx, 0, nz); c->SFxxz = a2_new(0, nx, 0, nz); c->SFxzz = a2_new(0, nx, 0, nz);
+  c->SFxxzz = a2_new(0, nx, 0, nz);
+  for (int l = 1; l <= c->grids; l++) {
+    int n = c->nxg[l], m = c->nzg[l];
+    c->S0[l] = a2_new(-3, n + 3, -3, m + 3); c->S1[l] = a2_new(-3, n + 3, -3, m + 3); c->S2[l] = a2_new(-3, n + 3, -3, m + 3);
+  }
+  c->Mc = (double*)calloc((size_t)(2 * c->kl + c->ku + 1) * c->ngc, sizeof(double));
+  c->RowB = (double*)calloc(c->ngc, sizeof(double)); c->Bc = (double*)calloc(c->ngc, sizeof(double));
+  c->ipiv = (int32_t*)calloc(c->ngc, sizeof(int32_t));
+  int64_t ntr = c->ntr;
+  double** tarr[] = {&c->rtr, &c->str, &c->Ttr, &c->Ctr, &c->rtr0, &c->rtr1, &c->rtr2, &c->str0, &c->str1,
+                     &c->str2, &c->Ttr0, &c->Ttr1, &c->Ttr2};
+  for (int j = 0; j < 13; j++) *tarr[j] = (double*)calloc((size_t)ntr, sizeof(double));
+  c->nempty = (int32_t*)calloc((size_t)(nx + 1) * (nz + 1), sizeof(int32_t));
+  long nw = (long)(nx + 7) * (nz + 7);
+  c->TxS = (double*)calloc(nw, sizeof(double)); c->CxS = (double*)calloc(nw, sizeof(double));
+  c->shape_sum = (double*)calloc(nw, sizeof(double));
+  chebyshev_prep_stream(c);
+  fd_coefficients(c);
+  /* build_grid (uniform, restart=0): PS/grid.f90:37-153 */
+  for (int i = 0; i <= nx; i++) AT(c->xg, i, 0) = c->dr * (double)i;
+  for (int k = 0; k <= nz; k++) AT(c->zg, 0, k) = c->ds * (double)k;
+  for (int i = 1; i <= 6; i++) { AT(c->xg, -i, 0) = -AT(c->xg, i, 0); AT(c->xg, nx + i, 0) = c->aspect - AT(c->xg, nx - i, 0) + c->aspect; }
+  for (int k = 1; k <= 6; k++) { AT(c->zg, 0, -k) = -AT(c->zg, 0, k); AT(c->zg, 0, nz + k) = 1.0 - AT(c->zg, 0, nz - k) + 1.0; }
+  AT(c->xg, 0, 0) = 0.0; AT(c->zg, 0, 0) = 0.0; AT(c->xg, nx, 0) = c->aspect; AT(c->zg, 0, nz) = 1.0;
+  derivative_eigenvalues(c);
+  /* stability_RK4_smoother: PS/smoother.f90:719-740 */
+  {
+    double Tss = -c->Trr_base / c->ds2, Trr = -c->Trr_base / c->dr2;
+    for (int k = 0; k <= nz; k++)
+      for (int i = 0; i <= nx; i++) {
+        double eR = fabs(Trr / pow(1.0, 2.0)) + fabs(Tss / pow(1.0, 2.0));
+        double eI = 0.0;
+        AT(c->dts_array, i, k) = 2.0 * eR / (pow(eR, 2.0) + pow(eI, 2.0));
+      }
+  }
+  eigenvalue_prep(c);
+  stream_space_matrix_prep(c);
+  /* stability_vis_smoother: PS/smoother.f90:761-782 */
+  for (int l = 1; l <= c->grids; l++) {
+    double Tss_so = -c->Trr_base_so / c->ds2g[l], Trr_so = -c->Trr_base_so / c->dr2g[l];
+    for (int k = 0; k <= c->nzg[l]; k++)
+      for (int i = 0; i <= c->nxg[l]; i++) {
+        double eR = fabs(Trr_so / pow(1.0, 2.0)) + fabs(Tss_so / pow(1.0, 2.0));
+        double eI = 0.0;
+        AT(c->dt_vis[l], i, k) = 2.0 * eR / (pow(eR, 2.0) + pow(eI, 2.0));
+      }
+  }
+  *out = c;
+  return PT_OK;
+}
+
+void pto_destroy(pto_case* c) {
+  if (!c) return;
+  A2* fs[] = {&c->T, &c->SF, &c->u, &c->w, &c->Tvis, &c->Tbuoy, &c->Tratio, &c->T0, &c->Cvis, &c->Cbuoy,
+              &c->Cnew, &c->C0, &c->conduct, &c->conduct_r, &c->conduct_s, &c->tsa, &c->Textend,
+              &c->dts_array, &c->xg, &c->zg, &c->SFxx, &c->SFzz, &c->SFxxx, &c->SFzzz, &c->SFxxxx,
+              &c->SFzzzz, &c->SFxz, &c->SFxxz, &c->SFxzz, &c->SFxxzz};
+  for (unsigned j = 0; j < sizeof fs / sizeof fs[0]; j++) a2_free(fs[j]);
+  for (int l = 1; l <= c->grids; l++) {
+    A2* ls[] = {&c->error[l], &c->vis_grid[l], &c->vis_grid_static[l], &c->vis_x[l], &c->vis_z[l], &c->vis_xx[l],
+                &c->vis_zz[l], &c->vis_xz[l], &c->vis_xxpzz[l], &c->vis_xxmzz[l], &c->vis_smooth[l],
+                &c->dt_vis[l], &c->RHS[l], &c->residual[l], &c->dt_stream[l], &c->ratio[l],
+                &c->S0[l], &c->S1[l], &c->S2[l]};
+    for (unsigned j = 0; j < sizeof ls / sizeof ls[0]; j++) a2_free(ls[j]);
+  }
+  free(c->DERr_u); free(c->DERr_w); free(c->DERr_tracer); free(c->Mc); free(c->RowB); free(c->Bc); free(c->ipiv);
+  double* tarr[] = {c->rtr, c->str, c->Ttr, c->Ctr, c->rtr0, c->rtr1, c->rtr2, c->str0, c->str1, c->str2, c->Ttr0, c->Ttr1, c->Ttr2};
+  for (int j = 0; j < 13; j++) free(tarr[j]);
+  free(c->nempty); free(c->TxS); free(c->CxS); free(c->shape_sum);
+  free(c);
+}
+
+void pto_dims(pto_case* c, int32_t* nx, int32_t* nz, int32_t* grids, int64_t* ntr) {
+  *nx = c->nx; *nz = c->nz; *grids = c->grids; *ntr = c->ntr;
+}
+
+/* tail of initial_temperature + initialize_tracers: PS/basics.f90:352-359,
+ * PS/composition.f90:848; PS/main.f90:228-229 */
+int pto_load_state(pto_case* c, const double* T_nodes, double Tbot_iso, double Ttop_iso,
+                   const double* rtr, const double* str, const double* Ttr, const double* Ctr) {
+  int nx = c->nx, nz = c->nz;
+  for (int k = 0; k <= nz; k++)
+    for (int i = 0; i <= nx; i++) AT(c->T, i, k) = T_nodes[(long)k * (nx + 1) + i];
+  c->Tbot_iso = Tbot_iso; c->Ttop_iso = Ttop_iso;
+  for (int i = -3; i <= nx + 3; i++) { AT(c->T, i, 0) = Tbot_iso; AT(c->T, i, nz) = Ttop_iso; }
+  enforce_bcs_T(c, &c->T);
+  memcpy(c->rtr, rtr, sizeof(double) * c->ntr); memcpy(c->str, str, sizeof(double) * c->ntr);
+  memcpy(c->Ttr, Ttr, sizeof(double) * c->ntr); memcpy(c->Ctr, Ctr, sizeof(double) * c->ntr);
+  a2_copy(&c->T0, &c->T);
+  int rc = tracers_to_corners(c, c->rtr, c->str, c->Ttr);
+  if (rc) return rc;
+  memset(c->SF.p, 0, sizeof(double) * (size_t)c->SF.ni * c->SF.nk);
+  for (int l = 2; l <= c->grids; l++) memset(c->error[l].p, 0, sizeof(double) * (size_t)c->error[l].ni * c->error[l].nk);
+  c->time = 0.0;
+  return PT_OK;
+}
+
+/* ---- exchange ------------------------------------------------------------------ */
+static A2* field_ptr(pto_case* c, int32_t id) {
+  switch (id) {
+    case PT_FIELD_SF: return &c->SF;       case PT_FIELD_T: return &c->Tratio;
+    case PT_FIELD_TVIS: return &c->Tvis;   case PT_FIELD_TBUOY: return &c->Tbuoy;
+    case PT_FIELD_U: return &c->u;         case PT_FIELD_W: return &c->w;
+    case PT_FIELD_CNEW: return &c->Cnew;   case PT_FIELD_CVIS: return &c->Cvis;
+    case PT_FIELD_CBUOY: return &c->Cbuoy; case PT_FIELD_CONDUCT: return &c->conduct;
+    case PT_FIELD_VIS: return &c->vis_grid[1]; case PT_FIELD_RESIDUAL: return &c->residual[1];
+    case PT_FIELD_SOURCE: return &c->tsa;
+    default: return NULL;
+  }
+}
+int64_t pto_field_extent(pto_case* c, int32_t id) {
+  A2* a = field_ptr(c, id);
+  return a ? (int64_t)a->ni * a->nk : -1;
+}
+int pto_get_field(pto_case* c, int32_t id, double* out) {
+  A2* a = field_ptr(c, id);
+  if (!a) return PT_ERR_INVALID;
+  memcpy(out, a->p, sizeof(double) * (size_t)a->ni * a->nk);
+  return PT_OK;
+}
+int pto_set_field(pto_case* c, int32_t id, const double* in) {
+  A2* a = field_ptr(c, id);
+  if (!a) return PT_ERR_INVALID;
+  memcpy(a->p, in, sizeof(double) * (size_t)a->ni * a->nk);
+  return PT_OK;
+}
+void pto_get_tracers(pto_case* c, double* rtr, double* str, double* Ttr, double* Ctr) {
+  size_t b = sizeof(double) * (size_t)c->ntr;
+  if (rtr) memcpy(rtr, c->rtr, b);
+  if (str) memcpy(str, c->str, b);
+  if (Ttr) memcpy(Ttr, c->Ttr, b);
+  if (Ctr) memcpy(Ctr, c->Ctr, b);
+}
+void pto_get_nempty(pto_case* c, int32_t* out) {
+  memcpy(out, c->nempty, sizeof(int32_t) * (size_t)(c->nx + 1) * (c->nz + 1));
+}
+void pto_get_info(pto_case* c, pt_solve_info* info) {
+  memset(info, 0, sizeof *info);
+  info->q = c->last_q; info->residual = c->last_res; info->res_ref = c->last_res_ref;
+  info->dt = c->dt; info->time = c->time; info->solves = c->solves; info->cycles = c->cycles;
+  info->n_eta = c->n_eta; info->n_temp = c->n_temp; info->n_comp = c->n_comp; info->eig_fine = c->eig_fine;
+}
+
+/* ---- unit hooks ------------------------------------------------------------------ */
+void pto_fd_tables(pto_case* c, double* D1, double* D2, double* D3, double* D4, double* Dtab) {
+  memcpy(D1, c->D1, sizeof c->D1); memcpy(D2, c->D2, sizeof c->D2);
+  memcpy(D3, c->D3, sizeof c->D3); memcpy(D4, c->D4, sizeof c->D4);
+  for (int m = 1; m <= 4; m++) memcpy(Dtab + (m - 1) * 7, c->D[m], sizeof(double) * 7);
+}
+void pto_rkc_coefficients(pto_case* c, double* mewt, double* mew, double* v, double* cc, double* stability) {
+  int s = c->P.nstageS;
+  for (int j = 0; j <= s; j++) { mewt[j] = c->mewtChebS[j]; mew[j] = c->mewChebS[j]; v[j] = c->vChebS[j]; cc[j] = c->cChebS[j]; }
+  *stability = c->RKC1_stability;
+}
+void pto_eigen_bases(pto_case* c, double* out) {
+  out[0] = c->Tr_base; out[1] = c->Trr_base; out[2] = c->Trrr_base; out[3] = c->Trrrr_base;
+  out[4] = c->Trs_base; out[5] = c->Trrs_base; out[6] = c->Trrss_base; out[7] = c->Tr_base_so;
+  out[8] = c->Trr_base_so;
+  /* dt for conduct = 1 everywhere (the value stability_RK4 returns) */
+  double Tss = -c->Trr_base / c->ds2, Trr = -c->Trr_base / c->dr2;
+  double eD = 1.0 * (fabs(Trr / pow(1.0, 2.0)) + fabs(Tss / pow(1.0, 2.0)));
+  out[9] = c->P.courant * (2.8 / eD);
+}
+int pto_apply_operator(pto_case* c, int level, const double* psi, double* out) {
+  if (level < 1 || level > c->grids) return PT_ERR_INVALID;
+  int n = c->nxg[level], m = c->nzg[level];
+  A2 S = a2_new(-3, n + 3, -3, m + 3);
+  for (int k = 0; k <= m; k++) for (int i = 0; i <= n; i++) AT(S, i, k) = psi[(long)k * (n + 1) + i];
+  enforce_bcs_stream(c, level, &S);
+  stream_derivatives(c, level, &S);
+  for (int k = 1; k <= m - 1; k++)
+    for (int i = 1; i <= n - 1; i++) {
+      double v = AT(c->vis_grid[level], i, k);
+      double t2 = v * (AT(c->SFxxxx, i, k) + AT(c->SFzzzz, i, k) + 2.0 * AT(c->SFxxzz, i, k));
+      out[(long)(k - 1) * (n - 1) + (i - 1)] = t2;
+    }
+  a2_free(&S);
+  return PT_OK;
+}
+int pto_interpolate(pto_case* c, int32_t field_id, int64_t n, const double* r, const double* s, double* out) {
+  A2* f = field_ptr(c, field_id);
+  if (!f || (field_id != PT_FIELD_U && field_id != PT_FIELD_W && field_id != PT_FIELD_SOURCE)) return PT_ERR_INVALID;
+  double* der = (double*)calloc((size_t)3 * (c->nx + 1) * (c->nz + 7), sizeof(double));
+  compute_derivatives(c, f, der);
+  for (int64_t j = 0; j < n; j++) out[j] = interpolate(c, r[j], s[j], der, f);
+  free(der);
+  return PT_OK;
+}
+/* mzran: PS/composition.f90:7-20, thread-0 seeds PS/basics.f90:321-324 */
+typedef struct { int32_t i, j, k, n; } mz_state;
+static double mzran(mz_state* st) {
+  int32_t ran_int = st->i - st->k;
+  if (ran_int < 0) ran_int = ran_int + 2147483579;
+  st->i = st->j; st->j = st->k; st->k = ran_int;
+  st->n = (int32_t)((uint32_t)69069 * (uint32_t)st->n + (uint32_t)1013904243);
+  ran_int = (int32_t)((uint32_t)ran_int + (uint32_t)st->n);
+  return 0.5 + (0.2328306e-9) * (double)ran_int;
+}
+void pto_mzran(int64_t n, double* out) {
+  mz_state st = {521288629, 362436069, 16163801, 1131199299};
+  for (int64_t j = 0; j < n; j++) out[j] = mzran(&st);
+}
+void pto_initial_positions(pto_case* c, double* rtr, double* str) {
+  mz_state st = {521288629, 362436069, 16163801, 1131199299};
+  int64_t id = 0;
+  for (int i = 0; i <= c->nx; i++) {
+    double rcell = c->dr * (double)i;
+    for (int k = 0; k <= c->nz; k++) {
+      double scell = c->ds * (double)k;
+      for (int p = 1; p <= c->P.tpc; p++) {
+        double pr = mzran(&st), ps = mzran(&st);
+        pr = (pr - 0.5) * c->dr; ps = (ps - 0.5) * c->ds;
+        double r = rcell + pr, s = scell + ps;
+        if (r <= 0.0) r = 0.0;
+        if (r >= c->aspect) r = c->aspect;
+        if (s <= 0.0) s = 0.0;
+        if (s >= 1.0) s = 1.0;
+        rtr[id] = r; str[id] = s; id++;
+      }
+    }
+  }
+}

@@ -1,0 +1,137 @@
+"""Host-side mirror of the reference's hot-path subroutines over the C ABI.
+
+`Simulation` owns one device context.  Its methods carry the names of the
+reference subroutines they replace (PS/iterative_solver.f90 `multigrid`,
+PS/momentum.f90 `compute_velocities`/`update_velocities`, PS/energy.f90
+`energy_time`, PS/composition.f90 `tracers_to_corners`), take no state
+arguments, and raise `PtError` with the reference's own message where the
+reference would `stop`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+
+
+class PtError(RuntimeError):
+    def __init__(self, code: int, where: str, detail: str = ""):
+        self.code = code
+        msg = _abi.ERROR_TEXT.get(code, f"error {code}")
+        super().__init__(f"{where}: {msg}" + (f" ({detail})" if detail else ""))
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Simulation:
+    def __init__(self, params: _abi.PtParams, lib=None):
+        self.lib = lib or _abi.load_library()
+        self.ctx = C.c_void_p()
+        self._check(self.lib.pt_create(C.byref(params), C.byref(self.ctx)), "pt_create")
+        nx, nz, g = C.c_int32(), C.c_int32(), C.c_int32()
+        ntr = C.c_int64()
+        self._check(self.lib.pt_get_dims(self.ctx, C.byref(nx), C.byref(nz), C.byref(g), C.byref(ntr)),
+                    "pt_get_dims")
+        self.nx, self.nz, self.grids, self.ntr = nx.value, nz.value, g.value, ntr.value
+        self.params = params
+
+    # -- errors ------------------------------------------------------------
+    def _check(self, rc: int, where: str):
+        if rc != _abi.PT_OK:
+            detail = self.lib.pt_last_error()
+            raise PtError(rc, where, detail.decode() if detail else "")
+
+    def close(self):
+        if self.ctx:
+            self.lib.pt_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state -------------------------------------------------------------
+    def load_state(self, case):
+        T = np.ascontiguousarray(case.T_nodes, dtype=np.float64)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (case.rtr, case.str_, case.Ttr, case.Ctr)]
+        if arrs[0].size != self.ntr:
+            raise ValueError(f"case has {arrs[0].size} tracers, context expects {self.ntr}")
+        self._check(self.lib.pt_load_state(self.ctx, _dp(T), case.Tbot_iso, case.Ttop_iso,
+                                           *[_dp(a) for a in arrs]), "pt_load_state")
+
+    # -- reference subroutines -----------------------------------------------
+    def multigrid(self):
+        self._check(self.lib.pt_multigrid(self.ctx), "multigrid")
+
+    def compute_velocities(self):
+        self._check(self.lib.pt_compute_velocities(self.ctx), "compute_velocities")
+
+    def update_velocities(self):
+        self._check(self.lib.pt_update_velocities(self.ctx), "update_velocities")
+
+    def energy_time(self):
+        self._check(self.lib.pt_energy_time(self.ctx), "energy_time")
+
+    def tracers_to_corners(self):
+        self._check(self.lib.pt_tracers_to_corners(self.ctx), "tracers_to_corners")
+
+    def time_step(self):
+        self._check(self.lib.pt_time_step(self.ctx), "time_step")
+
+    def run_steps(self, n: int):
+        self._check(self.lib.pt_run_steps(self.ctx, int(n)), "run_steps")
+
+    def synchronize(self):
+        self._check(self.lib.pt_synchronize(self.ctx), "synchronize")
+
+    # -- exchange ------------------------------------------------------------
+    def field(self, name: str) -> np.ndarray:
+        """Field in Fortran layout: array[k + h, i + h] for the reference
+        element (i, k) with halo h (RESIDUAL: array[k-1, i-1])."""
+        fid = _abi.FIELD[name]
+        n = self.lib.pt_field_extent(self.ctx, fid)
+        out = np.empty(n, dtype=np.float64)
+        self._check(self.lib.pt_get_field(self.ctx, fid, _dp(out)), f"get_field({name})")
+        if fid == _abi.FIELD["RESIDUAL"]:
+            return out.reshape(self.nz - 1, self.nx - 1)
+        h = _abi.FIELD_HALO[fid]
+        return out.reshape(self.nz + 1 + 2 * h, self.nx + 1 + 2 * h)
+
+    def set_field(self, name: str, arr: np.ndarray):
+        fid = _abi.FIELD[name]
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        if a.size != self.lib.pt_field_extent(self.ctx, fid):
+            raise ValueError("field size mismatch")
+        self._check(self.lib.pt_set_field(self.ctx, fid, _dp(a)), f"set_field({name})")
+
+    def tracers(self):
+        out = [np.empty(self.ntr, dtype=np.float64) for _ in range(4)]
+        self._check(self.lib.pt_get_tracers(self.ctx, *[_dp(a) for a in out]), "get_tracers")
+        return tuple(out)
+
+    def nempty(self) -> np.ndarray:
+        out = np.empty((self.nz + 1) * (self.nx + 1), dtype=np.int32)
+        self._check(self.lib.pt_get_nempty(self.ctx, out.ctypes.data_as(C.POINTER(C.c_int32))),
+                    "get_nempty")
+        return out.reshape(self.nz + 1, self.nx + 1)
+
+    def info(self) -> dict:
+        inf = _abi.PtSolveInfo()
+        self._check(self.lib.pt_get_info(self.ctx, C.byref(inf)), "get_info")
+        return {name: getattr(inf, name) for name, _ in _abi.PtSolveInfo._fields_}
+
+    # -- timing ----------------------------------------------------------------
+    def enable_kernel_timing(self, on: bool = True):
+        self._check(self.lib.pt_enable_kernel_timing(self.ctx, 1 if on else 0), "enable_kernel_timing")
+
+    def kernel_time(self, which: int):
+        ms = C.c_double()
+        n = C.c_int64()
+        self._check(self.lib.pt_kernel_time(self.ctx, which, C.byref(ms), C.byref(n)), "kernel_time")
+        return ms.value, n.value

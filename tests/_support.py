@@ -1,0 +1,306 @@
+"""Test support: the CPU oracle (oracle/liboracle.so) behind the same Python
+interface as projectracer_amd.Simulation, and a runner for the flang build
+of the reference (oracle/_ref/ref_driver).  Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+from projectracer_amd import _abi
+from projectracer_amd.simulation import PtError
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+REF_BIN = ROOT / "oracle" / "_ref" / "ref_driver"
+GOLDEN = ROOT / "tests" / "golden"
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+_olib = None
+
+
+def oracle_lib():
+    global _olib
+    if _olib is None:
+        if not ORACLE_SO.exists():
+            subprocess.run(["make", "-C", str(ROOT / "oracle"), "liboracle.so"], check=True,
+                           capture_output=True)
+        lib = C.CDLL(str(ORACLE_SO))
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        sig = {
+            "pto_create": (C.c_int, [C.POINTER(_abi.PtParams), C.POINTER(vp)]),
+            "pto_destroy": (None, [vp]),
+            "pto_dims": (None, [vp, ip, ip, ip, C.POINTER(C.c_int64)]),
+            "pto_load_state": (C.c_int, [vp, dp, C.c_double, C.c_double, dp, dp, dp, dp]),
+            "pto_multigrid": (C.c_int, [vp]),
+            "pto_compute_velocities": (None, [vp]),
+            "pto_update_velocities": (C.c_int, [vp]),
+            "pto_energy_time": (C.c_int, [vp]),
+            "pto_tracers_to_corners": (C.c_int, [vp]),
+            "pto_time_step": (C.c_int, [vp]),
+            "pto_field_extent": (C.c_int64, [vp, C.c_int32]),
+            "pto_get_field": (C.c_int, [vp, C.c_int32, dp]),
+            "pto_set_field": (C.c_int, [vp, C.c_int32, dp]),
+            "pto_get_tracers": (None, [vp, dp, dp, dp, dp]),
+            "pto_get_nempty": (None, [vp, ip]),
+            "pto_get_info": (None, [vp, C.POINTER(_abi.PtSolveInfo)]),
+            "pto_fd_tables": (None, [vp, dp, dp, dp, dp, dp]),
+            "pto_rkc_coefficients": (None, [vp, dp, dp, dp, dp, dp]),
+            "pto_eigen_bases": (None, [vp, dp]),
+            "pto_apply_operator": (C.c_int, [vp, C.c_int, dp, dp]),
+            "pto_interpolate": (C.c_int, [vp, C.c_int32, C.c_int64, dp, dp, dp]),
+            "pto_band_lu_solve": (C.c_int, [C.c_int, C.c_int, C.c_int, dp, ip, dp]),
+            "pto_mzran": (None, [C.c_int64, dp]),
+            "pto_initial_positions": (None, [vp, dp, dp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _olib = lib
+    return _olib
+
+
+class OracleSim:
+    """Same interface as projectracer_amd.Simulation, backed by the oracle."""
+
+    def __init__(self, params: _abi.PtParams):
+        self.lib = oracle_lib()
+        self.ctx = C.c_void_p()
+        rc = self.lib.pto_create(C.byref(params), C.byref(self.ctx))
+        if rc:
+            raise PtError(rc, "pto_create")
+        nx, nz, g, ntr = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int64()
+        self.lib.pto_dims(self.ctx, C.byref(nx), C.byref(nz), C.byref(g), C.byref(ntr))
+        self.nx, self.nz, self.grids, self.ntr = nx.value, nz.value, g.value, ntr.value
+        self.params = params
+
+    def close(self):
+        if self.ctx:
+            self.lib.pto_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, where):
+        if rc:
+            raise PtError(rc, where)
+
+    def load_state(self, case):
+        T = np.ascontiguousarray(case.T_nodes, dtype=np.float64)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (case.rtr, case.str_, case.Ttr, case.Ctr)]
+        self._chk(self.lib.pto_load_state(self.ctx, _dp(T), case.Tbot_iso, case.Ttop_iso,
+                                          *[_dp(a) for a in arrs]), "load_state")
+
+    def multigrid(self):
+        self._chk(self.lib.pto_multigrid(self.ctx), "multigrid")
+
+    def compute_velocities(self):
+        self.lib.pto_compute_velocities(self.ctx)
+
+    def update_velocities(self):
+        self._chk(self.lib.pto_update_velocities(self.ctx), "update_velocities")
+
+    def energy_time(self):
+        self._chk(self.lib.pto_energy_time(self.ctx), "energy_time")
+
+    def tracers_to_corners(self):
+        self._chk(self.lib.pto_tracers_to_corners(self.ctx), "tracers_to_corners")
+
+    def time_step(self):
+        self._chk(self.lib.pto_time_step(self.ctx), "time_step")
+
+    def run_steps(self, n):
+        for _ in range(n):
+            self.time_step()
+
+    def field(self, name):
+        fid = _abi.FIELD[name]
+        n = self.lib.pto_field_extent(self.ctx, fid)
+        out = np.empty(n, dtype=np.float64)
+        self._chk(self.lib.pto_get_field(self.ctx, fid, _dp(out)), "field")
+        if fid == _abi.FIELD["RESIDUAL"]:
+            return out.reshape(self.nz - 1, self.nx - 1)
+        h = _abi.FIELD_HALO[fid]
+        return out.reshape(self.nz + 1 + 2 * h, self.nx + 1 + 2 * h)
+
+    def set_field(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        self._chk(self.lib.pto_set_field(self.ctx, _abi.FIELD[name], _dp(a)), "set_field")
+
+    def tracers(self):
+        out = [np.empty(self.ntr) for _ in range(4)]
+        self.lib.pto_get_tracers(self.ctx, *[_dp(a) for a in out])
+        return tuple(out)
+
+    def nempty(self):
+        out = np.empty((self.nz + 1) * (self.nx + 1), dtype=np.int32)
+        self.lib.pto_get_nempty(self.ctx, out.ctypes.data_as(C.POINTER(C.c_int32)))
+        return out.reshape(self.nz + 1, self.nx + 1)
+
+    def info(self):
+        inf = _abi.PtSolveInfo()
+        self.lib.pto_get_info(self.ctx, C.byref(inf))
+        return {name: getattr(inf, name) for name, _ in _abi.PtSolveInfo._fields_}
+
+
+# ---------------------------------------------------------------------------
+# flang-built reference
+# ---------------------------------------------------------------------------
+def _nml_value(v):
+    if isinstance(v, float):
+        return repr(v).replace("e", "d") if "e" in repr(v) else repr(v) + "d0"
+    return str(v)
+
+
+def write_inputs_nml(path: Path, p: _abi.PtParams):
+    d = _abi.params_dict(p)
+    f = _nml_value
+    txt = f"""&inputs
+aspect={f(d['aspect'])}, nx={d['nx']}, nz={d['nz']}, Database=2, order=4, gridX=0, gridZ=0,
+thickness=0.1d0, nbound=8, spaceX=1.d0, spaceZ=1.d0,
+iterate=1, MG=1, non_Newtonian=0, tolerance={f(d['tolerance'])}, Nmulti={d['Nmulti']},
+mingrid={d['mingrid']}, grids={d['grids']}, eig_ratio={f(d['eig_ratio'])}, vis_iter={d['vis_iter']},
+max_cycles={d['max_cycles']}, tol_coarse=1.d-3, fp_max=100, splitting=0, RKC1=1,
+nstageS={d['nstageS']}, dampingS={f(d['dampingS'])}, c_2={f(d['c_2'])}, c_1={f(d['c_1'])},
+tend=1.d0, tsnap=1.d0, Chebyshev=0, nstage=3, damping=100.d0, aRK2=0.5d0, torder=4, movie=0,
+tmovie_snap=1.d0, Nlocal=1000, Nsnap=100, local_time=0, dt_manual=0.125d0,
+courant={f(d['courant'])}, courantT={f(d['courantT'])}, courantC={f(d['courantC'])},
+courant_stream={f(d['courant_stream'])}, RK_update=1, vel_update=1, CFL=0,
+comp=1, ntype=1, iC_continuous=1, boundary_tracers=1, original=1, shape_function=1, buffer=10,
+equalize=0, tracer=1, tpc={d['tpc']}, dist_factor=1.d-4,
+restart=0, initial_temp=2, initial_comp=0, initial_temp_smooth=0, delta_visT_init=2.0d0,
+add_layer=0, dlayer=-1.d0, dent=0.5d0, klayer=35.d0, Tbot={d['Tbot']}, Vbc={d['Vbc']}
+/
+&Rayleigh_numbers
+RaT={f(d['RaT'])}, RaC(1)={f(d['RaC'])}
+/
+&heating_conductivity
+iheattype=0, H={f(d['H'])}, conduct_factor(1)={f(d['conduct_factor'])}, Htype(1)={f(d['Htype'])}
+/
+&viscosity
+plastic=0, ivis={d['ivis']}, visP={f(d['visP'])}, visT={f(d['visT'])}, delta_visT={f(d['delta_visT'])},
+eig_ratio_min={f(d['eig_ratio_min'])}, visC(1)={f(d['visC'])}, delta_visC(1)={f(d['delta_visC'])},
+yield=1.d10, yield_gradient=0.d0, vis_star=0.d0, mpla=0.052822d0, Cpla=2.5453d0, alpha=-10.d0
+/
+"""
+    path.write_text(txt)
+
+
+def read_state(path: Path):
+    """Parse state_NNNNN.bin written by ref_driver.f90 (write_state)."""
+    raw = path.read_bytes()
+    off = 0
+
+    def take(n, dt):
+        nonlocal off
+        a = np.frombuffer(raw, dtype=dt, count=n, offset=off)
+        off += a.nbytes
+        return a
+
+    nx, nz, ntr, step = (int(v) for v in take(4, np.int32))
+    dt, time, Tb, Tt = take(4, np.float64)
+    s3 = (nz + 7, nx + 7)
+    s5 = (nz + 11, nx + 11)
+    out = dict(nx=nx, nz=nz, ntr=ntr, step=step, dt=dt, time=time, Tbot_iso=Tb, Ttop_iso=Tt)
+    for name in ("SF", "T", "U", "W", "TVIS", "TBUOY"):
+        out[name] = take(s3[0] * s3[1], np.float64).reshape(s3).copy()
+    for name in ("CNEW", "CVIS", "CONDUCT"):
+        out[name] = take(s5[0] * s5[1], np.float64).reshape(s5).copy()
+    out["VIS"] = take(s3[0] * s3[1], np.float64).reshape(s3).copy()
+    out["nempty"] = take((nx + 1) * (nz + 1), np.int32).reshape(nz + 1, nx + 1).copy()
+    for name in ("rtr", "str", "Ttr", "Ctr"):
+        out[name] = take(ntr, np.float64).copy()
+    return out
+
+
+def read_init(path: Path, nstageS=35):
+    raw = path.read_bytes()
+    ints = np.frombuffer(raw, dtype=np.int32, count=5)
+    nx, nz, grids, ntr, s = (int(v) for v in ints)
+    d = np.frombuffer(raw, dtype=np.float64, offset=20)
+    o = 0
+    tabs = {}
+    for name in ("D1", "D2", "D3", "D4"):
+        tabs[name] = d[o:o + 7].copy(); o += 7
+    tabs["D"] = d[o:o + 28].reshape(7, 4).T.copy(); o += 28    # D(1:4,-3:3), m fastest
+    tabs["bases"] = d[o:o + 9].copy(); o += 9
+    tabs["mewt"] = d[o:o + s].copy(); o += s
+    tabs["mew"] = d[o:o + s - 1].copy(); o += s - 1
+    tabs["v"] = d[o:o + s - 1].copy(); o += s - 1
+    tabs["cc"] = d[o:o + s - 1].copy(); o += s - 1
+    tabs["stability"] = d[o]; o += 1
+    tabs["dts"], tabs["delta_T"], tabs["delta_C"] = d[o], d[o + 1], d[o + 2]
+    tabs.update(nx=nx, nz=nz, grids=grids, ntr=ntr)
+    return tabs
+
+
+def parse_solves(stdout: str):
+    """Per-solve (tstep, q, residual, res_ref) lines the reference prints at
+    PS/iterative_solver.f90:338 (format 2i5,2g20.8); per-cycle lines
+    (i5,3g20.8) have a float in the second column and are skipped."""
+    out = []
+    for line in stdout.splitlines():
+        parts = line.split()
+        if len(parts) != 4:
+            continue
+        try:
+            t, q = int(parts[0]), int(parts[1])
+            out.append((t, q, float(parts[2]), float(parts[3])))
+        except ValueError:
+            continue
+    return out
+
+
+def run_reference(case, nrun: int, dump_every: int = 0, inject: bool = True, threads: int = 1,
+                  workdir: Path | None = None, timeout: int = 1200):
+    """Run the flang-built reference on `case` for nrun steps; returns
+    (states dict step->state, solves list, stdout)."""
+    if not REF_BIN.exists():
+        raise FileNotFoundError(REF_BIN)
+    tmp = Path(workdir) if workdir else Path(tempfile.mkdtemp(prefix="ptref_"))
+    tmp.mkdir(parents=True, exist_ok=True)
+    write_inputs_nml(tmp / "inputs.nml", case.params)
+    (tmp / "driver.nml").write_text(
+        f"&driver\nnrun={nrun}, inject={1 if inject else 0}, dump_every={dump_every}\n/\n")
+    if inject:
+        with open(tmp / "inject.bin", "wb") as fh:
+            fh.write(np.array([case.Tbot_iso, case.Ttop_iso], dtype=np.float64).tobytes())
+            fh.write(np.ascontiguousarray(case.T_nodes, dtype=np.float64).tobytes())
+            for a in (case.rtr, case.str_, case.Ttr, case.Ctr):
+                fh.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_STACKSIZE="1G")
+    proc = subprocess.run(["bash", "-c", f"ulimit -s unlimited && exec {REF_BIN}"], cwd=tmp, env=env,
+                          capture_output=True, text=True, timeout=timeout)
+    if proc.returncode != 0:
+        raise RuntimeError(f"reference failed rc={proc.returncode}: {proc.stdout[-2000:]} {proc.stderr[-2000:]}")
+    states = {}
+    for f in sorted(tmp.glob("state_*.bin")):
+        st = read_state(f)
+        states[st["step"]] = st
+    init = read_init(tmp / "init.bin")
+    return states, parse_solves(proc.stdout), proc.stdout, init
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / (den if den > 0 else 1.0))
+
+
+def interior(field, h, nx, nz):
+    """Node region (0:nx,0:nz) of a field stored with halo h."""
+    return field[h:h + nz + 1, h:h + nx + 1]
