@@ -1127,9 +1127,14 @@ static void smoother_time(pto_case* c) {
   a2_copy(&c->Cvis, &c->Cnew); a2_copy(&c->Cbuoy, &c->Cnew);
   int nn = 0, trigger = 0;
   A2 T0 = a2_new(-c->span, c->nx + c->span, -c->span, c->nz + c->span);
+  /* smoothness(diff,Cvis(tt,:,:)) passes the halo-5 C array to a dummy
+   * declared with the halo-3 shape (PS/smoother.f90:79, :602): by sequence
+   * association the test reads the C storage with the halo-3 row length. */
+  A2 Cseq = c->Cvis;
+  Cseq.ilo = -c->span; Cseq.klo = -c->span; Cseq.ni = c->nx + 2 * c->span + 1; Cseq.nk = c->nz + 2 * c->span + 1;
   for (;;) {
     for (int n = 1; n <= Niter; n++) {
-      double diff = smoothness(c, &c->Cvis);
+      double diff = smoothness(c, &Cseq);
       if (diff < c->delta_C) { trigger = 1; break; }
       double cc;
       if (nn == 0) {

@@ -124,8 +124,8 @@ def _finish(name, p, T_nodes, r, s, Ttr, Ctr):
 def isoviscous(nx=64, nz=64, tpc=20, **over) -> Case:
     """Config: lambda=1, Ra_T=1e5, eta=1; T=(1-z)+0.01cos(pi x)sin(pi z)
     (initial_temp=2, PS/initial_conditions.f90:60-61) evaluated with libm."""
-    p = default_params(aspect=1.0, nx=nx, nz=nz, grids=6, tpc=tpc, RaT=1e5, RaC=0.0,
-                       ivis=0, courant=0.5, **over)
+    p = default_params(aspect=1.0, nx=nx, nz=nz, tpc=tpc, RaT=1e5, RaC=0.0,
+                       ivis=0, **{"courant": 0.5, "grids": 6, **over})
     aspect = p.aspect
     dr, ds = aspect / nx, 1.0 / nz
 
@@ -147,9 +147,8 @@ def thermochem(nx=384, nz=128, tpc=40, aspect=3.0, RaT=1e6, RaC=5e5, visT=1e3, v
                layer=0.2, seed=1, **over) -> Case:
     """Config: dense layer C=1 for z<layer+0.01cos(2 pi x/lambda), T=(1-z)
     plus uniform noise of amplitude 0.01 (seed), eta(T,C) with ivis=2."""
-    p = default_params(aspect=aspect, nx=nx, nz=nz, grids=6, tpc=tpc, RaT=RaT, RaC=RaC,
-                       ivis=2, visT=visT, visC=visC, delta_visT=2.0, delta_visC=2.0,
-                       courant=0.5, **over)
+    p = default_params(aspect=aspect, nx=nx, nz=nz, tpc=tpc, RaT=RaT, RaC=RaC, visT=visT, visC=visC,
+                       **{"ivis": 2, "delta_visT": 2.0, "delta_visC": 2.0, "courant": 0.5, "grids": 6, **over})
     dr, ds = aspect / nx, 1.0 / nz
     rng = np.random.default_rng(seed)
     x = dr * np.arange(nx + 1)
@@ -168,9 +167,9 @@ def thermochem_large(nx=1024, nz=256, tpc=50, aspect=4.0, seed=2, **over) -> Cas
     """The 1024x256 workload: Ra_T=1e7, Ra_C=3e6, eta=exp(-ln(1e5)T+ln(30)C),
     two-layer C with interface at z=0.3, T=(1-z)+0.01 sum_{k=1..8} a_k
     cos(k pi x/lambda) sin(pi z), a_k ~ N(0,1) (seed)."""
-    p = default_params(aspect=aspect, nx=nx, nz=nz, grids=6, tpc=tpc, RaT=1e7, RaC=3e6,
-                       ivis=2, visT=1e5, visC=1.0 / 30.0, delta_visT=2.0, delta_visC=2.0,
-                       courant=0.5, **over)
+    p = default_params(aspect=aspect, nx=nx, nz=nz, tpc=tpc,
+                       **{"RaT": 1e7, "RaC": 3e6, "ivis": 2, "visT": 1e5, "visC": 1.0 / 30.0,
+                          "delta_visT": 2.0, "delta_visC": 2.0, "courant": 0.5, "grids": 6, **over})
     dr, ds = aspect / nx, 1.0 / nz
     rng = np.random.default_rng(seed)
     a = rng.standard_normal(8)

@@ -1,0 +1,340 @@
+// pt_api.hip -- C ABI entry points (include/projectracer_gpu.h).
+#include <cstring>
+#include <string>
+#include <vector>
+#include "pt_internal.hpp"
+
+namespace pt {
+
+static thread_local std::string g_err;
+void set_error(const std::string& s) { g_err = s; }
+
+// Per-kernel HIP-event timing on the context stream.  Events are recorded
+// around launches and folded into the total once the ring fills.
+static void timer_flush(KTimer& t) {
+  if (t.n == 0) return;
+  (void)hipEventSynchronize(t.end[t.n - 1]);
+  for (int j = 0; j < t.n; j++) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t.beg[j], t.end[j]) == hipSuccess) t.ms += ms;
+  }
+  t.launches += t.n;
+  t.n = 0;
+}
+void timer_begin(Context& c, int which) {
+  KTimer& t = c.kt[which];
+  if (t.n == 1024) timer_flush(t);
+  (void)hipEventRecord(t.beg[t.n], c.stream);
+}
+void timer_end(Context& c, int which) {
+  KTimer& t = c.kt[which];
+  (void)hipEventRecord(t.end[t.n], c.stream);
+  t.n++;
+}
+
+static int check_supported(const pt_params& p) {
+  if (p.nx < 8 || p.nz < 8 || p.grids < 2 || p.grids > MAXL || p.tpc < 1 || p.mingrid < 2) {
+    set_error("nx, nz >= 8, 2 <= grids <= 6, tpc >= 1, mingrid >= 2 required");
+    return PT_ERR_INVALID;
+  }
+  if (p.nstageS < 2 || p.nstageS > 60 || p.Nmulti < 1 || p.max_cycles < 1 || !(p.tolerance > 0.0)) {
+    set_error("2 <= nstageS <= 60, Nmulti >= 1, max_cycles >= 1, tolerance > 0 required");
+    return PT_ERR_INVALID;
+  }
+  if (p.ivis < 0 || p.ivis > 2 || p.RaT == 0.0 || (p.Vbc != 0 && p.Vbc != 1) || (p.Tbot != 0 && p.Tbot != 1)) {
+    set_error("supported: ivis in {0,1,2}, RaT != 0, Vbc and Tbot in {0,1}");
+    return PT_ERR_UNSUPPORTED;
+  }
+  if (p.precision != PT_PREC_FP64 && p.precision != PT_PREC_MIXED) {
+    set_error("precision must be PT_PREC_FP64 or PT_PREC_MIXED");
+    return PT_ERR_INVALID;
+  }
+  if (p.guess != PT_GUESS_WARM) {
+    set_error("only the warm-start initial guess is built");
+    return PT_ERR_UNSUPPORTED;
+  }
+  return PT_OK;
+}
+
+static Context* ctx_of(void* p) { return static_cast<Context*>(p); }
+
+static double* field_ptr(Context& c, int id, size_t* n) {
+  size_t n3 = (size_t)(c.T.nx + 7) * (c.T.nz + 7), n5 = (size_t)(c.T.nx + 11) * (c.T.nz + 11);
+  switch (id) {
+    case PT_FIELD_SF: *n = n3; return c.SF;
+    case PT_FIELD_T: *n = n3; return c.Tratio;
+    case PT_FIELD_TVIS: *n = n3; return c.Tvis;
+    case PT_FIELD_TBUOY: *n = n3; return c.Tbuoy;
+    case PT_FIELD_U: *n = n3; return c.u;
+    case PT_FIELD_W: *n = n3; return c.w;
+    case PT_FIELD_CNEW: *n = n5; return c.Cnew;
+    case PT_FIELD_CVIS: *n = n5; return c.Cvis;
+    case PT_FIELD_CBUOY: *n = n5; return c.Cbuoy;
+    case PT_FIELD_CONDUCT: *n = n5; return c.cond;
+    case PT_FIELD_VIS: *n = n3; return c.L[1].v.vis;
+    case PT_FIELD_RESIDUAL: *n = (size_t)(c.T.nx - 1) * (c.T.nz - 1); return c.L[1].res;
+    case PT_FIELD_SOURCE: *n = n3; return c.src;
+    default: *n = 0; return nullptr;
+  }
+}
+
+}  // namespace pt
+
+using namespace pt;
+
+extern "C" {
+
+const char* pt_last_error(void) { return g_err.c_str(); }
+
+void pt_default_params(pt_params* p) {
+  std::memset(p, 0, sizeof *p);
+  p->aspect = 1.0; p->nx = 64; p->nz = 64; p->grids = 6; p->mingrid = 4;
+  p->tolerance = 1e-8; p->max_cycles = 1000; p->Nmulti = 1; p->eig_ratio = 1.0; p->vis_iter = 12;
+  p->nstageS = 35; p->dampingS = 1000.0; p->c_2 = 0.95; p->c_1 = 0.9;
+  p->courant = 0.5; p->courantT = 0.99; p->courantC = 0.99; p->courant_stream = 0.99;
+  p->tpc = 20; p->Tbot = 0; p->Vbc = 0; p->RaT = 1e5; p->RaC = 0.0;
+  p->H = 0.0; p->conduct_factor = 1.0; p->Htype = 0.0;
+  p->ivis = 0; p->visP = 1.0; p->visT = 1e3; p->delta_visT = 2.0; p->eig_ratio_min = 2.0;
+  p->visC = 10.0; p->delta_visC = 2.0; p->precision = PT_PREC_FP64; p->guess = PT_GUESS_WARM;
+}
+
+int pt_create(const pt_params* p, void** out) {
+  if (!p || !out) { set_error("null argument"); return PT_ERR_INVALID; }
+  *out = nullptr;
+  PT_TRY(check_supported(*p));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_error("no HIP device visible");
+    return PT_ERR_NO_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  int dev = 0;
+  PT_HIP(hipGetDevice(&dev));
+  PT_HIP(hipGetDeviceProperties(&prop, dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_error(std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
+    return PT_ERR_NO_DEVICE;
+  }
+  Context* c = new Context();
+  std::memset(&c->L, 0, sizeof c->L);
+  c->P = *p;
+  c->device = dev;
+  int rc = build_tables(c->P, c->T);
+  if (rc != PT_OK) { delete c; return rc; }
+  c->P.nx = c->T.nx; c->P.nz = c->T.nz; c->P.grids = c->T.grids;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    delete c;
+    return PT_ERR_HIP;
+  }
+  for (auto& t : c->kt)
+    for (int j = 0; j < 1024; j++) {
+      (void)hipEventCreate(&t.beg[j]);
+      (void)hipEventCreate(&t.end[j]);
+    }
+  rc = solver_alloc(*c);
+  if (rc == PT_OK) rc = tracers_alloc(*c);
+  if (rc != PT_OK) {
+    solver_free(*c);
+    tracers_free(*c);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return PT_OK;
+}
+
+int pt_destroy(void* p) {
+  Context* c = ctx_of(p);
+  if (!c) return PT_OK;
+  (void)hipStreamSynchronize(c->stream);
+  solver_free(*c);
+  tracers_free(*c);
+  for (auto& t : c->kt)
+    for (int j = 0; j < 1024; j++) {
+      (void)hipEventDestroy(t.beg[j]);
+      (void)hipEventDestroy(t.end[j]);
+    }
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PT_OK;
+}
+
+int pt_get_dims(void* p, int32_t* nx, int32_t* nz, int32_t* grids, int64_t* ntr) {
+  Context* c = ctx_of(p);
+  if (!c) return PT_ERR_INVALID;
+  if (nx) *nx = c->T.nx;
+  if (nz) *nz = c->T.nz;
+  if (grids) *grids = c->T.grids;
+  if (ntr) *ntr = c->T.ntr;
+  return PT_OK;
+}
+
+int pt_load_state(void* p, const double* T_nodes, double Tbot_iso, double Ttop_iso, const double* rtr,
+                  const double* str, const double* Ttr, const double* Ctr) {
+  Context* c = ctx_of(p);
+  if (!c || !T_nodes || !rtr || !str || !Ttr || !Ctr) { set_error("null argument"); return PT_ERR_INVALID; }
+  return load_state(*c, T_nodes, Tbot_iso, Ttop_iso, rtr, str, Ttr, Ctr);
+}
+
+int pt_multigrid(void* p) {
+  Context* c = ctx_of(p);
+  if (!c) return PT_ERR_INVALID;
+  PT_TRY(multigrid(*c));
+  PT_HIP(hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+int pt_compute_velocities(void* p) {
+  Context* c = ctx_of(p);
+  if (!c) return PT_ERR_INVALID;
+  PT_TRY(compute_velocities(*c));
+  PT_HIP(hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+int pt_update_velocities(void* p) {
+  Context* c = ctx_of(p);
+  if (!c) return PT_ERR_INVALID;
+  PT_TRY(multigrid(*c));
+  PT_TRY(compute_velocities(*c));
+  PT_HIP(hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+int pt_energy_time(void* p) {
+  Context* c = ctx_of(p);
+  if (!c) return PT_ERR_INVALID;
+  PT_TRY(energy_time(*c));
+  PT_HIP(hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+int pt_tracers_to_corners(void* p) {
+  Context* c = ctx_of(p);
+  if (!c) return PT_ERR_INVALID;
+  size_t b3 = sizeof(double) * (size_t)(c->T.nx + 7) * (c->T.nz + 7);
+  PT_HIP(hipMemcpyAsync(c->T0, c->Tratio, b3, hipMemcpyDeviceToDevice, c->stream));
+  PT_TRY(tracers_to_corners(*c, c->r, c->s, c->Tt));
+  PT_HIP(hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+static int time_step_async(Context& c) {
+  PT_TRY(multigrid(c));
+  PT_TRY(compute_velocities(c));
+  PT_TRY(energy_time(c));
+  c.time = c.time + c.dt;
+  return PT_OK;
+}
+
+int pt_time_step(void* p) {
+  Context* c = ctx_of(p);
+  if (!c) return PT_ERR_INVALID;
+  PT_TRY(time_step_async(*c));
+  PT_HIP(hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+int pt_run_steps(void* p, int32_t n) {
+  Context* c = ctx_of(p);
+  if (!c || n < 0) return PT_ERR_INVALID;
+  for (int j = 0; j < n; j++) PT_TRY(time_step_async(*c));
+  PT_HIP(hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+int64_t pt_field_extent(void* p, int32_t id) {
+  Context* c = ctx_of(p);
+  if (!c) return -1;
+  size_t n;
+  return field_ptr(*c, id, &n) ? (int64_t)n : -1;
+}
+
+int pt_get_field(void* p, int32_t id, double* host) {
+  Context* c = ctx_of(p);
+  if (!c || !host) return PT_ERR_INVALID;
+  size_t n;
+  double* d = field_ptr(*c, id, &n);
+  if (!d) { set_error("unknown field id"); return PT_ERR_INVALID; }
+  if (id == PT_FIELD_RESIDUAL) {
+    // level-1 residual is stored with the level pitch; return (1:nx-1,1:nz-1)
+    int nx = c->T.nx, nz = c->T.nz, pitch = nx + 7;
+    PT_HIP(hipMemcpy2DAsync(host, sizeof(double) * (nx - 1), d + (long)4 * pitch + 4, sizeof(double) * pitch,
+                            sizeof(double) * (nx - 1), nz - 1, hipMemcpyDeviceToHost, c->stream));
+  } else {
+    PT_HIP(hipMemcpyAsync(host, d, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  }
+  PT_HIP(hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+int pt_set_field(void* p, int32_t id, const double* host) {
+  Context* c = ctx_of(p);
+  if (!c || !host || id == PT_FIELD_RESIDUAL) return PT_ERR_INVALID;
+  size_t n;
+  double* d = field_ptr(*c, id, &n);
+  if (!d) { set_error("unknown field id"); return PT_ERR_INVALID; }
+  PT_HIP(hipMemcpyAsync(d, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  PT_HIP(hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+int pt_get_tracers(void* p, double* rtr, double* str, double* Ttr, double* Ctr) {
+  Context* c = ctx_of(p);
+  if (!c) return PT_ERR_INVALID;
+  size_t b = sizeof(double) * (size_t)c->T.ntr;
+  if (rtr) PT_HIP(hipMemcpyAsync(rtr, c->r, b, hipMemcpyDeviceToHost, c->stream));
+  if (str) PT_HIP(hipMemcpyAsync(str, c->s, b, hipMemcpyDeviceToHost, c->stream));
+  if (Ttr) PT_HIP(hipMemcpyAsync(Ttr, c->Tt, b, hipMemcpyDeviceToHost, c->stream));
+  if (Ctr) PT_HIP(hipMemcpyAsync(Ctr, c->Ct, b, hipMemcpyDeviceToHost, c->stream));
+  PT_HIP(hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+int pt_get_nempty(void* p, int32_t* out) {
+  Context* c = ctx_of(p);
+  if (!c || !out) return PT_ERR_INVALID;
+  PT_HIP(hipMemcpyAsync(out, c->nempty, sizeof(int32_t) * (size_t)(c->T.nx + 1) * (c->T.nz + 1),
+                        hipMemcpyDeviceToHost, c->stream));
+  PT_HIP(hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+int pt_get_info(void* p, pt_solve_info* info) {
+  Context* c = ctx_of(p);
+  if (!c || !info) return PT_ERR_INVALID;
+  std::memset(info, 0, sizeof *info);
+  info->q = c->last_q; info->residual = c->residual1; info->res_ref = c->res_ref1;
+  info->dt = c->dt; info->time = c->time; info->solves = c->solves; info->cycles = c->cycles;
+  info->n_eta = c->n_eta; info->n_temp = c->n_temp; info->n_comp = c->n_comp; info->eig_fine = c->eig_fine;
+  return PT_OK;
+}
+
+int pt_synchronize(void* p) {
+  Context* c = ctx_of(p);
+  if (!c) return PT_ERR_INVALID;
+  PT_HIP(hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+int pt_enable_kernel_timing(void* p, int32_t on) {
+  Context* c = ctx_of(p);
+  if (!c) return PT_ERR_INVALID;
+  PT_HIP(hipStreamSynchronize(c->stream));
+  for (auto& t : c->kt) { timer_flush(t); t.ms = 0.0; t.launches = 0; }
+  c->timing = on != 0;
+  return PT_OK;
+}
+
+int pt_kernel_time(void* p, int32_t which, double* ms, int64_t* launches) {
+  Context* c = ctx_of(p);
+  if (!c || which < 0 || which >= PT_K_COUNT) return PT_ERR_INVALID;
+  PT_HIP(hipStreamSynchronize(c->stream));
+  timer_flush(c->kt[which]);
+  if (ms) *ms = c->kt[which].ms;
+  if (launches) *launches = c->kt[which].launches;
+  return PT_OK;
+}
+
+}  // extern "C"

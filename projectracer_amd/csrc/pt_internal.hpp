@@ -1,0 +1,164 @@
+// pt_internal.hpp -- device context of the gfx950 hot path.
+//
+// Layout in HBM (all FP64 unless noted; every 2-D field is column-major,
+// i fastest, exactly the Fortran array with its halo so exchanges with the
+// host are plain copies):
+//   node fields, halo 3 : pitch nx+7  (SF, Tratio, T0, Tvis, Tbuoy, u, w, source,
+//                                       conduct_r, conduct_s)
+//   node fields, halo 5 : pitch nx+11 (Cnew, C0, Cvis, Cbuoy, conduct, Textend)
+//   multigrid level l   : pitch n_l+7, halo 3 for every per-level array
+//                         (error, RKC scratch, vis_* set, RHS, residual, dt_stream)
+//   interpolation table : (i, k, field, d) with d = {f, d/dr, d2/dr2, d3/dr3},
+//                         k fastest so the 7 rows a tracer reads are contiguous
+//   tracers             : SoA, ID order for r/s/T/C and the RK4 registers
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "../../include/projectracer_gpu.h"
+
+namespace pt {
+
+constexpr int MAXL = 6;
+constexpr int H3 = 3;   // span / span_interp
+constexpr int H5 = 5;   // spanT
+
+// ---- host-side tables computed once at create (pt_setup.cpp) -------------
+struct Tables {
+  int nx, nz, grids, nsteps, multi[52];
+  double aspect, dr, ds, dr2, ds2, dr3, ds3, dr4, ds4;
+  double factorial[5], dr_power[5], ds_power[5];
+  int nxg[MAXL + 1], nzg[MAXL + 1];
+  double drg[MAXL + 1], dsg[MAXL + 1], dr2g[MAXL + 1], ds2g[MAXL + 1], dr3g[MAXL + 1], ds3g[MAXL + 1],
+      dr4g[MAXL + 1], ds4g[MAXL + 1], drdsg[MAXL + 1], drds2g[MAXL + 1], dr2dsg[MAXL + 1], dr2ds2g[MAXL + 1];
+  double D1[7], D2[7], D3[7], D4[7];
+  double Drs[7][7], Drss[7][7], Drrs[7][7], Drrss[7][7];   // [k+3][i+3]
+  double D[5][7];                                          // D(m,j) = D[m][j+3]
+  double Tr_base, Trr_base, Trrr_base, Trrrr_base, Trs_base, Trrs_base, Trrss_base, Tr_base_so, Trr_base_so;
+  double SR1[MAXL + 1], SR2[MAXL + 1], SR4[MAXL + 1], SR5[MAXL + 1], SR6[MAXL + 1];
+  double SI1[MAXL + 1], SI2[MAXL + 1], SI4[MAXL + 1], SI5[MAXL + 1], SI6[MAXL + 1];
+  double SM1[7][7], SM2[7][7], SM4[7][7], SM5[7][7], SM6[7][7];   // [dk+3][di+3]
+  double mewt[64], mew[64], vv[64], cc[64], RKC1_stability;
+  double delta_T, delta_C;
+  double dts;                    // dts_array value (uniform grid => constant)
+  double dt_vis[MAXL + 1];       // dt_vis per level (constant per level)
+  int kl, ku, ngc;
+  int64_t ntr;
+};
+int build_tables(const pt_params& p, Tables& t);   // returns PT_* status
+
+// ---- stencil constants passed to kernels by value ----------------------
+struct Stencil {
+  double D1[5], D2[5];           // offsets -2..2
+  double D3[7], D4[7];           // offsets -3..3
+  double Drs[5][5], Drss[5][5], Drrs[5][5], Drrss[5][5];   // [b+2][a+2]
+};
+
+struct LevelGeom {
+  int n, m, pitch;               // n_l, m_l, pitch = n_l + 7
+  double dr, ds, dr2, ds2, dr3, ds3, dr4, ds4, drds, drds2, dr2ds, dr2ds2;
+};
+
+// viscosity field set of one level (primary or scratch)
+struct VisSet {
+  double *vis, *vx, *vz, *vxx, *vzz, *vxz, *vxxpzz, *vxxmzz, *vsmooth;
+};
+
+struct Level {
+  LevelGeom g;
+  double *err, *bufB, *bufC;      // psi correction and RKC rotating buffers
+  double *rhs, *res, *dts, *ratio;
+  double *vis_static;
+  VisSet v, vs;                   // primary / scratch viscosity sets
+  // FP32 mirrors for the mixed-precision smoother
+  float *ferr, *fB, *fC, *fS0, *frhs, *fdts, *fvis, *fvxm, *fvxz, *fvx, *fvz;
+  size_t nelem;                   // (n+7)*(m+7)
+};
+
+struct KTimer {
+  hipEvent_t beg[1024], end[1024];
+  int n = 0;
+  double ms = 0.0;
+  int64_t launches = 0;
+};
+
+struct Context {
+  pt_params P;
+  Tables T;
+  Stencil st;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  Level L[MAXL + 1];
+  // node fields
+  double *SF, *Tratio, *T0, *Tvis, *Tbuoy, *u, *w, *src, *cond_r, *cond_s;
+  double* dtab;                   // D(1:3,-3:3) interpolation weights on the device
+  double *Cnew, *C0, *Cvis, *Cbuoy, *cond, *Textend;
+  double* itab;                   // interpolation table (u, w, source)
+  // coarse grid
+  double *Mc, *RowB, *Bc; int32_t* ipiv; int32_t* d_info;
+  // tracers (ID order)
+  double *r, *s, *Tt, *Ct, *r0, *s0, *T0t, *r1, *s1, *T1t, *r2, *s2, *T2t;
+  // projection work
+  int32_t *cellkey, *count, *start, *fill, *perm, *nempty;
+  double *TxS, *CxS, *SS;   // unused by the gather path; kept for reuse as scratch
+  int32_t* d_flags;         // [0]=tracer outside domain
+  // reductions
+  double* red;              // block partials
+  double* red_out;          // device scalars (pinned mirror below)
+  double* h_red;            // pinned host mirror
+  size_t red_cap;
+  // scan scratch
+  int32_t* scan_tmp; size_t scan_cap;
+  // state
+  double Tbot_iso = 0, Ttop_iso = 0, time = 0, dt = 0;
+  double res_ref1 = 0, residual1 = 0;
+  int last_q = 0; int64_t solves = 0, cycles = 0;
+  int n_eta = 0, n_temp = 0, n_comp = 0; double eig_fine = 0;
+  bool first_solve = true;
+  // mixed precision work
+  double* dres64;           // FP64 residual for defect correction
+  // timing
+  bool timing = false;
+  KTimer kt[PT_K_COUNT];
+};
+
+// error reporting
+void set_error(const std::string& s);
+#define PT_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      ::pt::set_error(std::string(#call) + ": " + hipGetErrorString(e_));              \
+      return PT_ERR_HIP;                                                               \
+    }                                                                                  \
+  } while (0)
+#define PT_TRY(call)              \
+  do {                            \
+    int rc_ = (call);             \
+    if (rc_ != PT_OK) return rc_; \
+  } while (0)
+
+// ---- solver (pt_solver.hip) ------------------------------------------------
+int solver_alloc(Context& c);
+int solver_free(Context& c);
+int multigrid(Context& c);
+int compute_velocities(Context& c);
+int compute_rhs1(Context& c);
+// ---- tracers (pt_tracers.hip) ------------------------------------------------
+int tracers_alloc(Context& c);
+int tracers_free(Context& c);
+int tracers_to_corners(Context& c, const double* r, const double* s, const double* Tt);
+int energy_time(Context& c);
+int load_state(Context& c, const double* T_nodes, double Tbot, double Ttop, const double* rtr,
+               const double* str, const double* Ttr, const double* Ctr);
+// ---- reductions (pt_solver.hip) ----------------------------------------------
+enum RedOp { RED_SUM_ABS = 0, RED_MAX = 1, RED_MIN = 2 };
+int reduce_to_host(Context& c, const double* d, int n, int m, int pitch, int off, RedOp op, double* out);
+__global__ void k_reduce_final(const double* partial, int nb, int op, double* out);
+
+// timing helpers
+void timer_begin(Context& c, int which);
+void timer_end(Context& c, int which);
+
+}  // namespace pt

@@ -1,0 +1,1144 @@
+// pt_solver.hip -- stream-function multigrid on gfx950.
+//
+// Kernels (one HIP kernel per reference loop nest or fused group):
+//   k_stage<R>      fused stream_derivatives + stream_space + RKC1 stage update
+//                   + enforceBCs_stream (ghost write-through), LDS tile with a
+//                   3-wide halo       (PS/iterative_solver.f90:973-1018, 890-911,
+//                                      1561-1597, 1616-1664)
+//   k_residual<R>   stream_derivatives + compute_residual with per-block |r| sums
+//                                      (PS/iterative_solver.f90:415-448)
+//   k_restrict      full weighting     (PS/iterative_solver.f90:1107-1145)
+//   k_prolong       bilinear prolongation + BCs (PS/iterative_solver.f90:343-396)
+//   k_rhs1          compute_RHS level 1 (PS/iterative_solver.f90:1075-1101)
+//   k_visf / k_vis_* viscosity hierarchy (PS/rheology.f90:2-23, PS/smoother.f90:454-580,
+//                   PS/iterative_solver.f90:472-690, 1147-1211)
+//   k_coarse_*      banded coarse operator, DGBEQU row scaling, DGBTF2 LU and
+//                   DGBTRS solve on one wavefront (PS/iterative_solver.f90:1701-1887)
+//   k_velocities    compute_velocities + velocityBCs (PS/momentum.f90:655-710,
+//                   PS/composition.f90:1295-1325)
+// FP64 kernels are compiled without FMA contraction so each point is
+// evaluated in the reference's operation order.
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+#include "pt_internal.hpp"
+
+namespace pt {
+
+#define IDX(i, k, pitch) ((long)((k) + 3) * (pitch) + ((i) + 3))
+
+// ---------------------------------------------------------------------------
+// stencil evaluation of stream_space on an LDS tile
+// ---------------------------------------------------------------------------
+struct Coef {
+  const void *vis, *vxm, *vxz, *vx, *vz, *rhs, *dts;
+};
+
+template <typename R>
+struct StageArgs {
+  const R* Sx;      // stencil input
+  const R* S0;      // pointwise
+  const R* S2;      // pointwise
+  R* Y;             // output (+ mirrored ghosts)
+  const R *vis, *vxm, *vxz, *vx, *vz, *rhs, *dts;
+  R c0, c1, c2, cf; // Y = ((c0*S0 + c1*S1) + c2*S2) + ((cf*dt)*F)   (stage >= 2)
+  int stage1;       // Y = S0 + (cf*dt)*F
+  int s0zero;       // S0 == 0 (mixed-precision correction from zero)
+  int ivis0;
+  int n, m, pitch;
+  R sz;             // ghost sign in z (free-slip -1, rigid +1)
+  R dr2, dr3, dr4, ds2, ds3, ds4, drds, dr2ds, drds2, dr2ds2;
+  R D1[5], D2[5], D3[7], D4[7];
+  R Drs[5][5], Drss[5][5], Drrs[5][5], Drrss[5][5];
+  const int* halt;  // device flag: skip work when nonzero (may be null)
+};
+
+constexpr int TX = 64, TZ = 4, PZ = 2;
+constexpr int TW = TX + 6, TH = TZ * PZ + 6;
+
+template <typename R>
+__device__ __forceinline__ R stream_space_tile(const R (*t)[TW], int ti, int tk, const StageArgs<R>& a, long g) {
+  // stream_derivatives (uniform grid), reference summation order, zero stencil weights skipped
+  R sxx = 0, szz = 0, sxxx = 0, szzz = 0, sxxxx = 0, szzzz = 0;
+#pragma unroll
+  for (int q = -2; q <= 2; q++) sxx = sxx + a.D2[q + 2] * t[tk][ti + q];
+#pragma unroll
+  for (int q = -3; q <= 3; q++) if (q != 0) sxxx = sxxx + a.D3[q + 3] * t[tk][ti + q];
+#pragma unroll
+  for (int q = -3; q <= 3; q++) sxxxx = sxxxx + a.D4[q + 3] * t[tk][ti + q];
+#pragma unroll
+  for (int q = -2; q <= 2; q++) szz = szz + a.D2[q + 2] * t[tk + q][ti];
+#pragma unroll
+  for (int q = -3; q <= 3; q++) if (q != 0) szzz = szzz + a.D3[q + 3] * t[tk + q][ti];
+#pragma unroll
+  for (int q = -3; q <= 3; q++) szzzz = szzzz + a.D4[q + 3] * t[tk + q][ti];
+  R SFxx = sxx / a.dr2, SFxxx = sxxx / a.dr3, SFxxxx = sxxxx / a.dr4;
+  R SFzz = szz / a.ds2, SFzzz = szzz / a.ds3, SFzzzz = szzzz / a.ds4;
+  R sxz = 0, sxxz = 0, sxzz = 0, sxxzz = 0;
+#pragma unroll
+  for (int b = -2; b <= 2; b++)
+#pragma unroll
+    for (int q = -2; q <= 2; q++) {
+      R v = t[tk + b][ti + q];
+      if (b != 0 && q != 0) sxz = sxz + a.Drs[b + 2][q + 2] * v;
+      if (b != 0) sxxz = sxxz + a.Drrs[b + 2][q + 2] * v;
+      if (q != 0) sxzz = sxzz + a.Drss[b + 2][q + 2] * v;
+      sxxzz = sxxzz + a.Drrss[b + 2][q + 2] * v;
+    }
+  R SFxz = sxz / a.drds, SFxxz = sxxz / a.dr2ds, SFxzz = sxzz / a.drds2, SFxxzz = sxxzz / a.dr2ds2;
+  R rhs = a.rhs[g];
+  R vis = a.vis[g];
+  R s;
+  if (a.ivis0) {
+    R t2 = vis * (SFxxxx + SFzzzz + (R)2 * SFxxzz);
+    s = t2 - rhs;
+  } else {
+    R t1 = a.vxm[g] * (SFxx - SFzz);
+    R t2 = vis * (SFxxxx + SFzzzz + (R)2 * SFxxzz);
+    R t4 = ((R)4 * a.vxz[g]) * SFxz;
+    R t5 = ((R)2 * a.vx[g]) * (SFxxx + SFxzz);
+    R t6 = ((R)2 * a.vz[g]) * (SFzzz + SFxxz);
+    s = t1 + t2 + t4 + t5 + t6 - rhs;
+  }
+  return -s;
+}
+
+template <typename R>
+__device__ __forceinline__ void load_tile(R (*t)[TW], const R* S, int i0, int k0, int n, int m, int pitch) {
+  for (int idx = threadIdx.y * TX + threadIdx.x; idx < TH * TW; idx += TX * TZ) {
+    int rr = idx / TW, cc = idx - rr * TW;
+    int gi = i0 - 3 + cc, gk = k0 - 3 + rr;
+    R v = 0;
+    if (gi <= n + 3 && gk <= m + 3) v = S[IDX(gi, gk, pitch)];
+    t[rr][cc] = v;
+  }
+}
+
+// store Y(i,k) and its ghost images (enforceBCs_stream: antisymmetric side
+// walls, sz-symmetric top/bottom, psi = 0 on the walls which are never written)
+template <typename R>
+__device__ __forceinline__ void store_with_ghosts(R* Y, int i, int k, int n, int m, int pitch, R y, R sz) {
+  Y[IDX(i, k, pitch)] = y;
+  bool lo_i = i <= 3, hi_i = i >= n - 3, lo_k = k <= 3, hi_k = k >= m - 3;
+  if (lo_i) Y[IDX(-i, k, pitch)] = -y;
+  if (hi_i) Y[IDX(2 * n - i, k, pitch)] = -y;
+  if (lo_k) {
+    Y[IDX(i, -k, pitch)] = sz * y;
+    if (lo_i) Y[IDX(-i, -k, pitch)] = -(sz * y);
+    if (hi_i) Y[IDX(2 * n - i, -k, pitch)] = -(sz * y);
+  }
+  if (hi_k) {
+    Y[IDX(i, 2 * m - k, pitch)] = sz * y;
+    if (lo_i) Y[IDX(-i, 2 * m - k, pitch)] = -(sz * y);
+    if (hi_i) Y[IDX(2 * n - i, 2 * m - k, pitch)] = -(sz * y);
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(TX* TZ) k_stage(StageArgs<R> a) {
+  if (a.halt && *a.halt) return;
+  __shared__ R tile[TH][TW];
+  const int i0 = 1 + blockIdx.x * TX, k0 = 1 + blockIdx.y * TZ * PZ;
+  load_tile<R>(tile, a.Sx, i0, k0, a.n, a.m, a.pitch);
+  __syncthreads();
+#pragma unroll
+  for (int pz = 0; pz < PZ; pz++) {
+    int i = i0 + threadIdx.x, k = k0 + threadIdx.y + pz * TZ;
+    if (i > a.n - 1 || k > a.m - 1) continue;
+    long g = IDX(i, k, a.pitch);
+    int ti = threadIdx.x + 3, tk = threadIdx.y + pz * TZ + 3;
+    R F = stream_space_tile<R>(tile, ti, tk, a, g);
+    R dt = a.dts[g];
+    R s0 = a.s0zero ? (R)0 : a.S0[g];
+    R y;
+    if (a.stage1) {
+      y = s0 + (a.cf * dt) * F;
+    } else {
+      R s1 = tile[tk][ti];
+      R s2 = a.S2[g];
+      y = ((a.c0 * s0 + a.c1 * s1) + a.c2 * s2) + ((a.cf * dt) * F);
+    }
+    store_with_ghosts<R>(a.Y, i, k, a.n, a.m, a.pitch, y, a.sz);
+  }
+}
+
+// residual(i,k) = stream_space(i,k,S); block partial sums of |residual|
+template <typename R>
+__global__ void __launch_bounds__(TX* TZ) k_residual(StageArgs<R> a, R* res, double* partial) {
+  if (a.halt && *a.halt) return;
+  __shared__ R tile[TH][TW];
+  __shared__ double red[TX * TZ];
+  const int i0 = 1 + blockIdx.x * TX, k0 = 1 + blockIdx.y * TZ * PZ;
+  load_tile<R>(tile, a.Sx, i0, k0, a.n, a.m, a.pitch);
+  __syncthreads();
+  double acc = 0.0;
+#pragma unroll
+  for (int pz = 0; pz < PZ; pz++) {
+    int i = i0 + threadIdx.x, k = k0 + threadIdx.y + pz * TZ;
+    if (i > a.n - 1 || k > a.m - 1) continue;
+    long g = IDX(i, k, a.pitch);
+    R F = stream_space_tile<R>(tile, threadIdx.x + 3, threadIdx.y + pz * TZ + 3, a, g);
+    res[g] = F;
+    acc = acc + fabs((double)F);
+  }
+  int tid = threadIdx.y * TX + threadIdx.x;
+  red[tid] = acc;
+  __syncthreads();
+  for (int s = TX * TZ / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] = red[tid] + red[tid + s];
+    __syncthreads();
+  }
+  if (tid == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// reductions: fixed-shape block partials then one block in fixed order
+// ---------------------------------------------------------------------------
+__global__ void k_reduce2d(const double* d, int n, int m, int pitch, int off, int op, double* partial) {
+  __shared__ double red[256];
+  int tid = threadIdx.x;
+  double acc = (op == RED_SUM_ABS) ? 0.0 : (op == RED_MAX ? -INFINITY : INFINITY);
+  long total = (long)n * m;
+  for (long idx = (long)blockIdx.x * 256 + tid; idx < total; idx += (long)gridDim.x * 256) {
+    int i = (int)(idx % n), k = (int)(idx / n);
+    double v = d[(long)(k + off) * pitch + (i + off)];
+    if (op == RED_SUM_ABS) acc = acc + fabs(v);
+    else if (op == RED_MAX) acc = fmax(acc, v);
+    else acc = fmin(acc, v);
+  }
+  red[tid] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      double x = red[tid], y = red[tid + s];
+      red[tid] = (op == RED_SUM_ABS) ? x + y : (op == RED_MAX ? fmax(x, y) : fmin(x, y));
+    }
+    __syncthreads();
+  }
+  if (tid == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_reduce_final(const double* partial, int nb, int op, double* out) {
+  __shared__ double red[256];
+  int tid = threadIdx.x;
+  double acc = (op == RED_SUM_ABS) ? 0.0 : (op == RED_MAX ? -INFINITY : INFINITY);
+  for (int j = tid; j < nb; j += 256) {
+    double v = partial[j];
+    acc = (op == RED_SUM_ABS) ? acc + v : (op == RED_MAX ? fmax(acc, v) : fmin(acc, v));
+  }
+  red[tid] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      double x = red[tid], y = red[tid + s];
+      red[tid] = (op == RED_SUM_ABS) ? x + y : (op == RED_MAX ? fmax(x, y) : fmin(x, y));
+    }
+    __syncthreads();
+  }
+  if (tid == 0) *out = red[0];
+}
+
+int reduce_to_host(Context& c, const double* d, int n, int m, int pitch, int off, RedOp op, double* out) {
+  long total = (long)n * m;
+  int nb = (int)std::min<long>(1024, (total + 255) / 256);
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_reduce2d, dim3(nb), dim3(256), 0, c.stream, d, n, m, pitch, off, (int)op, c.red);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, nb, (int)op, c.red_out);
+  PT_HIP(hipMemcpyAsync(c.h_red, c.red_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipStreamSynchronize(c.stream));
+  *out = c.h_red[0];
+  return PT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// grid transfer
+// ---------------------------------------------------------------------------
+// restrict / restrict_coefficients: out(i,k) (coarse interior) from in (fine)
+template <typename RI, typename RO>
+__global__ void k_restrict(const RI* in, int fpitch, RO* out, int cn, int cm, int cpitch) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, k = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > cn - 1 || k > cm - 1) return;
+  int ii = 2 * i, kk = 2 * k;
+  double corners = ((double)in[IDX(ii - 1, kk - 1, fpitch)] + (double)in[IDX(ii - 1, kk + 1, fpitch)] +
+                    (double)in[IDX(ii + 1, kk + 1, fpitch)] + (double)in[IDX(ii + 1, kk - 1, fpitch)]) / 16.0;
+  double sides = ((double)in[IDX(ii - 1, kk, fpitch)] + (double)in[IDX(ii, kk - 1, fpitch)] +
+                  (double)in[IDX(ii + 1, kk, fpitch)] + (double)in[IDX(ii, kk + 1, fpitch)]) / 8.0;
+  double centre = (double)in[IDX(ii, kk, fpitch)] / 4.0;
+  out[IDX(i, k, cpitch)] = (RO)(corners + sides + centre);
+}
+
+// prolong: fine += bilinear(coarse), then the fine ghosts (four parity classes
+// of PS/iterative_solver.f90:364-394 evaluated per fine point)
+template <typename RC, typename RF>
+__global__ void k_prolong(const RC* ec, int cpitch, RF* ef, int fn, int fm, int fpitch, RF sz) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, k = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > fn - 1 || k > fm - 1) return;
+  RF add;
+  bool ie = (i % 2) == 0, ke = (k % 2) == 0;
+  if (ie && ke) add = (RF)ec[IDX(i / 2, k / 2, cpitch)];
+  else if (ie && !ke) add = ((RF)ec[IDX(i / 2, (k - 1) / 2, cpitch)] + (RF)ec[IDX(i / 2, (k + 1) / 2, cpitch)]) / (RF)2;
+  else if (!ie && ke) add = ((RF)ec[IDX((i - 1) / 2, k / 2, cpitch)] + (RF)ec[IDX((i + 1) / 2, k / 2, cpitch)]) / (RF)2;
+  else
+    add = ((RF)ec[IDX((i - 1) / 2, (k - 1) / 2, cpitch)] + (RF)ec[IDX((i + 1) / 2, (k - 1) / 2, cpitch)] +
+           (RF)ec[IDX((i + 1) / 2, (k + 1) / 2, cpitch)] + (RF)ec[IDX((i - 1) / 2, (k + 1) / 2, cpitch)]) / (RF)4;
+  RF y = ef[IDX(i, k, fpitch)] + add;
+  store_with_ghosts<RF>(ef, i, k, fn, fm, fpitch, y, sz);
+}
+
+// compute_RHS(1): RaT*d(Tbuoy)/dx - RaC*d(Cbuoy)/dx (5-point D1)
+__global__ void k_rhs1(const double* Tb, const double* Cb, double* rhs, int nx, int nz, int p3, int p5, double dr,
+                       double RaT, double RaC, double d0, double d1, double d2, double d3, double d4) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, k = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > nx - 1 || k > nz - 1) return;
+  const double D[5] = {d0, d1, d2, d3, d4};
+  double r = 0.0;
+  if (RaT != 0.0) {
+    double s = 0.0;
+    for (int q = -2; q <= 2; q++) s = s + D[q + 2] * Tb[IDX(i + q, k, p3)];
+    double Tr = s / dr;
+    double Tx = Tr / 1.0;
+    r = RaT * Tx;
+  }
+  double s = 0.0;
+  for (int q = -2; q <= 2; q++) s = s + D[q + 2] * Cb[(long)(k + 5) * p5 + (i + q + 5)];
+  double Cr = s / dr;
+  double Cx = Cr / 1.0;
+  r = r - RaC * Cx;
+  rhs[IDX(i, k, p3)] = r;
+}
+
+// compute_velocities: w = -D1_x psi / dr, u = D1_z psi / ds on (0:nx,0:nz)
+__global__ void k_velocities(const double* SF, double* u, double* w, int nx, int nz, int p, double dr, double ds,
+                             double d0, double d1, double d2, double d3, double d4) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > nx || k > nz) return;
+  const double D[5] = {d0, d1, d2, d3, d4};
+  double sx = 0.0, sz = 0.0;
+  for (int q = -2; q <= 2; q++) sx = sx + D[q + 2] * SF[IDX(i + q, k, p)];
+  for (int q = -2; q <= 2; q++) sz = sz + D[q + 2] * SF[IDX(i, k + q, p)];
+  w[IDX(i, k, p)] = -(sx / dr);
+  u[IDX(i, k, p)] = sz / ds;
+}
+
+// velocityBCs (PS/composition.f90:1295-1325) as a gather over the ghost frame:
+// every ghost takes its value from the interior image with the reference's sign
+__global__ void k_velocity_bcs(double* u, double* w, int nx, int nz, int p, int Vbc) {
+  int i = -3 + (int)(blockIdx.x * blockDim.x + threadIdx.x), k = -3 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i > nx + 3 || k > nz + 3) return;
+  bool gi = i < 0 || i > nx, gk = k < 0 || k > nz;
+  int mi = i < 0 ? -i : (i > nx ? 2 * nx - i : i);
+  int mk = k < 0 ? -k : (k > nz ? 2 * nz - k : k);
+  // w: symmetric in x, antisymmetric in z; w=0 on k=0,nz (all columns)
+  double wv, uv;
+  if (!gi && !gk) {
+    wv = w[IDX(i, k, p)]; uv = u[IDX(i, k, p)];
+  } else {
+    double wm = w[IDX(mi, mk, p)], um = u[IDX(mi, mk, p)];
+    wv = gk ? -wm : wm;
+    double su = (Vbc == 0) ? 1.0 : -1.0;
+    uv = gi ? -um : um;
+    if (gk) uv = su * uv;
+  }
+  if (k == 0 || k == nz) wv = 0.0;
+  if (i == 0 || i == nx) uv = 0.0;
+  if (Vbc == 1 && (k == 0 || k == nz)) uv = 0.0;
+  if (gi || gk || k == 0 || k == nz || i == 0 || i == nx) {
+    w[IDX(i, k, p)] = wv;
+    u[IDX(i, k, p)] = uv;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// viscosity hierarchy
+// ---------------------------------------------------------------------------
+__global__ void k_visf(const double* Tvis, const double* Cvis, double* vs, int nx, int nz, int p3, int p5, int ivis,
+                       double lvT, double lvC, double lvP, double ds, int RaTnz) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > nx || k > nz) return;
+  double zg = ds * (double)k;
+  if (k == nz) zg = 1.0;
+  double v = 1.0;
+  if (ivis == 1) v = exp(-(Tvis[IDX(i, k, p3)] * lvT) + (1.0 - zg) * lvP);
+  else if (ivis == 2) {
+    double dp = 0.0 + Cvis[(long)(k + 5) * p5 + (i + 5)] * lvC;
+    if (RaTnz) v = exp(-(Tvis[IDX(i, k, p3)] * lvT) - dp + (1.0 - zg) * lvP);
+    else v = exp(-dp + (1.0 - zg) * lvP);
+  }
+  vs[IDX(i, k, p3)] = v;
+}
+
+// viscosityBCs (PS/iterative_solver.f90:666-690): side ghosts on rows
+// -3..m+3 then top/bottom ghosts on columns 0..n.  The side pass reads the
+// ghost rows before the top/bottom pass rewrites them, as the reference.
+__global__ void k_vis_bcs_side(double* v, int n, int m, int p) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;   // over rows -3..m+3, 3 ghosts each side
+  int rows = m + 7;
+  if (t >= rows * 3) return;
+  int k = -3 + t / 3, i = 1 + t % 3;
+  v[IDX(-i, k, p)] = v[IDX(i, k, p)];
+  v[IDX(n + i, k, p)] = v[IDX(n - i, k, p)];
+}
+__global__ void k_vis_bcs_tb(double* v, int n, int m, int p) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (n + 1) * 3) return;
+  int i = t / 3, k = 1 + t % 3;
+  v[IDX(i, -k, p)] = v[IDX(i, k, p)];
+  v[IDX(i, m + k, p)] = v[IDX(i, m - k, p)];
+}
+
+// viscosityBCs_initialize (PS/iterative_solver.f90:636-664)
+__global__ void k_vis_bcs_init_boundary(double* v, const double* f, int n, int m, int p, int fn, int fm, int fp) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t <= m - 2) {   // k = 1..m-1
+    int k = t + 1;
+    v[IDX(0, k, p)] = f[IDX(0, 2 * k, fp)];
+    v[IDX(n, k, p)] = f[IDX(fn, 2 * k, fp)];
+  }
+  if (t <= n) {
+    int i = t;
+    v[IDX(i, 0, p)] = f[IDX(2 * i, 0, fp)];
+    v[IDX(i, m, p)] = f[IDX(2 * i, fm, fp)];
+  }
+}
+__global__ void k_vis_bcs_init_side(double* v, int n, int m, int p) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (m + 1) * 3) return;
+  int k = t / 3, i = 1 + t % 3;
+  v[IDX(-i, k, p)] = v[IDX(i, k, p)];
+  v[IDX(n + i, k, p)] = v[IDX(n - i, k, p)];
+}
+__global__ void k_vis_bcs_init_tb(double* v, int n, int m, int p) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (n + 7) * 3) return;
+  int i = -3 + t / 3, k = 1 + t % 3;
+  v[IDX(i, -k, p)] = v[IDX(i, k, p)];
+  v[IDX(i, m + k, p)] = v[IDX(i, m - k, p)];
+}
+
+// viscosity_gradients, uniform grid (PS/iterative_solver.f90:488-520)
+__global__ void k_vis_grad(VisSet s, int n, int m, int p, int full, int ivis0, double dr, double ds, double dr2,
+                           double ds2, double drds, double dtv) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > n || k > m) return;
+  long g = IDX(i, k, p);
+  if (ivis0) {
+    s.vx[g] = 0; s.vz[g] = 0; s.vxx[g] = 0; s.vzz[g] = 0; s.vxz[g] = 0; s.vxxpzz[g] = 0; s.vxxmzz[g] = 0;
+    return;
+  }
+  const double* v = s.vis;
+  if (full) {
+    // Drs_so(:,-1)=(.25,0,-.25), (:,0)=0, (:,1)=(-.25,0,.25); sum in column-major order
+    double sxz = 0.0;
+    sxz = sxz + 0.25 * v[IDX(i - 1, k - 1, p)];
+    sxz = sxz + 0.0 * v[IDX(i, k - 1, p)];
+    sxz = sxz + (-0.25) * v[IDX(i + 1, k - 1, p)];
+    sxz = sxz + 0.0 * v[IDX(i - 1, k, p)];
+    sxz = sxz + 0.0 * v[IDX(i, k, p)];
+    sxz = sxz + 0.0 * v[IDX(i + 1, k, p)];
+    sxz = sxz + (-0.25) * v[IDX(i - 1, k + 1, p)];
+    sxz = sxz + 0.0 * v[IDX(i, k + 1, p)];
+    sxz = sxz + 0.25 * v[IDX(i + 1, k + 1, p)];
+    s.vxz[g] = sxz / drds;
+    double sx = 0.0;
+    sx = sx + (-0.5) * v[IDX(i - 1, k, p)];
+    sx = sx + 0.0 * v[IDX(i, k, p)];
+    sx = sx + 0.5 * v[IDX(i + 1, k, p)];
+    s.vx[g] = sx / dr;
+  }
+  double sxx = 0.0;
+  sxx = sxx + 1.0 * v[IDX(i - 1, k, p)];
+  sxx = sxx + (-2.0) * v[IDX(i, k, p)];
+  sxx = sxx + 1.0 * v[IDX(i + 1, k, p)];
+  double vxx = sxx / dr2;
+  s.vxx[g] = vxx;
+  if (full) {
+    double sz = 0.0;
+    sz = sz + (-0.5) * v[IDX(i, k - 1, p)];
+    sz = sz + 0.0 * v[IDX(i, k, p)];
+    sz = sz + 0.5 * v[IDX(i, k + 1, p)];
+    s.vz[g] = sz / ds;
+  }
+  double szz = 0.0;
+  szz = szz + 1.0 * v[IDX(i, k - 1, p)];
+  szz = szz + (-2.0) * v[IDX(i, k, p)];
+  szz = szz + 1.0 * v[IDX(i, k + 1, p)];
+  double vzz = szz / ds2;
+  s.vzz[g] = vzz;
+  double xp = vxx + vzz;
+  s.vxxpzz[g] = xp;
+  if (full) s.vxxmzz[g] = vxx - vzz;
+  s.vsmooth[g] = dtv * xp;
+}
+
+// test_eigenvalues, RKC1 branch (PS/iterative_solver.f90:1187-1208) + block min of ratio
+__global__ void k_test_eig(VisSet s, double* ratio, double* dts, int n, int m, int p, double SR1, double SR2,
+                           double SR4, double SR5, double SR6, double SI1, double SI2, double SI4, double SI5,
+                           double SI6, double ae, double be, double ctemp, double* partial) {
+  __shared__ double red[256];
+  int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, k = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  double rmin = INFINITY;
+  if (i <= n - 1 && k <= m - 1) {
+    long g = IDX(i, k, p);
+    double vis = s.vis[g];
+    double eI = fabs(s.vxxpzz[g] * SI1) + vis * SI2 + fabs(s.vxz[g] * SI4) + fabs(s.vx[g] * SI5) + fabs(s.vz[g] * SI6);
+    double eR = s.vxxmzz[g] * SR1 + vis * SR2 + s.vxz[g] * SR4 + s.vx[g] * SR5 + s.vz[g] * SR6;
+    double r = fabs(eR / eI);
+    ratio[g] = r;
+    double qa = eR / ae, qb = eI / be;
+    dts[g] = ctemp * eR / (qa * qa + qb * qb);
+    rmin = r;
+  }
+  red[tid] = rmin;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) red[tid] = fmin(red[tid], red[tid + st]);
+    __syncthreads();
+  }
+  if (tid == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ void k_vis_update(double* v, const double* sm, int n, int m, int p, double c) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > n || k > m) return;
+  long g = IDX(i, k, p);
+  v[g] = v[g] + c * sm[g];
+}
+
+// ---------------------------------------------------------------------------
+// coarsest grid: banded operator, row equilibration, LU, solve
+// ---------------------------------------------------------------------------
+struct CoarseArgs {
+  const double *vis, *vxm, *vxz, *vx, *vz;
+  int nxc, nzc, p, kl, ku, ngc, ivis0, Vbc;
+  double SM1[7][7], SM2[7][7], SM4[7][7], SM5[7][7], SM6[7][7];
+};
+
+__device__ double stream_space_matrix(const CoarseArgs& a, int ip, int kp, int iq, int kq, int sval) {
+  int di = iq - ip, dk = kq - kp;
+  long g = IDX(ip, kp, a.p);
+  double v;
+  if (a.ivis0) v = a.vis[g] * a.SM2[dk + 3][di + 3];
+  else
+    v = a.vxm[g] * a.SM1[dk + 3][di + 3] + a.vis[g] * a.SM2[dk + 3][di + 3] + a.vxz[g] * a.SM4[dk + 3][di + 3] +
+        a.vx[g] * a.SM5[dk + 3][di + 3] + a.vz[g] * a.SM6[dk + 3][di + 3];
+  if (sval == -1) v = -v;
+  return v;
+}
+
+// one thread per matrix row p (Compute_Matrix_coarse, analysis=0)
+__global__ void k_coarse_assemble(CoarseArgs a, double* Mc, double* RowB) {
+  int p = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > a.ngc) return;
+  const int limit[7] = {0, 2, 2, 3, 2, 2, 0};
+  int ld = 2 * a.kl + a.ku + 1, kk1 = a.kl + a.ku + 1, N = a.nzc + 1;
+  int kp = p - N * ((p - 1) / N) - 1, ip = (p - 1) / N;
+  auto M = [&](int r, int q) -> double& { return Mc[(long)(q - 1) * ld + (r - 1)]; };
+  if (ip > 0 && ip < a.nxc && kp > 0 && kp < a.nzc) {
+    for (int j = -3; j <= 3; j++) {
+      int iq = j + ip;
+      if (iq >= 0 && iq <= a.nxc)
+        for (int kq = kp - limit[j + 3]; kq <= kp + limit[j + 3]; kq++)
+          if (kq >= 0 && kq <= a.nzc) {
+            int q = N * iq + kq + 1;
+            M(kk1 + p - q, q) = stream_space_matrix(a, ip, kp, iq, kq, 1);
+          }
+    }
+    for (int j = -3; j <= 3; j++) {
+      int iq = j + ip;
+      for (int kq = kp - limit[j + 3]; kq <= kp + limit[j + 3]; kq++)
+        if (iq < 0 || iq > a.nxc || kq < 0 || kq > a.nzc) {
+          int iiq, kkq, xv, zv;
+          if (iq < 0) { iiq = -iq; xv = -1; }
+          else if (iq > a.nxc) { iiq = 2 * a.nxc - iq; xv = -1; }
+          else { iiq = iq; xv = 1; }
+          if (kq < 0) { kkq = -kq; zv = a.Vbc == 0 ? -1 : 1; }
+          else if (kq > a.nzc) { kkq = 2 * a.nzc - kq; zv = a.Vbc == 0 ? -1 : 1; }
+          else { kkq = kq; zv = 1; }
+          int q = N * iiq + kkq + 1;
+          M(kk1 + p - q, q) = M(kk1 + p - q, q) + stream_space_matrix(a, ip, kp, iq, kq, xv * zv);
+        }
+    }
+  } else {
+    M(kk1, p) = 1.0;   // impermeable boundary rows
+  }
+}
+
+// DGBEQU row scale factors and row scaling; one thread per row
+__global__ void k_coarse_equilibrate(double* Mc, double* RowB, int n, int kl, int ku) {
+  int p = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > n) return;
+  int ld = 2 * kl + ku + 1, kk1 = kl + ku + 1;
+  double r = 0.0;
+  int qlo = std::max(1, p - kl), qhi = std::min(n, p + ku);
+  for (int q = qlo; q <= qhi; q++) {
+    double a = fabs(Mc[(long)(q - 1) * ld + (kk1 + p - q - 1)]);
+    if (a > r) r = a;
+  }
+  const double smlnum = 2.2250738585072014e-308, bignum = 1.0 / smlnum;
+  r = fmin(fmax(r, smlnum), bignum);
+  double rb = 1.0 / r;
+  RowB[p - 1] = rb;
+  for (int q = qlo; q <= qhi; q++) {
+    double& m = Mc[(long)(q - 1) * ld + (kk1 + p - q - 1)];
+    m = rb * m;
+  }
+}
+
+// DGBTF2 (LAPACK unblocked band LU with partial pivoting) on one wavefront
+__global__ void __launch_bounds__(64) k_band_lu(double* ab, int32_t* ipiv, int n, int kl, int ku, int32_t* info) {
+  const int ld = 2 * kl + ku + 1, kv = ku + kl;
+  const int lane = threadIdx.x;
+#define AB(r, q) ab[(long)((q) - 1) * ld + ((r) - 1)]
+  for (int j = ku + 2 + lane; j <= std::min(kv, n); j += 64)
+    for (int i = kv - j + 2; i <= kl; i++) AB(i, j) = 0.0;
+  __syncthreads();
+  int ju = 1, inf = 0;
+  for (int j = 1; j <= n; j++) {
+    if (j + kv <= n)
+      for (int i = 1 + lane; i <= kl; i += 64) AB(i, j + kv) = 0.0;
+    int km = std::min(kl, n - j);
+    // IDAMAX over AB(kv+1 .. kv+1+km, j): first index of the maximum |.|
+    double best = -1.0; int bi = 0x7fffffff;
+    for (int t = lane; t <= km; t += 64) {
+      double a = fabs(AB(kv + 1 + t, j));
+      if (a > best) { best = a; bi = t; }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      double ob = __shfl_down(best, off);
+      int oi = __shfl_down(bi, off);
+      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+    int jp = __shfl(bi, 0) + 1;
+    __syncthreads();
+    if (lane == 0) ipiv[j - 1] = jp + j - 1;
+    double piv = AB(kv + jp, j);
+    if (piv != 0.0) {
+      ju = std::max(ju, std::min(j + ku + jp - 1, n));
+      if (jp != 1)
+        for (int t = lane; t <= ju - j; t += 64) {
+          double* pa = &ab[(long)(j - 1 + t) * ld + (kv + jp - 1 - t)];
+          double* pb = &ab[(long)(j - 1 + t) * ld + (kv - t)];
+          double tmp = *pa; *pa = *pb; *pb = tmp;
+        }
+      __syncthreads();
+      if (km > 0) {
+        double rcp = 1.0 / AB(kv + 1, j);
+        for (int t = 1 + lane; t <= km; t += 64) AB(kv + 1 + t, j) = rcp * AB(kv + 1 + t, j);
+        __syncthreads();
+        // DGER: columns are independent (the pivot-row element y(jj) is not
+        // updated), so the (jj, t) pairs are spread over the wavefront
+        const int total = (ju - j) * km;
+        for (int idx = lane; idx < total; idx += 64) {
+          int jj = 1 + idx / km, t = 1 + idx % km;
+          double yv = ab[(long)(j - 1 + jj) * ld + (kv - jj)];
+          double temp = -1.0 * yv;
+          double& x = ab[(long)(j - 1 + jj) * ld + (kv - jj + t)];
+          x = x + AB(kv + 1 + t, j) * temp;
+        }
+      }
+    } else if (inf == 0) {
+      inf = j;
+    }
+    __syncthreads();
+  }
+  if (lane == 0) *info = inf;
+#undef AB
+}
+
+// Compute_coarse_vector + DGBTRS('N') + Stream_grid_coarse + enforceBCs_stream
+template <typename RO>
+__global__ void __launch_bounds__(64) k_coarse_solve(const double* ab, const int32_t* ipiv, const double* RowB,
+                                                     const void* rhs_in, int rhs_is_float, RO* err, int n, int kl,
+                                                     int ku, int nxc, int nzc, int p, RO sz, const int* halt) {
+  if (halt && *halt) return;
+  __shared__ double b[4096];
+  const int lane = threadIdx.x, ld = 2 * kl + ku + 1, kd = ku + kl + 1, N = nzc + 1;
+  for (int q = lane; q < n; q += 64) {
+    int kq = q % N, iq = q / N;
+    double v = 0.0;
+    if (iq > 0 && iq < nxc && kq > 0 && kq < nzc) {
+      double r = rhs_is_float ? (double)((const float*)rhs_in)[IDX(iq, kq, p)] : ((const double*)rhs_in)[IDX(iq, kq, p)];
+      v = r * RowB[q];
+    }
+    b[q] = v;
+  }
+  __syncthreads();
+#define AB(r, q) ab[(long)((q) - 1) * ld + ((r) - 1)]
+  for (int j = 1; j <= n - 1; j++) {
+    int lm = std::min(kl, n - j);
+    int l = ipiv[j - 1];
+    if (l != j) {
+      if (lane == 0) { double t = b[l - 1]; b[l - 1] = b[j - 1]; b[j - 1] = t; }
+      __syncthreads();
+    }
+    double temp = -1.0 * b[j - 1];
+    for (int t = 1 + lane; t <= lm; t += 64) b[j - 1 + t] = b[j - 1 + t] + AB(kd + t, j) * temp;
+    __syncthreads();
+  }
+  const int kk = kl + ku, kp1 = kk + 1;
+  for (int j = n; j >= 1; j--) {
+    double bj = b[j - 1];
+    __syncthreads();
+    if (bj != 0.0) {
+      int l = kp1 - j;
+      double x = bj / AB(kp1, j);
+      if (lane == 0) b[j - 1] = x;
+      int lo = std::max(1, j - kk);
+      for (int i = j - 1 - lane; i >= lo; i -= 64) b[i - 1] = b[i - 1] - x * AB(l + i, j);
+    }
+    __syncthreads();
+  }
+#undef AB
+  for (int q = lane; q < n; q += 64) {
+    int kq = q % N, iq = q / N;
+    if (iq > 0 && iq < nxc && kq > 0 && kq < nzc) store_with_ghosts<RO>(err, iq, kq, nxc, nzc, p, (RO)b[q], sz);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host orchestration
+// ---------------------------------------------------------------------------
+static dim3 grid2(int ni, int nk, dim3 b) { return dim3((ni + b.x - 1) / b.x, (nk + b.y - 1) / b.y); }
+
+template <typename R>
+static StageArgs<R> stage_base(Context& c, int l, bool fp32) {
+  StageArgs<R> a;
+  std::memset(&a, 0, sizeof a);
+  Level& L = c.L[l];
+  const LevelGeom& g = L.g;
+  a.n = g.n; a.m = g.m; a.pitch = g.pitch;
+  a.sz = (R)(c.P.Vbc == 0 ? -1.0 : 1.0);
+  a.dr2 = (R)g.dr2; a.dr3 = (R)g.dr3; a.dr4 = (R)g.dr4; a.ds2 = (R)g.ds2; a.ds3 = (R)g.ds3; a.ds4 = (R)g.ds4;
+  a.drds = (R)g.drds; a.dr2ds = (R)g.dr2ds; a.drds2 = (R)g.drds2; a.dr2ds2 = (R)g.dr2ds2;
+  for (int q = 0; q < 5; q++) { a.D1[q] = (R)c.st.D1[q]; a.D2[q] = (R)c.st.D2[q]; }
+  for (int q = 0; q < 7; q++) { a.D3[q] = (R)c.st.D3[q]; a.D4[q] = (R)c.st.D4[q]; }
+  for (int b = 0; b < 5; b++)
+    for (int q = 0; q < 5; q++) {
+      a.Drs[b][q] = (R)c.st.Drs[b][q]; a.Drss[b][q] = (R)c.st.Drss[b][q];
+      a.Drrs[b][q] = (R)c.st.Drrs[b][q]; a.Drrss[b][q] = (R)c.st.Drrss[b][q];
+    }
+  a.ivis0 = c.P.ivis == 0;
+  if (!fp32) {
+    a.vis = (const R*)L.v.vis; a.vxm = (const R*)L.v.vxxmzz; a.vxz = (const R*)L.v.vxz;
+    a.vx = (const R*)L.v.vx; a.vz = (const R*)L.v.vz; a.rhs = (const R*)L.rhs; a.dts = (const R*)L.dts;
+  } else {
+    a.vis = (const R*)L.fvis; a.vxm = (const R*)L.fvxm; a.vxz = (const R*)L.fvxz;
+    a.vx = (const R*)L.fvx; a.vz = (const R*)L.fvz; a.rhs = (const R*)L.frhs; a.dts = (const R*)L.fdts;
+  }
+  a.halt = nullptr;
+  return a;
+}
+
+// iterate_stream (RKC1, Nmulti sweeps) followed by compute_residual
+// (PS/iterative_solver.f90:1418-1614, 415-448).  `s0zero` runs the smoother
+// on a correction that starts from zero (mixed precision).
+template <typename R>
+static int iterate_stream_t(Context& c, int l, int p, bool fp32, double* resmag) {
+  Level& L = c.L[l];
+  const LevelGeom& g = L.g;
+  const int s = c.P.nstageS;
+  dim3 blk(TX, TZ), grd((g.n - 1 + TX - 1) / TX, (g.m - 1 + TZ * PZ - 1) / (TZ * PZ));
+  R* err = fp32 ? (R*)L.ferr : (R*)L.err;
+  R* B = fp32 ? (R*)L.fB : (R*)L.bufB;
+  R* Cb = fp32 ? (R*)L.fC : (R*)L.bufC;
+  bool timed = (l == 1) && c.timing;
+  for (int it = 1; it <= c.P.Nmulti; it++) {
+    int ntemp = c.P.Nmulti - it + 1;
+    double temp = (c.P.Nmulti > 1) ? (c.P.c_2 - c.P.c_1) * (double)(ntemp - 1) / (double)(1 - c.P.Nmulti) : 0.0;
+    double cc = temp + c.P.c_2;
+    StageArgs<R> a = stage_base<R>(c, l, fp32);
+    // stage 1: S1 = S0 + mewt1*c*dt*F(S0) -> B
+    a.Sx = err; a.S0 = err; a.S2 = err; a.Y = B; a.stage1 = 1; a.cf = (R)(c.T.mewt[1] * cc);
+    if (timed) timer_begin(c, PT_K_RKC_STAGE_L1);
+    hipLaunchKernelGGL(k_stage<R>, grd, blk, 0, c.stream, a);
+    if (timed) timer_end(c, PT_K_RKC_STAGE_L1);
+    // rotating buffers: S1 in `cur`, S2 in `prev`; Y goes to the S2 buffer
+    // except at stage 2 (S2 aliases S0) and at the last stage (Y -> err)
+    R* cur = B;
+    R* prev = err;   // S2 = S0 interior at stage 2
+    R* spare = Cb;
+    for (int j = 2; j <= s; j++) {
+      a.stage1 = 0;
+      a.Sx = cur; a.S0 = err; a.S2 = prev;
+      a.c0 = (R)c.T.cc[j]; a.c1 = (R)c.T.mew[j]; a.c2 = (R)c.T.vv[j]; a.cf = (R)(c.T.mewt[j] * cc);
+      R* out;
+      if (j == s) out = err;
+      else if (prev == err) out = spare;
+      else out = prev;
+      a.Y = out;
+      if (timed) timer_begin(c, PT_K_RKC_STAGE_L1);
+      hipLaunchKernelGGL(k_stage<R>, grd, blk, 0, c.stream, a);
+      if (timed) timer_end(c, PT_K_RKC_STAGE_L1);
+      if (j < s) {
+        R* np = cur;
+        cur = out;
+        prev = np;
+        spare = (out == spare) ? nullptr : spare;
+      }
+    }
+  }
+  // residual of the smoothed field
+  StageArgs<R> a = stage_base<R>(c, l, fp32);
+  a.Sx = err;
+  R* res = fp32 ? (R*)L.fB : (R*)L.res;   // FP32 residual reuses the B buffer
+  int nb = grd.x * grd.y;
+  hipLaunchKernelGGL(k_residual<R>, grd, blk, 0, c.stream, a, res, c.red);
+  if (resmag) {
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, nb, (int)RED_SUM_ABS, c.red_out);
+    PT_HIP(hipMemcpyAsync(c.h_red, c.red_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    PT_HIP(hipStreamSynchronize(c.stream));
+    *resmag = c.h_red[0] / (double)(g.n - 2) / (double)(g.m - 2);
+  }
+  (void)p;
+  return PT_OK;
+}
+
+// smoother_vis (PS/smoother.f90:454-580) on the field set `vs` of level l
+static int smoother_vis(Context& c, int l, VisSet& vs, int interpolate) {
+  Level& L = c.L[l];
+  const LevelGeom& g = L.g;
+  const Tables& t = c.T;
+  auto bcs = [&]() {
+    int rows = (g.m + 7) * 3;
+    hipLaunchKernelGGL(k_vis_bcs_side, dim3((rows + 255) / 256), dim3(256), 0, c.stream, vs.vis, g.n, g.m, g.pitch);
+    int cols = (g.n + 1) * 3;
+    hipLaunchKernelGGL(k_vis_bcs_tb, dim3((cols + 255) / 256), dim3(256), 0, c.stream, vs.vis, g.n, g.m, g.pitch);
+  };
+  bcs();
+  int option, Niter;
+  double lb, la;
+  if (interpolate == 0) { option = 1; Niter = 3; lb = -3.0; la = -2.0; }
+  else { option = 0; Niter = c.P.vis_iter; lb = -2.0; la = -1.0; }
+  int Nvis_store = 0, nn_store = 1, n = 0;
+  bool trigger = false;
+  double ratio_min = 1.0e100;
+  dim3 b16(64, 4);
+  dim3 gnodes = grid2(g.n + 1, g.m + 1, b16);
+  dim3 gint = grid2(g.n - 1, g.m - 1, b16);
+  const int ivis0 = c.P.ivis == 0;
+  double ae = -(t.RKC1_stability / 2.0), be = (double)c.P.nstageS;
+  double ctemp = c.P.courant_stream * (t.RKC1_stability / (ae * ae));
+  for (;;) {
+    for (int nn = nn_store; nn <= Niter; nn++) {
+      hipLaunchKernelGGL(k_vis_grad, gnodes, b16, 0, c.stream, vs, g.n, g.m, g.pitch, Nvis_store >= 1 ? 1 : 0, ivis0,
+                         g.dr, g.ds, g.dr2, g.ds2, g.drds, t.dt_vis[l]);
+      if (interpolate == 0 && (Nvis_store >= 1 || ivis0)) {
+        hipLaunchKernelGGL(k_test_eig, gint, b16, 0, c.stream, vs, L.ratio, L.dts, g.n, g.m, g.pitch, t.SR1[l],
+                           t.SR2[l], t.SR4[l], t.SR5[l], t.SR6[l], t.SI1[l], t.SI2[l], t.SI4[l], t.SI5[l], t.SI6[l],
+                           ae, be, ctemp, c.red);
+        int nb = gint.x * gint.y;
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, nb, (int)RED_MIN, c.red_out);
+        PT_HIP(hipMemcpyAsync(c.h_red, c.red_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        PT_HIP(hipStreamSynchronize(c.stream));
+        ratio_min = c.h_red[0];
+      }
+      if (option == 1) {
+        if ((Nvis_store >= 1 && ratio_min > c.P.eig_ratio_min) || ivis0 || n > 20) {
+          c.eig_fine = ratio_min;   // module variables eig_fine, n_eta: last call wins
+          c.n_eta = n;
+          trigger = true;
+          break;
+        }
+      } else if (interpolate >= 1 && (Nvis_store >= 1 || ivis0)) {
+        trigger = true;
+        break;
+      }
+      int ntemp = Niter - nn + 1;
+      double cc = 2.0 / (-lb - la + (lb - la) * std::cos((double)(2 * ntemp - 1) * 3.1415926535897932 / (double)(2 * Niter)));
+      cc = 0.99 * cc;
+      hipLaunchKernelGGL(k_vis_update, gnodes, b16, 0, c.stream, vs.vis, vs.vsmooth, g.n, g.m, g.pitch, cc);
+      bcs();
+    }
+    if (trigger) break;
+    Nvis_store++;
+    nn_store = 1;
+    if (option == 1) { Niter = 6; lb = -2.0; la = -1.0; }
+    n++;
+  }
+  return PT_OK;
+}
+
+// compute_viscosity (Newtonian, static=1): PS/iterative_solver.f90:560-634.
+// The reference saves and restores the finer level's fields around the
+// restriction; here the finer level is smoothed in its scratch set instead.
+static int compute_viscosity(Context& c, int l) {
+  Level& L = c.L[l];
+  const LevelGeom& g = L.g;
+  dim3 b16(64, 4);
+  size_t bytes = sizeof(double) * L.nelem;
+  if (l == 1) {
+    const double lvT = std::log(c.P.visT), lvC = std::log(c.P.visC), lvP = std::log(c.P.visP);
+    hipLaunchKernelGGL(k_visf, grid2(g.n + 1, g.m + 1, b16), b16, 0, c.stream, c.Tvis, c.Cvis, L.vis_static, g.n, g.m,
+                       g.pitch, g.n + 11, c.P.ivis, lvT, lvC, lvP, c.T.ds, c.P.RaT != 0.0 ? 1 : 0);
+    PT_HIP(hipMemcpyAsync(L.v.vis, L.vis_static, bytes, hipMemcpyDeviceToDevice, c.stream));
+    return smoother_vis(c, 1, L.v, 0);
+  }
+  Level& F = c.L[l - 1];
+  size_t fbytes = sizeof(double) * F.nelem;
+  PT_HIP(hipMemcpyAsync(F.vs.vis, F.vis_static, fbytes, hipMemcpyDeviceToDevice, c.stream));
+  PT_TRY(smoother_vis(c, l - 1, F.vs, 1));
+  hipLaunchKernelGGL((k_restrict<double, double>), grid2(g.n - 1, g.m - 1, b16), b16, 0, c.stream, F.vs.vis,
+                     F.g.pitch, L.v.vis, g.n, g.m, g.pitch);
+  int tmax = std::max(g.n + 1, g.m - 1);
+  hipLaunchKernelGGL(k_vis_bcs_init_boundary, dim3((tmax + 255) / 256), dim3(256), 0, c.stream, L.v.vis, F.vs.vis,
+                     g.n, g.m, g.pitch, F.g.n, F.g.m, F.g.pitch);
+  int ts = (g.m + 1) * 3;
+  hipLaunchKernelGGL(k_vis_bcs_init_side, dim3((ts + 255) / 256), dim3(256), 0, c.stream, L.v.vis, g.n, g.m, g.pitch);
+  int tt = (g.n + 7) * 3;
+  hipLaunchKernelGGL(k_vis_bcs_init_tb, dim3((tt + 255) / 256), dim3(256), 0, c.stream, L.v.vis, g.n, g.m, g.pitch);
+  PT_HIP(hipMemcpyAsync(L.vis_static, L.v.vis, bytes, hipMemcpyDeviceToDevice, c.stream));
+  return smoother_vis(c, l, L.v, 0);
+}
+
+static int coarse_factor(Context& c) {
+  const Tables& t = c.T;
+  Level& G = c.L[t.grids];
+  CoarseArgs a;
+  a.vis = G.v.vis; a.vxm = G.v.vxxmzz; a.vxz = G.v.vxz; a.vx = G.v.vx; a.vz = G.v.vz;
+  a.nxc = G.g.n; a.nzc = G.g.m; a.p = G.g.pitch; a.kl = t.kl; a.ku = t.ku; a.ngc = t.ngc;
+  a.ivis0 = c.P.ivis == 0; a.Vbc = c.P.Vbc;
+  std::memcpy(a.SM1, t.SM1, sizeof a.SM1); std::memcpy(a.SM2, t.SM2, sizeof a.SM2);
+  std::memcpy(a.SM4, t.SM4, sizeof a.SM4); std::memcpy(a.SM5, t.SM5, sizeof a.SM5);
+  std::memcpy(a.SM6, t.SM6, sizeof a.SM6);
+  size_t mbytes = sizeof(double) * (size_t)(2 * t.kl + t.ku + 1) * t.ngc;
+  PT_HIP(hipMemsetAsync(c.Mc, 0, mbytes, c.stream));
+  hipLaunchKernelGGL(k_coarse_assemble, dim3((t.ngc + 127) / 128), dim3(128), 0, c.stream, a, c.Mc, c.RowB);
+  hipLaunchKernelGGL(k_coarse_equilibrate, dim3((t.ngc + 127) / 128), dim3(128), 0, c.stream, c.Mc, c.RowB, t.ngc,
+                     t.kl, t.ku);
+  hipLaunchKernelGGL(k_band_lu, dim3(1), dim3(64), 0, c.stream, c.Mc, c.ipiv, t.ngc, t.kl, t.ku, c.d_info);
+  int32_t info = 0;
+  PT_HIP(hipMemcpyAsync(&info, c.d_info, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipStreamSynchronize(c.stream));
+  if (info != 0) {
+    set_error("coarse-grid band LU: zero pivot at column " + std::to_string(info));
+    return PT_ERR_SINGULAR_COARSE;
+  }
+  return PT_OK;
+}
+
+int compute_rhs1(Context& c) {
+  Level& L = c.L[1];
+  dim3 b16(64, 4);
+  const double* D1 = c.T.D1;
+  hipLaunchKernelGGL(k_rhs1, grid2(c.T.nx - 1, c.T.nz - 1, b16), b16, 0, c.stream, c.Tbuoy, c.Cbuoy, L.rhs, c.T.nx,
+                     c.T.nz, L.g.pitch, c.T.nx + 11, c.T.dr, c.P.RaT, c.P.RaC, D1[1], D1[2], D1[3], D1[4], D1[5]);
+  return PT_OK;
+}
+
+// FP64 copies of the FP32 mirrors (mixed precision): coefficients per solve
+__global__ void k_to_f32(const double* a, float* b, long n) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = (float)a[t];
+}
+__global__ void k_add_f32_to_f64(double* psi, const float* e, int n, int m, int p, double sz) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, k = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > n - 1 || k > m - 1) return;
+  double y = psi[IDX(i, k, p)] + (double)e[IDX(i, k, p)];
+  store_with_ghosts<double>(psi, i, k, n, m, p, y, sz);
+}
+
+static int to_f32(Context& c, const double* a, float* b, size_t n) {
+  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, a, b, (long)n);
+  return PT_OK;
+}
+
+// multigrid: PS/iterative_solver.f90:124-341
+int multigrid(Context& c) {
+  const Tables& t = c.T;
+  const bool mixed = c.P.precision == PT_PREC_MIXED;
+  Level& L1 = c.L[1];
+  size_t b1 = sizeof(double) * L1.nelem;
+  dim3 b16(64, 4);
+  double sz = c.P.Vbc == 0 ? -1.0 : 1.0;
+  PT_HIP(hipMemcpyAsync(L1.err, c.SF, b1, hipMemcpyDeviceToDevice, c.stream));
+  PT_TRY(compute_rhs1(c));
+  double rr;
+  PT_TRY(reduce_to_host(c, L1.rhs, t.nx - 1, t.nz - 1, L1.g.pitch, 4, RED_SUM_ABS, &rr));
+  c.res_ref1 = rr / (double)(t.nx - 2) / (double)(t.nz - 2);
+  for (int l = 1; l <= t.grids; l++) PT_TRY(compute_viscosity(c, l));
+  if (c.first_solve || c.P.ivis > 0) {
+    PT_TRY(coarse_factor(c));
+    c.first_solve = false;
+  }
+  if (mixed) {
+    for (int l = 1; l <= t.grids; l++) {
+      Level& L = c.L[l];
+      PT_TRY(to_f32(c, L.v.vis, L.fvis, L.nelem)); PT_TRY(to_f32(c, L.v.vxxmzz, L.fvxm, L.nelem));
+      PT_TRY(to_f32(c, L.v.vxz, L.fvxz, L.nelem)); PT_TRY(to_f32(c, L.v.vx, L.fvx, L.nelem));
+      PT_TRY(to_f32(c, L.v.vz, L.fvz, L.nelem)); PT_TRY(to_f32(c, L.dts, L.fdts, L.nelem));
+      if (l >= 2) PT_TRY(to_f32(c, L.err, L.ferr, L.nelem));
+    }
+  }
+  int q = 1;
+  double resmag1 = 0.0;
+  bool done = false;
+  for (;;) {
+    for (int p = 1; p <= t.nsteps; p++) {
+      int l = t.multi[p];
+      Level& L = c.L[l];
+      if (t.multi[p - 1] < t.multi[p] && l >= 2) {
+        // compute_RHS(l) = restrict(residual(l-1))
+        Level& F = c.L[l - 1];
+        if (!mixed)
+          hipLaunchKernelGGL((k_restrict<double, double>), grid2(L.g.n - 1, L.g.m - 1, b16), b16, 0, c.stream, F.res,
+                             F.g.pitch, L.rhs, L.g.n, L.g.m, L.g.pitch);
+        else if (l == 2)
+          hipLaunchKernelGGL((k_restrict<double, float>), grid2(L.g.n - 1, L.g.m - 1, b16), b16, 0, c.stream, F.res,
+                             F.g.pitch, L.frhs, L.g.n, L.g.m, L.g.pitch);
+        else
+          hipLaunchKernelGGL((k_restrict<float, float>), grid2(L.g.n - 1, L.g.m - 1, b16), b16, 0, c.stream, F.fB,
+                             F.g.pitch, L.frhs, L.g.n, L.g.m, L.g.pitch);
+      }
+      if (l < t.grids) {
+        if (!mixed) {
+          PT_TRY(iterate_stream_t<double>(c, l, p, false, l == 1 ? &resmag1 : nullptr));
+        } else if (l == 1) {
+          // FP64 defect -> FP32 RKC1 smoothing of the correction -> FP64 update
+          StageArgs<double> a = stage_base<double>(c, 1, false);
+          a.Sx = L.err;
+          dim3 blk(TX, TZ), grd((L.g.n - 1 + TX - 1) / TX, (L.g.m - 1 + TZ * PZ - 1) / (TZ * PZ));
+          hipLaunchKernelGGL(k_residual<double>, grd, blk, 0, c.stream, a, c.dres64, c.red);
+          PT_TRY(to_f32(c, c.dres64, L.frhs, L.nelem));
+          PT_HIP(hipMemsetAsync(L.ferr, 0, sizeof(float) * L.nelem, c.stream));
+          PT_TRY(iterate_stream_t<float>(c, 1, p, true, nullptr));
+          hipLaunchKernelGGL(k_add_f32_to_f64, grid2(L.g.n - 1, L.g.m - 1, b16), b16, 0, c.stream, L.err, L.ferr,
+                             L.g.n, L.g.m, L.g.pitch, sz);
+          hipLaunchKernelGGL(k_residual<double>, grd, blk, 0, c.stream, a, L.res, c.red);
+          int nb = grd.x * grd.y;
+          hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, nb, (int)RED_SUM_ABS, c.red_out);
+          PT_HIP(hipMemcpyAsync(c.h_red, c.red_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+          PT_HIP(hipStreamSynchronize(c.stream));
+          resmag1 = c.h_red[0] / (double)(L.g.n - 2) / (double)(L.g.m - 2);
+        } else {
+          PT_TRY(iterate_stream_t<float>(c, l, p, true, nullptr));
+        }
+        if (l == 1 && resmag1 > 1.0e10 * c.res_ref1) {
+          set_error("residual " + std::to_string(resmag1) + " > 1e10 * res_ref");
+          return PT_ERR_HIGH_RESIDUAL;
+        }
+      } else {
+        if (!mixed)
+          hipLaunchKernelGGL(k_coarse_solve<double>, dim3(1), dim3(64), 0, c.stream, c.Mc, c.ipiv, c.RowB,
+                             (const void*)L.rhs, 0, L.err, t.ngc, t.kl, t.ku, L.g.n, L.g.m, L.g.pitch, sz,
+                             (const int*)nullptr);
+        else
+          hipLaunchKernelGGL(k_coarse_solve<float>, dim3(1), dim3(64), 0, c.stream, c.Mc, c.ipiv, c.RowB,
+                             (const void*)L.frhs, 1, L.ferr, t.ngc, t.kl, t.ku, L.g.n, L.g.m, L.g.pitch, (float)sz,
+                             (const int*)nullptr);
+      }
+      if (l == 1 && (resmag1 / c.res_ref1) <= c.P.tolerance) { done = true; break; }
+      if (t.multi[p + 1] < t.multi[p]) {
+        Level& F = c.L[l - 1];
+        dim3 gp = grid2(F.g.n - 1, F.g.m - 1, b16);
+        if (!mixed)
+          hipLaunchKernelGGL((k_prolong<double, double>), gp, b16, 0, c.stream, L.err, L.g.pitch, F.err, F.g.n, F.g.m,
+                             F.g.pitch, sz);
+        else if (l - 1 == 1)
+          hipLaunchKernelGGL((k_prolong<float, double>), gp, b16, 0, c.stream, L.ferr, L.g.pitch, F.err, F.g.n,
+                             F.g.m, F.g.pitch, sz);
+        else
+          hipLaunchKernelGGL((k_prolong<float, float>), gp, b16, 0, c.stream, L.ferr, L.g.pitch, F.ferr, F.g.n,
+                             F.g.m, F.g.pitch, (float)sz);
+      }
+    }
+    c.cycles++;
+    if (done || (resmag1 / c.res_ref1) <= c.P.tolerance) break;
+    q++;
+    if (q > c.P.max_cycles) {
+      set_error("q > max_cycles = " + std::to_string(c.P.max_cycles));
+      return PT_ERR_MAX_CYCLES;
+    }
+  }
+  PT_HIP(hipMemcpyAsync(c.SF, L1.err, b1, hipMemcpyDeviceToDevice, c.stream));
+  c.last_q = q;
+  c.residual1 = resmag1;
+  c.solves++;
+  return PT_OK;
+}
+
+int compute_velocities(Context& c) {
+  dim3 b16(64, 4);
+  const double* D1 = c.T.D1;
+  int p = c.T.nx + 7;
+  hipLaunchKernelGGL(k_velocities, grid2(c.T.nx + 1, c.T.nz + 1, b16), b16, 0, c.stream, c.SF, c.u, c.w, c.T.nx,
+                     c.T.nz, p, c.T.dr, c.T.ds, D1[1], D1[2], D1[3], D1[4], D1[5]);
+  hipLaunchKernelGGL(k_velocity_bcs, grid2(c.T.nx + 7, c.T.nz + 7, b16), b16, 0, c.stream, c.u, c.w, c.T.nx, c.T.nz,
+                     p, c.P.Vbc);
+  return PT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// allocation
+// ---------------------------------------------------------------------------
+static int dalloc(void** p, size_t bytes) {
+  PT_HIP(hipMalloc(p, bytes));
+  PT_HIP(hipMemset(*p, 0, bytes));
+  return PT_OK;
+}
+#define DALLOC(ptr, bytes) PT_TRY(dalloc((void**)&(ptr), (bytes)))
+
+int solver_alloc(Context& c) {
+  const Tables& t = c.T;
+  for (int l = 1; l <= t.grids; l++) {
+    Level& L = c.L[l];
+    LevelGeom& g = L.g;
+    g.n = t.nxg[l]; g.m = t.nzg[l]; g.pitch = g.n + 7;
+    g.dr = t.drg[l]; g.ds = t.dsg[l]; g.dr2 = t.dr2g[l]; g.ds2 = t.ds2g[l]; g.dr3 = t.dr3g[l]; g.ds3 = t.ds3g[l];
+    g.dr4 = t.dr4g[l]; g.ds4 = t.ds4g[l]; g.drds = t.drdsg[l]; g.drds2 = t.drds2g[l]; g.dr2ds = t.dr2dsg[l];
+    g.dr2ds2 = t.dr2ds2g[l];
+    L.nelem = (size_t)(g.n + 7) * (g.m + 7);
+    size_t b = sizeof(double) * L.nelem, f = sizeof(float) * L.nelem;
+    DALLOC(L.err, b); DALLOC(L.bufB, b); DALLOC(L.bufC, b); DALLOC(L.rhs, b); DALLOC(L.res, b);
+    DALLOC(L.dts, b); DALLOC(L.ratio, b); DALLOC(L.vis_static, b);
+    VisSet* sets[2] = {&L.v, &L.vs};
+    for (VisSet* s : sets) {
+      DALLOC(s->vis, b); DALLOC(s->vx, b); DALLOC(s->vz, b); DALLOC(s->vxx, b); DALLOC(s->vzz, b);
+      DALLOC(s->vxz, b); DALLOC(s->vxxpzz, b); DALLOC(s->vxxmzz, b); DALLOC(s->vsmooth, b);
+    }
+    if (c.P.precision == PT_PREC_MIXED) {
+      DALLOC(L.ferr, f); DALLOC(L.fB, f); DALLOC(L.fC, f); DALLOC(L.fS0, f); DALLOC(L.frhs, f);
+      DALLOC(L.fdts, f); DALLOC(L.fvis, f); DALLOC(L.fvxm, f); DALLOC(L.fvxz, f); DALLOC(L.fvx, f); DALLOC(L.fvz, f);
+    }
+  }
+  if (c.P.precision == PT_PREC_MIXED) DALLOC(c.dres64, sizeof(double) * c.L[1].nelem);
+  size_t mb = sizeof(double) * (size_t)(2 * t.kl + t.ku + 1) * t.ngc;
+  if (t.ngc > 4096) {
+    set_error("coarsest grid has more than 4096 unknowns");
+    return PT_ERR_UNSUPPORTED;
+  }
+  DALLOC(c.Mc, mb); DALLOC(c.RowB, sizeof(double) * t.ngc); DALLOC(c.Bc, sizeof(double) * t.ngc);
+  DALLOC(c.ipiv, sizeof(int32_t) * t.ngc); DALLOC(c.d_info, sizeof(int32_t));
+  c.red_cap = 1 << 20;
+  DALLOC(c.red, sizeof(double) * c.red_cap);
+  DALLOC(c.red_out, sizeof(double) * 16);
+  PT_HIP(hipHostMalloc((void**)&c.h_red, sizeof(double) * 16, hipHostMallocDefault));
+  // stencil constants
+  for (int q = 0; q < 5; q++) { c.st.D1[q] = t.D1[q + 1]; c.st.D2[q] = t.D2[q + 1]; }
+  for (int q = 0; q < 7; q++) { c.st.D3[q] = t.D3[q]; c.st.D4[q] = t.D4[q]; }
+  for (int b = 0; b < 5; b++)
+    for (int q = 0; q < 5; q++) {
+      c.st.Drs[b][q] = t.Drs[b + 1][q + 1]; c.st.Drss[b][q] = t.Drss[b + 1][q + 1];
+      c.st.Drrs[b][q] = t.Drrs[b + 1][q + 1]; c.st.Drrss[b][q] = t.Drrss[b + 1][q + 1];
+    }
+  return PT_OK;
+}
+
+int solver_free(Context& c) {
+  for (int l = 1; l <= c.T.grids; l++) {
+    Level& L = c.L[l];
+    void* ps[] = {L.err, L.bufB, L.bufC, L.rhs, L.res, L.dts, L.ratio, L.vis_static, L.v.vis, L.v.vx, L.v.vz,
+                  L.v.vxx, L.v.vzz, L.v.vxz, L.v.vxxpzz, L.v.vxxmzz, L.v.vsmooth, L.vs.vis, L.vs.vx, L.vs.vz,
+                  L.vs.vxx, L.vs.vzz, L.vs.vxz, L.vs.vxxpzz, L.vs.vxxmzz, L.vs.vsmooth, L.ferr, L.fB, L.fC, L.fS0,
+                  L.frhs, L.fdts, L.fvis, L.fvxm, L.fvxz, L.fvx, L.fvz};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+  }
+  void* ps[] = {c.Mc, c.RowB, c.Bc, c.ipiv, c.d_info, c.red, c.red_out, c.dres64};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  if (c.h_red) (void)hipHostFree(c.h_red);
+  return PT_OK;
+}
+
+}  // namespace pt

@@ -1,0 +1,819 @@
+// pt_tracers.hip -- Lagrangian tracer pipeline on gfx950.
+//
+//   k_bin            nint binning of tracer positions to node dual cells + domain
+//                    check + per-cell counts (PS/composition.f90:1002-1011)
+//   k_scan_*         exclusive scan of the counts (cell offsets)
+//   k_scatter/k_canon counting sort of tracer indices by cell; each cell's
+//                    segment is put in ascending tracer-ID order so the
+//                    projection sums in a fixed order (no float atomics)
+//   k_project        node-owner gather over the 3x3 neighbouring cells:
+//                    sum T*w, C*w, w in canonical order, Tratio/Cnew = ratio
+//                    where sum w > 0, nempty = cell count
+//                    (PS/composition.f90:942-1070, shape_func :896-921)
+//   k_tbc_* / k_cbc_* enforceBCs / extendT / enforceBCs_comp
+//                    (PS/energy.f90:558-630, PS/composition.f90:1279-1292)
+//   k_filter_*       smoother_time / smoother_time_T Richardson passes
+//                    (PS/smoother.f90:582-672) with max-reductions for smoothness
+//   k_conduct        load_conductivity_array (PS/energy.f90:59-86)
+//   k_source         tracer_space_uniform (PS/energy.f90:135-151)
+//   k_itab           compute_derivatives for u, w and the source into one
+//                    interleaved interpolation table (PS/composition.f90:1457-1478)
+//   k_rk_stage<S>    interpolate (x3) + the RK4 stage update S of energy_time
+//                    (PS/composition.f90:1359-1393, PS/energy.f90:258-537)
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+#include "pt_internal.hpp"
+
+namespace pt {
+
+#define I3(i, k, p) ((long)((k) + 3) * (p) + ((i) + 3))
+#define I5(i, k, p) ((long)((k) + 5) * (p) + ((i) + 5))
+
+static dim3 grid2(int ni, int nk, dim3 b) { return dim3((ni + b.x - 1) / b.x, (nk + b.y - 1) / b.y); }
+
+// nint(x) for the reference's NINT: round half away from zero
+__device__ __forceinline__ int nint(double x) { return (int)round(x); }
+
+// ---------------------------------------------------------------------------
+// binning + counting sort
+// ---------------------------------------------------------------------------
+__global__ void k_bin(const double* r, const double* s, int64_t n, int nx, int nz, double dr, double ds,
+                      double aspect, int32_t* key, int32_t* count, int32_t* flags) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double rv = r[j], sv = s[j];
+  if (rv < 0.0 || rv > aspect || sv < 0.0 || sv > 1.0 || rv != rv || sv != sv) {
+    atomicOr(&flags[0], 1);
+    key[j] = 0;
+    return;
+  }
+  int i = nint(rv / dr), k = nint(sv / ds);
+  int c = k * (nx + 1) + i;
+  key[j] = c;
+  atomicAdd(&count[c], 1);
+}
+
+// exclusive scan over n int32 in three phases (block scans of 1024)
+__global__ void k_scan_block(const int32_t* in, int32_t* out, int32_t* bsum, int n) {
+  __shared__ int32_t s[1024];
+  int t = threadIdx.x, g = blockIdx.x * 1024 + t;
+  int v = g < n ? in[g] : 0;
+  s[t] = v;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int x = t >= off ? s[t - off] : 0;
+    __syncthreads();
+    s[t] += x;
+    __syncthreads();
+  }
+  if (g < n) out[g] = s[t] - v;
+  if (t == 1023) bsum[blockIdx.x] = s[1023];
+}
+__global__ void k_scan_sums(int32_t* bsum, int nb) {
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int b = 0; b < nb; b++) { int v = bsum[b]; bsum[b] = acc; acc += v; }
+  }
+}
+__global__ void k_scan_add(int32_t* out, const int32_t* bsum, int n) {
+  int g = blockIdx.x * 1024 + threadIdx.x;
+  if (g < n) out[g] += bsum[blockIdx.x];
+}
+
+__global__ void k_scatter(const int32_t* key, int64_t n, const int32_t* start, int32_t* fill, int32_t* perm) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int c = key[j];
+  int pos = start[c] + atomicAdd(&fill[c], 1);
+  perm[pos] = (int32_t)j;
+}
+
+// ascending tracer index inside each cell segment (insertion sort; segments
+// hold ~tpc entries)
+__global__ void k_canon(const int32_t* start, const int32_t* count, int32_t* perm, int ncell) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int b = start[c], m = count[c];
+  for (int a = 1; a < m; a++) {
+    int v = perm[b + a], q = a - 1;
+    while (q >= 0 && perm[b + q] > v) { perm[b + q + 1] = perm[b + q]; q--; }
+    perm[b + q + 1] = v;
+  }
+}
+
+// node-owner projection (tracers_to_corners_iC_continuous main body)
+__global__ void k_project(const double* r, const double* s, const double* Tt, const double* Ct, const int32_t* start,
+                          const int32_t* count, const int32_t* perm, double* Tratio, double* Cnew, int32_t* nempty,
+                          int nx, int nz, double dr, double ds) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > nx || k > nz) return;
+  double ri = (double)i * dr, sk = (double)k * ds;
+  double txs = 0.0, cxs = 0.0, ssum = 0.0;
+  for (int a = -1; a <= 1; a++) {
+    int ic = i + a;
+    if (ic < 0 || ic > nx) continue;
+    for (int b = -1; b <= 1; b++) {
+      int kc = k + b;
+      if (kc < 0 || kc > nz) continue;
+      int c = kc * (nx + 1) + ic;
+      int beg = start[c], m = count[c];
+      for (int q = 0; q < m; q++) {
+        int j = perm[beg + q];
+        double rv = r[j], sv = s[j];
+        double wa = 1.0 - fabs(rv - ri) / dr;
+        wa = wa < 0.0 ? 0.0 : wa;
+        double wb = 1.0 - fabs(sv - sk) / ds;
+        wb = wb < 0.0 ? 0.0 : wb;
+        double sf = wa * wb;
+        txs = txs + Tt[j] * sf;
+        cxs = cxs + Ct[j] * sf;
+        ssum = ssum + sf;
+      }
+    }
+  }
+  nempty[k * (nx + 1) + i] = count[k * (nx + 1) + i];
+  if (ssum > 0.0) {
+    Tratio[I3(i, k, nx + 7)] = txs / ssum;
+    Cnew[I5(i, k, nx + 11)] = cxs / ssum;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// boundary conditions
+// ---------------------------------------------------------------------------
+// side ghosts (symmetric), rows 0..nz, for a field with halo h and pitch p
+__global__ void k_sym_side(double* f, int nx, int nz, int h, int p) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (nz + 1) * h) return;
+  int k = t / h, i = 1 + t % h;
+  long row = (long)(k + h) * p + h;
+  f[row - i] = f[row + i];
+  f[row + nx + i] = f[row + nx - i];
+}
+// enforceBCs_comp top/bottom: symmetric rows for all columns -h..nx+h
+__global__ void k_sym_tb(double* f, int nx, int nz, int h, int p) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (nx + 1 + 2 * h) * h) return;
+  int i = -h + t / h, k = 1 + t % h;
+  f[(long)(h - k) * p + (i + h)] = f[(long)(h + k) * p + (i + h)];
+  f[(long)(h + nz + k) * p + (i + h)] = f[(long)(h + nz - k) * p + (i + h)];
+}
+// T wall rows: ghost = 2*Twall + (2/D2(0))*heating*ds2 - mirror (isothermal) or
+// mirror (insulating bottom), then the wall rows themselves (enforceBCs/extendT)
+__global__ void k_t_tb(double* f, const double* Cvis, int nx, int nz, int h, int p, int p5, int Tbot, double Tb,
+                       double Tt, double two_over_d20, double ds2, double H, double Htype) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (nx + 1 + 2 * h) * h) return;
+  int i = -h + t / h, k = 1 + t % h;
+  long ci = i + 5;
+  double cb = Cvis[(long)(0 + 5) * p5 + ci], ct = Cvis[(long)(nz + 5) * p5 + ci];
+  double hb = Htype * cb + H * (1.0 - (0.0 + cb));
+  double ht = Htype * ct + H * (1.0 - (0.0 + ct));
+  double in_b = f[(long)(h + k) * p + (i + h)];
+  double in_t = f[(long)(h + nz - k) * p + (i + h)];
+  if (Tbot == 0) f[(long)(h - k) * p + (i + h)] = 2.0 * Tb + two_over_d20 * hb * ds2 - in_b;
+  else f[(long)(h - k) * p + (i + h)] = in_b;
+  f[(long)(h + nz + k) * p + (i + h)] = 2.0 * Tt + two_over_d20 * ht * ds2 - in_t;
+}
+__global__ void k_t_walls(double* f, int nx, int nz, int h, int p, int Tbot, double Tb, double Tt) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nx + 1 + 2 * h) return;
+  int i = -h + t;
+  if (Tbot == 0) f[(long)h * p + (i + h)] = Tb;
+  f[(long)(h + nz) * p + (i + h)] = Tt;
+}
+__global__ void k_copy_nodes(const double* a, int pa, int ha, double* b, int pb, int hb, int nx, int nz) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > nx || k > nz) return;
+  b[(long)(k + hb) * pb + (i + hb)] = a[(long)(k + ha) * pa + (i + ha)];
+}
+
+static void launch1(Context& c, void (*kern)(double*, int, int, int, int), double* f, int nthreads, int nx, int nz,
+                    int h, int p) {
+  hipLaunchKernelGGL(kern, dim3((nthreads + 255) / 256), dim3(256), 0, c.stream, f, nx, nz, h, p);
+}
+
+// enforceBCs(T) on a halo-3 field
+static void t_bcs(Context& c, double* f) {
+  int nx = c.T.nx, nz = c.T.nz;
+  launch1(c, k_sym_side, f, (nz + 1) * 3, nx, nz, 3, nx + 7);
+  int n = (nx + 7) * 3;
+  hipLaunchKernelGGL(k_t_tb, dim3((n + 255) / 256), dim3(256), 0, c.stream, f, c.Cvis, nx, nz, 3, nx + 7, nx + 11,
+                     c.P.Tbot, c.Tbot_iso, c.Ttop_iso, 2.0 / c.T.D2[3], c.T.ds2, c.P.H, c.P.Htype);
+  hipLaunchKernelGGL(k_t_walls, dim3((nx + 7 + 255) / 256), dim3(256), 0, c.stream, f, nx, nz, 3, nx + 7, c.P.Tbot,
+                     c.Tbot_iso, c.Ttop_iso);
+}
+// enforceBCs_comp on a halo-5 field
+static void c_bcs(Context& c, double* f) {
+  int nx = c.T.nx, nz = c.T.nz;
+  launch1(c, k_sym_side, f, (nz + 1) * 5, nx, nz, 5, nx + 11);
+  launch1(c, k_sym_tb, f, (nx + 11) * 5, nx, nz, 5, nx + 11);
+}
+// extendT: Textend(0:nx,0:nz) = Tratio, halo-5 wall ghosts
+static void extend_t(Context& c) {
+  int nx = c.T.nx, nz = c.T.nz;
+  dim3 b16(64, 4);
+  hipLaunchKernelGGL(k_copy_nodes, grid2(nx + 1, nz + 1, b16), b16, 0, c.stream, c.Tratio, nx + 7, 3, c.Textend,
+                     nx + 11, 5, nx, nz);
+  launch1(c, k_sym_side, c.Textend, (nz + 1) * 5, nx, nz, 5, nx + 11);
+  int n = (nx + 11) * 5;
+  hipLaunchKernelGGL(k_t_tb, dim3((n + 255) / 256), dim3(256), 0, c.stream, c.Textend, c.Cvis, nx, nz, 5, nx + 11,
+                     nx + 11, c.P.Tbot, c.Tbot_iso, c.Ttop_iso, 2.0 / c.T.D2[3], c.T.ds2, c.P.H, c.P.Htype);
+  hipLaunchKernelGGL(k_t_walls, dim3((nx + 11 + 255) / 256), dim3(256), 0, c.stream, c.Textend, nx, nz, 5, nx + 11,
+                     c.P.Tbot, c.Tbot_iso, c.Ttop_iso);
+}
+
+// ---------------------------------------------------------------------------
+// filters
+// ---------------------------------------------------------------------------
+// smoothness (PS/smoother.f90:75-95): block maxima over i=0..nx-1, k=1..nz-1
+__global__ void k_smoothness(const double* f, int nx, int nz, int h, int p, double* partial) {
+  __shared__ double red[256];
+  int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x, k = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  double d = 0.0;
+  if (i <= nx - 1 && k <= nz - 1) {
+    auto F = [&](int a, int b) { return f[(long)(b + h) * p + (a + h)]; };
+    double t = F(i, k);
+    double a1 = fabs(F(i, k - 1) - t), a2 = fabs(F(i + 1, k - 1) - t), a3 = fabs(F(i + 1, k) - t),
+           a4 = fabs(F(i, k + 1) - t), a5 = fabs(F(i + 1, k + 1) - t);
+    d = fmax(fmax(fmax(fmax(a1, a2), a3), a4), a5);
+  }
+  red[tid] = d;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) red[tid] = fmax(red[tid], red[tid + st]);
+    __syncthreads();
+  }
+  if (tid == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// one Richardson pass: out = T0 + (c*dts)*lap(T0) (clamped to [0,1] for C)
+__global__ void k_filter(const double* T0, double* out, int nx, int nz, int h, int p, double cdts, int clamp01,
+                         double dr, double ds, double dr2, double ds2, double d10, double d11, double d12, double d13,
+                         double d14, double d20, double d21, double d22, double d23, double d24) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > nx || k > nz) return;
+  const double D1[5] = {d10, d11, d12, d13, d14}, D2[5] = {d20, d21, d22, d23, d24};
+  auto F = [&](int a, int b) { return T0[(long)(b + h) * p + (a + h)]; };
+  double sr = 0.0, ss = 0.0, srr = 0.0, sss = 0.0;
+  for (int q = -2; q <= 2; q++) sr = sr + D1[q + 2] * F(i + q, k);
+  for (int q = -2; q <= 2; q++) ss = ss + D1[q + 2] * F(i, k + q);
+  for (int q = -2; q <= 2; q++) srr = srr + D2[q + 2] * F(i + q, k);
+  for (int q = -2; q <= 2; q++) sss = sss + D2[q + 2] * F(i, k + q);
+  double Tr = sr / dr, Ts = ss / ds, Trr = srr / dr2, Tss = sss / ds2;
+  double Tx = Tr / 1.0, Tz = Ts / 1.0;
+  double Txx = (Trr - Tx * 0.0) / 1.0, Tzz = (Tss - Tz * 0.0) / 1.0;
+  double temp = Txx + Tzz;
+  double v = F(i, k) + cdts * temp;
+  if (clamp01) {
+    if (v < 0.0) v = 0.0;
+    if (v > 1.0) v = 1.0;
+  }
+  out[(long)(k + h) * p + (i + h)] = v;
+}
+
+static int smoothness_host(Context& c, const double* f, int h, int p, double* out) {
+  dim3 b16(64, 4);
+  dim3 g = grid2(c.T.nx, c.T.nz - 1, b16);
+  hipLaunchKernelGGL(k_smoothness, g, b16, 0, c.stream, f, c.T.nx, c.T.nz, h, p, c.red);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, (int)(g.x * g.y), (int)RED_MAX, c.red_out);
+  PT_HIP(hipMemcpyAsync(c.h_red, c.red_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipStreamSynchronize(c.stream));
+  *out = c.h_red[0];
+  return PT_OK;
+}
+
+static void filter_pass(Context& c, const double* T0, double* out, int h, double cdts, int clamp01) {
+  dim3 b16(64, 4);
+  const double* D1 = c.T.D1;
+  const double* D2 = c.T.D2;
+  hipLaunchKernelGGL(k_filter, grid2(c.T.nx + 1, c.T.nz + 1, b16), b16, 0, c.stream, T0, out, c.T.nx, c.T.nz, h,
+                     c.T.nx + 1 + 2 * h, cdts, clamp01, c.T.dr, c.T.ds, c.T.dr2, c.T.ds2, D1[1], D1[2], D1[3], D1[4],
+                     D1[5], D2[1], D2[2], D2[3], D2[4], D2[5]);
+}
+
+// smoother_time (C): PS/smoother.f90:582-636
+static int smoother_time(Context& c) {
+  size_t b5 = sizeof(double) * (size_t)(c.T.nx + 11) * (c.T.nz + 11);
+  PT_HIP(hipMemcpyAsync(c.Cvis, c.Cnew, b5, hipMemcpyDeviceToDevice, c.stream));
+  PT_HIP(hipMemcpyAsync(c.Cbuoy, c.Cnew, b5, hipMemcpyDeviceToDevice, c.stream));
+  int Niter = 10, nn = 0;
+  double lb = -5.0, la = -1.0;
+  bool trigger = false;
+  for (;;) {
+    if (nn > 100000) {
+      set_error("smoother_time: C filter did not reach delta_C");
+      return PT_ERR_INVALID;
+    }
+    for (int n = 1; n <= Niter; n++) {
+      // the reference passes the halo-5 C array to smoothness, whose dummy has
+      // the halo-3 shape (PS/smoother.f90:79, :602): sequence association
+      // makes it read the C storage with the halo-3 row length
+      double diff;
+      PT_TRY(smoothness_host(c, c.Cvis, 3, c.T.nx + 7, &diff));
+      if (diff < c.T.delta_C) { trigger = true; break; }
+      double cc;
+      if (nn == 0) {
+        int ntemp = Niter - n + 1;
+        cc = 2.0 / (-lb - la + (lb - la) * std::cos((double)(2 * ntemp - 1) * 3.1415926535897932 / (double)(2 * Niter)));
+      } else {
+        cc = 1.0;
+      }
+      // T0 = Cvis (copy), Cvis = filter(T0); c*dts*temp = (c*dts)*temp
+      PT_HIP(hipMemcpyAsync(c.TxS, c.Cvis, b5, hipMemcpyDeviceToDevice, c.stream));
+      filter_pass(c, c.TxS, c.Cvis, 5, cc * (c.T.dts * c.P.courantC), 1);
+      c_bcs(c, c.Cvis);
+    }
+    if (trigger) break;
+    Niter = 1;
+    nn++;
+  }
+  c.n_comp = nn;
+  return PT_OK;
+}
+
+// smoother_time_T: PS/smoother.f90:638-672 (Tin = Tratio)
+static int smoother_time_T(Context& c) {
+  size_t b3 = sizeof(double) * (size_t)(c.T.nx + 7) * (c.T.nz + 7);
+  PT_HIP(hipMemcpyAsync(c.Tvis, c.Tratio, b3, hipMemcpyDeviceToDevice, c.stream));
+  PT_HIP(hipMemcpyAsync(c.Tbuoy, c.Tratio, b3, hipMemcpyDeviceToDevice, c.stream));
+  if (c.P.RaT == 0.0 || c.P.ivis == 0) return PT_OK;
+  int n = 0;
+  for (;;) {
+    if (n > 100000) {
+      set_error("smoother_time_T: T filter did not reach delta_T");
+      return PT_ERR_INVALID;
+    }
+    double diff;
+    PT_TRY(smoothness_host(c, c.Tvis, 3, c.T.nx + 7, &diff));
+    if (diff < c.T.delta_T) { c.n_temp = n; break; }
+    PT_HIP(hipMemcpyAsync(c.CxS, c.Tvis, b3, hipMemcpyDeviceToDevice, c.stream));
+    filter_pass(c, c.CxS, c.Tvis, 3, c.T.dts * c.P.courantT, 0);
+    t_bcs(c, c.Tvis);
+    n++;
+  }
+  return PT_OK;
+}
+
+// load_conductivity_array: PS/energy.f90:59-86
+__global__ void k_conduct_nodes(const double* Cvis, double* cond, int nx, int nz, double cf) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > nx || k > nz) return;
+  cond[I5(i, k, nx + 11)] = 1.0 + Cvis[I5(i, k, nx + 11)] * (cf - 1.0);
+}
+__global__ void k_conduct_grad(const double* cond, double* cr, double* cs, int nx, int nz, double dr, double ds,
+                               double d0, double d1, double d2, double d3, double d4) {
+  int i = -3 + (int)(blockIdx.x * blockDim.x + threadIdx.x), k = -3 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i > nx + 3 || k > nz + 3) return;
+  const double D[5] = {d0, d1, d2, d3, d4};
+  double sr = 0.0, ss = 0.0;
+  for (int q = -2; q <= 2; q++) sr = sr + D[q + 2] * cond[I5(i + q, k, nx + 11)];
+  for (int q = -2; q <= 2; q++) ss = ss + D[q + 2] * cond[I5(i, k + q, nx + 11)];
+  cr[I3(i, k, nx + 7)] = sr / dr;
+  cs[I3(i, k, nx + 7)] = ss / ds;
+}
+static void load_conductivity(Context& c) {
+  int nx = c.T.nx, nz = c.T.nz;
+  dim3 b16(64, 4);
+  hipLaunchKernelGGL(k_conduct_nodes, grid2(nx + 1, nz + 1, b16), b16, 0, c.stream, c.Cvis, c.cond, nx, nz,
+                     c.P.conduct_factor);
+  launch1(c, k_sym_side, c.cond, (nz + 1) * 5, nx, nz, 5, nx + 11);
+  launch1(c, k_sym_tb, c.cond, (nx + 11) * 5, nx, nz, 5, nx + 11);
+  const double* D1 = c.T.D1;
+  hipLaunchKernelGGL(k_conduct_grad, grid2(nx + 7, nz + 7, b16), b16, 0, c.stream, c.cond, c.cond_r, c.cond_s, nx, nz,
+                     c.T.dr, c.T.ds, D1[1], D1[2], D1[3], D1[4], D1[5]);
+}
+
+// tracers_to_corners: PS/composition.f90:923-1087
+int tracers_to_corners(Context& c, const double* r, const double* s, const double* Tt) {
+  const int nx = c.T.nx, nz = c.T.nz, ncell = (nx + 1) * (nz + 1);
+  const int64_t n = c.T.ntr;
+  size_t b3 = sizeof(double) * (size_t)(nx + 7) * (nz + 7);
+  if (c.timing) timer_begin(c, PT_K_PROJECT);
+  PT_HIP(hipMemcpyAsync(c.Tratio, c.T0, b3, hipMemcpyDeviceToDevice, c.stream));
+  PT_HIP(hipMemsetAsync(c.count, 0, sizeof(int32_t) * ncell, c.stream));
+  PT_HIP(hipMemsetAsync(c.fill, 0, sizeof(int32_t) * ncell, c.stream));
+  PT_HIP(hipMemsetAsync(c.d_flags, 0, sizeof(int32_t) * 4, c.stream));
+  unsigned nbt = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_bin, dim3(nbt), dim3(256), 0, c.stream, r, s, n, nx, nz, c.T.dr, c.T.ds, c.T.aspect, c.cellkey,
+                     c.count, c.d_flags);
+  int nb = (ncell + 1023) / 1024;
+  hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(1024), 0, c.stream, c.count, c.start, c.scan_tmp, ncell);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(64), 0, c.stream, c.scan_tmp, nb);
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, c.stream, c.start, c.scan_tmp, ncell);
+  hipLaunchKernelGGL(k_scatter, dim3(nbt), dim3(256), 0, c.stream, c.cellkey, n, c.start, c.fill, c.perm);
+  hipLaunchKernelGGL(k_canon, dim3((ncell + 255) / 256), dim3(256), 0, c.stream, c.start, c.count, c.perm, ncell);
+  dim3 b16(64, 4);
+  hipLaunchKernelGGL(k_project, grid2(nx + 1, nz + 1, b16), b16, 0, c.stream, r, s, Tt, c.Ct, c.start, c.count, c.perm,
+                     c.Tratio, c.Cnew, c.nempty, nx, nz, c.T.dr, c.T.ds);
+  if (c.timing) timer_end(c, PT_K_PROJECT);
+  int32_t flag = 0;
+  PT_HIP(hipMemcpyAsync(&flag, c.d_flags, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipStreamSynchronize(c.stream));
+  if (flag) {
+    set_error("tracer position outside [0,aspect]x[0,1]");
+    return PT_ERR_TRACER_OUTSIDE;
+  }
+  c_bcs(c, c.Cnew);
+  PT_TRY(smoother_time(c));
+  t_bcs(c, c.Tratio);
+  PT_TRY(smoother_time_T(c));
+  load_conductivity(c);
+  return PT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// source term, derivative tables, RK stages
+// ---------------------------------------------------------------------------
+__global__ void k_source(const double* Tx_, const double* cond, const double* cr, const double* cs, const double* Cvis,
+                         double* src, int nx, int nz, double dr, double ds, double dr2, double ds2, double H,
+                         double Htype, double d10, double d11, double d12, double d13, double d14, double d20,
+                         double d21, double d22, double d23, double d24) {
+  int i = -3 + (int)(blockIdx.x * blockDim.x + threadIdx.x), k = -3 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i > nx + 3 || k > nz + 3) return;
+  const double D1[5] = {d10, d11, d12, d13, d14}, D2[5] = {d20, d21, d22, d23, d24};
+  const int p5 = nx + 11;
+  double sx = 0, sz = 0, sxx = 0, szz = 0;
+  for (int q = -2; q <= 2; q++) sx = sx + D1[q + 2] * Tx_[I5(i + q, k, p5)];
+  for (int q = -2; q <= 2; q++) sz = sz + D1[q + 2] * Tx_[I5(i, k + q, p5)];
+  for (int q = -2; q <= 2; q++) sxx = sxx + D2[q + 2] * Tx_[I5(i + q, k, p5)];
+  for (int q = -2; q <= 2; q++) szz = szz + D2[q + 2] * Tx_[I5(i, k + q, p5)];
+  double Tx = sx / dr, Tz = sz / ds, Txx = sxx / dr2, Tzz = szz / ds2;
+  double kx = cr[I3(i, k, nx + 7)], kz = cs[I3(i, k, nx + 7)];
+  double cv = Cvis[I5(i, k, p5)];
+  double h = Htype * cv + H * (1.0 - (0.0 + cv));
+  src[I3(i, k, nx + 7)] = cond[I5(i, k, p5)] * (Txx + Tzz) + (kx * Tx + kz * Tz) + h;
+}
+
+// interpolation table: tab[((i*(nz+7)) + (k+3))*12 + f*4 + d], f in {u,w,src},
+// d = {value, DERr(1), DERr(2), DERr(3)}; i in 0..nx, k in -3..nz+3
+__global__ void k_itab(const double* u, const double* w, const double* src, double* tab, int nx, int nz,
+                       const double* Dm, double p1, double p2, double p3) {
+  int k = -3 + (int)(blockIdx.x * blockDim.x + threadIdx.x), i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > nx || k > nz + 3) return;
+  const double* fs[3] = {u, w, src};
+  const double pw[3] = {p1, p2, p3};
+  long base = ((long)i * (nz + 7) + (k + 3)) * 12;
+  for (int f = 0; f < 3; f++) {
+    const double* F = fs[f];
+    tab[base + f * 4] = F[I3(i, k, nx + 7)];
+    for (int jj = 1; jj <= 3; jj++) {
+      double sacc = 0.0;
+      for (int a = -3; a <= 3; a++) sacc = sacc + Dm[(jj - 1) * 7 + a + 3] * F[I3(i + a, k, nx + 7)];
+      tab[base + f * 4 + jj] = sacc / pw[jj - 1];
+    }
+  }
+}
+
+// hr**3 as the correctly rounded cube (the reference evaluates hr**real(3,8)
+// with libm pow); FMA-based double-double product, rounded once
+__device__ __forceinline__ double cube_cr(double x) {
+  double p = x * x;
+  double e = __fma_rn(x, x, -p);
+  double t = p * x;
+  double te = __fma_rn(p, x, -t);
+  return t + (te + e * x);
+}
+
+struct InterpConst {
+  int nx, nz;
+  double dr, ds, f1, f2, f3;     // factorial(1..3)
+  double dsp1, dsp2, dsp3;       // ds_power(1..3)
+  double Dm[3][7];               // D(1..3, -3..3)
+};
+
+// interpolate(r,s,DERr,f) for the three fields sharing one cell lookup
+__device__ __forceinline__ void interp3(const double* __restrict__ tab, int nz, const InterpConst& C, double r,
+                                        double s, double out[3]) {
+  int i = (int)round(r / C.dr), k = (int)round(s / C.ds);
+  // positions reaching this point passed the projection's domain check;
+  // the clamp only guards the table reads
+  i = i < 0 ? 0 : (i > C.nx ? C.nx : i);
+  k = k < 0 ? 0 : (k > C.nz ? C.nz : k);
+  double hr = r - (double)i * C.dr;
+  double a1 = hr / C.f1, a2 = (hr * hr) / C.f2, a3 = cube_cr(hr) / C.f3;
+  double hs = s - (double)k * C.ds;
+  double b1 = hs / C.f1, b2 = (hs * hs) / C.f2, b3 = cube_cr(hs) / C.f3;
+  const double* row = tab + ((long)i * (nz + 7) + (k - 3 + 3)) * 12;
+  double fR[3][7];
+#pragma unroll
+  for (int jj = 0; jj < 7; jj++) {
+    const double* e = row + jj * 12;
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+      double v = e[f * 4];
+      v = v + e[f * 4 + 1] * a1;
+      v = v + e[f * 4 + 2] * a2;
+      v = v + e[f * 4 + 3] * a3;
+      fR[f][jj] = v;
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < 3; f++) {
+    double d1 = 0.0, d2 = 0.0, d3 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 7; a++) {
+      d1 = d1 + C.Dm[0][a] * fR[f][a];
+      d2 = d2 + C.Dm[1][a] * fR[f][a];
+      d3 = d3 + C.Dm[2][a] * fR[f][a];
+    }
+    d1 = d1 / C.dsp1; d2 = d2 / C.dsp2; d3 = d3 / C.dsp3;
+    double fi = fR[f][3];
+    fi = fi + d1 * b1;
+    fi = fi + d2 * b2;
+    fi = fi + d3 * b3;
+    out[f] = fi;
+  }
+}
+
+struct RkArgs {
+  double *r, *s, *T;                 // r, s, Ttr (accumulators after stage 1)
+  double *r0, *s0, *T0;              // step-start copies
+  double *r1, *s1, *T1, *r2, *s2, *T2;
+  const double* tab;
+  int64_t n;
+  int nz;
+  double dtstage;                    // stage-specific step
+  InterpConst C;
+};
+
+// stage 1: PS/energy.f90:258-301
+__global__ void k_rk1(RkArgs a) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.n) return;
+  double rv = a.r[j], sv = a.s[j], Tv = a.T[j];
+  a.r0[j] = rv; a.s0[j] = sv; a.T0[j] = Tv;
+  double o[3];
+  interp3(a.tab, a.nz, a.C, rv, sv, o);
+  a.r[j] = o[0]; a.s[j] = o[1]; a.T[j] = o[2];
+  a.r1[j] = rv + a.dtstage * o[0];
+  a.s1[j] = sv + a.dtstage * o[1];
+  a.T1[j] = Tv + a.dtstage * o[2];
+}
+// stage 2: PS/energy.f90:350-402
+__global__ void k_rk2(RkArgs a) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.n) return;
+  double o[3];
+  interp3(a.tab, a.nz, a.C, a.r1[j], a.s1[j], o);
+  a.r[j] = a.r[j] + 2.0 * o[0];
+  a.s[j] = a.s[j] + 2.0 * o[1];
+  a.T[j] = a.T[j] + 2.0 * o[2];
+  a.r2[j] = a.r0[j] + a.dtstage * o[0];
+  a.s2[j] = a.s0[j] + a.dtstage * o[1];
+  a.T2[j] = a.T0[j] + a.dtstage * o[2];
+}
+// stage 3: PS/energy.f90:450-473
+__global__ void k_rk3(RkArgs a) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.n) return;
+  double o[3];
+  interp3(a.tab, a.nz, a.C, a.r2[j], a.s2[j], o);
+  a.r[j] = a.r[j] + 2.0 * o[0];
+  a.r1[j] = a.r0[j] + a.dtstage * o[0];
+  a.s[j] = a.s[j] + 2.0 * o[1];
+  a.s1[j] = a.s0[j] + a.dtstage * o[1];
+  a.T[j] = a.T[j] + 2.0 * o[2];
+  a.T1[j] = a.T0[j] + a.dtstage * o[2];
+}
+// stage 4: PS/energy.f90:517-537 (dtstage = dt/6)
+__global__ void k_rk4(RkArgs a) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.n) return;
+  double o[3];
+  interp3(a.tab, a.nz, a.C, a.r1[j], a.s1[j], o);
+  a.r[j] = a.r0[j] + a.dtstage * (a.r[j] + o[0]);
+  a.s[j] = a.s0[j] + a.dtstage * (a.s[j] + o[1]);
+  a.T[j] = a.T0[j] + a.dtstage * (a.T[j] + o[2]);
+}
+
+static int source_and_tables(Context& c, bool velocity_too) {
+  int nx = c.T.nx, nz = c.T.nz;
+  dim3 b16(64, 4);
+  (void)velocity_too;
+  extend_t(c);
+  const double *D1 = c.T.D1, *D2 = c.T.D2;
+  hipLaunchKernelGGL(k_source, grid2(nx + 7, nz + 7, b16), b16, 0, c.stream, c.Textend, c.cond, c.cond_r, c.cond_s,
+                     c.Cvis, c.src, nx, nz, c.T.dr, c.T.ds, c.T.dr2, c.T.ds2, c.P.H, c.P.Htype, D1[1], D1[2], D1[3],
+                     D1[4], D1[5], D2[1], D2[2], D2[3], D2[4], D2[5]);
+  return PT_OK;
+}
+
+static int build_itab(Context& c) {
+  int nx = c.T.nx, nz = c.T.nz;
+  dim3 b(64, 4);
+  hipLaunchKernelGGL(k_itab, grid2(nz + 7, nx + 1, b), b, 0, c.stream, c.u, c.w, c.src, c.itab, nx, nz,
+                     (const double*)c.dtab, c.T.dr_power[1], c.T.dr_power[2], c.T.dr_power[3]);
+  return PT_OK;
+}
+
+static RkArgs rk_args(Context& c) {
+  RkArgs a;
+  a.r = c.r; a.s = c.s; a.T = c.Tt; a.r0 = c.r0; a.s0 = c.s0; a.T0 = c.T0t;
+  a.r1 = c.r1; a.s1 = c.s1; a.T1 = c.T1t; a.r2 = c.r2; a.s2 = c.s2; a.T2 = c.T2t;
+  a.tab = c.itab; a.n = c.T.ntr; a.nz = c.T.nz;
+  a.C.nx = c.T.nx; a.C.nz = c.T.nz;
+  a.C.dr = c.T.dr; a.C.ds = c.T.ds; a.C.f1 = c.T.factorial[1]; a.C.f2 = c.T.factorial[2]; a.C.f3 = c.T.factorial[3];
+  a.C.dsp1 = c.T.ds_power[1]; a.C.dsp2 = c.T.ds_power[2]; a.C.dsp3 = c.T.ds_power[3];
+  for (int m = 0; m < 3; m++)
+    for (int q = 0; q < 7; q++) a.C.Dm[m][q] = c.T.D[m + 1][q];
+  return a;
+}
+
+// stability_RK4 with tracers (CFL=0): PS/energy.f90:756-810
+__global__ void k_eig_max(const double* cond, const double* cr, const double* cs, int nx, int nz, double Tr_max,
+                          double Ts_max, double Trr_max, double Tss_max, double* partial) {
+  __shared__ double ra[256], rd[256];
+  int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  double eA = 0.0, eD = 0.0;
+  if (i <= nx && k <= nz) {
+    double cd = cond[I5(i, k, nx + 11)];
+    eA = fabs(-(cd * 0.0 / 1.0) + cr[I3(i, k, nx + 7)] / 1.0) * Tr_max +
+         fabs(-(cd * 0.0 / 1.0) + cs[I3(i, k, nx + 7)] / 1.0) * Ts_max;
+    eD = cd * (fabs(Trr_max / 1.0) + fabs(Tss_max / 1.0));
+  }
+  ra[tid] = eA; rd[tid] = eD;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) { ra[tid] = fmax(ra[tid], ra[tid + st]); rd[tid] = fmax(rd[tid], rd[tid + st]); }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    int b = blockIdx.y * gridDim.x + blockIdx.x;
+    partial[2 * b] = ra[0];
+    partial[2 * b + 1] = rd[0];
+  }
+}
+__global__ void k_eig_final(const double* partial, int nb, double* out) {
+  if (threadIdx.x == 0) {
+    double a = 0.0, d = 0.0;
+    for (int b = 0; b < nb; b++) { a = fmax(a, partial[2 * b]); d = fmax(d, partial[2 * b + 1]); }
+    out[0] = a; out[1] = d;
+  }
+}
+static int stability_rk4(Context& c) {
+  const Tables& t = c.T;
+  double Ts_max = t.Tr_base / t.ds, Tss_max = -t.Trr_base / t.ds2, Tr_max = t.Tr_base / t.dr,
+         Trr_max = -t.Trr_base / t.dr2;
+  dim3 b16(64, 4);
+  dim3 g = grid2(t.nx + 1, t.nz + 1, b16);
+  hipLaunchKernelGGL(k_eig_max, g, b16, 0, c.stream, c.cond, c.cond_r, c.cond_s, t.nx, t.nz, Tr_max, Ts_max, Trr_max,
+                     Tss_max, c.red);
+  hipLaunchKernelGGL(k_eig_final, dim3(1), dim3(64), 0, c.stream, c.red, (int)(g.x * g.y), c.red_out);
+  PT_HIP(hipMemcpyAsync(c.h_red, c.red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipStreamSynchronize(c.stream));
+  double eA = c.h_red[0], eD = c.h_red[1];
+  double dtA = 2.8 / eA, dtD = 2.8 / eD, dt;
+  if (eA == 0.0) dt = dtD;
+  else dt = 1.0 / (1.0 / dtA + 1.0 / dtD);
+  c.dt = c.P.courant * dt;
+  return PT_OK;
+}
+
+static int update_velocities(Context& c) {
+  PT_TRY(multigrid(c));
+  return compute_velocities(c);
+}
+
+static void rk_launch(Context& c, void (*kern)(RkArgs), RkArgs a) {
+  unsigned nb = (unsigned)((a.n + 255) / 256);
+  if (c.timing) timer_begin(c, PT_K_TRACER_STAGE);
+  hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, c.stream, a);
+  if (c.timing) timer_end(c, PT_K_TRACER_STAGE);
+}
+
+// energy_time: PS/energy.f90:181-556 (tracer=1, comp=1, torder=4, RK_update=1)
+int energy_time(Context& c) {
+  PT_TRY(stability_rk4(c));
+  const double dt = c.dt;
+  const int nx = c.T.nx, nz = c.T.nz;
+  size_t b3 = sizeof(double) * (size_t)(nx + 7) * (nz + 7), b5 = sizeof(double) * (size_t)(nx + 11) * (nz + 11);
+  PT_HIP(hipMemcpyAsync(c.C0, c.Cnew, b5, hipMemcpyDeviceToDevice, c.stream));
+  PT_HIP(hipMemcpyAsync(c.T0, c.Tratio, b3, hipMemcpyDeviceToDevice, c.stream));
+  RkArgs a = rk_args(c);
+  // stage 1
+  PT_TRY(source_and_tables(c, true));
+  PT_TRY(build_itab(c));
+  a.dtstage = 0.5 * dt;
+  rk_launch(c, k_rk1, a);
+  // stage 2
+  PT_TRY(tracers_to_corners(c, c.r1, c.s1, c.T1t));
+  PT_TRY(source_and_tables(c, false));
+  PT_TRY(update_velocities(c));
+  PT_TRY(build_itab(c));
+  a.dtstage = 0.5 * dt;
+  rk_launch(c, k_rk2, a);
+  // stage 3
+  PT_HIP(hipMemcpyAsync(c.Cnew, c.C0, b5, hipMemcpyDeviceToDevice, c.stream));
+  PT_TRY(tracers_to_corners(c, c.r2, c.s2, c.T2t));
+  PT_TRY(source_and_tables(c, false));
+  PT_TRY(update_velocities(c));
+  PT_TRY(build_itab(c));
+  a.dtstage = dt;
+  rk_launch(c, k_rk3, a);
+  // stage 4
+  PT_HIP(hipMemcpyAsync(c.Cnew, c.C0, b5, hipMemcpyDeviceToDevice, c.stream));
+  PT_TRY(tracers_to_corners(c, c.r1, c.s1, c.T1t));
+  PT_TRY(source_and_tables(c, false));
+  PT_TRY(update_velocities(c));
+  PT_TRY(build_itab(c));
+  a.dtstage = dt / 6.0;
+  rk_launch(c, k_rk4, a);
+  // wrap up
+  PT_HIP(hipMemcpyAsync(c.Cnew, c.C0, b5, hipMemcpyDeviceToDevice, c.stream));
+  PT_TRY(tracers_to_corners(c, c.r, c.s, c.Tt));
+  return PT_OK;
+}
+
+// initial state: PS/basics.f90:352-359, PS/composition.f90:848, PS/main.f90:228-229
+__global__ void k_load_T(const double* Tn, double* T, int nx, int nz) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > nx || k > nz) return;
+  T[I3(i, k, nx + 7)] = Tn[(long)k * (nx + 1) + i];
+}
+
+int load_state(Context& c, const double* T_nodes, double Tbot, double Ttop, const double* rtr, const double* str,
+               const double* Ttr, const double* Ctr) {
+  const int nx = c.T.nx, nz = c.T.nz;
+  const int64_t n = c.T.ntr;
+  c.Tbot_iso = Tbot; c.Ttop_iso = Ttop; c.time = 0.0; c.first_solve = true;
+  size_t nb = sizeof(double) * (size_t)(nx + 1) * (nz + 1);
+  double* dTn = c.src;   // staging (src is rebuilt every stage)
+  PT_HIP(hipMemcpyAsync(dTn, T_nodes, nb, hipMemcpyHostToDevice, c.stream));
+  PT_HIP(hipMemsetAsync(c.T0, 0, sizeof(double) * (size_t)(nx + 7) * (nz + 7), c.stream));
+  dim3 b16(64, 4);
+  hipLaunchKernelGGL(k_load_T, grid2(nx + 1, nz + 1, b16), b16, 0, c.stream, dTn, c.T0, nx, nz);
+  hipLaunchKernelGGL(k_t_walls, dim3((nx + 7 + 255) / 256), dim3(256), 0, c.stream, c.T0, nx, nz, 3, nx + 7, 0, Tbot,
+                     Ttop);
+  t_bcs(c, c.T0);
+  size_t tb = sizeof(double) * (size_t)n;
+  PT_HIP(hipMemcpyAsync(c.r, rtr, tb, hipMemcpyHostToDevice, c.stream));
+  PT_HIP(hipMemcpyAsync(c.s, str, tb, hipMemcpyHostToDevice, c.stream));
+  PT_HIP(hipMemcpyAsync(c.Tt, Ttr, tb, hipMemcpyHostToDevice, c.stream));
+  PT_HIP(hipMemcpyAsync(c.Ct, Ctr, tb, hipMemcpyHostToDevice, c.stream));
+  PT_HIP(hipMemsetAsync(c.Cnew, 0, sizeof(double) * (size_t)(nx + 11) * (nz + 11), c.stream));
+  PT_TRY(tracers_to_corners(c, c.r, c.s, c.Tt));
+  PT_HIP(hipMemsetAsync(c.SF, 0, sizeof(double) * (size_t)(nx + 7) * (nz + 7), c.stream));
+  for (int l = 2; l <= c.T.grids; l++) {
+    PT_HIP(hipMemsetAsync(c.L[l].err, 0, sizeof(double) * c.L[l].nelem, c.stream));
+    if (c.L[l].ferr) PT_HIP(hipMemsetAsync(c.L[l].ferr, 0, sizeof(float) * c.L[l].nelem, c.stream));
+  }
+  PT_HIP(hipStreamSynchronize(c.stream));
+  return PT_OK;
+}
+
+// ---------------------------------------------------------------------------
+static int dalloc(void** p, size_t bytes) {
+  PT_HIP(hipMalloc(p, bytes));
+  PT_HIP(hipMemset(*p, 0, bytes));
+  return PT_OK;
+}
+#define DALLOC(ptr, bytes) PT_TRY(dalloc((void**)&(ptr), (bytes)))
+
+int tracers_alloc(Context& c) {
+  const int nx = c.T.nx, nz = c.T.nz, ncell = (nx + 1) * (nz + 1);
+  size_t b3 = sizeof(double) * (size_t)(nx + 7) * (nz + 7), b5 = sizeof(double) * (size_t)(nx + 11) * (nz + 11);
+  double** f3[] = {&c.SF, &c.Tratio, &c.T0, &c.Tvis, &c.Tbuoy, &c.u, &c.w, &c.src, &c.cond_r, &c.cond_s};
+  for (double** p : f3) DALLOC(*p, b3);
+  double** f5[] = {&c.Cnew, &c.C0, &c.Cvis, &c.Cbuoy, &c.cond, &c.Textend, &c.TxS, &c.CxS};
+  for (double** p : f5) DALLOC(*p, b5);
+  DALLOC(c.itab, sizeof(double) * (size_t)(nx + 1) * (nz + 7) * 12);
+  // D(1..3,-3..3) table for k_itab
+  DALLOC(c.dtab, sizeof(double) * 21);
+  double Dm[21];
+  for (int m = 0; m < 3; m++)
+    for (int q = 0; q < 7; q++) Dm[m * 7 + q] = c.T.D[m + 1][q];
+  PT_HIP(hipMemcpy(c.dtab, Dm, sizeof Dm, hipMemcpyHostToDevice));
+  size_t tb = sizeof(double) * (size_t)c.T.ntr;
+  double** tr[] = {&c.r, &c.s, &c.Tt, &c.Ct, &c.r0, &c.s0, &c.T0t, &c.r1, &c.s1, &c.T1t, &c.r2, &c.s2, &c.T2t};
+  for (double** p : tr) DALLOC(*p, tb);
+  DALLOC(c.cellkey, sizeof(int32_t) * (size_t)c.T.ntr);
+  DALLOC(c.perm, sizeof(int32_t) * (size_t)c.T.ntr);
+  DALLOC(c.count, sizeof(int32_t) * ncell);
+  DALLOC(c.start, sizeof(int32_t) * ncell);
+  DALLOC(c.fill, sizeof(int32_t) * ncell);
+  DALLOC(c.nempty, sizeof(int32_t) * ncell);
+  DALLOC(c.d_flags, sizeof(int32_t) * 4);
+  c.scan_cap = (ncell + 1023) / 1024 + 1;
+  DALLOC(c.scan_tmp, sizeof(int32_t) * c.scan_cap);
+  if (c.scan_cap > 1024 * 64) {
+    set_error("grid too large for the single-block scan of block sums");
+    return PT_ERR_UNSUPPORTED;
+  }
+  return PT_OK;
+}
+
+int tracers_free(Context& c) {
+  void* ps[] = {c.SF, c.Tratio, c.T0, c.Tvis, c.Tbuoy, c.u, c.w, c.src, c.cond_r, c.cond_s, c.Cnew, c.C0, c.Cvis,
+                c.Cbuoy, c.cond, c.Textend, c.TxS, c.CxS, c.itab, c.dtab, c.r, c.s, c.Tt, c.Ct, c.r0, c.s0, c.T0t,
+                c.r1, c.s1, c.T1t, c.r2, c.s2, c.T2t, c.cellkey, c.perm, c.count, c.start, c.fill, c.nempty,
+                c.d_flags, c.scan_tmp};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  return PT_OK;
+}
+
+}  // namespace pt
