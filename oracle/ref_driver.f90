@@ -10,14 +10,15 @@
 !!     PS/basics.f90:352-353);
 !!   * the time loop runs a fixed number of steps (controls in driver.nml);
 !!   * the state is written in full FP64 to state_NNNNN.bin (stream access)
-!!     instead of the reference's snapshots.
+!!     instead of the reference's snapshots (dump_every<0: no state files;
+!!     each loop iteration's wall time is printed for timing runs).
 !! Nothing in the reference sources is modified.
 program ref_driver
  use basics
  use arrays
  implicit none
  integer*4 :: nrun, inject, dump_every, level, i, k, step_dump
- real*8 :: t1, t2, t_solve0
+ real*8 :: t1, t2, t_solve0, t_it0, t_it1
  real*8, allocatable :: Tn(:,:)
  character(len=32) :: fname
  namelist /driver/ nrun, inject, dump_every
@@ -76,10 +77,11 @@ program ref_driver
  call cpu_time(t1)
 !$ t1=omp_get_wtime_d()
  do tstep=0,nrun
+  t_it0=omp_get_wtime_d()
   call multigrid
   call compute_velocities
   step_dump=0
-  if (tstep.eq.nrun) step_dump=1
+  if ((tstep.eq.nrun).and.(dump_every.ge.0)) step_dump=1
   if ((dump_every.gt.0)) then
    if (mod(tstep,dump_every).eq.0) step_dump=1
   end if
@@ -87,7 +89,8 @@ program ref_driver
   if (tstep.eq.nrun) exit
   call energy_time
   time=time+dt
-  write(*,'(a,i6,2(g25.16))') " driver: step done",tstep,time,dt
+  t_it1=omp_get_wtime_d()
+  write(*,'(a,i6,3(g25.16))') " driver: step done",tstep,time,dt,t_it1-t_it0
  end do
  call cpu_time(t2)
 !$ t2=omp_get_wtime_d()
