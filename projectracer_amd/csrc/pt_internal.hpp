@@ -95,13 +95,17 @@ struct Context {
   double* dtab;                   // D(1:3,-3:3) interpolation weights on the device
   double *Cnew, *C0, *Cvis, *Cbuoy, *cond, *Textend;
   double* itab;                   // interpolation table (u, w, source)
-  // coarse grid
+  // coarse grid: band LU (DGBTF2) and the dense operator G = A^-1 diag(RowB)
+  // restricted to interior unknowns, built from the LU once per factorization
   double *Mc, *RowB, *Bc; int32_t* ipiv; int32_t* d_info;
+  double* Gc; double* Gwork; int nci;
   // tracers (ID order)
   double *r, *s, *Tt, *Ct, *r0, *s0, *T0t, *r1, *s1, *T1t, *r2, *s2, *T2t;
-  // projection work
-  int32_t *cellkey, *count, *start, *fill, *perm, *nempty;
-  double *TxS, *CxS, *SS;   // unused by the gather path; kept for reuse as scratch
+  // projection work: cell keys, stable (cell, index) sort, per-cell partial sums
+  int32_t *cellkey, *skey, *iota, *perm, *cstart, *cend, *nempty;
+  void* sort_tmp; size_t sort_bytes; int sort_bits;
+  double* cellsum;          // 27 per cell: (T*w, C*w, w) for the 3x3 neighbour nodes
+  double *TxS, *CxS;        // halo-5 scratch fields for the filters
   int32_t* d_flags;         // [0]=tracer outside domain
   // reductions
   double* red;              // block partials
@@ -152,6 +156,10 @@ int tracers_to_corners(Context& c, const double* r, const double* s, const doubl
 int energy_time(Context& c);
 int load_state(Context& c, const double* T_nodes, double Tbot, double Ttop, const double* rtr,
                const double* str, const double* Ttr, const double* Ctr);
+// ---- sorting (pt_sort.hip) ---------------------------------------------------
+int sort_pairs_temp_bytes(int64_t n, int bits, size_t* bytes);
+int sort_pairs(void* tmp, size_t bytes, const int32_t* kin, int32_t* kout, const int32_t* vin, int32_t* vout,
+               int64_t n, int bits, hipStream_t s);
 // ---- reductions (pt_solver.hip) ----------------------------------------------
 enum RedOp { RED_SUM_ABS = 0, RED_MAX = 1, RED_MIN = 2 };
 int reduce_to_host(Context& c, const double* d, int n, int m, int pitch, int off, RedOp op, double* out);

@@ -54,6 +54,7 @@ struct StageArgs {
 };
 
 constexpr int TX = 64, TZ = 4, PZ = 2;
+constexpr size_t SMALL_LDS_BYTES = 160 * 1024;
 constexpr int TW = TX + 6, TH = TZ * PZ + 6;
 
 template <typename R>
@@ -101,6 +102,54 @@ __device__ __forceinline__ R stream_space_tile(const R (*t)[TW], int ti, int tk,
     s = t1 + t2 + t4 + t5 + t6 - rhs;
   }
   return -s;
+}
+
+// stream_space on a flat array with pitch p (centre index ctr): same
+// arithmetic and order as stream_space_tile
+template <typename R>
+__device__ __forceinline__ R stream_space_flat(const R* S, int p, long ctr, const StageArgs<R>& a, long g) {
+  R sxx = 0, szz = 0, sxxx = 0, szzz = 0, sxxxx = 0, szzzz = 0;
+#pragma unroll
+  for (int q = -2; q <= 2; q++) sxx = sxx + a.D2[q + 2] * S[ctr + q];
+#pragma unroll
+  for (int q = -3; q <= 3; q++) if (q != 0) sxxx = sxxx + a.D3[q + 3] * S[ctr + q];
+#pragma unroll
+  for (int q = -3; q <= 3; q++) sxxxx = sxxxx + a.D4[q + 3] * S[ctr + q];
+#pragma unroll
+  for (int q = -2; q <= 2; q++) szz = szz + a.D2[q + 2] * S[ctr + (long)q * p];
+#pragma unroll
+  for (int q = -3; q <= 3; q++) if (q != 0) szzz = szzz + a.D3[q + 3] * S[ctr + (long)q * p];
+#pragma unroll
+  for (int q = -3; q <= 3; q++) szzzz = szzzz + a.D4[q + 3] * S[ctr + (long)q * p];
+  R SFxx = sxx / a.dr2, SFxxx = sxxx / a.dr3, SFxxxx = sxxxx / a.dr4;
+  R SFzz = szz / a.ds2, SFzzz = szzz / a.ds3, SFzzzz = szzzz / a.ds4;
+  R sxz = 0, sxxz = 0, sxzz = 0, sxxzz = 0;
+#pragma unroll
+  for (int b = -2; b <= 2; b++)
+#pragma unroll
+    for (int q = -2; q <= 2; q++) {
+      R v = S[ctr + (long)b * p + q];
+      if (b != 0 && q != 0) sxz = sxz + a.Drs[b + 2][q + 2] * v;
+      if (b != 0) sxxz = sxxz + a.Drrs[b + 2][q + 2] * v;
+      if (q != 0) sxzz = sxzz + a.Drss[b + 2][q + 2] * v;
+      sxxzz = sxxzz + a.Drrss[b + 2][q + 2] * v;
+    }
+  R SFxz = sxz / a.drds, SFxxz = sxxz / a.dr2ds, SFxzz = sxzz / a.drds2, SFxxzz = sxxzz / a.dr2ds2;
+  R rhs = a.rhs[g];
+  R vis = a.vis[g];
+  R sv;
+  if (a.ivis0) {
+    R t2 = vis * (SFxxxx + SFzzzz + (R)2 * SFxxzz);
+    sv = t2 - rhs;
+  } else {
+    R t1 = a.vxm[g] * (SFxx - SFzz);
+    R t2 = vis * (SFxxxx + SFzzzz + (R)2 * SFxxzz);
+    R t4 = ((R)4 * a.vxz[g]) * SFxz;
+    R t5 = ((R)2 * a.vx[g]) * (SFxxx + SFxzz);
+    R t6 = ((R)2 * a.vz[g]) * (SFzzz + SFxxz);
+    sv = t1 + t2 + t4 + t5 + t6 - rhs;
+  }
+  return -sv;
 }
 
 template <typename R>
@@ -189,6 +238,134 @@ __global__ void __launch_bounds__(TX* TZ) k_residual(StageArgs<R> a, R* res, dou
     __syncthreads();
   }
   if (tid == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// small levels: the whole RKC1 sweep (all stages) and compute_residual in one
+// workgroup with the three RKC buffers resident in LDS
+// ---------------------------------------------------------------------------
+struct RkcCoef {
+  double mewt[64], mew[64], vv[64], cc[64];
+};
+
+template <typename R>
+__global__ void __launch_bounds__(1024) k_smooth_small(StageArgs<R> a, R* err, R* res, int nstage, double cpar,
+                                                      RkcCoef co) {
+  extern __shared__ unsigned char smem_raw[];
+  R* buf0 = reinterpret_cast<R*>(smem_raw);
+  const int n = a.n, m = a.m, p = a.pitch;
+  const int ne = p * (m + 7);
+  R* buf1 = buf0 + ne;
+  R* buf2 = buf1 + ne;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int e = tid; e < ne; e += nt) {
+    buf0[e] = err[e];
+    buf1[e] = (R)0;
+    buf2[e] = (R)0;
+  }
+  __syncthreads();
+  const int npts = (n - 1) * (m - 1);
+  // stage 1: S1 = S0 + mewt1*c*dt*F(S0) -> buf1
+  {
+    const R cf = (R)(co.mewt[1] * cpar);
+    for (int e = tid; e < npts; e += nt) {
+      int i = 1 + e % (n - 1), k = 1 + e / (n - 1);
+      long g = IDX(i, k, p);
+      R F = stream_space_flat<R>(buf0, p, g, a, g);
+      R y = buf0[g] + (cf * a.dts[g]) * F;
+      store_with_ghosts<R>(buf1, i, k, n, m, p, y, a.sz);
+    }
+  }
+  __syncthreads();
+  R* cur = buf1;
+  R* prev = buf0;   // S2 = S0 at stage 2
+  R* spare = buf2;
+  for (int j = 2; j <= nstage; j++) {
+    R* out = (prev == buf0) ? spare : prev;
+    const R c0 = (R)co.cc[j], c1 = (R)co.mew[j], c2 = (R)co.vv[j], cf = (R)(co.mewt[j] * cpar);
+    for (int e = tid; e < npts; e += nt) {
+      int i = 1 + e % (n - 1), k = 1 + e / (n - 1);
+      long g = IDX(i, k, p);
+      R F = stream_space_flat<R>(cur, p, g, a, g);
+      R y = ((c0 * buf0[g] + c1 * cur[g]) + c2 * prev[g]) + ((cf * a.dts[g]) * F);
+      store_with_ghosts<R>(out, i, k, n, m, p, y, a.sz);
+    }
+    __syncthreads();
+    R* np = cur;
+    cur = out;
+    prev = np;
+    if (out == spare) spare = nullptr;
+  }
+  // cur holds the sweep result (psi correction): write back, then the residual
+  for (int e = tid; e < ne; e += nt) err[e] = cur[e];
+  for (int e = tid; e < npts; e += nt) {
+    int i = 1 + e % (n - 1), k = 1 + e / (n - 1);
+    long g = IDX(i, k, p);
+    res[g] = stream_space_flat<R>(cur, p, g, a, g);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// coarsest grid as a dense operator: G(:,q) = A^-1 (RowB(q) e_q) from the band
+// LU factors, interior unknowns only; one thread per column q, work vectors
+// stored row-interleaved so the threads' accesses coalesce
+// ---------------------------------------------------------------------------
+__global__ void k_coarse_inverse(const double* ab, const int32_t* ipiv, const double* RowB, double* Gwork, double* G,
+                                 int n, int kl, int ku, int nxc, int nzc) {
+  const int nci = (nxc - 1) * (nzc - 1);
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nci) return;
+  const int ld = 2 * kl + ku + 1, kd = ku + kl + 1, N = nzc + 1;
+  int iq = 1 + t % (nxc - 1), kq = 1 + t / (nxc - 1);
+  int q = N * iq + kq + 1;   // 1-based unknown index
+#define AB(r, c) ab[(long)((c) - 1) * ld + ((r) - 1)]
+#define Y(pp) Gwork[(long)((pp) - 1) * nci + t]
+  for (int pp = 1; pp <= n; pp++) Y(pp) = 0.0;
+  Y(q) = RowB[q - 1];
+  for (int j = 1; j <= n - 1; j++) {
+    int lm = min(kl, n - j);
+    int l = ipiv[j - 1];
+    if (l != j) { double tmp = Y(l); Y(l) = Y(j); Y(j) = tmp; }
+    double temp = -1.0 * Y(j);
+    for (int u = 1; u <= lm; u++) Y(j + u) = Y(j + u) + AB(kd + u, j) * temp;
+  }
+  const int kk = kl + ku, kp1 = kk + 1;
+  for (int j = n; j >= 1; j--) {
+    double bj = Y(j);
+    if (bj != 0.0) {
+      int l = kp1 - j;
+      double x = bj / AB(kp1, j);
+      Y(j) = x;
+      int lo = max(1, j - kk);
+      for (int i = j - 1; i >= lo; i--) Y(i) = Y(i) - x * AB(l + i, j);
+    }
+  }
+  for (int pi = 0; pi < nci; pi++) {
+    int ip = 1 + pi % (nxc - 1), kp = 1 + pi / (nxc - 1);
+    G[(long)t * nci + pi] = Y(N * ip + kp + 1);
+  }
+#undef AB
+#undef Y
+}
+
+// coarse correction: err(p) = sum_q G(p,q) rhs(q), fixed order in q
+template <typename RI, typename RO>
+__global__ void k_coarse_apply(const double* G, const RI* rhs, RO* err, int nxc, int nzc, int pitch, RO sz,
+                               const int* halt) {
+  if (halt && *halt) return;
+  extern __shared__ double rv[];
+  const int nci = (nxc - 1) * (nzc - 1);
+  for (int q = threadIdx.x; q < nci; q += blockDim.x) {
+    int iq = 1 + q % (nxc - 1), kq = 1 + q / (nxc - 1);
+    rv[q] = (double)rhs[IDX(iq, kq, pitch)];
+  }
+  __syncthreads();
+  int pi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pi >= nci) return;
+  double acc = 0.0;
+  for (int q = 0; q < nci; q++) acc = acc + G[(long)q * nci + pi] * rv[q];
+  int ip = 1 + pi % (nxc - 1), kp = 1 + pi / (nxc - 1);
+  store_with_ghosts<RO>(err, ip, kp, nxc, nzc, pitch, (RO)acc, sz);
 }
 
 // ---------------------------------------------------------------------------
@@ -743,6 +920,31 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, double* resmag)
   R* B = fp32 ? (R*)L.fB : (R*)L.bufB;
   R* Cb = fp32 ? (R*)L.fC : (R*)L.bufC;
   bool timed = (l == 1) && c.timing;
+  const size_t lds = 3 * L.nelem * sizeof(R);
+  if (l >= 2 && lds <= SMALL_LDS_BYTES && resmag == nullptr) {
+    // whole sweep + residual in one workgroup, buffers in LDS
+    static bool attr_set = false;
+    if (!attr_set) {
+      PT_HIP(hipFuncSetAttribute((const void*)k_smooth_small<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)SMALL_LDS_BYTES));
+      attr_set = true;
+    }
+    RkcCoef co;
+    std::memcpy(co.mewt, c.T.mewt, sizeof co.mewt); std::memcpy(co.mew, c.T.mew, sizeof co.mew);
+    std::memcpy(co.vv, c.T.vv, sizeof co.vv); std::memcpy(co.cc, c.T.cc, sizeof co.cc);
+    R* res = fp32 ? (R*)L.fB : (R*)L.res;
+    for (int it = 1; it <= c.P.Nmulti; it++) {
+      int ntemp = c.P.Nmulti - it + 1;
+      double temp = (c.P.Nmulti > 1) ? (c.P.c_2 - c.P.c_1) * (double)(ntemp - 1) / (double)(1 - c.P.Nmulti) : 0.0;
+      double cc = temp + c.P.c_2;
+      StageArgs<R> a = stage_base<R>(c, l, fp32);
+      // the residual buffer aliases fB in FP32 mode: the fused kernel keeps its
+      // RKC buffers in LDS, so fB is free
+      hipLaunchKernelGGL(k_smooth_small<R>, dim3(1), dim3(1024), lds, c.stream, a, err, res, s, cc, co);
+    }
+    (void)p;
+    return PT_OK;
+  }
   for (int it = 1; it <= c.P.Nmulti; it++) {
     int ntemp = c.P.Nmulti - it + 1;
     double temp = (c.P.Nmulti > 1) ? (c.P.c_2 - c.P.c_1) * (double)(ntemp - 1) / (double)(1 - c.P.Nmulti) : 0.0;
@@ -914,6 +1116,8 @@ static int coarse_factor(Context& c) {
     set_error("coarse-grid band LU: zero pivot at column " + std::to_string(info));
     return PT_ERR_SINGULAR_COARSE;
   }
+  hipLaunchKernelGGL(k_coarse_inverse, dim3((c.nci + 63) / 64), dim3(64), 0, c.stream, c.Mc, c.ipiv, c.RowB,
+                     c.Gwork, c.Gc, t.ngc, t.kl, t.ku, G.g.n, G.g.m);
   return PT_OK;
 }
 
@@ -1018,14 +1222,16 @@ int multigrid(Context& c) {
           return PT_ERR_HIGH_RESIDUAL;
         }
       } else {
+        // Compute_coarse_vector + DGBTRS + Stream_grid_coarse + enforceBCs_stream
+        // as one dense matvec with the operator built from the LU factors
+        dim3 gb((c.nci + 255) / 256);
+        size_t sh = sizeof(double) * c.nci;
         if (!mixed)
-          hipLaunchKernelGGL(k_coarse_solve<double>, dim3(1), dim3(64), 0, c.stream, c.Mc, c.ipiv, c.RowB,
-                             (const void*)L.rhs, 0, L.err, t.ngc, t.kl, t.ku, L.g.n, L.g.m, L.g.pitch, sz,
-                             (const int*)nullptr);
+          hipLaunchKernelGGL((k_coarse_apply<double, double>), gb, dim3(256), sh, c.stream, c.Gc, L.rhs, L.err, L.g.n,
+                             L.g.m, L.g.pitch, sz, (const int*)nullptr);
         else
-          hipLaunchKernelGGL(k_coarse_solve<float>, dim3(1), dim3(64), 0, c.stream, c.Mc, c.ipiv, c.RowB,
-                             (const void*)L.frhs, 1, L.ferr, t.ngc, t.kl, t.ku, L.g.n, L.g.m, L.g.pitch, (float)sz,
-                             (const int*)nullptr);
+          hipLaunchKernelGGL((k_coarse_apply<float, float>), gb, dim3(256), sh, c.stream, c.Gc, L.frhs, L.ferr,
+                             L.g.n, L.g.m, L.g.pitch, (float)sz, (const int*)nullptr);
       }
       if (l == 1 && (resmag1 / c.res_ref1) <= c.P.tolerance) { done = true; break; }
       if (t.multi[p + 1] < t.multi[p]) {
@@ -1108,6 +1314,9 @@ int solver_alloc(Context& c) {
     return PT_ERR_UNSUPPORTED;
   }
   DALLOC(c.Mc, mb); DALLOC(c.RowB, sizeof(double) * t.ngc); DALLOC(c.Bc, sizeof(double) * t.ngc);
+  c.nci = (t.nxg[t.grids] - 1) * (t.nzg[t.grids] - 1);
+  DALLOC(c.Gc, sizeof(double) * (size_t)c.nci * c.nci);
+  DALLOC(c.Gwork, sizeof(double) * (size_t)c.nci * t.ngc);
   DALLOC(c.ipiv, sizeof(int32_t) * t.ngc); DALLOC(c.d_info, sizeof(int32_t));
   c.red_cap = 1 << 20;
   DALLOC(c.red, sizeof(double) * c.red_cap);
@@ -1134,7 +1343,7 @@ int solver_free(Context& c) {
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
-  void* ps[] = {c.Mc, c.RowB, c.Bc, c.ipiv, c.d_info, c.red, c.red_out, c.dres64};
+  void* ps[] = {c.Mc, c.RowB, c.Bc, c.ipiv, c.d_info, c.red, c.red_out, c.dres64, c.Gc, c.Gwork};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c.h_red) (void)hipHostFree(c.h_red);

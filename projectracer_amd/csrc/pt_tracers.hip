@@ -36,79 +36,104 @@ static dim3 grid2(int ni, int nk, dim3 b) { return dim3((ni + b.x - 1) / b.x, (n
 __device__ __forceinline__ int nint(double x) { return (int)round(x); }
 
 // ---------------------------------------------------------------------------
-// binning + counting sort
+// binning, stable sort by cell, per-cell wavefront reductions, node combine
 // ---------------------------------------------------------------------------
+// key = dual cell of nint(r/dr), nint(s/ds); positions outside the domain
+// (or NaN) raise the flag and get the sentinel key ncell (sorted last)
 __global__ void k_bin(const double* r, const double* s, int64_t n, int nx, int nz, double dr, double ds,
-                      double aspect, int32_t* key, int32_t* count, int32_t* flags) {
+                      double aspect, int32_t* key, int32_t* flags) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   double rv = r[j], sv = s[j];
-  if (rv < 0.0 || rv > aspect || sv < 0.0 || sv > 1.0 || rv != rv || sv != sv) {
+  int ncell = (nx + 1) * (nz + 1);
+  if (!(rv >= 0.0 && rv <= aspect && sv >= 0.0 && sv <= 1.0)) {
     atomicOr(&flags[0], 1);
-    key[j] = 0;
+    key[j] = ncell;
     return;
   }
   int i = nint(rv / dr), k = nint(sv / ds);
-  int c = k * (nx + 1) + i;
-  key[j] = c;
-  atomicAdd(&count[c], 1);
+  key[j] = k * (nx + 1) + i;
 }
 
-// exclusive scan over n int32 in three phases (block scans of 1024)
-__global__ void k_scan_block(const int32_t* in, int32_t* out, int32_t* bsum, int n) {
-  __shared__ int32_t s[1024];
-  int t = threadIdx.x, g = blockIdx.x * 1024 + t;
-  int v = g < n ? in[g] : 0;
-  s[t] = v;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int x = t >= off ? s[t - off] : 0;
-    __syncthreads();
-    s[t] += x;
-    __syncthreads();
-  }
-  if (g < n) out[g] = s[t] - v;
-  if (t == 1023) bsum[blockIdx.x] = s[1023];
-}
-__global__ void k_scan_sums(int32_t* bsum, int nb) {
-  if (threadIdx.x == 0) {
-    int acc = 0;
-    for (int b = 0; b < nb; b++) { int v = bsum[b]; bsum[b] = acc; acc += v; }
-  }
-}
-__global__ void k_scan_add(int32_t* out, const int32_t* bsum, int n) {
-  int g = blockIdx.x * 1024 + threadIdx.x;
-  if (g < n) out[g] += bsum[blockIdx.x];
-}
-
-__global__ void k_scatter(const int32_t* key, int64_t n, const int32_t* start, int32_t* fill, int32_t* perm) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  int c = key[j];
-  int pos = start[c] + atomicAdd(&fill[c], 1);
-  perm[pos] = (int32_t)j;
-}
-
-// ascending tracer index inside each cell segment (insertion sort; segments
-// hold ~tpc entries)
-__global__ void k_canon(const int32_t* start, const int32_t* count, int32_t* perm, int ncell) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+// segment bounds of each cell in the sorted key array (empty cells: 0,0)
+__global__ void k_cell_bounds(const int32_t* skey, int64_t n, int ncell, int32_t* cstart, int32_t* cend) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int c = skey[p];
   if (c >= ncell) return;
-  int b = start[c], m = count[c];
-  for (int a = 1; a < m; a++) {
-    int v = perm[b + a], q = a - 1;
-    while (q >= 0 && perm[b + q] > v) { perm[b + q + 1] = perm[b + q]; q--; }
-    perm[b + q + 1] = v;
+  if (p == 0 || skey[p - 1] != c) cstart[c] = (int32_t)p;
+  if (p == n - 1 || skey[p + 1] != c) cend[c] = (int32_t)(p + 1);
+}
+
+// One wavefront per cell: lane l takes the l-th tracer of the cell's sorted
+// segment (64 at a time), forms the 9 bilinear weights of shape_func
+// (PS/composition.f90:896-921, the reference's 9 neighbour nodes of
+// nint(r/dr),nint(s/ds)) and T*w, C*w, w; a fixed xor-butterfly reduces the
+// 27 values across the wave, chunks are added in order.  No atomics: the
+// result is deterministic.
+__global__ void __launch_bounds__(256) k_cell_reduce(const double* __restrict__ r, const double* __restrict__ s,
+                                                     const double* __restrict__ Tt, const double* __restrict__ Ct,
+                                                     const int32_t* __restrict__ perm,
+                                                     const int32_t* __restrict__ cstart,
+                                                     const int32_t* __restrict__ cend, double* __restrict__ cellsum,
+                                                     int nx, int nz, double dr, double ds) {
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int ncell = (nx + 1) * (nz + 1);
+  if (c >= ncell) return;
+  const int i = c % (nx + 1), k = c / (nx + 1);
+  const int b0 = cstart[c], n = cend[c] - b0;
+  double acc[27];
+#pragma unroll
+  for (int q = 0; q < 27; q++) acc[q] = 0.0;
+  for (int base = 0; base < n; base += 64) {
+    double v[27];
+    if (base + lane < n) {
+      int j = perm[b0 + base + lane];
+      double rv = r[j], sv = s[j], T = Tt[j], Cv = Ct[j];
+      double wx[3], wz[3];
+#pragma unroll
+      for (int a = -1; a <= 1; a++) {
+        double ri = (double)(i + a) * dr;
+        double t = 1.0 - fabs(rv - ri) / dr;
+        wx[a + 1] = t < 0.0 ? 0.0 : t;
+        double sk = (double)(k + a) * ds;
+        double u = 1.0 - fabs(sv - sk) / ds;
+        wz[a + 1] = u < 0.0 ? 0.0 : u;
+      }
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+          double w = wx[a] * wz[b];
+          v[(a * 3 + b) * 3 + 0] = T * w;
+          v[(a * 3 + b) * 3 + 1] = Cv * w;
+          v[(a * 3 + b) * 3 + 2] = w;
+        }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 27; q++) v[q] = 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+      double x = v[q];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) x = x + __shfl_xor(x, off);
+      acc[q] = acc[q] + x;
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 27; q++) cellsum[(long)c * 27 + q] = acc[q];
   }
 }
 
-// node-owner projection (tracers_to_corners_iC_continuous main body)
-__global__ void k_project(const double* r, const double* s, const double* Tt, const double* Ct, const int32_t* start,
-                          const int32_t* count, const int32_t* perm, double* Tratio, double* Cnew, int32_t* nempty,
-                          int nx, int nz, double dr, double ds) {
+// node (i,k) sums the partials of the 9 cells around it in a fixed order;
+// Tratio, Cnew = ratio where the weight sum is positive, nempty = own count
+__global__ void k_node_combine(const double* __restrict__ cellsum, const int32_t* cstart, const int32_t* cend,
+                               double* Tratio, double* Cnew, int32_t* nempty, int nx, int nz) {
   int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
   if (i > nx || k > nz) return;
-  double ri = (double)i * dr, sk = (double)k * ds;
   double txs = 0.0, cxs = 0.0, ssum = 0.0;
   for (int a = -1; a <= 1; a++) {
     int ic = i + a;
@@ -116,23 +141,15 @@ __global__ void k_project(const double* r, const double* s, const double* Tt, co
     for (int b = -1; b <= 1; b++) {
       int kc = k + b;
       if (kc < 0 || kc > nz) continue;
-      int c = kc * (nx + 1) + ic;
-      int beg = start[c], m = count[c];
-      for (int q = 0; q < m; q++) {
-        int j = perm[beg + q];
-        double rv = r[j], sv = s[j];
-        double wa = 1.0 - fabs(rv - ri) / dr;
-        wa = wa < 0.0 ? 0.0 : wa;
-        double wb = 1.0 - fabs(sv - sk) / ds;
-        wb = wb < 0.0 ? 0.0 : wb;
-        double sf = wa * wb;
-        txs = txs + Tt[j] * sf;
-        cxs = cxs + Ct[j] * sf;
-        ssum = ssum + sf;
-      }
+      long c = (long)kc * (nx + 1) + ic;
+      int slot = ((-a) + 1) * 3 + ((-b) + 1);   // this node seen from cell (ic,kc)
+      txs = txs + cellsum[c * 27 + slot * 3 + 0];
+      cxs = cxs + cellsum[c * 27 + slot * 3 + 1];
+      ssum = ssum + cellsum[c * 27 + slot * 3 + 2];
     }
   }
-  nempty[k * (nx + 1) + i] = count[k * (nx + 1) + i];
+  long own = (long)k * (nx + 1) + i;
+  nempty[own] = cend[own] - cstart[own];
   if (ssum > 0.0) {
     Tratio[I3(i, k, nx + 7)] = txs / ssum;
     Cnew[I5(i, k, nx + 11)] = cxs / ssum;
@@ -393,21 +410,19 @@ int tracers_to_corners(Context& c, const double* r, const double* s, const doubl
   size_t b3 = sizeof(double) * (size_t)(nx + 7) * (nz + 7);
   if (c.timing) timer_begin(c, PT_K_PROJECT);
   PT_HIP(hipMemcpyAsync(c.Tratio, c.T0, b3, hipMemcpyDeviceToDevice, c.stream));
-  PT_HIP(hipMemsetAsync(c.count, 0, sizeof(int32_t) * ncell, c.stream));
-  PT_HIP(hipMemsetAsync(c.fill, 0, sizeof(int32_t) * ncell, c.stream));
   PT_HIP(hipMemsetAsync(c.d_flags, 0, sizeof(int32_t) * 4, c.stream));
+  PT_HIP(hipMemsetAsync(c.cstart, 0, sizeof(int32_t) * ncell, c.stream));
+  PT_HIP(hipMemsetAsync(c.cend, 0, sizeof(int32_t) * ncell, c.stream));
   unsigned nbt = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(k_bin, dim3(nbt), dim3(256), 0, c.stream, r, s, n, nx, nz, c.T.dr, c.T.ds, c.T.aspect, c.cellkey,
-                     c.count, c.d_flags);
-  int nb = (ncell + 1023) / 1024;
-  hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(1024), 0, c.stream, c.count, c.start, c.scan_tmp, ncell);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(64), 0, c.stream, c.scan_tmp, nb);
-  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, c.stream, c.start, c.scan_tmp, ncell);
-  hipLaunchKernelGGL(k_scatter, dim3(nbt), dim3(256), 0, c.stream, c.cellkey, n, c.start, c.fill, c.perm);
-  hipLaunchKernelGGL(k_canon, dim3((ncell + 255) / 256), dim3(256), 0, c.stream, c.start, c.count, c.perm, ncell);
+  hipLaunchKernelGGL(k_bin, dim3(nbt), dim3(256), 0, c.stream, r, s, n, nx, nz, c.T.dr, c.T.ds, c.T.aspect,
+                     c.cellkey, c.d_flags);
+  PT_TRY(sort_pairs(c.sort_tmp, c.sort_bytes, c.cellkey, c.skey, c.iota, c.perm, n, c.sort_bits, c.stream));
+  hipLaunchKernelGGL(k_cell_bounds, dim3(nbt), dim3(256), 0, c.stream, c.skey, n, ncell, c.cstart, c.cend);
+  hipLaunchKernelGGL(k_cell_reduce, dim3((ncell + 3) / 4), dim3(256), 0, c.stream, r, s, Tt, c.Ct, c.perm, c.cstart,
+                     c.cend, c.cellsum, nx, nz, c.T.dr, c.T.ds);
   dim3 b16(64, 4);
-  hipLaunchKernelGGL(k_project, grid2(nx + 1, nz + 1, b16), b16, 0, c.stream, r, s, Tt, c.Ct, c.start, c.count, c.perm,
-                     c.Tratio, c.Cnew, c.nempty, nx, nz, c.T.dr, c.T.ds);
+  hipLaunchKernelGGL(k_node_combine, grid2(nx + 1, nz + 1, b16), b16, 0, c.stream, c.cellsum, c.cstart, c.cend,
+                     c.Tratio, c.Cnew, c.nempty, nx, nz);
   if (c.timing) timer_end(c, PT_K_PROJECT);
   int32_t flag = 0;
   PT_HIP(hipMemcpyAsync(&flag, c.d_flags, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
@@ -773,6 +788,11 @@ static int dalloc(void** p, size_t bytes) {
 }
 #define DALLOC(ptr, bytes) PT_TRY(dalloc((void**)&(ptr), (bytes)))
 
+__global__ void k_iota(int32_t* v, int64_t n) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) v[j] = (int32_t)j;
+}
+
 int tracers_alloc(Context& c) {
   const int nx = c.T.nx, nz = c.T.nz, ncell = (nx + 1) * (nz + 1);
   size_t b3 = sizeof(double) * (size_t)(nx + 7) * (nz + 7), b5 = sizeof(double) * (size_t)(nx + 11) * (nz + 11);
@@ -790,27 +810,26 @@ int tracers_alloc(Context& c) {
   size_t tb = sizeof(double) * (size_t)c.T.ntr;
   double** tr[] = {&c.r, &c.s, &c.Tt, &c.Ct, &c.r0, &c.s0, &c.T0t, &c.r1, &c.s1, &c.T1t, &c.r2, &c.s2, &c.T2t};
   for (double** p : tr) DALLOC(*p, tb);
-  DALLOC(c.cellkey, sizeof(int32_t) * (size_t)c.T.ntr);
-  DALLOC(c.perm, sizeof(int32_t) * (size_t)c.T.ntr);
-  DALLOC(c.count, sizeof(int32_t) * ncell);
-  DALLOC(c.start, sizeof(int32_t) * ncell);
-  DALLOC(c.fill, sizeof(int32_t) * ncell);
+  size_t ib = sizeof(int32_t) * (size_t)c.T.ntr;
+  DALLOC(c.cellkey, ib); DALLOC(c.skey, ib); DALLOC(c.iota, ib); DALLOC(c.perm, ib);
+  DALLOC(c.cstart, sizeof(int32_t) * ncell); DALLOC(c.cend, sizeof(int32_t) * ncell);
   DALLOC(c.nempty, sizeof(int32_t) * ncell);
+  DALLOC(c.cellsum, sizeof(double) * 27 * (size_t)ncell);
   DALLOC(c.d_flags, sizeof(int32_t) * 4);
-  c.scan_cap = (ncell + 1023) / 1024 + 1;
-  DALLOC(c.scan_tmp, sizeof(int32_t) * c.scan_cap);
-  if (c.scan_cap > 1024 * 64) {
-    set_error("grid too large for the single-block scan of block sums");
-    return PT_ERR_UNSUPPORTED;
-  }
+  hipLaunchKernelGGL(k_iota, dim3((unsigned)((c.T.ntr + 255) / 256)), dim3(256), 0, 0, c.iota, c.T.ntr);
+  c.sort_bits = 1;
+  while ((1 << c.sort_bits) <= ncell) c.sort_bits++;
+  PT_TRY(sort_pairs_temp_bytes(c.T.ntr, c.sort_bits, &c.sort_bytes));
+  DALLOC(c.sort_tmp, c.sort_bytes + 256);
+  PT_HIP(hipDeviceSynchronize());
   return PT_OK;
 }
 
 int tracers_free(Context& c) {
   void* ps[] = {c.SF, c.Tratio, c.T0, c.Tvis, c.Tbuoy, c.u, c.w, c.src, c.cond_r, c.cond_s, c.Cnew, c.C0, c.Cvis,
                 c.Cbuoy, c.cond, c.Textend, c.TxS, c.CxS, c.itab, c.dtab, c.r, c.s, c.Tt, c.Ct, c.r0, c.s0, c.T0t,
-                c.r1, c.s1, c.T1t, c.r2, c.s2, c.T2t, c.cellkey, c.perm, c.count, c.start, c.fill, c.nempty,
-                c.d_flags, c.scan_tmp};
+                c.r1, c.s1, c.T1t, c.r2, c.s2, c.T2t, c.cellkey, c.skey, c.iota, c.perm, c.cstart, c.cend,
+                c.nempty, c.cellsum, c.d_flags, c.sort_tmp};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   return PT_OK;
