@@ -48,10 +48,20 @@ struct StageArgs {
   int n, m, pitch;
   R sz;             // ghost sign in z (free-slip -1, rigid +1)
   R dr2, dr3, dr4, ds2, ds3, ds4, drds, dr2ds, drds2, dr2ds2;
+  R rdr2, rdr3, rdr4, rds2, rds3, rds4, rdrds, rdr2ds, rdrds2, rdr2ds2;
   R D1[5], D2[5], D3[7], D4[7];
   R Drs[5][5], Drss[5][5], Drrs[5][5], Drrss[5][5];
   const int* halt;  // device flag: skip work when nonzero (may be null)
 };
+
+// Accumulation in the reference's order: separate multiply and add for the
+// FP64 path (bit-faithful); fused multiply-add for the FP32 smoother.
+__device__ __forceinline__ double acc_(double s, double c, double v) { return s + c * v; }
+__device__ __forceinline__ float acc_(float s, float c, float v) { return __fmaf_rn(c, v, s); }
+// derivative scaling: FP64 divides by the grid power as the reference does;
+// FP32 multiplies by the precomputed reciprocal
+__device__ __forceinline__ double scale_(double s, double d, double rd) { (void)rd; return s / d; }
+__device__ __forceinline__ float scale_(float s, float d, float rd) { (void)d; return s * rd; }
 
 constexpr int TX = 64, TZ = 4, PZ = 2;
 constexpr size_t SMALL_LDS_BYTES = 160 * 1024;
@@ -62,31 +72,32 @@ __device__ __forceinline__ R stream_space_tile(const R (*t)[TW], int ti, int tk,
   // stream_derivatives (uniform grid), reference summation order, zero stencil weights skipped
   R sxx = 0, szz = 0, sxxx = 0, szzz = 0, sxxxx = 0, szzzz = 0;
 #pragma unroll
-  for (int q = -2; q <= 2; q++) sxx = sxx + a.D2[q + 2] * t[tk][ti + q];
+  for (int q = -2; q <= 2; q++) sxx = acc_(sxx, a.D2[q + 2], t[tk][ti + q]);
 #pragma unroll
-  for (int q = -3; q <= 3; q++) if (q != 0) sxxx = sxxx + a.D3[q + 3] * t[tk][ti + q];
+  for (int q = -3; q <= 3; q++) if (q != 0) sxxx = acc_(sxxx, a.D3[q + 3], t[tk][ti + q]);
 #pragma unroll
-  for (int q = -3; q <= 3; q++) sxxxx = sxxxx + a.D4[q + 3] * t[tk][ti + q];
+  for (int q = -3; q <= 3; q++) sxxxx = acc_(sxxxx, a.D4[q + 3], t[tk][ti + q]);
 #pragma unroll
-  for (int q = -2; q <= 2; q++) szz = szz + a.D2[q + 2] * t[tk + q][ti];
+  for (int q = -2; q <= 2; q++) szz = acc_(szz, a.D2[q + 2], t[tk + q][ti]);
 #pragma unroll
-  for (int q = -3; q <= 3; q++) if (q != 0) szzz = szzz + a.D3[q + 3] * t[tk + q][ti];
+  for (int q = -3; q <= 3; q++) if (q != 0) szzz = acc_(szzz, a.D3[q + 3], t[tk + q][ti]);
 #pragma unroll
-  for (int q = -3; q <= 3; q++) szzzz = szzzz + a.D4[q + 3] * t[tk + q][ti];
-  R SFxx = sxx / a.dr2, SFxxx = sxxx / a.dr3, SFxxxx = sxxxx / a.dr4;
-  R SFzz = szz / a.ds2, SFzzz = szzz / a.ds3, SFzzzz = szzzz / a.ds4;
+  for (int q = -3; q <= 3; q++) szzzz = acc_(szzzz, a.D4[q + 3], t[tk + q][ti]);
+  R SFxx = scale_(sxx, a.dr2, a.rdr2), SFxxx = scale_(sxxx, a.dr3, a.rdr3), SFxxxx = scale_(sxxxx, a.dr4, a.rdr4);
+  R SFzz = scale_(szz, a.ds2, a.rds2), SFzzz = scale_(szzz, a.ds3, a.rds3), SFzzzz = scale_(szzzz, a.ds4, a.rds4);
   R sxz = 0, sxxz = 0, sxzz = 0, sxxzz = 0;
 #pragma unroll
   for (int b = -2; b <= 2; b++)
 #pragma unroll
     for (int q = -2; q <= 2; q++) {
       R v = t[tk + b][ti + q];
-      if (b != 0 && q != 0) sxz = sxz + a.Drs[b + 2][q + 2] * v;
-      if (b != 0) sxxz = sxxz + a.Drrs[b + 2][q + 2] * v;
-      if (q != 0) sxzz = sxzz + a.Drss[b + 2][q + 2] * v;
-      sxxzz = sxxzz + a.Drrss[b + 2][q + 2] * v;
+      if (b != 0 && q != 0) sxz = acc_(sxz, a.Drs[b + 2][q + 2], v);
+      if (b != 0) sxxz = acc_(sxxz, a.Drrs[b + 2][q + 2], v);
+      if (q != 0) sxzz = acc_(sxzz, a.Drss[b + 2][q + 2], v);
+      sxxzz = acc_(sxxzz, a.Drrss[b + 2][q + 2], v);
     }
-  R SFxz = sxz / a.drds, SFxxz = sxxz / a.dr2ds, SFxzz = sxzz / a.drds2, SFxxzz = sxxzz / a.dr2ds2;
+  R SFxz = scale_(sxz, a.drds, a.rdrds), SFxxz = scale_(sxxz, a.dr2ds, a.rdr2ds), SFxzz = scale_(sxzz, a.drds2, a.rdrds2),
+    SFxxzz = scale_(sxxzz, a.dr2ds2, a.rdr2ds2);
   R rhs = a.rhs[g];
   R vis = a.vis[g];
   R s;
@@ -110,31 +121,32 @@ template <typename R>
 __device__ __forceinline__ R stream_space_flat(const R* S, int p, long ctr, const StageArgs<R>& a, long g) {
   R sxx = 0, szz = 0, sxxx = 0, szzz = 0, sxxxx = 0, szzzz = 0;
 #pragma unroll
-  for (int q = -2; q <= 2; q++) sxx = sxx + a.D2[q + 2] * S[ctr + q];
+  for (int q = -2; q <= 2; q++) sxx = acc_(sxx, a.D2[q + 2], S[ctr + q]);
 #pragma unroll
-  for (int q = -3; q <= 3; q++) if (q != 0) sxxx = sxxx + a.D3[q + 3] * S[ctr + q];
+  for (int q = -3; q <= 3; q++) if (q != 0) sxxx = acc_(sxxx, a.D3[q + 3], S[ctr + q]);
 #pragma unroll
-  for (int q = -3; q <= 3; q++) sxxxx = sxxxx + a.D4[q + 3] * S[ctr + q];
+  for (int q = -3; q <= 3; q++) sxxxx = acc_(sxxxx, a.D4[q + 3], S[ctr + q]);
 #pragma unroll
-  for (int q = -2; q <= 2; q++) szz = szz + a.D2[q + 2] * S[ctr + (long)q * p];
+  for (int q = -2; q <= 2; q++) szz = acc_(szz, a.D2[q + 2], S[ctr + (long)q * p]);
 #pragma unroll
-  for (int q = -3; q <= 3; q++) if (q != 0) szzz = szzz + a.D3[q + 3] * S[ctr + (long)q * p];
+  for (int q = -3; q <= 3; q++) if (q != 0) szzz = acc_(szzz, a.D3[q + 3], S[ctr + (long)q * p]);
 #pragma unroll
-  for (int q = -3; q <= 3; q++) szzzz = szzzz + a.D4[q + 3] * S[ctr + (long)q * p];
-  R SFxx = sxx / a.dr2, SFxxx = sxxx / a.dr3, SFxxxx = sxxxx / a.dr4;
-  R SFzz = szz / a.ds2, SFzzz = szzz / a.ds3, SFzzzz = szzzz / a.ds4;
+  for (int q = -3; q <= 3; q++) szzzz = acc_(szzzz, a.D4[q + 3], S[ctr + (long)q * p]);
+  R SFxx = scale_(sxx, a.dr2, a.rdr2), SFxxx = scale_(sxxx, a.dr3, a.rdr3), SFxxxx = scale_(sxxxx, a.dr4, a.rdr4);
+  R SFzz = scale_(szz, a.ds2, a.rds2), SFzzz = scale_(szzz, a.ds3, a.rds3), SFzzzz = scale_(szzzz, a.ds4, a.rds4);
   R sxz = 0, sxxz = 0, sxzz = 0, sxxzz = 0;
 #pragma unroll
   for (int b = -2; b <= 2; b++)
 #pragma unroll
     for (int q = -2; q <= 2; q++) {
       R v = S[ctr + (long)b * p + q];
-      if (b != 0 && q != 0) sxz = sxz + a.Drs[b + 2][q + 2] * v;
-      if (b != 0) sxxz = sxxz + a.Drrs[b + 2][q + 2] * v;
-      if (q != 0) sxzz = sxzz + a.Drss[b + 2][q + 2] * v;
-      sxxzz = sxxzz + a.Drrss[b + 2][q + 2] * v;
+      if (b != 0 && q != 0) sxz = acc_(sxz, a.Drs[b + 2][q + 2], v);
+      if (b != 0) sxxz = acc_(sxxz, a.Drrs[b + 2][q + 2], v);
+      if (q != 0) sxzz = acc_(sxzz, a.Drss[b + 2][q + 2], v);
+      sxxzz = acc_(sxxzz, a.Drrss[b + 2][q + 2], v);
     }
-  R SFxz = sxz / a.drds, SFxxz = sxxz / a.dr2ds, SFxzz = sxzz / a.drds2, SFxxzz = sxxzz / a.dr2ds2;
+  R SFxz = scale_(sxz, a.drds, a.rdrds), SFxxz = scale_(sxxz, a.dr2ds, a.rdr2ds), SFxzz = scale_(sxzz, a.drds2, a.rdrds2),
+    SFxxzz = scale_(sxxzz, a.dr2ds2, a.rdr2ds2);
   R rhs = a.rhs[g];
   R vis = a.vis[g];
   R sv;
@@ -888,6 +900,10 @@ static StageArgs<R> stage_base(Context& c, int l, bool fp32) {
   a.sz = (R)(c.P.Vbc == 0 ? -1.0 : 1.0);
   a.dr2 = (R)g.dr2; a.dr3 = (R)g.dr3; a.dr4 = (R)g.dr4; a.ds2 = (R)g.ds2; a.ds3 = (R)g.ds3; a.ds4 = (R)g.ds4;
   a.drds = (R)g.drds; a.dr2ds = (R)g.dr2ds; a.drds2 = (R)g.drds2; a.dr2ds2 = (R)g.dr2ds2;
+  a.rdr2 = (R)(1.0 / g.dr2); a.rdr3 = (R)(1.0 / g.dr3); a.rdr4 = (R)(1.0 / g.dr4);
+  a.rds2 = (R)(1.0 / g.ds2); a.rds3 = (R)(1.0 / g.ds3); a.rds4 = (R)(1.0 / g.ds4);
+  a.rdrds = (R)(1.0 / g.drds); a.rdr2ds = (R)(1.0 / g.dr2ds); a.rdrds2 = (R)(1.0 / g.drds2);
+  a.rdr2ds2 = (R)(1.0 / g.dr2ds2);
   for (int q = 0; q < 5; q++) { a.D1[q] = (R)c.st.D1[q]; a.D2[q] = (R)c.st.D2[q]; }
   for (int q = 0; q < 7; q++) { a.D3[q] = (R)c.st.D3[q]; a.D4[q] = (R)c.st.D4[q]; }
   for (int b = 0; b < 5; b++)
