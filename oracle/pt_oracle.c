@@ -1692,23 +1692,40 @@ void pto_mzran(int64_t n, double* out) {
   mz_state st = {521288629, 362436069, 16163801, 1131199299};
   for (int64_t j = 0; j < n; j++) out[j] = mzran(&st);
 }
-void pto_initial_positions(pto_case* c, double* rtr, double* str) {
-  mz_state st = {521288629, 362436069, 16163801, 1131199299};
+/* initialize_tracers positions (PS/composition.f90:786-847): node columns
+ * split into nthreads contiguous ranges (:790-802), each range drawing from
+ * its own mzran stream; stream j's seeds derive from stream j-1's initial
+ * seeds as in PS/basics.f90:325-332 */
+void pto_initial_positions(pto_case* c, int nthreads, double* rtr, double* str) {
+  if (nthreads < 1) nthreads = 1;
+  mz_state seed = {521288629, 362436069, 16163801, 1131199299};
+  int nx_start = 0;
   int64_t id = 0;
-  for (int i = 0; i <= c->nx; i++) {
-    double rcell = c->dr * (double)i;
-    for (int k = 0; k <= c->nz; k++) {
-      double scell = c->ds * (double)k;
-      for (int p = 1; p <= c->P.tpc; p++) {
-        double pr = mzran(&st), ps = mzran(&st);
-        pr = (pr - 0.5) * c->dr; ps = (ps - 0.5) * c->ds;
-        double r = rcell + pr, s = scell + ps;
-        if (r <= 0.0) r = 0.0;
-        if (r >= c->aspect) r = c->aspect;
-        if (s <= 0.0) s = 0.0;
-        if (s >= 1.0) s = 1.0;
-        rtr[id] = r; str[id] = s; id++;
+  for (int rank = 0; rank < nthreads; rank++) {
+    int nx_finish = (rank == nthreads - 1) ? c->nx : nx_start + ((c->nx + 1) / nthreads - 1);
+    mz_state st = seed;
+    for (int i = nx_start; i <= nx_finish; i++) {
+      double rcell = c->dr * (double)i;
+      for (int k = 0; k <= c->nz; k++) {
+        double scell = c->ds * (double)k;
+        for (int p = 1; p <= c->P.tpc; p++) {
+          double pr = mzran(&st), ps = mzran(&st);
+          pr = (pr - 0.5) * c->dr; ps = (ps - 0.5) * c->ds;
+          double r = rcell + pr, s = scell + ps;
+          if (r <= 0.0) r = 0.0;
+          if (r >= c->aspect) r = c->aspect;
+          if (s <= 0.0) s = 0.0;
+          if (s >= 1.0) s = 1.0;
+          rtr[id] = r; str[id] = s; id++;
+        }
       }
     }
+    nx_start = nx_finish + 1;
+    int32_t ran_int = seed.i - seed.k;
+    if (ran_int < 0) ran_int = ran_int + 2147483579;
+    mz_state next;
+    next.i = seed.j; next.j = seed.k; next.k = ran_int;
+    next.n = (int32_t)((uint32_t)69069 * (uint32_t)seed.n + (uint32_t)1013904243);
+    seed = next;
   }
 }

@@ -63,9 +63,9 @@ int  pto_interpolate(pto_case* c, int32_t field_id, int64_t n, const double* r,
 int  pto_band_lu_solve(int n, int kl, int ku, double* ab, int32_t* ipiv, double* b);
 /* mzran stream 0 (PS/composition.f90:7-20 with seeds PS/basics.f90:321-324) */
 void pto_mzran(int64_t n, double* out);
-/* initialize_tracers with one OpenMP thread (PS/composition.f90:786-847):
- * positions only; ntr = (nx+1)(nz+1)tpc */
-void pto_initial_positions(pto_case* c, double* rtr, double* str);
+/* initialize_tracers positions as the reference draws them with nthreads
+ * OpenMP threads (PS/composition.f90:786-847); ntr = (nx+1)(nz+1)tpc */
+void pto_initial_positions(pto_case* c, int nthreads, double* rtr, double* str);
 
 #ifdef __cplusplus
 }

@@ -124,8 +124,8 @@ def _finish(name, p, T_nodes, r, s, Ttr, Ctr):
 def isoviscous(nx=64, nz=64, tpc=20, **over) -> Case:
     """Config: lambda=1, Ra_T=1e5, eta=1; T=(1-z)+0.01cos(pi x)sin(pi z)
     (initial_temp=2, PS/initial_conditions.f90:60-61) evaluated with libm."""
-    p = default_params(aspect=1.0, nx=nx, nz=nz, tpc=tpc, RaT=1e5, RaC=0.0,
-                       ivis=0, **{"courant": 0.5, "grids": 6, **over})
+    p = default_params(nx=nx, nz=nz, tpc=tpc, RaT=1e5, RaC=0.0,
+                       ivis=0, **{"aspect": 1.0, "courant": 0.5, "grids": 6, **over})
     aspect = p.aspect
     dr, ds = aspect / nx, 1.0 / nz
 

@@ -59,7 +59,7 @@ def oracle_lib():
             "pto_interpolate": (C.c_int, [vp, C.c_int32, C.c_int64, dp, dp, dp]),
             "pto_band_lu_solve": (C.c_int, [C.c_int, C.c_int, C.c_int, dp, ip, dp]),
             "pto_mzran": (None, [C.c_int64, dp]),
-            "pto_initial_positions": (None, [vp, dp, dp]),
+            "pto_initial_positions": (None, [vp, C.c_int, dp, dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
