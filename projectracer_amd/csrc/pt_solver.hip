@@ -51,7 +51,6 @@ struct StageArgs {
   R rdr2, rdr3, rdr4, rds2, rds3, rds4, rdrds, rdr2ds, rdrds2, rdr2ds2;
   R D1[5], D2[5], D3[7], D4[7];
   R Drs[5][5], Drss[5][5], Drrs[5][5], Drrss[5][5];
-  const int* halt;  // device flag: skip work when nonzero (may be null)
 };
 
 // Accumulation in the reference's order: separate multiply and add for the
@@ -67,99 +66,79 @@ constexpr int TX = 64, TZ = 4, PZ = 2;
 constexpr size_t SMALL_LDS_BYTES = 160 * 1024;
 constexpr int TW = TX + 6, TH = TZ * PZ + 6;
 
+// pointwise operands of stream_space at one point (read once per launch)
 template <typename R>
-__device__ __forceinline__ R stream_space_tile(const R (*t)[TW], int ti, int tk, const StageArgs<R>& a, long g) {
-  // stream_derivatives (uniform grid), reference summation order, zero stencil weights skipped
-  R sxx = 0, szz = 0, sxxx = 0, szzz = 0, sxxxx = 0, szzzz = 0;
-#pragma unroll
-  for (int q = -2; q <= 2; q++) sxx = acc_(sxx, a.D2[q + 2], t[tk][ti + q]);
-#pragma unroll
-  for (int q = -3; q <= 3; q++) if (q != 0) sxxx = acc_(sxxx, a.D3[q + 3], t[tk][ti + q]);
-#pragma unroll
-  for (int q = -3; q <= 3; q++) sxxxx = acc_(sxxxx, a.D4[q + 3], t[tk][ti + q]);
-#pragma unroll
-  for (int q = -2; q <= 2; q++) szz = acc_(szz, a.D2[q + 2], t[tk + q][ti]);
-#pragma unroll
-  for (int q = -3; q <= 3; q++) if (q != 0) szzz = acc_(szzz, a.D3[q + 3], t[tk + q][ti]);
-#pragma unroll
-  for (int q = -3; q <= 3; q++) szzzz = acc_(szzzz, a.D4[q + 3], t[tk + q][ti]);
-  R SFxx = scale_(sxx, a.dr2, a.rdr2), SFxxx = scale_(sxxx, a.dr3, a.rdr3), SFxxxx = scale_(sxxxx, a.dr4, a.rdr4);
-  R SFzz = scale_(szz, a.ds2, a.rds2), SFzzz = scale_(szzz, a.ds3, a.rds3), SFzzzz = scale_(szzzz, a.ds4, a.rds4);
-  R sxz = 0, sxxz = 0, sxzz = 0, sxxzz = 0;
-#pragma unroll
-  for (int b = -2; b <= 2; b++)
-#pragma unroll
-    for (int q = -2; q <= 2; q++) {
-      R v = t[tk + b][ti + q];
-      if (b != 0 && q != 0) sxz = acc_(sxz, a.Drs[b + 2][q + 2], v);
-      if (b != 0) sxxz = acc_(sxxz, a.Drrs[b + 2][q + 2], v);
-      if (q != 0) sxzz = acc_(sxzz, a.Drss[b + 2][q + 2], v);
-      sxxzz = acc_(sxxzz, a.Drrss[b + 2][q + 2], v);
-    }
-  R SFxz = scale_(sxz, a.drds, a.rdrds), SFxxz = scale_(sxxz, a.dr2ds, a.rdr2ds), SFxzz = scale_(sxzz, a.drds2, a.rdrds2),
-    SFxxzz = scale_(sxxzz, a.dr2ds2, a.rdr2ds2);
-  R rhs = a.rhs[g];
-  R vis = a.vis[g];
-  R s;
-  if (a.ivis0) {
-    R t2 = vis * (SFxxxx + SFzzzz + (R)2 * SFxxzz);
-    s = t2 - rhs;
+struct PointCoef {
+  R rhs, vis, vxm, vxz, vx, vz;
+};
+
+template <typename R, bool IV0>
+__device__ __forceinline__ PointCoef<R> load_pc(const StageArgs<R>& a, long g) {
+  PointCoef<R> c;
+  c.rhs = a.rhs[g];
+  c.vis = a.vis[g];
+  if constexpr (!IV0) {
+    c.vxm = a.vxm[g]; c.vxz = a.vxz[g]; c.vx = a.vx[g]; c.vz = a.vz[g];
   } else {
-    R t1 = a.vxm[g] * (SFxx - SFzz);
-    R t2 = vis * (SFxxxx + SFzzzz + (R)2 * SFxxzz);
-    R t4 = ((R)4 * a.vxz[g]) * SFxz;
-    R t5 = ((R)2 * a.vx[g]) * (SFxxx + SFxzz);
-    R t6 = ((R)2 * a.vz[g]) * (SFzzz + SFxxz);
-    s = t1 + t2 + t4 + t5 + t6 - rhs;
+    c.vxm = c.vxz = c.vx = c.vz = (R)0;
   }
-  return -s;
+  return c;
 }
 
-// stream_space on a flat array with pitch p (centre index ctr): same
-// arithmetic and order as stream_space_tile
-template <typename R>
-__device__ __forceinline__ R stream_space_flat(const R* S, int p, long ctr, const StageArgs<R>& a, long g) {
+// stream_derivatives (uniform grid) + stream_space at one point, reference
+// summation order per derivative, zero stencil weights skipped.  `at(b, q)`
+// reads psi(i+q, k+b).  With IV0 (isoviscous, eta_x = eta_z = eta_xz =
+// eta_xx-eta_zz = 0) only the three derivatives stream_space uses are formed;
+// each is accumulated exactly as in the general case.
+template <typename R, bool IV0, typename At>
+__device__ __forceinline__ R stream_space_pt(At at, const StageArgs<R>& a, const PointCoef<R>& pc) {
   R sxx = 0, szz = 0, sxxx = 0, szzz = 0, sxxxx = 0, szzzz = 0;
+  if constexpr (!IV0) {
 #pragma unroll
-  for (int q = -2; q <= 2; q++) sxx = acc_(sxx, a.D2[q + 2], S[ctr + q]);
+    for (int q = -2; q <= 2; q++) sxx = acc_(sxx, a.D2[q + 2], at(0, q));
 #pragma unroll
-  for (int q = -3; q <= 3; q++) if (q != 0) sxxx = acc_(sxxx, a.D3[q + 3], S[ctr + q]);
+    for (int q = -3; q <= 3; q++) if (q != 0) sxxx = acc_(sxxx, a.D3[q + 3], at(0, q));
+  }
 #pragma unroll
-  for (int q = -3; q <= 3; q++) sxxxx = acc_(sxxxx, a.D4[q + 3], S[ctr + q]);
+  for (int q = -3; q <= 3; q++) sxxxx = acc_(sxxxx, a.D4[q + 3], at(0, q));
+  if constexpr (!IV0) {
 #pragma unroll
-  for (int q = -2; q <= 2; q++) szz = acc_(szz, a.D2[q + 2], S[ctr + (long)q * p]);
+    for (int q = -2; q <= 2; q++) szz = acc_(szz, a.D2[q + 2], at(q, 0));
 #pragma unroll
-  for (int q = -3; q <= 3; q++) if (q != 0) szzz = acc_(szzz, a.D3[q + 3], S[ctr + (long)q * p]);
+    for (int q = -3; q <= 3; q++) if (q != 0) szzz = acc_(szzz, a.D3[q + 3], at(q, 0));
+  }
 #pragma unroll
-  for (int q = -3; q <= 3; q++) szzzz = acc_(szzzz, a.D4[q + 3], S[ctr + (long)q * p]);
-  R SFxx = scale_(sxx, a.dr2, a.rdr2), SFxxx = scale_(sxxx, a.dr3, a.rdr3), SFxxxx = scale_(sxxxx, a.dr4, a.rdr4);
-  R SFzz = scale_(szz, a.ds2, a.rds2), SFzzz = scale_(szzz, a.ds3, a.rds3), SFzzzz = scale_(szzzz, a.ds4, a.rds4);
+  for (int q = -3; q <= 3; q++) szzzz = acc_(szzzz, a.D4[q + 3], at(q, 0));
   R sxz = 0, sxxz = 0, sxzz = 0, sxxzz = 0;
 #pragma unroll
   for (int b = -2; b <= 2; b++)
 #pragma unroll
     for (int q = -2; q <= 2; q++) {
-      R v = S[ctr + (long)b * p + q];
-      if (b != 0 && q != 0) sxz = acc_(sxz, a.Drs[b + 2][q + 2], v);
-      if (b != 0) sxxz = acc_(sxxz, a.Drrs[b + 2][q + 2], v);
-      if (q != 0) sxzz = acc_(sxzz, a.Drss[b + 2][q + 2], v);
+      R v = at(b, q);
+      if constexpr (!IV0) {
+        if (b != 0 && q != 0) sxz = acc_(sxz, a.Drs[b + 2][q + 2], v);
+        if (b != 0) sxxz = acc_(sxxz, a.Drrs[b + 2][q + 2], v);
+        if (q != 0) sxzz = acc_(sxzz, a.Drss[b + 2][q + 2], v);
+      }
       sxxzz = acc_(sxxzz, a.Drrss[b + 2][q + 2], v);
     }
-  R SFxz = scale_(sxz, a.drds, a.rdrds), SFxxz = scale_(sxxz, a.dr2ds, a.rdr2ds), SFxzz = scale_(sxzz, a.drds2, a.rdrds2),
-    SFxxzz = scale_(sxxzz, a.dr2ds2, a.rdr2ds2);
-  R rhs = a.rhs[g];
-  R vis = a.vis[g];
+  R SFxxxx = scale_(sxxxx, a.dr4, a.rdr4), SFzzzz = scale_(szzzz, a.ds4, a.rds4);
+  R SFxxzz = scale_(sxxzz, a.dr2ds2, a.rdr2ds2);
   R sv;
-  if (a.ivis0) {
-    R t2 = vis * (SFxxxx + SFzzzz + (R)2 * SFxxzz);
-    sv = t2 - rhs;
+  if constexpr (IV0) {
+    R t2 = pc.vis * (SFxxxx + SFzzzz + (R)2 * SFxxzz);
+    sv = t2 - pc.rhs;
   } else {
-    R t1 = a.vxm[g] * (SFxx - SFzz);
-    R t2 = vis * (SFxxxx + SFzzzz + (R)2 * SFxxzz);
-    R t4 = ((R)4 * a.vxz[g]) * SFxz;
-    R t5 = ((R)2 * a.vx[g]) * (SFxxx + SFxzz);
-    R t6 = ((R)2 * a.vz[g]) * (SFzzz + SFxxz);
-    sv = t1 + t2 + t4 + t5 + t6 - rhs;
+    R SFxx = scale_(sxx, a.dr2, a.rdr2), SFxxx = scale_(sxxx, a.dr3, a.rdr3);
+    R SFzz = scale_(szz, a.ds2, a.rds2), SFzzz = scale_(szzz, a.ds3, a.rds3);
+    R SFxz = scale_(sxz, a.drds, a.rdrds), SFxxz = scale_(sxxz, a.dr2ds, a.rdr2ds);
+    R SFxzz = scale_(sxzz, a.drds2, a.rdrds2);
+    R t1 = pc.vxm * (SFxx - SFzz);
+    R t2 = pc.vis * (SFxxxx + SFzzzz + (R)2 * SFxxzz);
+    R t4 = ((R)4 * pc.vxz) * SFxz;
+    R t5 = ((R)2 * pc.vx) * (SFxxx + SFxzz);
+    R t6 = ((R)2 * pc.vz) * (SFzzz + SFxxz);
+    sv = t1 + t2 + t4 + t5 + t6 - pc.rhs;
   }
   return -sv;
 }
@@ -195,51 +174,69 @@ __device__ __forceinline__ void store_with_ghosts(R* Y, int i, int k, int n, int
   }
 }
 
-template <typename R>
+// one RKC1 stage on a TX x (TZ*PZ) tile; the pointwise operands are loaded
+// before the tile barrier so their latency overlaps the tile fill
+template <typename R, bool IV0>
 __global__ void __launch_bounds__(TX* TZ) k_stage(StageArgs<R> a) {
-  if (a.halt && *a.halt) return;
   __shared__ R tile[TH][TW];
   const int i0 = 1 + blockIdx.x * TX, k0 = 1 + blockIdx.y * TZ * PZ;
   load_tile<R>(tile, a.Sx, i0, k0, a.n, a.m, a.pitch);
+  R dt[PZ], s0[PZ], s2[PZ];
+  PointCoef<R> pc[PZ];
+  const int i = i0 + threadIdx.x;
+#pragma unroll
+  for (int pz = 0; pz < PZ; pz++) {
+    int k = k0 + threadIdx.y + pz * TZ;
+    dt[pz] = s0[pz] = s2[pz] = (R)0;
+    if (i > a.n - 1 || k > a.m - 1) continue;
+    long g = IDX(i, k, a.pitch);
+    dt[pz] = a.dts[g];
+    if (!a.s0zero) s0[pz] = a.S0[g];
+    if (!a.stage1) s2[pz] = a.S2[g];
+    pc[pz] = load_pc<R, IV0>(a, g);
+  }
   __syncthreads();
 #pragma unroll
   for (int pz = 0; pz < PZ; pz++) {
-    int i = i0 + threadIdx.x, k = k0 + threadIdx.y + pz * TZ;
+    int k = k0 + threadIdx.y + pz * TZ;
     if (i > a.n - 1 || k > a.m - 1) continue;
-    long g = IDX(i, k, a.pitch);
     int ti = threadIdx.x + 3, tk = threadIdx.y + pz * TZ + 3;
-    R F = stream_space_tile<R>(tile, ti, tk, a, g);
-    R dt = a.dts[g];
-    R s0 = a.s0zero ? (R)0 : a.S0[g];
+    R F = stream_space_pt<R, IV0>([&](int b, int q) { return tile[tk + b][ti + q]; }, a, pc[pz]);
     R y;
     if (a.stage1) {
-      y = s0 + (a.cf * dt) * F;
+      y = s0[pz] + (a.cf * dt[pz]) * F;
     } else {
       R s1 = tile[tk][ti];
-      R s2 = a.S2[g];
-      y = ((a.c0 * s0 + a.c1 * s1) + a.c2 * s2) + ((a.cf * dt) * F);
+      y = ((a.c0 * s0[pz] + a.c1 * s1) + a.c2 * s2[pz]) + ((a.cf * dt[pz]) * F);
     }
     store_with_ghosts<R>(a.Y, i, k, a.n, a.m, a.pitch, y, a.sz);
   }
 }
 
 // residual(i,k) = stream_space(i,k,S); block partial sums of |residual|
-template <typename R>
+template <typename R, bool IV0>
 __global__ void __launch_bounds__(TX* TZ) k_residual(StageArgs<R> a, R* res, double* partial) {
-  if (a.halt && *a.halt) return;
   __shared__ R tile[TH][TW];
   __shared__ double red[TX * TZ];
   const int i0 = 1 + blockIdx.x * TX, k0 = 1 + blockIdx.y * TZ * PZ;
   load_tile<R>(tile, a.Sx, i0, k0, a.n, a.m, a.pitch);
+  PointCoef<R> pc[PZ];
+  const int i = i0 + threadIdx.x;
+#pragma unroll
+  for (int pz = 0; pz < PZ; pz++) {
+    int k = k0 + threadIdx.y + pz * TZ;
+    if (i > a.n - 1 || k > a.m - 1) continue;
+    pc[pz] = load_pc<R, IV0>(a, IDX(i, k, a.pitch));
+  }
   __syncthreads();
   double acc = 0.0;
 #pragma unroll
   for (int pz = 0; pz < PZ; pz++) {
-    int i = i0 + threadIdx.x, k = k0 + threadIdx.y + pz * TZ;
+    int k = k0 + threadIdx.y + pz * TZ;
     if (i > a.n - 1 || k > a.m - 1) continue;
-    long g = IDX(i, k, a.pitch);
-    R F = stream_space_tile<R>(tile, threadIdx.x + 3, threadIdx.y + pz * TZ + 3, a, g);
-    res[g] = F;
+    int ti = threadIdx.x + 3, tk = threadIdx.y + pz * TZ + 3;
+    R F = stream_space_pt<R, IV0>([&](int b, int q) { return tile[tk + b][ti + q]; }, a, pc[pz]);
+    res[IDX(i, k, a.pitch)] = F;
     acc = acc + fabs((double)F);
   }
   int tid = threadIdx.y * TX + threadIdx.x;
@@ -254,38 +251,61 @@ __global__ void __launch_bounds__(TX* TZ) k_residual(StageArgs<R> a, R* res, dou
 
 // ---------------------------------------------------------------------------
 // small levels: the whole RKC1 sweep (all stages) and compute_residual in one
-// workgroup with the three RKC buffers resident in LDS
+// workgroup with the three RKC buffers resident in LDS.  Each thread owns up
+// to PPT interior points; their indices and pointwise operands are fixed for
+// the sweep and kept in registers.
 // ---------------------------------------------------------------------------
 struct RkcCoef {
   double mewt[64], mew[64], vv[64], cc[64];
 };
+constexpr int SMALL_THREADS = 1024;
 
-template <typename R>
-__global__ void __launch_bounds__(1024) k_smooth_small(StageArgs<R> a, R* err, R* res, int nstage, double cpar,
-                                                      RkcCoef co) {
+template <typename R, bool IV0, int PPT>
+__global__ void __launch_bounds__(SMALL_THREADS) k_smooth_small(StageArgs<R> a, R* err, R* res, int nstage,
+                                                               double cpar, RkcCoef co) {
   extern __shared__ unsigned char smem_raw[];
   R* buf0 = reinterpret_cast<R*>(smem_raw);
   const int n = a.n, m = a.m, p = a.pitch;
   const int ne = p * (m + 7);
   R* buf1 = buf0 + ne;
   R* buf2 = buf1 + ne;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for (int e = tid; e < ne; e += nt) {
+  const int tid = threadIdx.x;
+  for (int e = tid; e < ne; e += SMALL_THREADS) {
     buf0[e] = err[e];
     buf1[e] = (R)0;
     buf2[e] = (R)0;
   }
+  const int nxi = n - 1, npts = nxi * (m - 1);
+  int pi[PPT], pk[PPT], pg[PPT];
+  R pdt[PPT];
+  PointCoef<R> pc[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; q++) {
+    int e = tid + q * SMALL_THREADS;
+    pi[q] = 0;
+    pk[q] = 0;
+    pg[q] = 0;
+    pdt[q] = (R)0;
+    if (e < npts) {
+      int k = e / nxi;
+      pi[q] = 1 + (e - k * nxi);
+      pk[q] = 1 + k;
+      pg[q] = (int)IDX(pi[q], pk[q], p);
+      pdt[q] = a.dts[pg[q]];
+      pc[q] = load_pc<R, IV0>(a, pg[q]);
+    }
+  }
   __syncthreads();
-  const int npts = (n - 1) * (m - 1);
   // stage 1: S1 = S0 + mewt1*c*dt*F(S0) -> buf1
   {
     const R cf = (R)(co.mewt[1] * cpar);
-    for (int e = tid; e < npts; e += nt) {
-      int i = 1 + e % (n - 1), k = 1 + e / (n - 1);
-      long g = IDX(i, k, p);
-      R F = stream_space_flat<R>(buf0, p, g, a, g);
-      R y = buf0[g] + (cf * a.dts[g]) * F;
-      store_with_ghosts<R>(buf1, i, k, n, m, p, y, a.sz);
+#pragma unroll
+    for (int q = 0; q < PPT; q++) {
+      if (pi[q] == 0) continue;
+      const int g = pg[q];
+      R F = stream_space_pt<R, IV0>([&](int b, int c) { return buf0[g + b * p + c]; }, a, pc[q]);
+      R y = buf0[g] + (cf * pdt[q]) * F;
+      store_with_ghosts<R>(buf1, pi[q], pk[q], n, m, p, y, a.sz);
     }
   }
   __syncthreads();
@@ -295,12 +315,13 @@ __global__ void __launch_bounds__(1024) k_smooth_small(StageArgs<R> a, R* err, R
   for (int j = 2; j <= nstage; j++) {
     R* out = (prev == buf0) ? spare : prev;
     const R c0 = (R)co.cc[j], c1 = (R)co.mew[j], c2 = (R)co.vv[j], cf = (R)(co.mewt[j] * cpar);
-    for (int e = tid; e < npts; e += nt) {
-      int i = 1 + e % (n - 1), k = 1 + e / (n - 1);
-      long g = IDX(i, k, p);
-      R F = stream_space_flat<R>(cur, p, g, a, g);
-      R y = ((c0 * buf0[g] + c1 * cur[g]) + c2 * prev[g]) + ((cf * a.dts[g]) * F);
-      store_with_ghosts<R>(out, i, k, n, m, p, y, a.sz);
+#pragma unroll
+    for (int q = 0; q < PPT; q++) {
+      if (pi[q] == 0) continue;
+      const int g = pg[q];
+      R F = stream_space_pt<R, IV0>([&](int b, int c) { return cur[g + b * p + c]; }, a, pc[q]);
+      R y = ((c0 * buf0[g] + c1 * cur[g]) + c2 * prev[g]) + ((cf * pdt[q]) * F);
+      store_with_ghosts<R>(out, pi[q], pk[q], n, m, p, y, a.sz);
     }
     __syncthreads();
     R* np = cur;
@@ -309,11 +330,12 @@ __global__ void __launch_bounds__(1024) k_smooth_small(StageArgs<R> a, R* err, R
     if (out == spare) spare = nullptr;
   }
   // cur holds the sweep result (psi correction): write back, then the residual
-  for (int e = tid; e < ne; e += nt) err[e] = cur[e];
-  for (int e = tid; e < npts; e += nt) {
-    int i = 1 + e % (n - 1), k = 1 + e / (n - 1);
-    long g = IDX(i, k, p);
-    res[g] = stream_space_flat<R>(cur, p, g, a, g);
+  for (int e = tid; e < ne; e += SMALL_THREADS) err[e] = cur[e];
+#pragma unroll
+  for (int q = 0; q < PPT; q++) {
+    if (pi[q] == 0) continue;
+    const int g = pg[q];
+    res[g] = stream_space_pt<R, IV0>([&](int b, int c) { return cur[g + b * p + c]; }, a, pc[q]);
   }
 }
 
@@ -362,9 +384,7 @@ __global__ void k_coarse_inverse(const double* ab, const int32_t* ipiv, const do
 
 // coarse correction: err(p) = sum_q G(p,q) rhs(q), fixed order in q
 template <typename RI, typename RO>
-__global__ void k_coarse_apply(const double* G, const RI* rhs, RO* err, int nxc, int nzc, int pitch, RO sz,
-                               const int* halt) {
-  if (halt && *halt) return;
+__global__ void k_coarse_apply(const double* G, const RI* rhs, RO* err, int nxc, int nzc, int pitch, RO sz) {
   extern __shared__ double rv[];
   const int nci = (nxc - 1) * (nzc - 1);
   for (int q = threadIdx.x; q < nci; q += blockDim.x) {
@@ -835,56 +855,6 @@ __global__ void __launch_bounds__(64) k_band_lu(double* ab, int32_t* ipiv, int n
 #undef AB
 }
 
-// Compute_coarse_vector + DGBTRS('N') + Stream_grid_coarse + enforceBCs_stream
-template <typename RO>
-__global__ void __launch_bounds__(64) k_coarse_solve(const double* ab, const int32_t* ipiv, const double* RowB,
-                                                     const void* rhs_in, int rhs_is_float, RO* err, int n, int kl,
-                                                     int ku, int nxc, int nzc, int p, RO sz, const int* halt) {
-  if (halt && *halt) return;
-  __shared__ double b[4096];
-  const int lane = threadIdx.x, ld = 2 * kl + ku + 1, kd = ku + kl + 1, N = nzc + 1;
-  for (int q = lane; q < n; q += 64) {
-    int kq = q % N, iq = q / N;
-    double v = 0.0;
-    if (iq > 0 && iq < nxc && kq > 0 && kq < nzc) {
-      double r = rhs_is_float ? (double)((const float*)rhs_in)[IDX(iq, kq, p)] : ((const double*)rhs_in)[IDX(iq, kq, p)];
-      v = r * RowB[q];
-    }
-    b[q] = v;
-  }
-  __syncthreads();
-#define AB(r, q) ab[(long)((q) - 1) * ld + ((r) - 1)]
-  for (int j = 1; j <= n - 1; j++) {
-    int lm = std::min(kl, n - j);
-    int l = ipiv[j - 1];
-    if (l != j) {
-      if (lane == 0) { double t = b[l - 1]; b[l - 1] = b[j - 1]; b[j - 1] = t; }
-      __syncthreads();
-    }
-    double temp = -1.0 * b[j - 1];
-    for (int t = 1 + lane; t <= lm; t += 64) b[j - 1 + t] = b[j - 1 + t] + AB(kd + t, j) * temp;
-    __syncthreads();
-  }
-  const int kk = kl + ku, kp1 = kk + 1;
-  for (int j = n; j >= 1; j--) {
-    double bj = b[j - 1];
-    __syncthreads();
-    if (bj != 0.0) {
-      int l = kp1 - j;
-      double x = bj / AB(kp1, j);
-      if (lane == 0) b[j - 1] = x;
-      int lo = std::max(1, j - kk);
-      for (int i = j - 1 - lane; i >= lo; i -= 64) b[i - 1] = b[i - 1] - x * AB(l + i, j);
-    }
-    __syncthreads();
-  }
-#undef AB
-  for (int q = lane; q < n; q += 64) {
-    int kq = q % N, iq = q / N;
-    if (iq > 0 && iq < nxc && kq > 0 && kq < nzc) store_with_ghosts<RO>(err, iq, kq, nxc, nzc, p, (RO)b[q], sz);
-  }
-}
-
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
@@ -919,8 +889,56 @@ static StageArgs<R> stage_base(Context& c, int l, bool fp32) {
     a.vis = (const R*)L.fvis; a.vxm = (const R*)L.fvxm; a.vxz = (const R*)L.fvxz;
     a.vx = (const R*)L.fvx; a.vz = (const R*)L.fvz; a.rhs = (const R*)L.frhs; a.dts = (const R*)L.fdts;
   }
-  a.halt = nullptr;
   return a;
+}
+
+template <typename R, bool IV0>
+static void launch_stage(Context& c, dim3 grd, dim3 blk, const StageArgs<R>& a) {
+  hipLaunchKernelGGL((k_stage<R, IV0>), grd, blk, 0, c.stream, a);
+}
+template <typename R>
+static void launch_stage(Context& c, dim3 grd, dim3 blk, const StageArgs<R>& a) {
+  if (c.P.ivis == 0) launch_stage<R, true>(c, grd, blk, a);
+  else launch_stage<R, false>(c, grd, blk, a);
+}
+template <typename R>
+static void launch_residual(Context& c, dim3 grd, dim3 blk, const StageArgs<R>& a, R* res, double* partial) {
+  if (c.P.ivis == 0) hipLaunchKernelGGL((k_residual<R, true>), grd, blk, 0, c.stream, a, res, partial);
+  else hipLaunchKernelGGL((k_residual<R, false>), grd, blk, 0, c.stream, a, res, partial);
+}
+
+// points per thread of the single-workgroup smoother, 0 when the level is
+// too large for it (beyond 4 points, 2 with the full variable-viscosity
+// operator, the per-thread state no longer fits in 128 VGPRs)
+static int small_ppt(int n, int m, bool iv0) {
+  const int npts = (n - 1) * (m - 1);
+  const int pmax = iv0 ? 4 : 2;
+  for (int ppt = 1; ppt <= pmax; ppt *= 2)
+    if (npts <= ppt * SMALL_THREADS) return ppt;
+  return 0;
+}
+
+template <typename R, bool IV0, int PPT>
+static int launch_small_t(Context& c, size_t lds, const StageArgs<R>& a, R* err, R* res, int s, double cc,
+                          const RkcCoef& co) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    PT_HIP(hipFuncSetAttribute((const void*)k_smooth_small<R, IV0, PPT>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)SMALL_LDS_BYTES));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_smooth_small<R, IV0, PPT>), dim3(1), dim3(SMALL_THREADS), lds, c.stream, a, err, res, s, cc,
+                     co);
+  return PT_OK;
+}
+template <typename R, bool IV0>
+static int launch_small_v(Context& c, int ppt, size_t lds, const StageArgs<R>& a, R* err, R* res, int s, double cc,
+                          const RkcCoef& co) {
+  switch (ppt) {
+    case 1: return launch_small_t<R, IV0, 1>(c, lds, a, err, res, s, cc, co);
+    case 2: return launch_small_t<R, IV0, 2>(c, lds, a, err, res, s, cc, co);
+    default: return launch_small_t<R, IV0, 4>(c, lds, a, err, res, s, cc, co);
+  }
 }
 
 // iterate_stream (RKC1, Nmulti sweeps) followed by compute_residual
@@ -937,14 +955,9 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, double* resmag)
   R* Cb = fp32 ? (R*)L.fC : (R*)L.bufC;
   bool timed = (l == 1) && c.timing;
   const size_t lds = 3 * L.nelem * sizeof(R);
-  if (l >= 2 && lds <= SMALL_LDS_BYTES && resmag == nullptr) {
+  const int ppt = small_ppt(g.n, g.m, c.P.ivis == 0);
+  if (l >= 2 && ppt > 0 && lds <= SMALL_LDS_BYTES && resmag == nullptr) {
     // whole sweep + residual in one workgroup, buffers in LDS
-    static bool attr_set = false;
-    if (!attr_set) {
-      PT_HIP(hipFuncSetAttribute((const void*)k_smooth_small<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)SMALL_LDS_BYTES));
-      attr_set = true;
-    }
     RkcCoef co;
     std::memcpy(co.mewt, c.T.mewt, sizeof co.mewt); std::memcpy(co.mew, c.T.mew, sizeof co.mew);
     std::memcpy(co.vv, c.T.vv, sizeof co.vv); std::memcpy(co.cc, c.T.cc, sizeof co.cc);
@@ -956,7 +969,8 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, double* resmag)
       StageArgs<R> a = stage_base<R>(c, l, fp32);
       // the residual buffer aliases fB in FP32 mode: the fused kernel keeps its
       // RKC buffers in LDS, so fB is free
-      hipLaunchKernelGGL(k_smooth_small<R>, dim3(1), dim3(1024), lds, c.stream, a, err, res, s, cc, co);
+      if (c.P.ivis == 0) PT_TRY((launch_small_v<R, true>(c, ppt, lds, a, err, res, s, cc, co)));
+      else PT_TRY((launch_small_v<R, false>(c, ppt, lds, a, err, res, s, cc, co)));
     }
     (void)p;
     return PT_OK;
@@ -969,7 +983,7 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, double* resmag)
     // stage 1: S1 = S0 + mewt1*c*dt*F(S0) -> B
     a.Sx = err; a.S0 = err; a.S2 = err; a.Y = B; a.stage1 = 1; a.cf = (R)(c.T.mewt[1] * cc);
     if (timed) timer_begin(c, PT_K_RKC_STAGE_L1);
-    hipLaunchKernelGGL(k_stage<R>, grd, blk, 0, c.stream, a);
+    launch_stage<R>(c, grd, blk, a);
     if (timed) timer_end(c, PT_K_RKC_STAGE_L1);
     // rotating buffers: S1 in `cur`, S2 in `prev`; Y goes to the S2 buffer
     // except at stage 2 (S2 aliases S0) and at the last stage (Y -> err)
@@ -986,7 +1000,7 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, double* resmag)
       else out = prev;
       a.Y = out;
       if (timed) timer_begin(c, PT_K_RKC_STAGE_L1);
-      hipLaunchKernelGGL(k_stage<R>, grd, blk, 0, c.stream, a);
+      launch_stage<R>(c, grd, blk, a);
       if (timed) timer_end(c, PT_K_RKC_STAGE_L1);
       if (j < s) {
         R* np = cur;
@@ -1001,7 +1015,7 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, double* resmag)
   a.Sx = err;
   R* res = fp32 ? (R*)L.fB : (R*)L.res;   // FP32 residual reuses the B buffer
   int nb = grd.x * grd.y;
-  hipLaunchKernelGGL(k_residual<R>, grd, blk, 0, c.stream, a, res, c.red);
+  launch_residual<R>(c, grd, blk, a, res, c.red);
   if (resmag) {
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, nb, (int)RED_SUM_ABS, c.red_out);
     PT_HIP(hipMemcpyAsync(c.h_red, c.red_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -1218,13 +1232,13 @@ int multigrid(Context& c) {
           StageArgs<double> a = stage_base<double>(c, 1, false);
           a.Sx = L.err;
           dim3 blk(TX, TZ), grd((L.g.n - 1 + TX - 1) / TX, (L.g.m - 1 + TZ * PZ - 1) / (TZ * PZ));
-          hipLaunchKernelGGL(k_residual<double>, grd, blk, 0, c.stream, a, c.dres64, c.red);
+          launch_residual<double>(c, grd, blk, a, c.dres64, c.red);
           PT_TRY(to_f32(c, c.dres64, L.frhs, L.nelem));
           PT_HIP(hipMemsetAsync(L.ferr, 0, sizeof(float) * L.nelem, c.stream));
           PT_TRY(iterate_stream_t<float>(c, 1, p, true, nullptr));
           hipLaunchKernelGGL(k_add_f32_to_f64, grid2(L.g.n - 1, L.g.m - 1, b16), b16, 0, c.stream, L.err, L.ferr,
                              L.g.n, L.g.m, L.g.pitch, sz);
-          hipLaunchKernelGGL(k_residual<double>, grd, blk, 0, c.stream, a, L.res, c.red);
+          launch_residual<double>(c, grd, blk, a, L.res, c.red);
           int nb = grd.x * grd.y;
           hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, nb, (int)RED_SUM_ABS, c.red_out);
           PT_HIP(hipMemcpyAsync(c.h_red, c.red_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -1244,10 +1258,10 @@ int multigrid(Context& c) {
         size_t sh = sizeof(double) * c.nci;
         if (!mixed)
           hipLaunchKernelGGL((k_coarse_apply<double, double>), gb, dim3(256), sh, c.stream, c.Gc, L.rhs, L.err, L.g.n,
-                             L.g.m, L.g.pitch, sz, (const int*)nullptr);
+                             L.g.m, L.g.pitch, sz);
         else
           hipLaunchKernelGGL((k_coarse_apply<float, float>), gb, dim3(256), sh, c.stream, c.Gc, L.frhs, L.ferr,
-                             L.g.n, L.g.m, L.g.pitch, (float)sz, (const int*)nullptr);
+                             L.g.n, L.g.m, L.g.pitch, (float)sz);
       }
       if (l == 1 && (resmag1 / c.res_ref1) <= c.P.tolerance) { done = true; break; }
       if (t.multi[p + 1] < t.multi[p]) {
