@@ -158,7 +158,8 @@ int pt_run_steps(void* ctx, int32_t n);
 int64_t pt_field_extent(void* ctx, int32_t field_id);
 int pt_get_field(void* ctx, int32_t field_id, double* host);
 int pt_set_field(void* ctx, int32_t field_id, const double* host);
-/* Tracers in reference ID order (the device keeps them cell-sorted). */
+/* Tracers in reference ID order (the device stores them in ID order; the
+ * projection works through a cell-sorted permutation it keeps separately). */
 int pt_get_tracers(void* ctx, double* rtr, double* str, double* Ttr, double* Ctr);
 /* nempty(0:nx,0:nz): tracers binned to each node's dual cell (INT32). */
 int pt_get_nempty(void* ctx, int32_t* nempty);

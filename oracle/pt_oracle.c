@@ -17,6 +17,9 @@
  *    shipped stdout:10 (s=35, damping 1000).
  *  - derivative_eigenvalues/stability_RK4: dt = 2.59875e-5 at 100x100,
  *    courant 0.99 from the shipped stats.dat row 2.
+ *  - tracers_to_corners, smoother_time, smoother_time_T on the initial
+ *    cloud: shipped T10000 (nempty exact, Tvis/Cnew/Cvis to 5 digits) and a
+ *    brute-force projection on a tiny cloud.
  *  - stream_derivatives/stream_space/enforceBCs_stream: sine-mode operator
  *    symbols (closed form) and flang-reference solves.
  *  - multigrid (all of a1-a15), compute_velocities, energy_time, interpolate,
@@ -25,8 +28,12 @@
  *    exactness of interpolate, projection partition of unity, restriction
  *    and prolongation of constants/linears).
  *  - band LU: scipy.linalg.lapack dgbtrf/dgbtrs golden vectors.
- *  - mzran: integer recurrence checked against the reference's initial
- *    tracer cloud produced by the flang build.
+ *  - mzran / initialize_tracers positions: the reference's own initial
+ *    clouds with 2 and 3 OpenMP threads (flang build), bit for bit, and the
+ *    shipped T10000 snapshot (2-thread cloud projected and filtered).
+ *  - parity unpinned: the insulating-bottom (Tbot=1) and rigid top/bottom
+ *    (Vbc=1) boundary branches and the ivis=1 viscosity law are restated
+ *    but no fixture or closed form covers them.
  */
 #include "pt_oracle.h"
 #include <math.h>
