@@ -339,6 +339,131 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_smooth_small(StageArgs<R> a, 
   }
 }
 
+// FP32 isoviscous sweep for the mixed-precision smoother, column-blocked:
+// each thread owns RB vertically adjacent points of one column.  The x-D2
+// sums of the RB+4 rows its 5x5 neighbourhood spans are formed once and
+// reused by every point's z-D2 sum (psi_xxzz = D2_z(D2_x psi), the product
+// weights D2(b)D2(q) factorised), and every LDS value is read once per
+// stage.  S0 stays in registers.  Factorised sums round differently from
+// the reference's 5x5 product weights, so only the FP32 path uses this.
+template <int RB>
+__global__ void __launch_bounds__(SMALL_THREADS) k_smooth_cb(StageArgs<float> a, float* err, float* res, int nstage,
+                                                            double cpar, RkcCoef co) {
+  extern __shared__ unsigned char smem_raw[];
+  float* buf0 = reinterpret_cast<float*>(smem_raw);
+  const int n = a.n, m = a.m, p = a.pitch;
+  const int ne = p * (m + 7);
+  float* buf1 = buf0 + ne;
+  float* buf2 = buf1 + ne;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < ne; e += SMALL_THREADS) {
+    buf0[e] = err[e];
+    buf1[e] = 0.0f;
+    buf2[e] = 0.0f;
+  }
+  const int nxi = n - 1, nkb = (m - 1 + RB - 1) / RB;
+  const bool active = tid < nxi * nkb;
+  int i = 1, k0 = 1;
+  float s0v[RB], dtv[RB], rhsv[RB], visv[RB];
+  bool on[RB];
+  if (active) {
+    int kb = tid / nxi;
+    i = 1 + (tid - kb * nxi);
+    k0 = 1 + kb * RB;
+  }
+#pragma unroll
+  for (int r = 0; r < RB; r++) {
+    int k = k0 + r;
+    on[r] = active && k <= m - 1;
+    s0v[r] = dtv[r] = rhsv[r] = visv[r] = 0.0f;
+    if (on[r]) {
+      long g = IDX(i, k, p);
+      s0v[r] = err[g];
+      dtv[r] = a.dts[g];
+      rhsv[r] = a.rhs[g];
+      visv[r] = a.vis[g];
+    }
+  }
+  const int rmax = m + 3;   // last stored row
+  // F = RHS - L psi at the thread's RB points, stencil values from S
+  auto eval = [&](const float* S, float* F) {
+    float v[5][RB + 4];
+#pragma unroll
+    for (int w = 0; w < RB + 4; w++) {
+      int row = min(k0 - 2 + w, rmax);
+#pragma unroll
+      for (int c = 0; c < 5; c++) v[c][w] = S[IDX(i - 2 + c, row, p)];
+    }
+    float colv[RB + 6];
+    colv[0] = S[IDX(i, k0 - 3, p)];
+#pragma unroll
+    for (int w = 0; w < RB + 4; w++) colv[w + 1] = v[2][w];
+    colv[RB + 5] = S[IDX(i, min(k0 + RB + 2, rmax), p)];
+    float t2[RB + 4];
+#pragma unroll
+    for (int w = 0; w < RB + 4; w++) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int c = 0; c < 5; c++) acc = __fmaf_rn(a.D2[c], v[c][w], acc);
+      t2[w] = acc;
+    }
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+      const int w = r + 2;
+      const int row = min(k0 + r, rmax);
+      float sxxxx = a.D4[0] * S[IDX(i - 3, row, p)];
+#pragma unroll
+      for (int c = 0; c < 5; c++) sxxxx = __fmaf_rn(a.D4[c + 1], v[c][w], sxxxx);
+      sxxxx = __fmaf_rn(a.D4[6], S[IDX(i + 3, row, p)], sxxxx);
+      float szzzz = 0.0f;
+#pragma unroll
+      for (int b = 0; b < 7; b++) szzzz = __fmaf_rn(a.D4[b], colv[r + b], szzzz);
+      float sxxzz = 0.0f;
+#pragma unroll
+      for (int b = 0; b < 5; b++) sxxzz = __fmaf_rn(a.D2[b], t2[w - 2 + b], sxxzz);
+      float lap2 = sxxxx * a.rdr4 + szzzz * a.rds4 + 2.0f * (sxxzz * a.rdr2ds2);
+      F[r] = rhsv[r] - visv[r] * lap2;
+    }
+  };
+  __syncthreads();
+  {
+    const float cf = (float)(co.mewt[1] * cpar);
+    float F[RB];
+    eval(buf0, F);
+#pragma unroll
+    for (int r = 0; r < RB; r++)
+      if (on[r]) store_with_ghosts<float>(buf1, i, k0 + r, n, m, p, s0v[r] + (cf * dtv[r]) * F[r], a.sz);
+  }
+  __syncthreads();
+  float* cur = buf1;
+  float* prev = buf0;   // S2 = S0 at stage 2
+  float* spare = buf2;
+  for (int j = 2; j <= nstage; j++) {
+    float* out = (prev == buf0) ? spare : prev;
+    const float c0 = (float)co.cc[j], c1 = (float)co.mew[j], c2 = (float)co.vv[j], cf = (float)(co.mewt[j] * cpar);
+    float F[RB];
+    eval(cur, F);
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+      if (!on[r]) continue;
+      long g = IDX(i, k0 + r, p);
+      float y = ((c0 * s0v[r] + c1 * cur[g]) + c2 * prev[g]) + ((cf * dtv[r]) * F[r]);
+      store_with_ghosts<float>(out, i, k0 + r, n, m, p, y, a.sz);
+    }
+    __syncthreads();
+    float* np = cur;
+    cur = out;
+    prev = np;
+    if (out == spare) spare = nullptr;
+  }
+  for (int e = tid; e < ne; e += SMALL_THREADS) err[e] = cur[e];
+  float F[RB];
+  eval(cur, F);
+#pragma unroll
+  for (int r = 0; r < RB; r++)
+    if (on[r]) res[IDX(i, k0 + r, p)] = F[r];
+}
+
 // ---------------------------------------------------------------------------
 // coarsest grid as a dense operator: G(:,q) = A^-1 (RowB(q) e_q) from the band
 // LU factors, interior unknowns only; one thread per column q, work vectors
@@ -941,6 +1066,31 @@ static int launch_small_v(Context& c, int ppt, size_t lds, const StageArgs<R>& a
   }
 }
 
+// rows per thread of the column-blocked FP32 sweep: 4 when that keeps at
+// least half the workgroup busy with one column block per thread, else 0
+// (smaller levels are faster with one point per thread)
+static int cb_rows(int n, int m) {
+  const int b4 = (n - 1) * ((m - 1 + 3) / 4);
+  return (b4 <= SMALL_THREADS && 2 * b4 >= SMALL_THREADS) ? 4 : 0;
+}
+template <int RB>
+static int launch_cb_t(Context& c, size_t lds, const StageArgs<float>& a, float* err, float* res, int s, double cc,
+                       const RkcCoef& co) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    PT_HIP(hipFuncSetAttribute((const void*)k_smooth_cb<RB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)SMALL_LDS_BYTES));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k_smooth_cb<RB>, dim3(1), dim3(SMALL_THREADS), lds, c.stream, a, err, res, s, cc, co);
+  return PT_OK;
+}
+static int launch_cb(Context& c, int rb, size_t lds, const StageArgs<float>& a, float* err, float* res, int s,
+                     double cc, const RkcCoef& co) {
+  (void)rb;
+  return launch_cb_t<4>(c, lds, a, err, res, s, cc, co);
+}
+
 // iterate_stream (RKC1, Nmulti sweeps) followed by compute_residual
 // (PS/iterative_solver.f90:1418-1614, 415-448).  `s0zero` runs the smoother
 // on a correction that starts from zero (mixed precision).
@@ -969,6 +1119,13 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, double* resmag)
       StageArgs<R> a = stage_base<R>(c, l, fp32);
       // the residual buffer aliases fB in FP32 mode: the fused kernel keeps its
       // RKC buffers in LDS, so fB is free
+      if constexpr (sizeof(R) == 4) {
+        const int rb = cb_rows(g.n, g.m);
+        if (c.P.ivis == 0 && rb > 0) {
+          PT_TRY(launch_cb(c, rb, lds, a, err, res, s, cc, co));
+          continue;
+        }
+      }
       if (c.P.ivis == 0) PT_TRY((launch_small_v<R, true>(c, ppt, lds, a, err, res, s, cc, co)));
       else PT_TRY((launch_small_v<R, false>(c, ppt, lds, a, err, res, s, cc, co)));
     }
@@ -1333,7 +1490,7 @@ int solver_alloc(Context& c) {
       DALLOC(s->vxz, b); DALLOC(s->vxxpzz, b); DALLOC(s->vxxmzz, b); DALLOC(s->vsmooth, b);
     }
     if (c.P.precision == PT_PREC_MIXED) {
-      DALLOC(L.ferr, f); DALLOC(L.fB, f); DALLOC(L.fC, f); DALLOC(L.fS0, f); DALLOC(L.frhs, f);
+      DALLOC(L.ferr, f); DALLOC(L.fB, f); DALLOC(L.fC, f); DALLOC(L.frhs, f);
       DALLOC(L.fdts, f); DALLOC(L.fvis, f); DALLOC(L.fvxm, f); DALLOC(L.fvxz, f); DALLOC(L.fvx, f); DALLOC(L.fvz, f);
     }
   }
@@ -1368,7 +1525,7 @@ int solver_free(Context& c) {
     Level& L = c.L[l];
     void* ps[] = {L.err, L.bufB, L.bufC, L.rhs, L.res, L.dts, L.ratio, L.vis_static, L.v.vis, L.v.vx, L.v.vz,
                   L.v.vxx, L.v.vzz, L.v.vxz, L.v.vxxpzz, L.v.vxxmzz, L.v.vsmooth, L.vs.vis, L.vs.vx, L.vs.vz,
-                  L.vs.vxx, L.vs.vzz, L.vs.vxz, L.vs.vxxpzz, L.vs.vxxmzz, L.vs.vsmooth, L.ferr, L.fB, L.fC, L.fS0,
+                  L.vs.vxx, L.vs.vzz, L.vs.vxz, L.vs.vxxpzz, L.vs.vxxmzz, L.vs.vsmooth, L.ferr, L.fB, L.fC,
                   L.frhs, L.fdts, L.fvis, L.fvxm, L.fvxz, L.fvx, L.fvz};
     for (void* p : ps)
       if (p) (void)hipFree(p);
