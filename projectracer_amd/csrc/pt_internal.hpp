@@ -72,7 +72,7 @@ struct Level {
   double *vis_static;
   VisSet v, vs;                   // primary / scratch viscosity sets
   // FP32 mirrors for the mixed-precision smoother
-  float *ferr, *fB, *fC, *frhs, *fdts, *fvis, *fvxm, *fvxz, *fvx, *fvz;
+  float *ferr, *fB, *fC, *fD, *fE, *frhs, *fdts, *fvis, *fvxm, *fvxz, *fvx, *fvz;
   size_t nelem;                   // (n+7)*(m+7)
 };
 

@@ -339,13 +339,61 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_smooth_small(StageArgs<R> a, 
   }
 }
 
-// FP32 isoviscous sweep for the mixed-precision smoother, column-blocked:
-// each thread owns RB vertically adjacent points of one column.  The x-D2
-// sums of the RB+4 rows its 5x5 neighbourhood spans are formed once and
-// reused by every point's z-D2 sum (psi_xxzz = D2_z(D2_x psi), the product
-// weights D2(b)D2(q) factorised), and every LDS value is read once per
-// stage.  S0 stays in registers.  Factorised sums round differently from
-// the reference's 5x5 product weights, so only the FP32 path uses this.
+// F = RHS - L psi (isoviscous, FP32) at the RB vertically adjacent points
+// (i, k0..k0+RB-1) of an LDS array with row pitch W whose column 0 is global
+// column `lo` and whose row 0 is global row -3.  The x-D2 sums of the RB+4
+// rows the 5x5 neighbourhood spans are formed once and reused by every
+// point's z-D2 sum (psi_xxzz = D2_z(D2_x psi), the product weights D2(b)D2(q)
+// factorised); each LDS value is read once.  Rows beyond rmax (points past
+// the last interior row of a partial block) are clamped; their F is unused.
+template <int RB>
+__device__ __forceinline__ void eval_cb(const float* S, int W, int lo, int i, int k0, int rmax,
+                                        const StageArgs<float>& a, const float* rhsv, const float* visv, float* F) {
+  auto at = [&](int ii, int kk) { return S[(kk + 3) * W + (ii - lo)]; };
+  float v[5][RB + 4];
+#pragma unroll
+  for (int w = 0; w < RB + 4; w++) {
+    const int row = min(k0 - 2 + w, rmax);
+#pragma unroll
+    for (int c = 0; c < 5; c++) v[c][w] = at(i - 2 + c, row);
+  }
+  float colv[RB + 6];
+  colv[0] = at(i, k0 - 3);
+#pragma unroll
+  for (int w = 0; w < RB + 4; w++) colv[w + 1] = v[2][w];
+  colv[RB + 5] = at(i, min(k0 + RB + 2, rmax));
+  float t2[RB + 4];
+#pragma unroll
+  for (int w = 0; w < RB + 4; w++) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 5; c++) acc = __fmaf_rn(a.D2[c], v[c][w], acc);
+    t2[w] = acc;
+  }
+#pragma unroll
+  for (int r = 0; r < RB; r++) {
+    const int w = r + 2;
+    const int row = min(k0 + r, rmax);
+    float sxxxx = a.D4[0] * at(i - 3, row);
+#pragma unroll
+    for (int c = 0; c < 5; c++) sxxxx = __fmaf_rn(a.D4[c + 1], v[c][w], sxxxx);
+    sxxxx = __fmaf_rn(a.D4[6], at(i + 3, row), sxxxx);
+    float szzzz = 0.0f;
+#pragma unroll
+    for (int b = 0; b < 7; b++) szzzz = __fmaf_rn(a.D4[b], colv[r + b], szzzz);
+    float sxxzz = 0.0f;
+#pragma unroll
+    for (int b = 0; b < 5; b++) sxxzz = __fmaf_rn(a.D2[b], t2[w - 2 + b], sxxzz);
+    const float lap2 = sxxxx * a.rdr4 + szzzz * a.rds4 + 2.0f * (sxxzz * a.rdr2ds2);
+    F[r] = rhsv[r] - visv[r] * lap2;
+  }
+}
+
+// FP32 isoviscous single-workgroup sweep for the mixed-precision smoother,
+// column-blocked: each thread owns RB vertically adjacent points of one
+// column (eval_cb); S0 and the pointwise operands stay in registers.
+// Factorised sums round differently from the reference's 5x5 product
+// weights, so only the FP32 path uses this.
 template <int RB>
 __global__ void __launch_bounds__(SMALL_THREADS) k_smooth_cb(StageArgs<float> a, float* err, float* res, int nstage,
                                                             double cpar, RkcCoef co) {
@@ -385,51 +433,11 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_smooth_cb(StageArgs<float> a,
     }
   }
   const int rmax = m + 3;   // last stored row
-  // F = RHS - L psi at the thread's RB points, stencil values from S
-  auto eval = [&](const float* S, float* F) {
-    float v[5][RB + 4];
-#pragma unroll
-    for (int w = 0; w < RB + 4; w++) {
-      int row = min(k0 - 2 + w, rmax);
-#pragma unroll
-      for (int c = 0; c < 5; c++) v[c][w] = S[IDX(i - 2 + c, row, p)];
-    }
-    float colv[RB + 6];
-    colv[0] = S[IDX(i, k0 - 3, p)];
-#pragma unroll
-    for (int w = 0; w < RB + 4; w++) colv[w + 1] = v[2][w];
-    colv[RB + 5] = S[IDX(i, min(k0 + RB + 2, rmax), p)];
-    float t2[RB + 4];
-#pragma unroll
-    for (int w = 0; w < RB + 4; w++) {
-      float acc = 0.0f;
-#pragma unroll
-      for (int c = 0; c < 5; c++) acc = __fmaf_rn(a.D2[c], v[c][w], acc);
-      t2[w] = acc;
-    }
-#pragma unroll
-    for (int r = 0; r < RB; r++) {
-      const int w = r + 2;
-      const int row = min(k0 + r, rmax);
-      float sxxxx = a.D4[0] * S[IDX(i - 3, row, p)];
-#pragma unroll
-      for (int c = 0; c < 5; c++) sxxxx = __fmaf_rn(a.D4[c + 1], v[c][w], sxxxx);
-      sxxxx = __fmaf_rn(a.D4[6], S[IDX(i + 3, row, p)], sxxxx);
-      float szzzz = 0.0f;
-#pragma unroll
-      for (int b = 0; b < 7; b++) szzzz = __fmaf_rn(a.D4[b], colv[r + b], szzzz);
-      float sxxzz = 0.0f;
-#pragma unroll
-      for (int b = 0; b < 5; b++) sxxzz = __fmaf_rn(a.D2[b], t2[w - 2 + b], sxxzz);
-      float lap2 = sxxxx * a.rdr4 + szzzz * a.rds4 + 2.0f * (sxxzz * a.rdr2ds2);
-      F[r] = rhsv[r] - visv[r] * lap2;
-    }
-  };
   __syncthreads();
   {
     const float cf = (float)(co.mewt[1] * cpar);
     float F[RB];
-    eval(buf0, F);
+    eval_cb<RB>(buf0, p, -3, i, k0, rmax, a, rhsv, visv, F);
 #pragma unroll
     for (int r = 0; r < RB; r++)
       if (on[r]) store_with_ghosts<float>(buf1, i, k0 + r, n, m, p, s0v[r] + (cf * dtv[r]) * F[r], a.sz);
@@ -442,7 +450,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_smooth_cb(StageArgs<float> a,
     float* out = (prev == buf0) ? spare : prev;
     const float c0 = (float)co.cc[j], c1 = (float)co.mew[j], c2 = (float)co.vv[j], cf = (float)(co.mewt[j] * cpar);
     float F[RB];
-    eval(cur, F);
+    eval_cb<RB>(cur, p, -3, i, k0, rmax, a, rhsv, visv, F);
 #pragma unroll
     for (int r = 0; r < RB; r++) {
       if (!on[r]) continue;
@@ -458,10 +466,134 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_smooth_cb(StageArgs<float> a,
   }
   for (int e = tid; e < ne; e += SMALL_THREADS) err[e] = cur[e];
   float F[RB];
-  eval(cur, F);
+  eval_cb<RB>(cur, p, -3, i, k0, rmax, a, rhsv, visv, F);
 #pragma unroll
   for (int r = 0; r < RB; r++)
     if (on[r]) res[IDX(i, k0 + r, p)] = F[r];
+}
+
+// ---------------------------------------------------------------------------
+// FP32 isoviscous RKC1 sweep segments on column strips (temporal blocking),
+// for levels too large for one workgroup.  Workgroup b owns the core columns
+// [c0, c1) and keeps whole columns (rows -3..m+3) of the core widened by 3
+// columns per fused stage in LDS.  Stage t of a segment is computed on the
+// core widened by 3 columns per remaining stage, so after the segment the
+// core holds its last stage with no exchange between workgroups (overlap
+// columns are computed by both neighbours).  Per point the arithmetic is
+// eval_cb's, so the result does not depend on the strip decomposition.
+// Threads: x = strip column (64 lanes), y = 16 blocks of 4 rows (m <= 65).
+// ---------------------------------------------------------------------------
+constexpr int STRIP_TX = 64, STRIP_TY = 16, STRIP_RB = 4;
+
+struct StripArgs {
+  const float* S0;    // sweep start (pointwise; stencil source of stage 1)
+  const float* Gcur;  // stage j0-1, read when j0 >= 3
+  const float* Gprev; // stage j0-2, read when j0 >= 3
+  float* Ocur;        // stage j1 on the core columns (+ ghost images)
+  float* Oprev;       // stage j1-1 on the core columns; null on the last segment
+  double cpar;        // Courant factor of the sweep
+  int j0, j1;         // stages of this segment (1-based, j1 > j0)
+  int cw;             // core columns per workgroup
+  int W;              // LDS strip width in columns
+  RkcCoef co;
+};
+
+__global__ void __launch_bounds__(STRIP_TX* STRIP_TY) k_strip_cb(StageArgs<float> a, StripArgs s) {
+  extern __shared__ unsigned char smem_raw[];
+  constexpr int RB = STRIP_RB;
+  const int n = a.n, m = a.m, p = a.pitch;
+  const int halo = 3 * (s.j1 - s.j0 + 1);
+  const int c0 = 1 + blockIdx.x * s.cw, c1 = min(c0 + s.cw, n);   // core [c0, c1)
+  const int lo = max(-3, c0 - halo), hi = min(n + 3, c1 - 1 + halo);
+  const int W = s.W, rows = m + 7;
+  float* const buf = reinterpret_cast<float*>(smem_raw);   // three strips of W*rows
+  const int bsz = W * rows;
+  const int tid = threadIdx.y * STRIP_TX + threadIdx.x, nthr = STRIP_TX * STRIP_TY;
+  const float* cur_src = (s.j0 == 1) ? s.S0 : s.Gcur;
+  const bool has_prev = s.j0 >= 3;
+  for (int e = tid; e < W * rows; e += nthr) {
+    int rr = e / W, lc = e - rr * W;
+    int gc = lo + lc;
+    bool in = gc <= hi;
+    long g = IDX(gc, rr - 3, p);
+    buf[e] = in ? cur_src[g] : 0.0f;
+    buf[bsz + e] = (in && has_prev) ? s.Gprev[g] : 0.0f;
+    buf[2 * bsz + e] = 0.0f;
+  }
+  // this thread's column and row block; pointwise operands in registers
+  const int ic = c0 - halo + 3 + threadIdx.x;
+  const int k0 = 1 + threadIdx.y * RB;
+  float s0v[RB], dtv[RB], rhsv[RB], visv[RB];
+  bool on[RB];
+#pragma unroll
+  for (int r = 0; r < RB; r++) {
+    const int k = k0 + r;
+    on[r] = ic >= 1 && ic <= n - 1 && k <= m - 1;
+    s0v[r] = dtv[r] = rhsv[r] = visv[r] = 0.0f;
+    if (on[r]) {
+      long g = IDX(ic, k, p);
+      s0v[r] = s.S0[g];
+      dtv[r] = a.dts[g];
+      rhsv[r] = a.rhs[g];
+      visv[r] = a.vis[g];
+    }
+  }
+  const int rmax = m + 3;
+  auto put = [&](float* B, int ii, int kk, float v) {
+    if (ii >= lo && ii <= hi) B[(kk + 3) * W + (ii - lo)] = v;
+  };
+  __syncthreads();
+  int ci = 0, pi = 1;
+  for (int j = s.j0; j <= s.j1; j++) {
+    const int t = j - s.j0;
+    const int rlo = max(1, c0 - halo + 3 * (t + 1)), rhi = min(n - 1, c1 - 1 + halo - 3 * (t + 1));
+    const int oi = 3 - ci - pi;
+    const float* cur = buf + ci * bsz;
+    const float* prev = buf + pi * bsz;
+    float* out = buf + oi * bsz;
+    const float cc0 = (float)s.co.cc[j], cc1 = (float)s.co.mew[j], cc2 = (float)s.co.vv[j];
+    const float cf = (float)(s.co.mewt[j] * s.cpar);
+    if (ic >= rlo && ic <= rhi && k0 <= m - 1) {
+      float F[RB];
+      eval_cb<RB>(cur, W, lo, ic, k0, rmax, a, rhsv, visv, F);
+#pragma unroll
+      for (int r = 0; r < RB; r++) {
+        if (!on[r]) continue;
+        const int k = k0 + r;
+        const int lg = (k + 3) * W + (ic - lo);
+        float y;
+        if (j == 1) y = s0v[r] + (cf * dtv[r]) * F[r];
+        else y = ((cc0 * s0v[r] + cc1 * cur[lg]) + cc2 * prev[lg]) + ((cf * dtv[r]) * F[r]);
+        // store_with_ghosts into the strip
+        const float sz = a.sz;
+        const bool lo_i = ic <= 3, hi_i = ic >= n - 3, lo_k = k <= 3, hi_k = k >= m - 3;
+        put(out, ic, k, y);
+        if (lo_i) put(out, -ic, k, -y);
+        if (hi_i) put(out, 2 * n - ic, k, -y);
+        if (lo_k) {
+          put(out, ic, -k, sz * y);
+          if (lo_i) put(out, -ic, -k, -(sz * y));
+          if (hi_i) put(out, 2 * n - ic, -k, -(sz * y));
+        }
+        if (hi_k) {
+          put(out, ic, 2 * m - k, sz * y);
+          if (lo_i) put(out, -ic, 2 * m - k, -(sz * y));
+          if (hi_i) put(out, 2 * n - ic, 2 * m - k, -(sz * y));
+        }
+      }
+    }
+    __syncthreads();
+    pi = ci;
+    ci = oi;
+  }
+  // core columns of the last two stages to global memory, with ghost images
+  const int cwc = c1 - c0, npts = cwc * (m - 1);
+  for (int e = tid; e < npts; e += nthr) {
+    const int k = 1 + e / cwc, i = c0 + (e - (k - 1) * cwc);
+    const int lg = (k + 3) * W + (i - lo);
+    store_with_ghosts<float>(s.Ocur, i, k, n, m, p, buf[ci * bsz + lg], a.sz);
+    if (s.Oprev) store_with_ghosts<float>(s.Oprev, i, k, n, m, p, buf[pi * bsz + lg], a.sz);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1091,6 +1223,68 @@ static int launch_cb(Context& c, int rb, size_t lds, const StageArgs<float>& a, 
   return launch_cb_t<4>(c, lds, a, err, res, s, cc, co);
 }
 
+// strip sweep of an FP32 isoviscous level: core columns per workgroup and
+// stages per segment (cw == 0: not eligible).  The widest stage region (core
+// + 6 columns per further stage) spans at most the 64 column lanes; the
+// three strip buffers must fit LDS; rows are covered by 16 blocks of 4.
+struct StripCfg {
+  int cw = 0, ns = 0, W = 0;
+};
+static StripCfg strip_cfg(const LevelGeom& g) {
+  StripCfg sc;
+  if (g.m - 1 > STRIP_TY * STRIP_RB) return sc;
+  const size_t rows = (size_t)g.m + 7;
+  for (int ns = 5; ns >= 2; ns--) {
+    for (int cw = STRIP_TX + 6 - 6 * ns; cw >= 16; cw -= 4) {
+      const int W = std::min(cw + 6 * ns, g.n + 7);
+      if (3 * (size_t)W * rows * sizeof(float) <= SMALL_LDS_BYTES) {
+        sc.cw = cw; sc.ns = ns; sc.W = W;
+        return sc;
+      }
+    }
+  }
+  return sc;
+}
+
+// one FP32 RKC1 sweep of level l as segments of fused stages.  The error
+// buffer holds S0 throughout; segments read one buffer pair and write the
+// other; the sweep result replaces the error buffer by a pointer swap.
+static int strip_sweep(Context& c, int l, const StripCfg& sc, double cpar, const RkcCoef& co) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    PT_HIP(hipFuncSetAttribute((const void*)k_strip_cb, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)SMALL_LDS_BYTES));
+    attr_set = true;
+  }
+  Level& L = c.L[l];
+  float** f[5] = {&L.ferr, &L.fB, &L.fC, &L.fD, &L.fE};
+  const LevelGeom& g = L.g;
+  const int s = c.P.nstageS;
+  const int nseg = (s + sc.ns - 1) / sc.ns, base = s / nseg, rem = s % nseg;
+  if (base < 2) {
+    set_error("strip sweep needs at least 2 stages per segment");
+    return PT_ERR_UNSUPPORTED;
+  }
+  const int nblk = (g.n - 1 + sc.cw - 1) / sc.cw;
+  const size_t lds = 3 * (size_t)sc.W * (g.m + 7) * sizeof(float);
+  StageArgs<float> a = stage_base<float>(c, l, true);
+  StripArgs sa;
+  sa.S0 = *f[0]; sa.cpar = cpar; sa.cw = sc.cw; sa.W = sc.W; sa.co = co;
+  int rd = 1, wr = 3;   // read pair f[rd], f[rd+1]; write pair f[wr], f[wr+1]
+  int j0 = 1;
+  for (int seg = 0; seg < nseg; seg++) {
+    const int len = base + (seg < rem ? 1 : 0), j1 = j0 + len - 1;
+    sa.j0 = j0; sa.j1 = j1;
+    sa.Gcur = *f[rd]; sa.Gprev = *f[rd + 1];
+    sa.Ocur = *f[wr]; sa.Oprev = (j1 == s) ? nullptr : *f[wr + 1];
+    hipLaunchKernelGGL(k_strip_cb, dim3(nblk), dim3(STRIP_TX, STRIP_TY), lds, c.stream, a, sa);
+    std::swap(rd, wr);
+    j0 = j1 + 1;
+  }
+  std::swap(*f[0], *f[rd]);   // the last segment's output becomes the error buffer
+  return PT_OK;
+}
+
 // iterate_stream (RKC1, Nmulti sweeps) followed by compute_residual
 // (PS/iterative_solver.f90:1418-1614, 415-448).  `s0zero` runs the smoother
 // on a correction that starts from zero (mixed precision).
@@ -1106,6 +1300,25 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, double* resmag)
   bool timed = (l == 1) && c.timing;
   const size_t lds = 3 * L.nelem * sizeof(R);
   const int ppt = small_ppt(g.n, g.m, c.P.ivis == 0);
+  if constexpr (sizeof(R) == 4) {
+    // FP32 isoviscous levels too large for one workgroup: strip sweeps
+    const StripCfg sc = strip_cfg(g);
+    if (l >= 2 && c.P.ivis == 0 && ppt == 0 && sc.cw > 0 && resmag == nullptr) {
+      RkcCoef co;
+      std::memcpy(co.mewt, c.T.mewt, sizeof co.mewt); std::memcpy(co.mew, c.T.mew, sizeof co.mew);
+      std::memcpy(co.vv, c.T.vv, sizeof co.vv); std::memcpy(co.cc, c.T.cc, sizeof co.cc);
+      for (int it = 1; it <= c.P.Nmulti; it++) {
+        int ntemp = c.P.Nmulti - it + 1;
+        double temp = (c.P.Nmulti > 1) ? (c.P.c_2 - c.P.c_1) * (double)(ntemp - 1) / (double)(1 - c.P.Nmulti) : 0.0;
+        PT_TRY(strip_sweep(c, l, sc, temp + c.P.c_2, co));
+      }
+      StageArgs<R> a = stage_base<R>(c, l, fp32);
+      a.Sx = (R*)L.ferr;
+      launch_residual<R>(c, grd, blk, a, (R*)L.fB, c.red);
+      (void)p;
+      return PT_OK;
+    }
+  }
   if (l >= 2 && ppt > 0 && lds <= SMALL_LDS_BYTES && resmag == nullptr) {
     // whole sweep + residual in one workgroup, buffers in LDS
     RkcCoef co;
@@ -1490,7 +1703,7 @@ int solver_alloc(Context& c) {
       DALLOC(s->vxz, b); DALLOC(s->vxxpzz, b); DALLOC(s->vxxmzz, b); DALLOC(s->vsmooth, b);
     }
     if (c.P.precision == PT_PREC_MIXED) {
-      DALLOC(L.ferr, f); DALLOC(L.fB, f); DALLOC(L.fC, f); DALLOC(L.frhs, f);
+      DALLOC(L.ferr, f); DALLOC(L.fB, f); DALLOC(L.fC, f); DALLOC(L.fD, f); DALLOC(L.fE, f); DALLOC(L.frhs, f);
       DALLOC(L.fdts, f); DALLOC(L.fvis, f); DALLOC(L.fvxm, f); DALLOC(L.fvxz, f); DALLOC(L.fvx, f); DALLOC(L.fvz, f);
     }
   }
@@ -1525,7 +1738,7 @@ int solver_free(Context& c) {
     Level& L = c.L[l];
     void* ps[] = {L.err, L.bufB, L.bufC, L.rhs, L.res, L.dts, L.ratio, L.vis_static, L.v.vis, L.v.vx, L.v.vz,
                   L.v.vxx, L.v.vzz, L.v.vxz, L.v.vxxpzz, L.v.vxxmzz, L.v.vsmooth, L.vs.vis, L.vs.vx, L.vs.vz,
-                  L.vs.vxx, L.vs.vzz, L.vs.vxz, L.vs.vxxpzz, L.vs.vxxmzz, L.vs.vsmooth, L.ferr, L.fB, L.fC,
+                  L.vs.vxx, L.vs.vzz, L.vs.vxz, L.vs.vxxpzz, L.vs.vxxmzz, L.vs.vsmooth, L.ferr, L.fB, L.fC, L.fD, L.fE,
                   L.frhs, L.fdts, L.fvis, L.fvxm, L.fvxz, L.fvx, L.fvz};
     for (void* p : ps)
       if (p) (void)hipFree(p);
