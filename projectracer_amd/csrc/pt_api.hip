@@ -284,10 +284,13 @@ int pt_get_tracers(void* p, double* rtr, double* str, double* Ttr, double* Ctr) 
   Context* c = ctx_of(p);
   if (!c) return PT_ERR_INVALID;
   size_t b = sizeof(double) * (size_t)c->T.ntr;
-  if (rtr) PT_HIP(hipMemcpyAsync(rtr, c->r, b, hipMemcpyDeviceToHost, c->stream));
-  if (str) PT_HIP(hipMemcpyAsync(str, c->s, b, hipMemcpyDeviceToHost, c->stream));
-  if (Ttr) PT_HIP(hipMemcpyAsync(Ttr, c->Tt, b, hipMemcpyDeviceToHost, c->stream));
-  if (Ctr) PT_HIP(hipMemcpyAsync(Ctr, c->Ct, b, hipMemcpyDeviceToHost, c->stream));
+  double *dr = nullptr, *ds = nullptr, *dT = nullptr, *dC = nullptr;
+  PT_TRY(tracers_in_id_order(*c, rtr ? &dr : nullptr, str ? &ds : nullptr, Ttr ? &dT : nullptr,
+                             Ctr ? &dC : nullptr));
+  if (rtr) PT_HIP(hipMemcpyAsync(rtr, dr, b, hipMemcpyDeviceToHost, c->stream));
+  if (str) PT_HIP(hipMemcpyAsync(str, ds, b, hipMemcpyDeviceToHost, c->stream));
+  if (Ttr) PT_HIP(hipMemcpyAsync(Ttr, dT, b, hipMemcpyDeviceToHost, c->stream));
+  if (Ctr) PT_HIP(hipMemcpyAsync(Ctr, dC, b, hipMemcpyDeviceToHost, c->stream));
   PT_HIP(hipStreamSynchronize(c->stream));
   return PT_OK;
 }

@@ -99,8 +99,10 @@ struct Context {
   // restricted to interior unknowns, built from the LU once per factorization
   double *Mc, *RowB, *Bc; int32_t* ipiv; int32_t* d_info;
   double* Gc; double* Gwork; int nci;
-  // tracers (ID order)
+  // tracers, SoA in cell order (re-sorted after every projection of the step's
+  // final positions); tid maps a storage slot to the tracer's reference ID
   double *r, *s, *Tt, *Ct, *r0, *s0, *T0t, *r1, *s1, *T1t, *r2, *s2, *T2t;
+  int32_t *tid, *tid_tmp;
   // projection work: cell keys, stable (cell, index) sort, per-cell partial sums
   int32_t *cellkey, *skey, *iota, *perm, *cstart, *cend, *nempty;
   void* sort_tmp; size_t sort_bytes; int sort_bits;
@@ -112,8 +114,6 @@ struct Context {
   double* red_out;          // device scalars (pinned mirror below)
   double* h_red;            // pinned host mirror
   size_t red_cap;
-  // scan scratch
-  int32_t* scan_tmp; size_t scan_cap;
   // state
   double Tbot_iso = 0, Ttop_iso = 0, time = 0, dt = 0;
   double res_ref1 = 0, residual1 = 0;
@@ -156,6 +156,8 @@ int tracers_to_corners(Context& c, const double* r, const double* s, const doubl
 int energy_time(Context& c);
 int load_state(Context& c, const double* T_nodes, double Tbot, double Ttop, const double* rtr,
                const double* str, const double* Ttr, const double* Ctr);
+// tracer state in reference ID order into device buffers (any may be null)
+int tracers_in_id_order(Context& c, double** r, double** s, double** Tt, double** Ct);
 // ---- sorting (pt_sort.hip) ---------------------------------------------------
 int sort_pairs_temp_bytes(int64_t n, int bits, size_t* bytes);
 int sort_pairs(void* tmp, size_t bytes, const int32_t* kin, int32_t* kout, const int32_t* vin, int32_t* vout,

@@ -12,8 +12,12 @@
 //   k_rhs1          compute_RHS level 1 (PS/iterative_solver.f90:1075-1101)
 //   k_visf / k_vis_* viscosity hierarchy (PS/rheology.f90:2-23, PS/smoother.f90:454-580,
 //                   PS/iterative_solver.f90:472-690, 1147-1211)
-//   k_coarse_*      banded coarse operator, DGBEQU row scaling, DGBTF2 LU and
-//                   DGBTRS solve on one wavefront (PS/iterative_solver.f90:1701-1887)
+//   k_coarse_*      banded coarse operator, DGBEQU row scaling, DGBTF2 LU on one
+//                   wavefront, and the solve (DGBTRS) as a dense interior operator
+//                   built once per factorization (PS/iterative_solver.f90:1701-1887)
+//   k_smooth_small, k_smooth_cb, k_strip_cb
+//                   whole RKC1 sweeps of coarse levels in LDS (one workgroup, or
+//                   column strips with several fused stages per launch)
 //   k_velocities    compute_velocities + velocityBCs (PS/momentum.f90:655-710,
 //                   PS/composition.f90:1295-1325)
 // FP64 kernels are compiled without FMA contraction so each point is

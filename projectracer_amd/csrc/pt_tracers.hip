@@ -1,24 +1,25 @@
 // pt_tracers.hip -- Lagrangian tracer pipeline on gfx950.
 //
 //   k_bin            nint binning of tracer positions to node dual cells + domain
-//                    check + per-cell counts (PS/composition.f90:1002-1011)
-//   k_scan_*         exclusive scan of the counts (cell offsets)
-//   k_scatter/k_canon counting sort of tracer indices by cell; each cell's
-//                    segment is put in ascending tracer-ID order so the
-//                    projection sums in a fixed order (no float atomics)
-//   k_project        node-owner gather over the 3x3 neighbouring cells:
-//                    sum T*w, C*w, w in canonical order, Tratio/Cnew = ratio
-//                    where sum w > 0, nempty = cell count
-//                    (PS/composition.f90:942-1070, shape_func :896-921)
-//   k_tbc_* / k_cbc_* enforceBCs / extendT / enforceBCs_comp
+//                    check (PS/composition.f90:1002-1011)
+//   (rocPRIM)        stable radix sort of (cell, slot) pairs: each cell's tracers
+//                    become one run in a fixed order
+//   k_cell_bounds    run bounds per cell (nempty = run length)
+//   k_cell_reduce    per-cell sums of T*w, C*w, w for the 3x3 neighbour nodes
+//                    (shape_func, PS/composition.f90:896-921), fixed order
+//   k_node_combine   node = sum of its 9 cells' partials in a fixed order;
+//                    Tratio/Cnew = ratio where sum w > 0 (PS/composition.f90:942-1070)
+//   k_reorder        keeps the tracer SoA in cell order (ID map for host copies)
+//   k_sym_* / k_t_*  enforceBCs / extendT / enforceBCs_comp
 //                    (PS/energy.f90:558-630, PS/composition.f90:1279-1292)
-//   k_filter_*       smoother_time / smoother_time_T Richardson passes
-//                    (PS/smoother.f90:582-672) with max-reductions for smoothness
-//   k_conduct        load_conductivity_array (PS/energy.f90:59-86)
+//   k_smoothness / k_filter
+//                    smoother_time / smoother_time_T Richardson passes
+//                    (PS/smoother.f90:582-672, 75-139)
+//   k_conduct_*      load_conductivity_array (PS/energy.f90:59-86)
 //   k_source         tracer_space_uniform (PS/energy.f90:135-151)
 //   k_itab           compute_derivatives for u, w and the source into one
 //                    interleaved interpolation table (PS/composition.f90:1457-1478)
-//   k_rk_stage<S>    interpolate (x3) + the RK4 stage update S of energy_time
+//   k_rk1..k_rk4     interpolate (x3) + the RK4 stage updates of energy_time
 //                    (PS/composition.f90:1359-1393, PS/energy.f90:258-537)
 #include <cmath>
 #include <cstring>
@@ -36,7 +37,7 @@ static dim3 grid2(int ni, int nk, dim3 b) { return dim3((ni + b.x - 1) / b.x, (n
 __device__ __forceinline__ int nint(double x) { return (int)round(x); }
 
 // ---------------------------------------------------------------------------
-// binning, stable sort by cell, per-cell wavefront reductions, node combine
+// binning, stable sort by cell, per-cell sums, node combine
 // ---------------------------------------------------------------------------
 // key = dual cell of nint(r/dr), nint(s/ds); positions outside the domain
 // (or NaN) raise the flag and get the sentinel key ncell (sorted last)
@@ -65,66 +66,92 @@ __global__ void k_cell_bounds(const int32_t* skey, int64_t n, int ncell, int32_t
   if (p == n - 1 || skey[p + 1] != c) cend[c] = (int32_t)(p + 1);
 }
 
-// One wavefront per cell: lane l takes the l-th tracer of the cell's sorted
-// segment (64 at a time), forms the 9 bilinear weights of shape_func
-// (PS/composition.f90:896-921, the reference's 9 neighbour nodes of
-// nint(r/dr),nint(s/ds)) and T*w, C*w, w; a fixed xor-butterfly reduces the
-// 27 values across the wave, chunks are added in order.  No atomics: the
-// result is deterministic.
-__global__ void __launch_bounds__(256) k_cell_reduce(const double* __restrict__ r, const double* __restrict__ s,
-                                                     const double* __restrict__ Tt, const double* __restrict__ Ct,
-                                                     const int32_t* __restrict__ perm,
-                                                     const int32_t* __restrict__ cstart,
-                                                     const int32_t* __restrict__ cend, double* __restrict__ cellsum,
-                                                     int nx, int nz, double dr, double ds) {
-  const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+// Per-cell partial sums of the projection.  A block owns PJ_CELLS
+// consecutive cells; their tracers are one contiguous run of the
+// (cell, slot)-sorted order, processed in chunks of PJ_CHUNK:
+//  phase 1, one thread per tracer: coalesced loads through the permutation
+//    (close to the identity, since the tracer arrays are kept in cell
+//    order), the 3+3 hat factors max(0,1-|r-ri|/dr), max(0,1-|s-sk|/ds) of
+//    shape_func (PS/composition.f90:896-921) for the nodes around the
+//    tracer's cell, staged in LDS with T and C;
+//  phase 2, one thread per (cell, neighbour node): w = wx*wz, and T*w, C*w,
+//    w summed over the cell's tracers in ascending slot order (across
+//    chunks too, so each sum is one sequential pass).
+// Fixed summation order, no atomics: the result is deterministic.
+constexpr int PJ_CELLS = 28;     // 28 cells x 9 neighbour nodes = 252 phase-2 threads
+constexpr int PJ_THREADS = 256;
+constexpr int PJ_CHUNK = 1024;
+__global__ void __launch_bounds__(PJ_THREADS) k_cell_reduce(const double* __restrict__ r, const double* __restrict__ s,
+                                                            const double* __restrict__ Tt, const double* __restrict__ Ct,
+                                                            const int32_t* __restrict__ perm,
+                                                            const int32_t* __restrict__ skey, int64_t n,
+                                                            const int32_t* __restrict__ cstart,
+                                                            const int32_t* __restrict__ cend,
+                                                            double* __restrict__ cellsum, int nx, int nz, double dr,
+                                                            double ds) {
+  __shared__ double hx[3][PJ_CHUNK], hz[3][PJ_CHUNK], sT[PJ_CHUNK], sC[PJ_CHUNK];
+  __shared__ int64_t range[2];
   const int ncell = (nx + 1) * (nz + 1);
-  if (c >= ncell) return;
-  const int i = c % (nx + 1), k = c / (nx + 1);
-  const int b0 = cstart[c], n = cend[c] - b0;
-  double acc[27];
-#pragma unroll
-  for (int q = 0; q < 27; q++) acc[q] = 0.0;
-  for (int base = 0; base < n; base += 64) {
-    double v[27];
-    if (base + lane < n) {
-      int j = perm[b0 + base + lane];
-      double rv = r[j], sv = s[j], T = Tt[j], Cv = Ct[j];
-      double wx[3], wz[3];
-#pragma unroll
-      for (int a = -1; a <= 1; a++) {
-        double ri = (double)(i + a) * dr;
-        double t = 1.0 - fabs(rv - ri) / dr;
-        wx[a + 1] = t < 0.0 ? 0.0 : t;
-        double sk = (double)(k + a) * ds;
-        double u = 1.0 - fabs(sv - sk) / ds;
-        wz[a + 1] = u < 0.0 ? 0.0 : u;
-      }
-#pragma unroll
-      for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int b = 0; b < 3; b++) {
-          double w = wx[a] * wz[b];
-          v[(a * 3 + b) * 3 + 0] = T * w;
-          v[(a * 3 + b) * 3 + 1] = Cv * w;
-          v[(a * 3 + b) * 3 + 2] = w;
-        }
-    } else {
-#pragma unroll
-      for (int q = 0; q < 27; q++) v[q] = 0.0;
+  const int cfirst = blockIdx.x * PJ_CELLS;
+  const int clast = min(cfirst + PJ_CELLS, ncell);   // exclusive
+  const int tid = threadIdx.x;
+  if (tid < 2) {
+    // lower bound of the block's first / one-past-last cell in the sorted keys
+    const int target = tid == 0 ? cfirst : clast;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+      const int64_t mid = lo + (hi - lo) / 2;
+      if (skey[mid] < target) lo = mid + 1;
+      else hi = mid;
     }
-#pragma unroll
-    for (int q = 0; q < 27; q++) {
-      double x = v[q];
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) x = x + __shfl_xor(x, off);
-      acc[q] = acc[q] + x;
-    }
+    range[tid] = lo;
   }
-  if (lane == 0) {
+  // phase-2 role: cell c, neighbour node (a, b) of the 3x3 around it
+  const int c = cfirst + tid / 9, ab = tid % 9, na = ab / 3, nb = ab % 3;
+  const bool owner = tid < 9 * PJ_CELLS && c < clast;
+  int64_t b0 = 0, b1 = 0;
+  if (owner) {
+    b0 = cstart[c];
+    b1 = cend[c];
+  }
+  double txs = 0.0, cxs = 0.0, ws = 0.0;
+  __syncthreads();
+  const int64_t t0 = range[0], t1 = range[1];
+  for (int64_t base = t0; base < t1; base += PJ_CHUNK) {
+    const int cnt = (int)min((int64_t)PJ_CHUNK, t1 - base);
+    for (int q = tid; q < cnt; q += PJ_THREADS) {
+      const int32_t j = perm[base + q];
+      const double rv = r[j], sv = s[j];
+      const int key = skey[base + q];
+      const int i = key % (nx + 1), k = key / (nx + 1);
 #pragma unroll
-    for (int q = 0; q < 27; q++) cellsum[(long)c * 27 + q] = acc[q];
+      for (int e = 0; e < 3; e++) {
+        const double tx = 1.0 - fabs(rv - (double)(i + e - 1) * dr) / dr;
+        hx[e][q] = tx < 0.0 ? 0.0 : tx;
+        const double tz = 1.0 - fabs(sv - (double)(k + e - 1) * ds) / ds;
+        hz[e][q] = tz < 0.0 ? 0.0 : tz;
+      }
+      sT[q] = Tt[j];
+      sC[q] = Ct[j];
+    }
+    __syncthreads();
+    if (owner) {
+      const int64_t lo = max(b0, base), hi = min(b1, base + cnt);
+      for (int64_t t = lo; t < hi; t++) {
+        const int q = (int)(t - base);
+        const double w = hx[na][q] * hz[nb][q];
+        txs = txs + sT[q] * w;
+        cxs = cxs + sC[q] * w;
+        ws = ws + w;
+      }
+    }
+    __syncthreads();
+  }
+  if (owner) {
+    double* out = cellsum + (long)c * 27 + ab * 3;
+    out[0] = txs;
+    out[1] = cxs;
+    out[2] = ws;
   }
 }
 
@@ -418,8 +445,8 @@ int tracers_to_corners(Context& c, const double* r, const double* s, const doubl
                      c.cellkey, c.d_flags);
   PT_TRY(sort_pairs(c.sort_tmp, c.sort_bytes, c.cellkey, c.skey, c.iota, c.perm, n, c.sort_bits, c.stream));
   hipLaunchKernelGGL(k_cell_bounds, dim3(nbt), dim3(256), 0, c.stream, c.skey, n, ncell, c.cstart, c.cend);
-  hipLaunchKernelGGL(k_cell_reduce, dim3((ncell + 3) / 4), dim3(256), 0, c.stream, r, s, Tt, c.Ct, c.perm, c.cstart,
-                     c.cend, c.cellsum, nx, nz, c.T.dr, c.T.ds);
+  hipLaunchKernelGGL(k_cell_reduce, dim3((ncell + PJ_CELLS - 1) / PJ_CELLS), dim3(PJ_THREADS), 0, c.stream, r, s, Tt,
+                     c.Ct, c.perm, c.skey, n, c.cstart, c.cend, c.cellsum, nx, nz, c.T.dr, c.T.ds);
   dim3 b16(64, 4);
   hipLaunchKernelGGL(k_node_combine, grid2(nx + 1, nz + 1, b16), b16, 0, c.stream, c.cellsum, c.cstart, c.cend,
                      c.Tratio, c.Cnew, c.nempty, nx, nz);
@@ -701,6 +728,8 @@ static void rk_launch(Context& c, void (*kern)(RkArgs), RkArgs a) {
 }
 
 // energy_time: PS/energy.f90:181-556 (tracer=1, comp=1, torder=4, RK_update=1)
+static int reorder_tracers(Context& c);
+
 int energy_time(Context& c) {
   PT_TRY(stability_rk4(c));
   const double dt = c.dt;
@@ -740,6 +769,7 @@ int energy_time(Context& c) {
   // wrap up
   PT_HIP(hipMemcpyAsync(c.Cnew, c.C0, b5, hipMemcpyDeviceToDevice, c.stream));
   PT_TRY(tracers_to_corners(c, c.r, c.s, c.Tt));
+  PT_TRY(reorder_tracers(c));
   return PT_OK;
 }
 
@@ -748,6 +778,50 @@ __global__ void k_load_T(const double* Tn, double* T, int nx, int nz) {
   int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
   if (i > nx || k > nz) return;
   T[I3(i, k, nx + 7)] = Tn[(long)k * (nx + 1) + i];
+}
+
+// Storage order of the tracers: after the projection of the step's final
+// positions the SoA arrays are permuted into that projection's cell order
+// (stable: ascending slot within a cell), so the next step's interpolation
+// gathers and projection reads walk the grid cell by cell.
+__global__ void k_reorder(const int32_t* __restrict__ perm, int64_t n, const double* __restrict__ r,
+                          const double* __restrict__ s, const double* __restrict__ T, const double* __restrict__ C,
+                          const int32_t* __restrict__ id, double* __restrict__ r2, double* __restrict__ s2,
+                          double* __restrict__ T2, double* __restrict__ C2, int32_t* __restrict__ id2) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int32_t q = perm[j];
+  r2[j] = r[q];
+  s2[j] = s[q];
+  T2[j] = T[q];
+  C2[j] = C[q];
+  id2[j] = id[q];
+}
+static int reorder_tracers(Context& c) {
+  const int64_t n = c.T.ntr;
+  hipLaunchKernelGGL(k_reorder, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, c.perm, n, c.r, c.s,
+                     c.Tt, c.Ct, c.tid, c.r1, c.s1, c.T1t, c.r2, c.tid_tmp);
+  std::swap(c.r, c.r1);
+  std::swap(c.s, c.s1);
+  std::swap(c.Tt, c.T1t);
+  std::swap(c.Ct, c.r2);
+  std::swap(c.tid, c.tid_tmp);
+  return PT_OK;
+}
+__global__ void k_to_id_order(const int32_t* __restrict__ id, int64_t n, const double* __restrict__ a,
+                              double* __restrict__ out) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) out[id[j]] = a[j];
+}
+// the RK4 registers r1, s1, T1t, r2 are free between steps and hold the result
+int tracers_in_id_order(Context& c, double** r, double** s, double** Tt, double** Ct) {
+  const int64_t n = c.T.ntr;
+  const dim3 g((unsigned)((n + 255) / 256)), b(256);
+  if (r) { hipLaunchKernelGGL(k_to_id_order, g, b, 0, c.stream, c.tid, n, c.r, c.r1); *r = c.r1; }
+  if (s) { hipLaunchKernelGGL(k_to_id_order, g, b, 0, c.stream, c.tid, n, c.s, c.s1); *s = c.s1; }
+  if (Tt) { hipLaunchKernelGGL(k_to_id_order, g, b, 0, c.stream, c.tid, n, c.Tt, c.T1t); *Tt = c.T1t; }
+  if (Ct) { hipLaunchKernelGGL(k_to_id_order, g, b, 0, c.stream, c.tid, n, c.Ct, c.r2); *Ct = c.r2; }
+  return PT_OK;
 }
 
 int load_state(Context& c, const double* T_nodes, double Tbot, double Ttop, const double* rtr, const double* str,
@@ -769,8 +843,10 @@ int load_state(Context& c, const double* T_nodes, double Tbot, double Ttop, cons
   PT_HIP(hipMemcpyAsync(c.s, str, tb, hipMemcpyHostToDevice, c.stream));
   PT_HIP(hipMemcpyAsync(c.Tt, Ttr, tb, hipMemcpyHostToDevice, c.stream));
   PT_HIP(hipMemcpyAsync(c.Ct, Ctr, tb, hipMemcpyHostToDevice, c.stream));
+  PT_HIP(hipMemcpyAsync(c.tid, c.iota, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, c.stream));
   PT_HIP(hipMemsetAsync(c.Cnew, 0, sizeof(double) * (size_t)(nx + 11) * (nz + 11), c.stream));
   PT_TRY(tracers_to_corners(c, c.r, c.s, c.Tt));
+  PT_TRY(reorder_tracers(c));
   PT_HIP(hipMemsetAsync(c.SF, 0, sizeof(double) * (size_t)(nx + 7) * (nz + 7), c.stream));
   for (int l = 2; l <= c.T.grids; l++) {
     PT_HIP(hipMemsetAsync(c.L[l].err, 0, sizeof(double) * c.L[l].nelem, c.stream));
@@ -812,6 +888,7 @@ int tracers_alloc(Context& c) {
   for (double** p : tr) DALLOC(*p, tb);
   size_t ib = sizeof(int32_t) * (size_t)c.T.ntr;
   DALLOC(c.cellkey, ib); DALLOC(c.skey, ib); DALLOC(c.iota, ib); DALLOC(c.perm, ib);
+  DALLOC(c.tid, ib); DALLOC(c.tid_tmp, ib);
   DALLOC(c.cstart, sizeof(int32_t) * ncell); DALLOC(c.cend, sizeof(int32_t) * ncell);
   DALLOC(c.nempty, sizeof(int32_t) * ncell);
   DALLOC(c.cellsum, sizeof(double) * 27 * (size_t)ncell);
@@ -828,7 +905,7 @@ int tracers_alloc(Context& c) {
 int tracers_free(Context& c) {
   void* ps[] = {c.SF, c.Tratio, c.T0, c.Tvis, c.Tbuoy, c.u, c.w, c.src, c.cond_r, c.cond_s, c.Cnew, c.C0, c.Cvis,
                 c.Cbuoy, c.cond, c.Textend, c.TxS, c.CxS, c.itab, c.dtab, c.r, c.s, c.Tt, c.Ct, c.r0, c.s0, c.T0t,
-                c.r1, c.s1, c.T1t, c.r2, c.s2, c.T2t, c.cellkey, c.skey, c.iota, c.perm, c.cstart, c.cend,
+                c.r1, c.s1, c.T1t, c.r2, c.s2, c.T2t, c.cellkey, c.skey, c.iota, c.perm, c.tid, c.tid_tmp, c.cstart, c.cend,
                 c.nempty, c.cellsum, c.d_flags, c.sort_tmp};
   for (void* p : ps)
     if (p) (void)hipFree(p);
