@@ -33,9 +33,18 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes of one level-1 RKC1 stage per interior point: read S0, S1,
-# S2, eta, eta_xx-eta_zz, eta_xz, eta_x, eta_z, RHS, dt_stream, write Y
-RKC_BYTES_PER_POINT = {"f64": 96, "f32": 48}
+# algorithmic bytes of one level-1 RKC1 stage per interior point: read S1
+# (stencil source), S0, S2, eta, RHS, dt_stream and write Y; the general
+# operator also reads eta_xx-eta_zz, eta_xz, eta_x, eta_z (96 B in FP64), the
+# isoviscous one does not (56 B)
+RKC_BYTES_PER_POINT_F64 = {"isoviscous": 56, "general": 96}
+# tracer pipeline, per tracer: the four RK4 stage kernels move 96+112+112+88 B
+# (positions, temperature, step-start copies, stage registers); one
+# projection reads r, s, T, C once (32 B)
+RK_BYTES_PER_TRACER_STEP = 408
+RK2_BYTES_PER_TRACER = 112   # stage 2: read r1, s1, r, s, T, r0, s0, T0; write r, s, T, r2, s2, T2
+PROJECT_BYTES_PER_TRACER = 32
+TRAFFIC_FILE = ROOT / "profiles" / "k_rk2_1024x256_traffic.json"
 
 
 def parse():
@@ -48,6 +57,11 @@ def parse():
     ap.add_argument("--tpc", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--paths", default="mixed,fp64",
+                    help="GPU paths to time (profiling runs may select one)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a 256 MiB read + 256 MiB write elementwise kernel, the "
+                         "reference for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
     return ap.parse_args()
 
 
@@ -81,6 +95,7 @@ def time_gpu(args, precision):
     kms, klaunch = sim.kernel_time(_abi.PT_K_RKC_STAGE_L1)
     tms, tlaunch = sim.kernel_time(_abi.PT_K_TRACER_STAGE)
     pms, plaunch = sim.kernel_time(_abi.PT_K_PROJECT)
+    r2ms, r2launch = sim.kernel_time(_abi.PT_K_TRACER_RK2)
     sim.enable_kernel_timing(False)
     out = dict(
         seconds=t1 - t0,
@@ -88,7 +103,7 @@ def time_gpu(args, precision):
         solves=inf1["solves"] - inf0["solves"],
         cycles=inf1["cycles"] - inf0["cycles"],
         rkc_ms=kms, rkc_launches=klaunch, tracer_ms=tms, tracer_launches=tlaunch,
-        project_ms=pms, project_launches=plaunch,
+        project_ms=pms, project_launches=plaunch, rk2_ms=r2ms, rk2_launches=r2launch,
         nx=sim.nx, nz=sim.nz, ntr=sim.ntr, last_residual_ratio=inf1["residual"] / inf1["res_ref"],
     )
     sim.close()
@@ -140,17 +155,46 @@ def main():
               "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     from projectracer_amd import _abi
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.mul(x, 2.0, out=y)
+        torch.cuda.synchronize()
+        del x, y
+    paths = set(args.paths.split(","))
+    if paths != {"mixed", "fp64"}:
+        # profiling run: time the selected path only, no JSON line
+        for name in sorted(paths):
+            prec = _abi.PT_PREC_MIXED if name == "mixed" else _abi.PT_PREC_FP64
+            r = time_gpu(args, prec)
+            print(f"{name}: {r['ms_per_step']:.2f} ms/step, {r['cycles']} cycles", flush=True)
+        return
     mixed = time_gpu(args, _abi.PT_PREC_MIXED)
     fp64 = time_gpu(args, _abi.PT_PREC_FP64)
     steps_s = args.steps / mixed["seconds"]
     fp64_steps_s = args.steps / fp64["seconds"]
     interior = (mixed["nx"] - 1) * (mixed["nz"] - 1)
-    # roofline of the dominant solver kernel (level-1 RKC1 stage), measured
-    # with HIP events on the library stream inside the timed region
+    # roofline: the dominant HBM-streaming kernel of the step, the RK4 tracer
+    # stage k_rk2 (one launch per step, 13.2 M tracers), timed with HIP events
+    # on the library stream inside the timed region of the headline run.  The
+    # solver's levels >= 2 are latency-bound and the level-1 stage kernel is
+    # too short (~5 us) for per-launch events to resolve; it is reported
+    # beside it.
+    ntr = mixed["ntr"]
+    rk2_us = 1e3 * mixed["rk2_ms"] / max(1, mixed["rk2_launches"])
+    bytes_per_launch = RK2_BYTES_PER_TRACER * ntr
+    achieved = bytes_per_launch / (rk2_us * 1e-6) / 1e9
+    traffic = None
+    if TRAFFIC_FILE.exists():
+        # HBM bytes per launch of the same kernel from the committed rocprofv3
+        # FETCH_SIZE / WRITE_SIZE runs, scaled by the copy-kernel calibration
+        traffic = json.loads(TRAFFIC_FILE.read_text()).get("hbm_bytes_per_launch")
     rk = fp64
-    avg_ms = rk["rkc_ms"] / max(1, rk["rkc_launches"])
-    bytes_per_launch = RKC_BYTES_PER_POINT["f64"] * interior
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    l1_us = 1e3 * rk["rkc_ms"] / max(1, rk["rkc_launches"])
+    l1_bytes = RKC_BYTES_PER_POINT_F64["isoviscous"] * interior
+    rk_ms_step = mixed["tracer_ms"] / args.steps
+    pj_ms_call = mixed["project_ms"] / max(1, mixed["project_launches"])
     result = {
         "metric": "time-steps/s & tracer-updates/s at 1024x256, mixed+learned vs plain FP64 GPU",
         "value": steps_s,
@@ -179,11 +223,28 @@ def main():
         },
         "speedup_vs_fp64_gpu": steps_s / fp64_steps_s,
         "roofline": {
-            "kernel": "k_stage<double> (level-1 RKC1 stage, FP64 path)",
+            "kernel": "k_rk2 (RK4 tracer stage 2: interpolation of u, w, source + stage update)",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "avg_launch_us": avg_ms * 1e3, "launches": rk["rkc_launches"],
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "avg_launch_us": rk2_us, "launches": mixed["rk2_launches"],
             "bytes_per_launch": bytes_per_launch,
+        },
+        "solver_stage_l1_fp64": {
+            "kernel": "k_stage<double, isoviscous> (level-1 RKC1 stage, FP64 path)",
+            "bytes_per_launch": l1_bytes, "avg_launch_us_events": l1_us, "launches": rk["rkc_launches"],
+            "achieved_GBs_events": l1_bytes / (l1_us * 1e-6) / 1e9,
+        },
+        "tracer_roofline": {
+            "rk4_stages": {
+                "bytes_per_tracer_step": RK_BYTES_PER_TRACER_STEP, "ms_per_step": rk_ms_step,
+                "achieved_GBs": RK_BYTES_PER_TRACER_STEP * ntr / (rk_ms_step * 1e-3) / 1e9,
+                "frac": RK_BYTES_PER_TRACER_STEP * ntr / (rk_ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            },
+            "projection": {
+                "bytes_per_tracer_call": PROJECT_BYTES_PER_TRACER, "ms_per_call": pj_ms_call,
+                "achieved_GBs": PROJECT_BYTES_PER_TRACER * ntr / (pj_ms_call * 1e-3) / 1e9,
+                "frac": PROJECT_BYTES_PER_TRACER * ntr / (pj_ms_call * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            },
         },
         "kernel_share_fp64": {
             "rkc_stage_l1_ms_per_step": fp64["rkc_ms"] / args.steps,

@@ -171,7 +171,11 @@ int pt_synchronize(void* ctx);
 /* Enable per-kernel HIP-event timing of the dominant kernels on the
  * context's stream.  While enabled, pt_kernel_time reports the accumulated
  * device time (ms) and launch count of kernel `which` (PT_K_*). */
-enum { PT_K_RKC_STAGE_L1 = 0, PT_K_TRACER_STAGE = 1, PT_K_PROJECT = 2, PT_K_COUNT = 3 };
+/*   PT_K_RKC_STAGE_L1  level-1 RKC1 stage launches of the solver
+ *   PT_K_TRACER_STAGE  the four RK4 tracer stage kernels of energy_time
+ *   PT_K_PROJECT       one projection (binning, sort, sums, node combine)
+ *   PT_K_TRACER_RK2    the second RK4 tracer stage kernel alone */
+enum { PT_K_RKC_STAGE_L1 = 0, PT_K_TRACER_STAGE = 1, PT_K_PROJECT = 2, PT_K_TRACER_RK2 = 3, PT_K_COUNT = 4 };
 int pt_enable_kernel_timing(void* ctx, int32_t on);
 int pt_kernel_time(void* ctx, int32_t which, double* ms, int64_t* launches);
 

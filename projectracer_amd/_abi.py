@@ -37,6 +37,7 @@ FIELD_HALO = {0: 3, 1: 3, 2: 3, 3: 3, 4: 3, 5: 3, 6: 5, 7: 5, 8: 5, 9: 5, 10: 3,
 PT_K_RKC_STAGE_L1 = 0
 PT_K_TRACER_STAGE = 1
 PT_K_PROJECT = 2
+PT_K_TRACER_RK2 = 3
 
 ERROR_TEXT = {
     PT_ERR_TRACER_OUTSIDE: "Tracer advected outside of domain - stopping",

@@ -66,7 +66,7 @@ __device__ __forceinline__ float acc_(float s, float c, float v) { return __fmaf
 __device__ __forceinline__ double scale_(double s, double d, double rd) { (void)rd; return s / d; }
 __device__ __forceinline__ float scale_(float s, float d, float rd) { (void)d; return s * rd; }
 
-constexpr int TX = 64, TZ = 4, PZ = 2;
+constexpr int TX = 64, TZ = 8, PZ = 1;
 constexpr size_t SMALL_LDS_BYTES = 160 * 1024;
 constexpr int TW = TX + 6, TH = TZ * PZ + 6;
 

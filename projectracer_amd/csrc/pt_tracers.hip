@@ -720,10 +720,14 @@ static int update_velocities(Context& c) {
   return compute_velocities(c);
 }
 
-static void rk_launch(Context& c, void (*kern)(RkArgs), RkArgs a) {
+// `which` names the stage's own timer (PT_K_TRACER_RK2 for stage 2, else -1);
+// every stage also counts towards PT_K_TRACER_STAGE
+static void rk_launch(Context& c, void (*kern)(RkArgs), RkArgs a, int which = -1) {
   unsigned nb = (unsigned)((a.n + 255) / 256);
   if (c.timing) timer_begin(c, PT_K_TRACER_STAGE);
+  if (c.timing && which >= 0) timer_begin(c, which);
   hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, c.stream, a);
+  if (c.timing && which >= 0) timer_end(c, which);
   if (c.timing) timer_end(c, PT_K_TRACER_STAGE);
 }
 
@@ -749,7 +753,7 @@ int energy_time(Context& c) {
   PT_TRY(update_velocities(c));
   PT_TRY(build_itab(c));
   a.dtstage = 0.5 * dt;
-  rk_launch(c, k_rk2, a);
+  rk_launch(c, k_rk2, a, PT_K_TRACER_RK2);
   // stage 3
   PT_HIP(hipMemcpyAsync(c.Cnew, c.C0, b5, hipMemcpyDeviceToDevice, c.stream));
   PT_TRY(tracers_to_corners(c, c.r2, c.s2, c.T2t));
