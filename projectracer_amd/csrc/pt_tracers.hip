@@ -521,7 +521,7 @@ __device__ __forceinline__ double cube_cr(double x) {
 
 struct InterpConst {
   int nx, nz;
-  double dr, ds, f1, f2, f3;     // factorial(1..3)
+  double dr, ds, f3;             // factorial(3)
   double dsp1, dsp2, dsp3;       // ds_power(1..3)
   double Dm[3][7];               // D(1..3, -3..3)
 };
@@ -534,10 +534,12 @@ __device__ __forceinline__ void interp3(const double* __restrict__ tab, int nz, 
   // the clamp only guards the table reads
   i = i < 0 ? 0 : (i > C.nx ? C.nx : i);
   k = k < 0 ? 0 : (k > C.nz ? C.nz : k);
+  // factorial(1) = 1 and factorial(2) = 2 exactly, so h/factorial(1) = h and
+  // h^2/factorial(2) = h^2 * 0.5 are the reference's quotients bit for bit
   double hr = r - (double)i * C.dr;
-  double a1 = hr / C.f1, a2 = (hr * hr) / C.f2, a3 = cube_cr(hr) / C.f3;
+  double a1 = hr, a2 = (hr * hr) * 0.5, a3 = cube_cr(hr) / C.f3;
   double hs = s - (double)k * C.ds;
-  double b1 = hs / C.f1, b2 = (hs * hs) / C.f2, b3 = cube_cr(hs) / C.f3;
+  double b1 = hs, b2 = (hs * hs) * 0.5, b3 = cube_cr(hs) / C.f3;
   const double* row = tab + ((long)i * (nz + 7) + (k - 3 + 3)) * 12;
   double fR[3][7];
 #pragma unroll
@@ -657,7 +659,7 @@ static RkArgs rk_args(Context& c) {
   a.r1 = c.r1; a.s1 = c.s1; a.T1 = c.T1t; a.r2 = c.r2; a.s2 = c.s2; a.T2 = c.T2t;
   a.tab = c.itab; a.n = c.T.ntr; a.nz = c.T.nz;
   a.C.nx = c.T.nx; a.C.nz = c.T.nz;
-  a.C.dr = c.T.dr; a.C.ds = c.T.ds; a.C.f1 = c.T.factorial[1]; a.C.f2 = c.T.factorial[2]; a.C.f3 = c.T.factorial[3];
+  a.C.dr = c.T.dr; a.C.ds = c.T.ds; a.C.f3 = c.T.factorial[3];
   a.C.dsp1 = c.T.ds_power[1]; a.C.dsp2 = c.T.ds_power[2]; a.C.dsp3 = c.T.ds_power[3];
   for (int m = 0; m < 3; m++)
     for (int q = 0; q < 7; q++) a.C.Dm[m][q] = c.T.D[m + 1][q];
