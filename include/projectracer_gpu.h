@@ -85,10 +85,13 @@ typedef struct pt_params {
   /* path selection (no reference counterpart) */
   int32_t precision;      /* PT_PREC_* */
   int32_t guess;          /* PT_GUESS_* */
+  /* &inputs: diagnostics */
+  double  dent;           /* reference height of the entrainment integral */
 } pt_params;
 
 /* Fill *p with the reference defaults used by this build (inputs.nml values
- * for the supported path; tolerance 1e-8, 6 levels, RKC1 35 stages). */
+ * for the supported path; tolerance 1e-8, 6 levels, RKC1 35 stages,
+ * dent 0.5). */
 void pt_default_params(pt_params* p);
 
 /* ---- field identifiers for pt_get_field / pt_set_field ----------------- */
@@ -121,6 +124,21 @@ typedef struct pt_solve_info {
   int32_t n_comp;         /* C filter passes of the last projection */
   double  eig_fine;       /* ratio_min of the last level-1 smoother_vis */
 } pt_solve_info;
+
+/* ---- per-step diagnostics (PS/basics.f90:363-462) ---------------------
+ * The values `stats` writes to stats.dat (comp=1, tracer=1 row, format at
+ * PS/basics.f90:421-422) and viscosity_smoother.dat (:453-455), computed on
+ * the device from the current state.  The first call after pt_load_state
+ * plays the reference's tstep=0 call: it fixes `mass` (the initial C
+ * integral) that entrainment is normalised by. */
+typedef struct pt_stats_row {
+  double  time, Tavg, vrms, Nu, Nu_bot, RaT_avg, mobility, strain_max, q1, q2, q3, q4;
+  int32_t empty_cells, tracer_min, tracer_max;
+  double  Cavg, ent;
+  double  vis_min, vis_max;   /* viscosity_smoother.dat */
+  int32_t n_eta, n_temp, n_comp;
+  double  eig_fine;
+} pt_stats_row;
 
 /* ---- lifecycle ---------------------------------------------------------
  * pt_create: compute_smoothing_parameters, multigrid_setup (optimize_grid),
@@ -164,6 +182,8 @@ int pt_get_tracers(void* ctx, double* rtr, double* str, double* Ttr, double* Ctr
 /* nempty(0:nx,0:nz): tracers binned to each node's dual cell (INT32). */
 int pt_get_nempty(void* ctx, int32_t* nempty);
 int pt_get_info(void* ctx, pt_solve_info* info);
+/* stats (PS/basics.f90:363-462) of the current state; see pt_stats_row */
+int pt_stats(void* ctx, pt_stats_row* row);
 int pt_synchronize(void* ctx);
 
 /* ---- timing hooks (no reference counterpart; PS/basics.f90:27-31 hold

@@ -112,6 +112,8 @@ struct pto_case {
   int n_eta, n_temp, n_comp; double eig_fine; int Nvis_store, nn_store;
   int last_q; double last_res, last_res_ref; int64_t solves, cycles;
   int err;
+  /* stats: entrainment normalisation (set by the first call after a load) */
+  double mass; int mass_set; int kent;
 };
 
 #define DER(a, c, jj, i, k) ((a)[((long)((k) + 3) * ((c)->nx + 1) + (i)) * 3 + ((jj) - 1)])
@@ -1586,6 +1588,7 @@ int pto_load_state(pto_case* c, const double* T_nodes, double Tbot_iso, double T
   memcpy(c->rtr, rtr, sizeof(double) * c->ntr); memcpy(c->str, str, sizeof(double) * c->ntr);
   memcpy(c->Ttr, Ttr, sizeof(double) * c->ntr); memcpy(c->Ctr, Ctr, sizeof(double) * c->ntr);
   a2_copy(&c->T0, &c->T);
+  c->mass_set = 0;
   int rc = tracers_to_corners(c, c->rtr, c->str, c->Ttr);
   if (rc) return rc;
   memset(c->SF.p, 0, sizeof(double) * (size_t)c->SF.ni * c->SF.nk);
@@ -1638,6 +1641,109 @@ void pto_get_info(pto_case* c, pt_solve_info* info) {
   info->q = c->last_q; info->residual = c->last_res; info->res_ref = c->last_res_ref;
   info->dt = c->dt; info->time = c->time; info->solves = c->solves; info->cycles = c->cycles;
   info->n_eta = c->n_eta; info->n_temp = c->n_temp; info->n_comp = c->n_comp; info->eig_fine = c->eig_fine;
+}
+
+
+/* ---- stats: PS/basics.f90:363-462 --------------------------------------------- */
+/* integrate1Dx, PS/basics.f90:494-516 (uniform grid, xr = 1): trapezoid rule */
+static double integrate1Dx_row(const A2* f, int k, int nx, double dr) {
+  double fint = 0.0;
+  for (int i = 0; i <= nx - 1; i++) fint = fint + 0.5 * (AT(*f, i, k) * 1.0 + AT(*f, i + 1, k) * 1.0) * dr;
+  return fint;
+}
+static double integrate1Dx_vec(const double* f, int nx, double dr) {
+  double fint = 0.0;
+  for (int i = 0; i <= nx - 1; i++) fint = fint + 0.5 * (f[i] * 1.0 + f[i + 1] * 1.0) * dr;
+  return fint;
+}
+/* integrate1Dz (:518-528) from row k0 (integrate1Dz_ent, :530-541, uses kent) */
+static double integrate1Dz_from(const double* f, int k0, int nz, double ds) {
+  double fint = 0.0;
+  for (int k = k0; k <= nz - 1; k++) fint = fint + 0.5 * (f[k] * 1.0 + f[k + 1] * 1.0) * ds;
+  return fint;
+}
+/* D1 . f(i, k-2..k+2) / ds / zs */
+static double ddz_node(pto_case* c, const A2* f, int i, int k) {
+  double s = 0.0;
+  for (int j = -2; j <= 2; j++) s = s + c->D1[j + 3] * AT(*f, i, k + j);
+  return s / c->ds / 1.0;
+}
+/* strain-rate invariant at node (i,k) from psi: compute_strain_rate_invariant_direct,
+ * PS/momentum.f90:73-93 (uniform grid branch) */
+static double strain_node(pto_case* c, const A2* e, int i, int k) {
+  double sxx = 0.0, szz = 0.0, sxz = 0.0;
+  for (int j = -2; j <= 2; j++) sxx = sxx + c->D2[j + 3] * AT(*e, i + j, k);
+  for (int j = -2; j <= 2; j++) szz = szz + c->D2[j + 3] * AT(*e, i, k + j);
+  for (int b = -2; b <= 2; b++)
+    for (int a = -2; a <= 2; a++) sxz = sxz + c->Drs[b + 3][a + 3] * AT(*e, i + a, k + b);
+  double Sxx = sxx / c->dr2, Szz = szz / c->ds2, Sxz = sxz / c->drds;
+  return pow(2.0 * pow(Sxz, 2.0) + 0.5 * pow(Szz - Sxx, 2.0), 0.5);
+}
+
+int pto_stats(pto_case* c, pt_stats_row* row) {
+  const int nx = c->nx, nz = c->nz;
+  memset(row, 0, sizeof *row);
+  double* fx = (double*)calloc((size_t)(nz + 1), sizeof(double));
+  double* tmp = (double*)calloc((size_t)(nx + 1), sizeof(double));
+  A2 g = a2_new(0, nx, 0, nz);
+  /* strain_max = maxval(strain(0:nx,nz,1)) from error(:,:,1) = psi */
+  double smax = -DBL_MAX;
+  for (int i = 0; i <= nx; i++) {
+    double sv = strain_node(c, &c->SF, i, nz);
+    if (sv > smax) smax = sv;
+  }
+  /* Cavg and entrainment (kent from entrainment_setup, PS/composition.f90:765-774) */
+  for (int k = 0; k <= nz; k++) fx[k] = integrate1Dx_row(&c->Cnew, k, nx, c->dr);
+  double Cavg = integrate1Dz_from(fx, 0, nz, c->ds) / c->aspect;
+  if (!c->mass_set) {
+    c->kent = 0;
+    for (int k = 0; k <= nz; k++) if (c->ds * (double)k <= c->P.dent) c->kent = k;
+    c->mass = Cavg;
+    c->mass_set = 1;
+  }
+  double ent = integrate1Dz_from(fx, c->kent, nz, c->ds) / c->aspect / c->mass;
+  int empty = 0, tmin = 2147483647, tmax = -1;
+  for (long q = 0; q < (long)(nx + 1) * (nz + 1); q++) {
+    int v = c->nempty[q];
+    if (v == 0) empty++;
+    if (v < tmin) tmin = v;
+    if (v > tmax) tmax = v;
+  }
+  /* vrms and mobility */
+  for (int k = 0; k <= nz; k++)
+    for (int i = 0; i <= nx; i++) AT(g, i, k) = pow(AT(c->u, i, k), 2.0) + pow(AT(c->w, i, k), 2.0);
+  for (int k = 0; k <= nz; k++) fx[k] = integrate1Dx_row(&g, k, nx, c->dr);
+  double vrms = pow(integrate1Dz_from(fx, 0, nz, c->ds) / c->aspect, 0.5);
+  double mobility = pow(integrate1Dx_row(&g, nz, nx, c->dr) / c->aspect, 0.5) / vrms;
+  /* T statistics: module T, conductivity, level-1 viscosity */
+  for (int k = 0; k <= nz; k++) fx[k] = integrate1Dx_row(&c->T, k, nx, c->dr);
+  double Tavg = integrate1Dz_from(fx, 0, nz, c->ds) / c->aspect;
+  for (int k = 0; k <= nz; k++)
+    for (int i = 0; i <= nx; i++) AT(g, i, k) = c->P.RaT / AT(c->conduct, i, k) / AT(c->vis_grid[1], i, k);
+  for (int k = 0; k <= nz; k++) fx[k] = integrate1Dx_row(&g, k, nx, c->dr);
+  double RaT_avg = integrate1Dz_from(fx, 0, nz, c->ds) / c->aspect;
+  for (int i = 0; i <= nx; i++) tmp[i] = -AT(c->conduct, i, nz) * ddz_node(c, &c->T, i, nz);
+  double Nu = integrate1Dx_vec(tmp, nx, c->dr) / integrate1Dx_row(&c->T, 0, nx, c->dr);
+  for (int i = 0; i <= nx; i++) tmp[i] = -AT(c->conduct, i, 0) * ddz_node(c, &c->T, i, 0);
+  double Nu_bot = integrate1Dx_vec(tmp, nx, c->dr) / c->aspect;
+  row->q1 = -AT(c->conduct, 0, nz) * ddz_node(c, &c->T, 0, nz);
+  row->q2 = -AT(c->conduct, nx, nz) * ddz_node(c, &c->T, nx, nz);
+  row->q3 = -AT(c->conduct, nx, 0) * ddz_node(c, &c->T, nx, 0);
+  row->q4 = -AT(c->conduct, 0, 0) * ddz_node(c, &c->T, 0, 0);
+  double vmin = DBL_MAX, vmax = -DBL_MAX;
+  for (int k = 0; k <= nz; k++)
+    for (int i = 0; i <= nx; i++) {
+      double v = AT(c->vis_grid[1], i, k);
+      if (v < vmin) vmin = v;
+      if (v > vmax) vmax = v;
+    }
+  row->time = c->time; row->Tavg = Tavg; row->vrms = vrms; row->Nu = Nu; row->Nu_bot = Nu_bot;
+  row->RaT_avg = RaT_avg; row->mobility = mobility; row->strain_max = smax;
+  row->empty_cells = empty; row->tracer_min = tmin; row->tracer_max = tmax;
+  row->Cavg = Cavg; row->ent = ent; row->vis_min = vmin; row->vis_max = vmax;
+  row->n_eta = c->n_eta; row->n_temp = c->n_temp; row->n_comp = c->n_comp; row->eig_fine = c->eig_fine;
+  a2_free(&g); free(fx); free(tmp);
+  return PT_OK;
 }
 
 /* ---- unit hooks ------------------------------------------------------------------ */

@@ -39,6 +39,9 @@ int  pto_set_field(pto_case* c, int32_t field_id, const double* in);
 void pto_get_tracers(pto_case* c, double* rtr, double* str, double* Ttr, double* Ctr);
 void pto_get_nempty(pto_case* c, int32_t* out);
 void pto_get_info(pto_case* c, pt_solve_info* info);
+/* stats (PS/basics.f90:363-462); the first call after pto_load_state fixes
+ * the entrainment normalisation as the reference's tstep=0 call does */
+int  pto_stats(pto_case* c, pt_stats_row* row);
 
 /* unit-level hooks for closed-form tests */
 /* stencil tables: D1..D4 (-3:3) and the interpolation table D(1:4,-3:3) */

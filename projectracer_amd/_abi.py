@@ -68,6 +68,7 @@ class PtParams(C.Structure):
         ("visP", C.c_double), ("visT", C.c_double), ("delta_visT", C.c_double),
         ("eig_ratio_min", C.c_double), ("visC", C.c_double), ("delta_visC", C.c_double),
         ("precision", C.c_int32), ("guess", C.c_int32),
+        ("dent", C.c_double),
     ]
 
 
@@ -77,6 +78,20 @@ class PtSolveInfo(C.Structure):
         ("dt", C.c_double), ("time", C.c_double), ("solves", C.c_int64),
         ("cycles", C.c_int64), ("n_eta", C.c_int32), ("n_temp", C.c_int32),
         ("n_comp", C.c_int32), ("eig_fine", C.c_double),
+    ]
+
+
+class PtStatsRow(C.Structure):
+    """pt_stats_row: the stats.dat / viscosity_smoother.dat values of
+    PS/basics.f90:363-462."""
+    _fields_ = [
+        ("time", C.c_double), ("Tavg", C.c_double), ("vrms", C.c_double), ("Nu", C.c_double),
+        ("Nu_bot", C.c_double), ("RaT_avg", C.c_double), ("mobility", C.c_double),
+        ("strain_max", C.c_double), ("q1", C.c_double), ("q2", C.c_double), ("q3", C.c_double),
+        ("q4", C.c_double), ("empty_cells", C.c_int32), ("tracer_min", C.c_int32),
+        ("tracer_max", C.c_int32), ("Cavg", C.c_double), ("ent", C.c_double),
+        ("vis_min", C.c_double), ("vis_max", C.c_double), ("n_eta", C.c_int32),
+        ("n_temp", C.c_int32), ("n_comp", C.c_int32), ("eig_fine", C.c_double),
     ]
 
 
@@ -90,7 +105,7 @@ def default_params(**overrides) -> PtParams:
         courant=0.5, courantT=0.99, courantC=0.99, courant_stream=0.99,
         tpc=20, Tbot=0, Vbc=0, RaT=1e5, RaC=0.0, H=0.0, conduct_factor=1.0, Htype=0.0,
         ivis=0, visP=1.0, visT=1e3, delta_visT=2.0, eig_ratio_min=2.0, visC=10.0,
-        delta_visC=2.0, precision=PT_PREC_FP64, guess=PT_GUESS_WARM)
+        delta_visC=2.0, precision=PT_PREC_FP64, guess=PT_GUESS_WARM, dent=0.5)
     for k, v in overrides.items():
         setattr(p, k, v)
     return p
@@ -136,6 +151,7 @@ def load_library(path: os.PathLike | None = None):
         "pt_get_tracers": (C.c_int, [vp, dp, dp, dp, dp]),
         "pt_get_nempty": (C.c_int, [vp, ip]),
         "pt_get_info": (C.c_int, [vp, C.POINTER(PtSolveInfo)]),
+        "pt_stats": (C.c_int, [vp, C.POINTER(PtStatsRow)]),
         "pt_synchronize": (C.c_int, [vp]),
         "pt_enable_kernel_timing": (C.c_int, [vp, C.c_int32]),
         "pt_kernel_time": (C.c_int, [vp, C.c_int32, dp, C.POINTER(C.c_int64)]),
@@ -155,5 +171,5 @@ EXPORTED_SYMBOLS = [
     "pt_multigrid", "pt_compute_velocities", "pt_update_velocities", "pt_energy_time",
     "pt_tracers_to_corners", "pt_time_step", "pt_run_steps", "pt_field_extent",
     "pt_get_field", "pt_set_field", "pt_get_tracers", "pt_get_nempty", "pt_get_info",
-    "pt_synchronize", "pt_enable_kernel_timing", "pt_kernel_time", "pt_last_error",
+    "pt_stats", "pt_synchronize", "pt_enable_kernel_timing", "pt_kernel_time", "pt_last_error",
 ]

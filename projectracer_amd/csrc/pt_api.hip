@@ -96,6 +96,7 @@ void pt_default_params(pt_params* p) {
   p->H = 0.0; p->conduct_factor = 1.0; p->Htype = 0.0;
   p->ivis = 0; p->visP = 1.0; p->visT = 1e3; p->delta_visT = 2.0; p->eig_ratio_min = 2.0;
   p->visC = 10.0; p->delta_visC = 2.0; p->precision = PT_PREC_FP64; p->guess = PT_GUESS_WARM;
+  p->dent = 0.5;
 }
 
 int pt_create(const pt_params* p, void** out) {
@@ -312,6 +313,12 @@ int pt_get_info(void* p, pt_solve_info* info) {
   info->dt = c->dt; info->time = c->time; info->solves = c->solves; info->cycles = c->cycles;
   info->n_eta = c->n_eta; info->n_temp = c->n_temp; info->n_comp = c->n_comp; info->eig_fine = c->eig_fine;
   return PT_OK;
+}
+
+int pt_stats(void* p, pt_stats_row* row) {
+  Context* c = ctx_of(p);
+  if (!c || !row) return PT_ERR_INVALID;
+  return stats(*c, row);
 }
 
 int pt_synchronize(void* p) {

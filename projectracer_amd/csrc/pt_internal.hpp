@@ -122,6 +122,11 @@ struct Context {
   bool first_solve = true;
   // mixed precision work
   double* dres64;           // FP64 residual for defect correction
+  // stats: the reference's module T is the initial field until the first
+  // energy_time; entrainment is normalised by the first call's C integral
+  bool t_initial = true;
+  double mass = 0.0; bool mass_set = false;
+  double* stats_rows = nullptr;
   // timing
   bool timing = false;
   KTimer kt[PT_K_COUNT];
@@ -158,6 +163,8 @@ int load_state(Context& c, const double* T_nodes, double Tbot, double Ttop, cons
                const double* str, const double* Ttr, const double* Ctr);
 // tracer state in reference ID order into device buffers (any may be null)
 int tracers_in_id_order(Context& c, double** r, double** s, double** Tt, double** Ct);
+// ---- diagnostics (pt_stats.hip) -------------------------------------------------
+int stats(Context& c, pt_stats_row* row);
 // ---- sorting (pt_sort.hip) ---------------------------------------------------
 int sort_pairs_temp_bytes(int64_t n, int bits, size_t* bytes);
 int sort_pairs(void* tmp, size_t bytes, const int32_t* kin, int32_t* kout, const int32_t* vin, int32_t* vout,

@@ -1437,7 +1437,9 @@ static int smoother_vis(Context& c, int l, VisSet& vs, int interpolate) {
         hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, nb, (int)RED_MIN, c.red_out);
         PT_HIP(hipMemcpyAsync(c.h_red, c.red_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
         PT_HIP(hipStreamSynchronize(c.stream));
-        ratio_min = c.h_red[0];
+        // the reference's loop starts from 1e100 and keeps it where every
+        // ratio is +Inf (eigI = 0, isoviscous): PS/smoother.f90:501-506
+        ratio_min = std::min(1.0e100, c.h_red[0]);
       }
       if (option == 1) {
         if ((Nvis_store >= 1 && ratio_min > c.P.eig_ratio_min) || ivis0 || n > 20) {
@@ -1747,7 +1749,7 @@ int solver_free(Context& c) {
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
-  void* ps[] = {c.Mc, c.RowB, c.Bc, c.ipiv, c.d_info, c.red, c.red_out, c.dres64, c.Gc, c.Gwork};
+  void* ps[] = {c.Mc, c.RowB, c.Bc, c.ipiv, c.d_info, c.red, c.red_out, c.dres64, c.Gc, c.Gwork, c.stats_rows};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c.h_red) (void)hipHostFree(c.h_red);

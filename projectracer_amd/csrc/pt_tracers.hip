@@ -776,6 +776,7 @@ int energy_time(Context& c) {
   PT_HIP(hipMemcpyAsync(c.Cnew, c.C0, b5, hipMemcpyDeviceToDevice, c.stream));
   PT_TRY(tracers_to_corners(c, c.r, c.s, c.Tt));
   PT_TRY(reorder_tracers(c));
+  c.t_initial = false;   // T = Tratio (PS/energy.f90:552)
   return PT_OK;
 }
 
@@ -835,6 +836,7 @@ int load_state(Context& c, const double* T_nodes, double Tbot, double Ttop, cons
   const int nx = c.T.nx, nz = c.T.nz;
   const int64_t n = c.T.ntr;
   c.Tbot_iso = Tbot; c.Ttop_iso = Ttop; c.time = 0.0; c.first_solve = true;
+  c.t_initial = true; c.mass_set = false;
   size_t nb = sizeof(double) * (size_t)(nx + 1) * (nz + 1);
   double* dTn = c.src;   // staging (src is rebuilt every stage)
   PT_HIP(hipMemcpyAsync(dTn, T_nodes, nb, hipMemcpyHostToDevice, c.stream));

@@ -26,7 +26,8 @@ module projectracer_gpu
   PT_FIELD_TBUOY = 3, PT_FIELD_U = 4, PT_FIELD_W = 5, PT_FIELD_CNEW = 6, PT_FIELD_CVIS = 7, &
   PT_FIELD_CBUOY = 8, PT_FIELD_CONDUCT = 9, PT_FIELD_VIS = 10, PT_FIELD_RESIDUAL = 11, &
   PT_FIELD_SOURCE = 12
- integer(c_int32_t), parameter :: PT_K_RKC_STAGE_L1 = 0, PT_K_TRACER_STAGE = 1, PT_K_PROJECT = 2
+ integer(c_int32_t), parameter :: PT_K_RKC_STAGE_L1 = 0, PT_K_TRACER_STAGE = 1, PT_K_PROJECT = 2, &
+  PT_K_TRACER_RK2 = 3
 
  !! namelist values (PS/basics.f90:54-66); layout of struct pt_params
  type, bind(C) :: pt_params
@@ -44,7 +45,18 @@ module projectracer_gpu
   integer(c_int32_t) :: ivis
   real(c_double) :: visP, visT, delta_visT, eig_ratio_min, visC, delta_visC
   integer(c_int32_t) :: precision, guess
+  real(c_double) :: dent
  end type pt_params
+
+ !! stats.dat / viscosity_smoother.dat values (PS/basics.f90:363-462)
+ type, bind(C) :: pt_stats_row
+  real(c_double) :: time, Tavg, vrms, Nu, Nu_bot, RaT_avg, mobility, strain_max, q1, q2, q3, q4
+  integer(c_int32_t) :: empty_cells, tracer_min, tracer_max
+  real(c_double) :: Cavg, ent
+  real(c_double) :: vis_min, vis_max
+  integer(c_int32_t) :: n_eta, n_temp, n_comp
+  real(c_double) :: eig_fine
+ end type pt_stats_row
 
  type, bind(C) :: pt_solve_info
   integer(c_int32_t) :: q
@@ -144,6 +156,11 @@ module projectracer_gpu
    import :: c_int, c_ptr, pt_solve_info
    type(c_ptr), value :: ctx
    type(pt_solve_info), intent(out) :: info
+  end function
+  integer(c_int) function pt_stats(ctx, row) bind(C, name="pt_stats")
+   import :: c_int, c_ptr, pt_stats_row
+   type(c_ptr), value :: ctx
+   type(pt_stats_row), intent(out) :: row
   end function
   integer(c_int) function pt_synchronize(ctx) bind(C, name="pt_synchronize")
    import :: c_int, c_ptr

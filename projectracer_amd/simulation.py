@@ -126,6 +126,13 @@ class Simulation:
         self._check(self.lib.pt_get_info(self.ctx, C.byref(inf)), "get_info")
         return {name: getattr(inf, name) for name, _ in _abi.PtSolveInfo._fields_}
 
+    def stats(self) -> dict:
+        """stats (PS/basics.f90:363-462): the stats.dat and
+        viscosity_smoother.dat values of the current state."""
+        row = _abi.PtStatsRow()
+        self._check(self.lib.pt_stats(self.ctx, C.byref(row)), "stats")
+        return {name: getattr(row, name) for name, _ in _abi.PtStatsRow._fields_}
+
     # -- timing ----------------------------------------------------------------
     def enable_kernel_timing(self, on: bool = True):
         self._check(self.lib.pt_enable_kernel_timing(self.ctx, 1 if on else 0), "enable_kernel_timing")

@@ -52,6 +52,7 @@ def oracle_lib():
             "pto_get_tracers": (None, [vp, dp, dp, dp, dp]),
             "pto_get_nempty": (None, [vp, ip]),
             "pto_get_info": (None, [vp, C.POINTER(_abi.PtSolveInfo)]),
+            "pto_stats": (C.c_int, [vp, C.POINTER(_abi.PtStatsRow)]),
             "pto_fd_tables": (None, [vp, dp, dp, dp, dp, dp]),
             "pto_rkc_coefficients": (None, [vp, dp, dp, dp, dp, dp]),
             "pto_eigen_bases": (None, [vp, dp]),
@@ -154,6 +155,11 @@ class OracleSim:
         inf = _abi.PtSolveInfo()
         self.lib.pto_get_info(self.ctx, C.byref(inf))
         return {name: getattr(inf, name) for name, _ in _abi.PtSolveInfo._fields_}
+
+    def stats(self):
+        row = _abi.PtStatsRow()
+        self._chk(self.lib.pto_stats(self.ctx, C.byref(row)), "stats")
+        return {name: getattr(row, name) for name, _ in _abi.PtStatsRow._fields_}
 
 
 # ---------------------------------------------------------------------------

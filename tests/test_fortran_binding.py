@@ -19,12 +19,13 @@ program binding_probe
  implicit none
  type(pt_params) :: p
  type(pt_solve_info) :: info
+ type(pt_stats_row) :: row
  type(c_ptr) :: ctx
  integer(c_int) :: ierr
  call pt_default_params(p)
  write(*,'(a,4i8)') "ints ", p%nx, p%nz, p%grids, p%nstageS
  write(*,'(a,4es25.17)') "reals ", p%tolerance, p%dampingS, p%c_2, p%courant_stream
- write(*,'(a,2i6)') "sizes ", int(c_sizeof(p)), int(c_sizeof(info))
+ write(*,'(a,3i6)') "sizes ", int(c_sizeof(p)), int(c_sizeof(info)), int(c_sizeof(row))
  p%nx = 2
  ierr = pt_create(p, ctx)
  write(*,'(a,i4)') "invalid ", ierr
@@ -48,6 +49,7 @@ def test_fortran_interface_compiles_links_and_matches_header(tmp_path):
     assert [int(v) for v in rows["ints"].split()] == [p.nx, p.nz, p.grids, p.nstageS]
     reals = [float(v) for v in rows["reals"].split()]
     assert reals == [p.tolerance, p.dampingS, p.c_2, p.courant_stream]
-    assert [int(v) for v in rows["sizes"].split()] == [C.sizeof(_abi.PtParams), C.sizeof(_abi.PtSolveInfo)]
+    assert [int(v) for v in rows["sizes"].split()] == [C.sizeof(_abi.PtParams), C.sizeof(_abi.PtSolveInfo),
+                                                        C.sizeof(_abi.PtStatsRow)]
     assert int(rows["invalid"]) == _abi.PT_ERR_INVALID
     assert "nx" in rows["message"]

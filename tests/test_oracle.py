@@ -379,3 +379,65 @@ def test_thermochem_96_one_step_matches_reference_build():
     assert sim.info()["q"] == g["solves_tq"][4][1]
     sim.compute_velocities()
     _check_state(sim, g, "s1", names, 1e-10)
+
+
+# ---------------------------------------------------------------------------
+# stats (PS/basics.f90:363-462) against the shipped stats.dat row 1 (t = 0)
+# ---------------------------------------------------------------------------
+# stats.dat row 1 of the shipped 100x100 run (g25.15): time, Tavg, vrms, Nu,
+# Nu_bot, RaT_avg, mobility, strain_max, q1..q4, empty, min, max, Cavg, ent
+SHIPPED_STATS_ROW1 = dict(
+    time=0.0, Tavg=0.500000000000000, vrms=0.354798091580736e-03, Nu=0.500000000000018,
+    Nu_bot=0.499999999999999, RaT_avg=100000.000000000, mobility=0.773460460515217,
+    strain_max=0.423294330063474e-01, q1=0.500000000000020, q2=0.500000000000020,
+    q3=0.499999999999998, q4=0.499999999999998, empty_cells=0, tracer_min=60, tracer_max=60,
+    Cavg=0.499999323770751, ent=0.203858152927001e-01)
+
+
+def shipped_initial_case(grids=2, tolerance=1e-12):
+    """The shipped run's t=0 state: 100x100, tpc=60, tracers from 2 mzran
+    streams, custom T and C (PS/initial_conditions.f90:42-43, 98-100 with
+    f0=0, klayer=35, dlayer=0.5, RaT=1e5, RaC=5e4); T at the nodes is the
+    analytic field (stats at t=0 integrate it, not the projection)."""
+    p = _abi.default_params(aspect=1.0, nx=100, nz=100, tpc=60, grids=grids, RaT=1e5, RaC=5e4, ivis=0,
+                            courant=0.99, delta_visC=1.0, tolerance=tolerance, dent=0.5)
+    sim = OracleSim(p)
+    r, s = np.zeros(sim.ntr), np.zeros(sim.ntr)
+    oracle_lib().pto_initial_positions(sim.ctx, 2, _dp(r), _dp(s))
+    RaT, RaC, kl, dl = 1e5, 5e4, 35.0, 0.5
+
+    def comp(z):
+        return 1.0 / (1.0 + np.exp(-2.0 * kl * (dl - z)))
+
+    def temp(z):
+        return (RaC * comp(z) + (RaT - RaC) * (1.0 - z)) / RaT
+
+    zn = np.arange(101) / 100.0
+    T_nodes = np.repeat(temp(zn)[:, None], 101, axis=1)
+    case = cases.Case("shipped_100x100", p, T_nodes, float(temp(0.0)), float(temp(1.0)),
+                      r, s, temp(s), comp(s))
+    return p, case
+
+
+def test_stats_of_the_shipped_initial_state():
+    """The shipped run solved its t=0 state with the direct banded solver;
+    the multigrid solve of the same discrete operator to a relative residual
+    of 1e-12 gives the same psi to ~1e-10, so the solve-dependent values
+    (vrms, mobility, strain_max) match to 1e-8, and the others (integrals of
+    the analytic T, the projected C, the conductivity) to the printed 15
+    digits."""
+    p, case = shipped_initial_case()
+    sim = OracleSim(p)
+    sim.load_state(case)
+    sim.multigrid()
+    sim.compute_velocities()
+    st = sim.stats()
+    print({k: st[k] for k in SHIPPED_STATS_ROW1})
+    for k, ref in SHIPPED_STATS_ROW1.items():
+        got = st[k]
+        if isinstance(ref, int):
+            assert got == ref, k
+        elif k in ("vrms", "mobility", "strain_max"):
+            assert got == pytest.approx(ref, rel=1e-8), k
+        else:
+            assert got == pytest.approx(ref, rel=5e-14, abs=5e-15), k
