@@ -95,6 +95,7 @@ struct Context {
   double* dtab;                   // D(1:3,-3:3) interpolation weights on the device
   double *Cnew, *C0, *Cvis, *Cbuoy, *cond, *Textend;
   double* itab;                   // interpolation table (u, w, source)
+  double* ctab;                   // per-node interpolation polynomial coefficients
   // coarse grid: band LU (DGBTF2) and the dense operator G = A^-1 diag(RowB)
   // restricted to interior unknowns, built from the LU once per factorization
   double *Mc, *RowB, *Bc; int32_t* ipiv; int32_t* d_info;

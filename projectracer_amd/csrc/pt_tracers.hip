@@ -522,13 +522,45 @@ __device__ __forceinline__ double cube_cr(double x) {
 struct InterpConst {
   int nx, nz;
   double dr, ds, f3;             // factorial(3)
+};
+struct InterpConstTab {
   double dsp1, dsp2, dsp3;       // ds_power(1..3)
   double Dm[3][7];               // D(1..3, -3..3)
 };
 
-// interpolate(r,s,DERr,f) for the three fields sharing one cell lookup
-__device__ __forceinline__ void interp3(const double* __restrict__ tab, int nz, const InterpConst& C, double r,
-                                        double s, double out[3]) {
+// Per-node coefficient table of the interpolation polynomial.  interpolate
+// (PS/composition.f90:1359-1393) evaluates, around the tracer's node (i,k),
+//   fR(jj) = T0(jj) + a1 T1(jj) + a2 T2(jj) + a3 T3(jj),   jj = k-3..k+3,
+//   out    = fR(k) + b1 d1 + b2 d2 + b3 d3,  d_q = sum_jj D(q,jj) fR(jj) / ds^q,
+// with T_p = DERr(p) (T_0 = f), a_p = hr^p/p!, b_q = hs^q/q!.  That is the
+// bicubic  out = sum_q b_q sum_p a_p C_pq  with  C_p0 = T_p(k),
+// C_pq = sum_jj D(q,jj) T_p(jj) / ds^q.  The C_pq depend on the node only, so
+// they are formed once per stage here (16 per field) and a tracer evaluates
+// 4+1 short sums instead of 7 rows of Taylor sums and three derivative sums
+// with their divisions.  Same polynomial; the rounding differs from the
+// reference's order at the level of the last bit.
+// ctab[node*48 + f*16 + q*4 + p], node = k*(nx+1) + i, f in {u, w, source}
+__global__ void k_ctab(const double* __restrict__ tab, double* __restrict__ ctab, int nx, int nz, InterpConstTab C) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > nx || k > nz) return;
+  const double* rows = tab + ((long)i * (nz + 7) + k) * 12;   // row kk = k-3 at offset 0
+  double* out = ctab + ((long)k * (nx + 1) + i) * 48;
+  const double dsp[3] = {C.dsp1, C.dsp2, C.dsp3};
+  for (int f = 0; f < 3; f++) {
+    for (int p = 0; p < 4; p++) {
+      out[f * 16 + p] = rows[3 * 12 + f * 4 + p];
+      for (int q = 1; q <= 3; q++) {
+        double acc = 0.0;
+        for (int jj = 0; jj < 7; jj++) acc = acc + C.Dm[q - 1][jj] * rows[jj * 12 + f * 4 + p];
+        out[f * 16 + q * 4 + p] = acc / dsp[q - 1];
+      }
+    }
+  }
+}
+
+// interpolate for the three fields at one position from the node's C_pq
+__device__ __forceinline__ void interp3(const double* __restrict__ ctab, const InterpConst& C, double r, double s,
+                                        double out[3]) {
   int i = (int)round(r / C.dr), k = (int)round(s / C.ds);
   // positions reaching this point passed the projection's domain check;
   // the clamp only guards the table reads
@@ -536,38 +568,27 @@ __device__ __forceinline__ void interp3(const double* __restrict__ tab, int nz, 
   k = k < 0 ? 0 : (k > C.nz ? C.nz : k);
   // factorial(1) = 1 and factorial(2) = 2 exactly, so h/factorial(1) = h and
   // h^2/factorial(2) = h^2 * 0.5 are the reference's quotients bit for bit
-  double hr = r - (double)i * C.dr;
-  double a1 = hr, a2 = (hr * hr) * 0.5, a3 = cube_cr(hr) / C.f3;
-  double hs = s - (double)k * C.ds;
-  double b1 = hs, b2 = (hs * hs) * 0.5, b3 = cube_cr(hs) / C.f3;
-  const double* row = tab + ((long)i * (nz + 7) + (k - 3 + 3)) * 12;
-  double fR[3][7];
-#pragma unroll
-  for (int jj = 0; jj < 7; jj++) {
-    const double* e = row + jj * 12;
-#pragma unroll
-    for (int f = 0; f < 3; f++) {
-      double v = e[f * 4];
-      v = v + e[f * 4 + 1] * a1;
-      v = v + e[f * 4 + 2] * a2;
-      v = v + e[f * 4 + 3] * a3;
-      fR[f][jj] = v;
-    }
-  }
+  const double hr = r - (double)i * C.dr;
+  const double a1 = hr, a2 = (hr * hr) * 0.5, a3 = cube_cr(hr) / C.f3;
+  const double hs = s - (double)k * C.ds;
+  const double b1 = hs, b2 = (hs * hs) * 0.5, b3 = cube_cr(hs) / C.f3;
+  const double2* cf = reinterpret_cast<const double2*>(ctab + ((long)k * (C.nx + 1) + i) * 48);
 #pragma unroll
   for (int f = 0; f < 3; f++) {
-    double d1 = 0.0, d2 = 0.0, d3 = 0.0;
+    double g[4];
 #pragma unroll
-    for (int a = 0; a < 7; a++) {
-      d1 = d1 + C.Dm[0][a] * fR[f][a];
-      d2 = d2 + C.Dm[1][a] * fR[f][a];
-      d3 = d3 + C.Dm[2][a] * fR[f][a];
+    for (int q = 0; q < 4; q++) {
+      const double2 lo = cf[f * 8 + q * 2], hi = cf[f * 8 + q * 2 + 1];
+      double v = lo.x;
+      v = v + lo.y * a1;
+      v = v + hi.x * a2;
+      v = v + hi.y * a3;
+      g[q] = v;
     }
-    d1 = d1 / C.dsp1; d2 = d2 / C.dsp2; d3 = d3 / C.dsp3;
-    double fi = fR[f][3];
-    fi = fi + d1 * b1;
-    fi = fi + d2 * b2;
-    fi = fi + d3 * b3;
+    double fi = g[0];
+    fi = fi + g[1] * b1;
+    fi = fi + g[2] * b2;
+    fi = fi + g[3] * b3;
     out[f] = fi;
   }
 }
@@ -576,9 +597,8 @@ struct RkArgs {
   double *r, *s, *T;                 // r, s, Ttr (accumulators after stage 1)
   double *r0, *s0, *T0;              // step-start copies
   double *r1, *s1, *T1, *r2, *s2, *T2;
-  const double* tab;
+  const double* tab;                 // ctab (per-node polynomial coefficients)
   int64_t n;
-  int nz;
   double dtstage;                    // stage-specific step
   InterpConst C;
 };
@@ -590,7 +610,7 @@ __global__ void k_rk1(RkArgs a) {
   double rv = a.r[j], sv = a.s[j], Tv = a.T[j];
   a.r0[j] = rv; a.s0[j] = sv; a.T0[j] = Tv;
   double o[3];
-  interp3(a.tab, a.nz, a.C, rv, sv, o);
+  interp3(a.tab, a.C, rv, sv, o);
   a.r[j] = o[0]; a.s[j] = o[1]; a.T[j] = o[2];
   a.r1[j] = rv + a.dtstage * o[0];
   a.s1[j] = sv + a.dtstage * o[1];
@@ -601,7 +621,7 @@ __global__ void k_rk2(RkArgs a) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= a.n) return;
   double o[3];
-  interp3(a.tab, a.nz, a.C, a.r1[j], a.s1[j], o);
+  interp3(a.tab, a.C, a.r1[j], a.s1[j], o);
   a.r[j] = a.r[j] + 2.0 * o[0];
   a.s[j] = a.s[j] + 2.0 * o[1];
   a.T[j] = a.T[j] + 2.0 * o[2];
@@ -614,7 +634,7 @@ __global__ void k_rk3(RkArgs a) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= a.n) return;
   double o[3];
-  interp3(a.tab, a.nz, a.C, a.r2[j], a.s2[j], o);
+  interp3(a.tab, a.C, a.r2[j], a.s2[j], o);
   a.r[j] = a.r[j] + 2.0 * o[0];
   a.r1[j] = a.r0[j] + a.dtstage * o[0];
   a.s[j] = a.s[j] + 2.0 * o[1];
@@ -627,7 +647,7 @@ __global__ void k_rk4(RkArgs a) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= a.n) return;
   double o[3];
-  interp3(a.tab, a.nz, a.C, a.r1[j], a.s1[j], o);
+  interp3(a.tab, a.C, a.r1[j], a.s1[j], o);
   a.r[j] = a.r0[j] + a.dtstage * (a.r[j] + o[0]);
   a.s[j] = a.s0[j] + a.dtstage * (a.s[j] + o[1]);
   a.T[j] = a.T0[j] + a.dtstage * (a.T[j] + o[2]);
@@ -650,6 +670,11 @@ static int build_itab(Context& c) {
   dim3 b(64, 4);
   hipLaunchKernelGGL(k_itab, grid2(nz + 7, nx + 1, b), b, 0, c.stream, c.u, c.w, c.src, c.itab, nx, nz,
                      (const double*)c.dtab, c.T.dr_power[1], c.T.dr_power[2], c.T.dr_power[3]);
+  InterpConstTab C;
+  C.dsp1 = c.T.ds_power[1]; C.dsp2 = c.T.ds_power[2]; C.dsp3 = c.T.ds_power[3];
+  for (int m = 0; m < 3; m++)
+    for (int q = 0; q < 7; q++) C.Dm[m][q] = c.T.D[m + 1][q];
+  hipLaunchKernelGGL(k_ctab, grid2(nx + 1, nz + 1, b), b, 0, c.stream, c.itab, c.ctab, nx, nz, C);
   return PT_OK;
 }
 
@@ -657,12 +682,9 @@ static RkArgs rk_args(Context& c) {
   RkArgs a;
   a.r = c.r; a.s = c.s; a.T = c.Tt; a.r0 = c.r0; a.s0 = c.s0; a.T0 = c.T0t;
   a.r1 = c.r1; a.s1 = c.s1; a.T1 = c.T1t; a.r2 = c.r2; a.s2 = c.s2; a.T2 = c.T2t;
-  a.tab = c.itab; a.n = c.T.ntr; a.nz = c.T.nz;
+  a.tab = c.ctab; a.n = c.T.ntr;
   a.C.nx = c.T.nx; a.C.nz = c.T.nz;
   a.C.dr = c.T.dr; a.C.ds = c.T.ds; a.C.f3 = c.T.factorial[3];
-  a.C.dsp1 = c.T.ds_power[1]; a.C.dsp2 = c.T.ds_power[2]; a.C.dsp3 = c.T.ds_power[3];
-  for (int m = 0; m < 3; m++)
-    for (int q = 0; q < 7; q++) a.C.Dm[m][q] = c.T.D[m + 1][q];
   return a;
 }
 
@@ -885,6 +907,7 @@ int tracers_alloc(Context& c) {
   double** f5[] = {&c.Cnew, &c.C0, &c.Cvis, &c.Cbuoy, &c.cond, &c.Textend, &c.TxS, &c.CxS};
   for (double** p : f5) DALLOC(*p, b5);
   DALLOC(c.itab, sizeof(double) * (size_t)(nx + 1) * (nz + 7) * 12);
+  DALLOC(c.ctab, sizeof(double) * (size_t)(nx + 1) * (nz + 1) * 48);
   // D(1..3,-3..3) table for k_itab
   DALLOC(c.dtab, sizeof(double) * 21);
   double Dm[21];
@@ -912,7 +935,7 @@ int tracers_alloc(Context& c) {
 
 int tracers_free(Context& c) {
   void* ps[] = {c.SF, c.Tratio, c.T0, c.Tvis, c.Tbuoy, c.u, c.w, c.src, c.cond_r, c.cond_s, c.Cnew, c.C0, c.Cvis,
-                c.Cbuoy, c.cond, c.Textend, c.TxS, c.CxS, c.itab, c.dtab, c.r, c.s, c.Tt, c.Ct, c.r0, c.s0, c.T0t,
+                c.Cbuoy, c.cond, c.Textend, c.TxS, c.CxS, c.itab, c.ctab, c.dtab, c.r, c.s, c.Tt, c.Ct, c.r0, c.s0, c.T0t,
                 c.r1, c.s1, c.T1t, c.r2, c.s2, c.T2t, c.cellkey, c.skey, c.iota, c.perm, c.tid, c.tid_tmp, c.cstart, c.cend,
                 c.nempty, c.cellsum, c.d_flags, c.sort_tmp};
   for (void* p : ps)
