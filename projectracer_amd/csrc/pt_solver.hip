@@ -14,7 +14,8 @@
 //                   PS/iterative_solver.f90:472-690, 1147-1211)
 //   k_coarse_*      banded coarse operator, DGBEQU row scaling, DGBTF2 LU on one
 //                   wavefront, and the solve (DGBTRS) as a dense interior operator
-//                   built once per factorization (PS/iterative_solver.f90:1701-1887)
+//                   built once per factorization (PS/iterative_solver.f90:1701-1887),
+//                   applied by k_coarse_apply (FP64) or k_coarse_apply_wg (mixed)
 //   k_smooth_small, k_smooth_cb, k_strip_cb
 //                   whole RKC1 sweeps of coarse levels in LDS (one workgroup, or
 //                   column strips with several fused stages per launch)
@@ -264,11 +265,25 @@ struct RkcCoef {
 };
 constexpr int SMALL_THREADS = 1024;
 
+// RKC1 coefficients of stages 0..63 held one stage per lane of every
+// wavefront, read at a (uniform) stage index with v_readlane: no memory
+// access in the stage loop
+struct LaneRkc {
+  double cc, mew, vv, mewt;
+};
+__device__ __forceinline__ LaneRkc lane_rkc(const RkcCoef& co) {
+  const int j = threadIdx.x & 63;
+  return LaneRkc{co.cc[j], co.mew[j], co.vv[j], co.mewt[j]};
+}
+__device__ __forceinline__ double lane_at(double v, int j) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+  return __hiloint2double(hi, lo);
+}
+
 template <typename R, bool IV0, int PPT>
-__global__ void __launch_bounds__(SMALL_THREADS) k_smooth_small(StageArgs<R> a, R* err, R* res, int nstage,
-                                                               double cpar, RkcCoef co) {
-  extern __shared__ unsigned char smem_raw[];
-  R* buf0 = reinterpret_cast<R*>(smem_raw);
+__device__ __forceinline__ void sweep_small(const StageArgs<R>& a, R* err, R* res, int nstage, double cpar,
+                                            const RkcCoef& co, R* buf0) {
   const int n = a.n, m = a.m, p = a.pitch;
   const int ne = p * (m + 7);
   R* buf1 = buf0 + ne;
@@ -299,10 +314,11 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_smooth_small(StageArgs<R> a, 
       pc[q] = load_pc<R, IV0>(a, pg[q]);
     }
   }
+  const LaneRkc lk = lane_rkc(co);
   __syncthreads();
   // stage 1: S1 = S0 + mewt1*c*dt*F(S0) -> buf1
   {
-    const R cf = (R)(co.mewt[1] * cpar);
+    const R cf = (R)(lane_at(lk.mewt, 1) * cpar);
 #pragma unroll
     for (int q = 0; q < PPT; q++) {
       if (pi[q] == 0) continue;
@@ -318,7 +334,8 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_smooth_small(StageArgs<R> a, 
   R* spare = buf2;
   for (int j = 2; j <= nstage; j++) {
     R* out = (prev == buf0) ? spare : prev;
-    const R c0 = (R)co.cc[j], c1 = (R)co.mew[j], c2 = (R)co.vv[j], cf = (R)(co.mewt[j] * cpar);
+    const R c0 = (R)lane_at(lk.cc, j), c1 = (R)lane_at(lk.mew, j), c2 = (R)lane_at(lk.vv, j);
+    const R cf = (R)(lane_at(lk.mewt, j) * cpar);
 #pragma unroll
     for (int q = 0; q < PPT; q++) {
       if (pi[q] == 0) continue;
@@ -341,6 +358,13 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_smooth_small(StageArgs<R> a, 
     const int g = pg[q];
     res[g] = stream_space_pt<R, IV0>([&](int b, int c) { return cur[g + b * p + c]; }, a, pc[q]);
   }
+}
+
+template <typename R, bool IV0, int PPT>
+__global__ void __launch_bounds__(SMALL_THREADS) k_smooth_small(StageArgs<R> a, R* err, R* res, int nstage,
+                                                               double cpar, RkcCoef co) {
+  extern __shared__ unsigned char smem_raw[];
+  sweep_small<R, IV0, PPT>(a, err, res, nstage, cpar, co, reinterpret_cast<R*>(smem_raw));
 }
 
 // F = RHS - L psi (isoviscous, FP32) at the RB vertically adjacent points
@@ -399,10 +423,8 @@ __device__ __forceinline__ void eval_cb(const float* S, int W, int lo, int i, in
 // Factorised sums round differently from the reference's 5x5 product
 // weights, so only the FP32 path uses this.
 template <int RB>
-__global__ void __launch_bounds__(SMALL_THREADS) k_smooth_cb(StageArgs<float> a, float* err, float* res, int nstage,
-                                                            double cpar, RkcCoef co) {
-  extern __shared__ unsigned char smem_raw[];
-  float* buf0 = reinterpret_cast<float*>(smem_raw);
+__device__ __forceinline__ void sweep_cb(const StageArgs<float>& a, float* err, float* res, int nstage, double cpar,
+                                         const RkcCoef& co, float* buf0) {
   const int n = a.n, m = a.m, p = a.pitch;
   const int ne = p * (m + 7);
   float* buf1 = buf0 + ne;
@@ -437,9 +459,10 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_smooth_cb(StageArgs<float> a,
     }
   }
   const int rmax = m + 3;   // last stored row
+  const LaneRkc lk = lane_rkc(co);
   __syncthreads();
   {
-    const float cf = (float)(co.mewt[1] * cpar);
+    const float cf = (float)(lane_at(lk.mewt, 1) * cpar);
     float F[RB];
     eval_cb<RB>(buf0, p, -3, i, k0, rmax, a, rhsv, visv, F);
 #pragma unroll
@@ -452,7 +475,8 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_smooth_cb(StageArgs<float> a,
   float* spare = buf2;
   for (int j = 2; j <= nstage; j++) {
     float* out = (prev == buf0) ? spare : prev;
-    const float c0 = (float)co.cc[j], c1 = (float)co.mew[j], c2 = (float)co.vv[j], cf = (float)(co.mewt[j] * cpar);
+    const float c0 = (float)lane_at(lk.cc, j), c1 = (float)lane_at(lk.mew, j), c2 = (float)lane_at(lk.vv, j);
+    const float cf = (float)(lane_at(lk.mewt, j) * cpar);
     float F[RB];
     eval_cb<RB>(cur, p, -3, i, k0, rmax, a, rhsv, visv, F);
 #pragma unroll
@@ -474,6 +498,13 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_smooth_cb(StageArgs<float> a,
 #pragma unroll
   for (int r = 0; r < RB; r++)
     if (on[r]) res[IDX(i, k0 + r, p)] = F[r];
+}
+
+template <int RB>
+__global__ void __launch_bounds__(SMALL_THREADS) k_smooth_cb(StageArgs<float> a, float* err, float* res, int nstage,
+                                                            double cpar, RkcCoef co) {
+  extern __shared__ unsigned char smem_raw[];
+  sweep_cb<RB>(a, err, res, nstage, cpar, co, reinterpret_cast<float*>(smem_raw));
 }
 
 // ---------------------------------------------------------------------------
@@ -724,25 +755,31 @@ int reduce_to_host(Context& c, const double* d, int n, int m, int pitch, int off
 // grid transfer
 // ---------------------------------------------------------------------------
 // restrict / restrict_coefficients: out(i,k) (coarse interior) from in (fine)
-template <typename RI, typename RO>
-__global__ void k_restrict(const RI* in, int fpitch, RO* out, int cn, int cm, int cpitch) {
-  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, k = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  if (i > cn - 1 || k > cm - 1) return;
+template <typename RI>
+__device__ __forceinline__ double restrict_val(const RI* in, int fpitch, int i, int k) {
   int ii = 2 * i, kk = 2 * k;
   double corners = ((double)in[IDX(ii - 1, kk - 1, fpitch)] + (double)in[IDX(ii - 1, kk + 1, fpitch)] +
                     (double)in[IDX(ii + 1, kk + 1, fpitch)] + (double)in[IDX(ii + 1, kk - 1, fpitch)]) / 16.0;
   double sides = ((double)in[IDX(ii - 1, kk, fpitch)] + (double)in[IDX(ii, kk - 1, fpitch)] +
                   (double)in[IDX(ii + 1, kk, fpitch)] + (double)in[IDX(ii, kk + 1, fpitch)]) / 8.0;
   double centre = (double)in[IDX(ii, kk, fpitch)] / 4.0;
-  out[IDX(i, k, cpitch)] = (RO)(corners + sides + centre);
+  return corners + sides + centre;
+}
+template <typename RI, typename RO>
+__device__ __forceinline__ void restrict_pt(const RI* in, int fpitch, RO* out, int cpitch, int i, int k) {
+  out[IDX(i, k, cpitch)] = (RO)restrict_val<RI>(in, fpitch, i, k);
+}
+template <typename RI, typename RO>
+__global__ void k_restrict(const RI* in, int fpitch, RO* out, int cn, int cm, int cpitch) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, k = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > cn - 1 || k > cm - 1) return;
+  restrict_pt<RI, RO>(in, fpitch, out, cpitch, i, k);
 }
 
 // prolong: fine += bilinear(coarse), then the fine ghosts (four parity classes
 // of PS/iterative_solver.f90:364-394 evaluated per fine point)
 template <typename RC, typename RF>
-__global__ void k_prolong(const RC* ec, int cpitch, RF* ef, int fn, int fm, int fpitch, RF sz) {
-  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, k = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  if (i > fn - 1 || k > fm - 1) return;
+__device__ __forceinline__ RF prolong_val(const RC* ec, int cpitch, const RF* ef, int fpitch, int i, int k) {
   RF add;
   bool ie = (i % 2) == 0, ke = (k % 2) == 0;
   if (ie && ke) add = (RF)ec[IDX(i / 2, k / 2, cpitch)];
@@ -751,8 +788,62 @@ __global__ void k_prolong(const RC* ec, int cpitch, RF* ef, int fn, int fm, int 
   else
     add = ((RF)ec[IDX((i - 1) / 2, (k - 1) / 2, cpitch)] + (RF)ec[IDX((i + 1) / 2, (k - 1) / 2, cpitch)] +
            (RF)ec[IDX((i + 1) / 2, (k + 1) / 2, cpitch)] + (RF)ec[IDX((i - 1) / 2, (k + 1) / 2, cpitch)]) / (RF)4;
-  RF y = ef[IDX(i, k, fpitch)] + add;
-  store_with_ghosts<RF>(ef, i, k, fn, fm, fpitch, y, sz);
+  return ef[IDX(i, k, fpitch)] + add;
+}
+template <typename RC, typename RF>
+__device__ __forceinline__ void prolong_pt(const RC* ec, int cpitch, RF* ef, int fn, int fm, int fpitch, RF sz, int i,
+                                           int k) {
+  store_with_ghosts<RF>(ef, i, k, fn, fm, fpitch, prolong_val<RC, RF>(ec, cpitch, ef, fpitch, i, k), sz);
+}
+template <typename RC, typename RF>
+__global__ void k_prolong(const RC* ec, int cpitch, RF* ef, int fn, int fm, int fpitch, RF sz) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, k = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > fn - 1 || k > fm - 1) return;
+  prolong_pt<RC, RF>(ec, cpitch, ef, fn, fm, fpitch, sz, i, k);
+}
+
+// coarse correction err = G rhs by one workgroup (mixed path): the column
+// range is split in CA_SEG fixed segments, each summed in order by its own
+// thread group (loads issued 8 at a time), and the segment sums are added in
+// segment order.  LDS: (1 + CA_SEG) * nci doubles.
+constexpr int CA_SEG = 4, CA_ROWS = SMALL_THREADS / CA_SEG;
+__device__ __forceinline__ void coarse_apply_wg(const double* G, const float* rhs, float* err, int nxc, int nzc,
+                                                int pitch, float sz, double* sm) {
+  const int nci = (nxc - 1) * (nzc - 1);
+  double* rv = sm;
+  double* part = sm + nci;
+  for (int q = threadIdx.x; q < nci; q += SMALL_THREADS) {
+    int iq = 1 + q % (nxc - 1), kq = 1 + q / (nxc - 1);
+    rv[q] = (double)rhs[IDX(iq, kq, pitch)];
+  }
+  __syncthreads();
+  const int seg = threadIdx.x / CA_ROWS, qs = (nci + CA_SEG - 1) / CA_SEG;
+  const int q0 = seg * qs, q1 = min(nci, q0 + qs);
+  for (int pi = threadIdx.x % CA_ROWS; pi < nci; pi += CA_ROWS) {
+    double acc = 0.0;
+    int q = q0;
+    for (; q + 8 <= q1; q += 8) {
+      double g[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) g[u] = G[(long)(q + u) * nci + pi];
+#pragma unroll
+      for (int u = 0; u < 8; u++) acc = acc + g[u] * rv[q + u];
+    }
+    for (; q < q1; q++) acc = acc + G[(long)q * nci + pi] * rv[q];
+    part[seg * nci + pi] = acc;
+  }
+  __syncthreads();
+  for (int pi = threadIdx.x; pi < nci; pi += SMALL_THREADS) {
+    double acc = part[pi];
+    for (int sgi = 1; sgi < CA_SEG; sgi++) acc = acc + part[sgi * nci + pi];
+    int ip = 1 + pi % (nxc - 1), kp = 1 + pi / (nxc - 1);
+    store_with_ghosts<float>(err, ip, kp, nxc, nzc, pitch, (float)acc, sz);
+  }
+}
+__global__ void __launch_bounds__(SMALL_THREADS) k_coarse_apply_wg(const double* G, const float* rhs, float* err,
+                                                                   int nxc, int nzc, int pitch, float sz) {
+  extern __shared__ double ca_sm[];
+  coarse_apply_wg(G, rhs, err, nxc, nzc, pitch, sz, ca_sm);
 }
 
 // compute_RHS(1): RaT*d(Tbuoy)/dx - RaC*d(Cbuoy)/dx (5-point D1)
@@ -1203,11 +1294,12 @@ static int launch_small_v(Context& c, int ppt, size_t lds, const StageArgs<R>& a
 }
 
 // rows per thread of the column-blocked FP32 sweep: 4 when that keeps at
-// least half the workgroup busy with one column block per thread, else 0
-// (smaller levels are faster with one point per thread)
+// least half the workgroup busy with one column block per thread, 1 when
+// every interior point has a thread of its own, else 0
 static int cb_rows(int n, int m) {
   const int b4 = (n - 1) * ((m - 1 + 3) / 4);
-  return (b4 <= SMALL_THREADS && 2 * b4 >= SMALL_THREADS) ? 4 : 0;
+  if (b4 <= SMALL_THREADS && 2 * b4 >= SMALL_THREADS) return 4;
+  return (n - 1) * (m - 1) <= SMALL_THREADS ? 1 : 0;
 }
 template <int RB>
 static int launch_cb_t(Context& c, size_t lds, const StageArgs<float>& a, float* err, float* res, int s, double cc,
@@ -1223,7 +1315,7 @@ static int launch_cb_t(Context& c, size_t lds, const StageArgs<float>& a, float*
 }
 static int launch_cb(Context& c, int rb, size_t lds, const StageArgs<float>& a, float* err, float* res, int s,
                      double cc, const RkcCoef& co) {
-  (void)rb;
+  if (rb == 1) return launch_cb_t<1>(c, lds, a, err, res, s, cc, co);
   return launch_cb_t<4>(c, lds, a, err, res, s, cc, co);
 }
 
@@ -1286,6 +1378,26 @@ static int strip_sweep(Context& c, int l, const StripCfg& sc, double cpar, const
     j0 = j1 + 1;
   }
   std::swap(*f[0], *f[rd]);   // the last segment's output becomes the error buffer
+  return PT_OK;
+}
+
+// first level of the fused sub-cycle, 0 when it does not apply: levels
+// ls..grids-1 are single-workgroup sweeps whose argument table fits the
+// kernel arguments, the coarse operator fits one workgroup, and in the mixed
+// path level ls-1 is an FP32 level (ls >= 3)
+static int launch_coarse_apply_wg(Context& c, const float* rhs, float* err, int n, int m, int pitch, float sz) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    PT_HIP(hipFuncSetAttribute((const void*)k_coarse_apply_wg, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)SMALL_LDS_BYTES));
+    attr_set = true;
+  }
+  const size_t lds = sizeof(double) * (1 + CA_SEG) * c.nci;
+  if (lds > SMALL_LDS_BYTES) {
+    set_error("coarsest grid too large for the single-workgroup coarse operator");
+    return PT_ERR_UNSUPPORTED;
+  }
+  hipLaunchKernelGGL(k_coarse_apply_wg, dim3(1), dim3(SMALL_THREADS), lds, c.stream, c.Gc, rhs, err, n, m, pitch, sz);
   return PT_OK;
 }
 
@@ -1636,8 +1748,7 @@ int multigrid(Context& c) {
           hipLaunchKernelGGL((k_coarse_apply<double, double>), gb, dim3(256), sh, c.stream, c.Gc, L.rhs, L.err, L.g.n,
                              L.g.m, L.g.pitch, sz);
         else
-          hipLaunchKernelGGL((k_coarse_apply<float, float>), gb, dim3(256), sh, c.stream, c.Gc, L.frhs, L.ferr,
-                             L.g.n, L.g.m, L.g.pitch, (float)sz);
+          PT_TRY(launch_coarse_apply_wg(c, L.frhs, L.ferr, L.g.n, L.g.m, L.g.pitch, (float)sz));
       }
       if (l == 1 && (resmag1 / c.res_ref1) <= c.P.tolerance) { done = true; break; }
       if (t.multi[p + 1] < t.multi[p]) {
