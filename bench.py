@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--event-steps", type=int, default=3,
                     help="steps run after the timed ones with per-kernel HIP events")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-restart-io", action="store_true",
+                    help="skip timing pt_write_restart / pt_read_restart after the headline run")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--paths", default="mixed,fp64",
                     help="GPU paths to time (profiling runs may select one)")
@@ -74,7 +76,7 @@ def make_case(args, precision):
     return case
 
 
-def time_gpu(args, precision):
+def time_gpu(args, precision, restart_io=False):
     from projectracer_amd import Simulation, _abi
     case = make_case(args, precision)
     sim = Simulation(case.params)
@@ -103,6 +105,18 @@ def time_gpu(args, precision):
     pms, plaunch = sim.kernel_time(_abi.PT_K_PROJECT)
     r2ms, r2launch = sim.kernel_time(_abi.PT_K_TRACER_RK2)
     sim.enable_kernel_timing(False)
+    restart = None
+    if restart_io:
+        # checkpoint / resume of the whole device state in the reference's files
+        with tempfile.TemporaryDirectory(prefix="ptrst_") as td:
+            t2 = time.perf_counter()
+            sim.print_restart(td)
+            t3 = time.perf_counter()
+            nbytes = sum(f.stat().st_size for f in Path(td).iterdir())
+            sim.read_restart(td, case.Tbot_iso, case.Ttop_iso)
+            t4 = time.perf_counter()
+        restart = dict(write_s=t3 - t2, read_s=t4 - t3, bytes=nbytes,
+                       write_GBs=nbytes / (t3 - t2) / 1e9, read_GBs=nbytes / (t4 - t3) / 1e9)
     out = dict(
         seconds=t1 - t0,
         ms_per_step=1000.0 * (t1 - t0) / args.steps,
@@ -111,6 +125,7 @@ def time_gpu(args, precision):
         rkc_ms=kms, rkc_launches=klaunch, tracer_ms=tms, tracer_launches=tlaunch,
         project_ms=pms, project_launches=plaunch, rk2_ms=r2ms, rk2_launches=r2launch,
         nx=sim.nx, nz=sim.nz, ntr=sim.ntr, last_residual_ratio=inf1["residual"] / inf1["res_ref"],
+        restart=restart,
     )
     sim.close()
     return out
@@ -176,7 +191,7 @@ def main():
             r = time_gpu(args, prec)
             print(f"{name}: {r['ms_per_step']:.2f} ms/step, {r['cycles']} cycles", flush=True)
         return
-    mixed = time_gpu(args, _abi.PT_PREC_MIXED)
+    mixed = time_gpu(args, _abi.PT_PREC_MIXED, restart_io=not args.no_restart_io)
     fp64 = time_gpu(args, _abi.PT_PREC_FP64)
     steps_s = args.steps / mixed["seconds"]
     fp64_steps_s = args.steps / fp64["seconds"]
@@ -258,6 +273,10 @@ def main():
             "projection_ms_per_step": fp64["project_ms"] / args.event_steps,
         },
     }
+    if mixed["restart"]:
+        # print_restart + restart=1 read of the whole state (Trestart, restart_tracers) to a
+        # temporary directory on the host
+        result["restart_io"] = mixed["restart"]
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(result), flush=True)

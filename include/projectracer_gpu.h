@@ -184,6 +184,25 @@ int pt_get_nempty(void* ctx, int32_t* nempty);
 int pt_get_info(void* ctx, pt_solve_info* info);
 /* stats (PS/basics.f90:363-462) of the current state; see pt_stats_row */
 int pt_stats(void* ctx, pt_stats_row* row);
+
+/* ---- checkpoint / resume in the reference's files --------------------
+ * pt_write_restart: print_restart (PS/main.f90:457-496) into directory dir:
+ *   Trestart        xg(-6:nx+6), zg(-6:nz+6), T(-3:nx+3,-3:nz+3),
+ *                   Cnew(1,-5:nx+5,-5:nz+5), SF(-3:nx+3,-3:nz+3)
+ *   restart_tracers rtr, str, Ctr, Ttr in tracer ID order
+ * each one Fortran unformatted sequential record (4-byte length before and
+ * after), the files of a RaT /= 0, comp=1, tracer=1, iC_continuous=1 run.
+ * T is the module T (the initial field before the first energy_time).
+ * pt_read_restart: the restart=1 start (PS/grid.f90:55-103, PS/main.f90:
+ *   204-209): reads both files, re-projects the tracers onto T and Cnew
+ *   (file values kept at nodes without tracers), keeps SF as the initial
+ *   guess, resets time to 0.  The walls take Tbot_iso/Ttop_iso from the
+ *   caller: the reference leaves them at 0 on restart.
+ * PT_ERR_INVALID when a file is missing, has another record length than
+ * this context's grid and tracer count, or another grid.  Records above
+ * 2^31-1 bytes are PT_ERR_UNSUPPORTED. */
+int pt_write_restart(void* ctx, const char* dir);
+int pt_read_restart(void* ctx, const char* dir, double Tbot_iso, double Ttop_iso);
 int pt_synchronize(void* ctx);
 
 /* ---- timing hooks (no reference counterpart; PS/basics.f90:27-31 hold

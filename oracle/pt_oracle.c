@@ -31,6 +31,8 @@
  *  - mzran / initialize_tracers positions: the reference's own initial
  *    clouds with 2 and 3 OpenMP threads (flang build), bit for bit, and the
  *    shipped T10000 snapshot (2-thread cloud projected and filtered).
+ *  - build_grid coordinates (print_restart's xg, zg): the shipped Trestart
+ *    record, bit for bit.
  *  - parity unpinned: the insulating-bottom (Tbot=1) and rigid top/bottom
  *    (Vbc=1) boundary branches and the ivis=1 viscosity law are restated
  *    but no fixture or closed form covers them.
@@ -1841,4 +1843,27 @@ void pto_initial_positions(pto_case* c, int nthreads, double* rtr, double* str) 
     next.n = (int32_t)((uint32_t)69069 * (uint32_t)seed.n + (uint32_t)1013904243);
     seed = next;
   }
+}
+
+/* build_grid (PS/grid.f90:46-116) for gridX = gridZ = 0: r = dr*i at the
+ * nodes, mirrored halos of span+span_interp = 6 points, exact end points */
+void pto_grid_coords(int nx, int nz, double aspect, double* xg, double* zg) {
+  const int h = 6;
+  const double dr = aspect / (double)nx, ds = 1.0 / (double)nz;
+  double* X = xg + h;
+  double* Z = zg + h;
+  for (int i = 0; i <= nx; i++) X[i] = dr * (double)i;
+  for (int k = 0; k <= nz; k++) Z[k] = ds * (double)k;
+  for (int i = 1; i <= h; i++) {
+    X[-i] = -X[i];
+    X[nx + i] = aspect - X[nx - i] + aspect;
+  }
+  for (int k = 1; k <= h; k++) {
+    Z[-k] = -Z[k];
+    Z[nz + k] = 1.0 - Z[nz - k] + 1.0;
+  }
+  X[0] = 0.0;
+  Z[0] = 0.0;
+  X[nx] = aspect;
+  Z[nz] = 1.0;
 }

@@ -69,6 +69,9 @@ void pto_mzran(int64_t n, double* out);
 /* initialize_tracers positions as the reference draws them with nthreads
  * OpenMP threads (PS/composition.f90:786-847); ntr = (nx+1)(nz+1)tpc */
 void pto_initial_positions(pto_case* c, int nthreads, double* rtr, double* str);
+/* node coordinates of a uniform grid with the halos print_restart writes
+ * (build_grid, PS/grid.f90:46-116): xg(-6:nx+6), zg(-6:nz+6) */
+void pto_grid_coords(int nx, int nz, double aspect, double* xg, double* zg);
 
 #ifdef __cplusplus
 }

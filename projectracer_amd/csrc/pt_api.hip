@@ -321,6 +321,18 @@ int pt_stats(void* p, pt_stats_row* row) {
   return stats(*c, row);
 }
 
+int pt_write_restart(void* p, const char* dir) {
+  Context* c = ctx_of(p);
+  if (!c || !dir) return PT_ERR_INVALID;
+  return write_restart(*c, dir);
+}
+
+int pt_read_restart(void* p, const char* dir, double Tbot_iso, double Ttop_iso) {
+  Context* c = ctx_of(p);
+  if (!c || !dir) return PT_ERR_INVALID;
+  return read_restart(*c, dir, Tbot_iso, Ttop_iso);
+}
+
 int pt_synchronize(void* p) {
   Context* c = ctx_of(p);
   if (!c) return PT_ERR_INVALID;

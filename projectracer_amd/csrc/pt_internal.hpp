@@ -164,6 +164,11 @@ int load_state(Context& c, const double* T_nodes, double Tbot, double Ttop, cons
                const double* str, const double* Ttr, const double* Ctr);
 // tracer state in reference ID order into device buffers (any may be null)
 int tracers_in_id_order(Context& c, double** r, double** s, double** Tt, double** Ct);
+int restart_state(Context& c, const double* T3, const double* C5, const double* SF3, double Tbot, double Ttop,
+                  const double* rtr, const double* str, const double* Ttr, const double* Ctr);
+int stream_bcs(Context& c, double* S);
+int write_restart(Context& c, const char* dir);
+int read_restart(Context& c, const char* dir, double Tbot, double Ttop);
 // ---- diagnostics (pt_stats.hip) -------------------------------------------------
 int stats(Context& c, pt_stats_row* row);
 // ---- sorting (pt_sort.hip) ---------------------------------------------------

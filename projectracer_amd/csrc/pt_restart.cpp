@@ -1,0 +1,157 @@
+// pt_restart.cpp -- checkpoint and resume in the reference's own files.
+//
+// print_restart (PS/main.f90:457-496) writes, for RaT != 0, comp = 1,
+// tracer = 1 and iC_continuous = 1:
+//   Trestart         one record: xg(-6:nx+6), zg(-6:nz+6), T(-3:nx+3,-3:nz+3),
+//                    Cnew(1,-5:nx+5,-5:nz+5), SF(-3:nx+3,-3:nz+3)
+//   restart_tracers  one record: rtr, str, Ctr, Ttr (ntr each, tracer ID order)
+// as Fortran unformatted sequential files: each record is framed by its
+// length in bytes as a 4-byte little-endian integer before and after.  The
+// restart=1 branch of build_grid (PS/grid.f90:55-103) reads them back.
+// Records longer than 2^31-1 bytes (beyond 67 M tracers) would need the
+// compilers' sub-record framing and are refused.
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "pt_internal.hpp"
+
+namespace pt {
+
+namespace {
+
+std::string join(const char* dir, const char* name) {
+  std::string d = dir ? dir : ".";
+  if (!d.empty() && d.back() != '/') d += '/';
+  return d + name;
+}
+
+struct File {
+  FILE* f = nullptr;
+  ~File() {
+    if (f) std::fclose(f);
+  }
+};
+
+// one record from the parts, framed by its length
+int write_record(const std::string& path, const std::vector<std::pair<const double*, size_t>>& parts) {
+  size_t bytes = 0;
+  for (auto& p : parts) bytes += sizeof(double) * p.second;
+  if (bytes > 0x7fffffffu) {
+    set_error(path + ": record of " + std::to_string(bytes) + " bytes exceeds the 4-byte record length");
+    return PT_ERR_UNSUPPORTED;
+  }
+  File fh;
+  fh.f = std::fopen(path.c_str(), "wb");
+  if (!fh.f) {
+    set_error("cannot open " + path + " for writing");
+    return PT_ERR_INVALID;
+  }
+  const uint32_t len = (uint32_t)bytes;
+  bool ok = std::fwrite(&len, 4, 1, fh.f) == 1;
+  for (auto& p : parts) ok = ok && std::fwrite(p.first, sizeof(double), p.second, fh.f) == p.second;
+  ok = ok && std::fwrite(&len, 4, 1, fh.f) == 1;
+  ok = (std::fclose(fh.f) == 0) && ok;
+  fh.f = nullptr;
+  if (!ok) {
+    set_error("write to " + path + " failed");
+    return PT_ERR_INVALID;
+  }
+  return PT_OK;
+}
+
+// the single record of `path`, which must hold exactly `count` doubles
+int read_record(const std::string& path, size_t count, std::vector<double>& out) {
+  File fh;
+  fh.f = std::fopen(path.c_str(), "rb");
+  if (!fh.f) {
+    set_error("cannot open " + path);
+    return PT_ERR_INVALID;
+  }
+  uint32_t head = 0, tail = 0;
+  out.resize(count);
+  const size_t bytes = sizeof(double) * count;
+  if (std::fread(&head, 4, 1, fh.f) != 1 || head != bytes) {
+    set_error(path + ": record of " + std::to_string(head) + " bytes, this context expects " + std::to_string(bytes));
+    return PT_ERR_INVALID;
+  }
+  if (std::fread(out.data(), sizeof(double), count, fh.f) != count || std::fread(&tail, 4, 1, fh.f) != 1 ||
+      tail != head) {
+    set_error(path + ": truncated or malformed record");
+    return PT_ERR_INVALID;
+  }
+  return PT_OK;
+}
+
+// xg, zg of a uniform grid as build_grid forms them (PS/grid.f90:46-116):
+// nodes r = dr*i, mirrored halos, exact end points
+void grid_coords(const Tables& t, std::vector<double>& xg, std::vector<double>& zg) {
+  const int h = 6;   // span + span_interp
+  xg.assign(t.nx + 1 + 2 * h, 0.0);
+  zg.assign(t.nz + 1 + 2 * h, 0.0);
+  auto X = [&](int i) -> double& { return xg[i + h]; };
+  auto Z = [&](int k) -> double& { return zg[k + h]; };
+  for (int i = 0; i <= t.nx; i++) X(i) = t.dr * (double)i;
+  for (int k = 0; k <= t.nz; k++) Z(k) = t.ds * (double)k;
+  for (int i = 1; i <= h; i++) {
+    X(-i) = -X(i);
+    X(t.nx + i) = t.aspect - X(t.nx - i) + t.aspect;
+  }
+  for (int k = 1; k <= h; k++) {
+    Z(-k) = -Z(k);
+    Z(t.nz + k) = 1.0 - Z(t.nz - k) + 1.0;
+  }
+  X(0) = 0.0; Z(0) = 0.0;
+  X(t.nx) = t.aspect; Z(t.nz) = 1.0;
+}
+
+}  // namespace
+
+int write_restart(Context& c, const char* dir) {
+  const Tables& t = c.T;
+  const size_t n3 = (size_t)(t.nx + 7) * (t.nz + 7), n5 = (size_t)(t.nx + 11) * (t.nz + 11);
+  const size_t ntr = (size_t)t.ntr;
+  std::vector<double> xg, zg, T(n3), C(n5), SF(n3), tr(4 * ntr);
+  grid_coords(t, xg, zg);
+  // the module T: the initial field until the first energy_time, then Tratio (PS/energy.f90:552)
+  const double* Tdev = c.t_initial ? c.T0 : c.Tratio;
+  PT_HIP(hipMemcpyAsync(T.data(), Tdev, sizeof(double) * n3, hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipMemcpyAsync(C.data(), c.Cnew, sizeof(double) * n5, hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipMemcpyAsync(SF.data(), c.SF, sizeof(double) * n3, hipMemcpyDeviceToHost, c.stream));
+  double *dr = nullptr, *ds = nullptr, *dT = nullptr, *dC = nullptr;
+  PT_TRY(tracers_in_id_order(c, &dr, &ds, &dT, &dC));
+  const size_t tb = sizeof(double) * ntr;
+  PT_HIP(hipMemcpyAsync(tr.data(), dr, tb, hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipMemcpyAsync(tr.data() + ntr, ds, tb, hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipMemcpyAsync(tr.data() + 2 * ntr, dC, tb, hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipMemcpyAsync(tr.data() + 3 * ntr, dT, tb, hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipStreamSynchronize(c.stream));
+  PT_TRY(write_record(join(dir, "Trestart"),
+                      {{xg.data(), xg.size()}, {zg.data(), zg.size()}, {T.data(), n3}, {C.data(), n5}, {SF.data(), n3}}));
+  return write_record(join(dir, "restart_tracers"), {{tr.data(), tr.size()}});
+}
+
+int read_restart(Context& c, const char* dir, double Tbot, double Ttop) {
+  const Tables& t = c.T;
+  const size_t n3 = (size_t)(t.nx + 7) * (t.nz + 7), n5 = (size_t)(t.nx + 11) * (t.nz + 11);
+  const size_t ntr = (size_t)t.ntr;
+  std::vector<double> xg, zg, rec, tr;
+  grid_coords(t, xg, zg);
+  PT_TRY(read_record(join(dir, "Trestart"), xg.size() + zg.size() + 2 * n3 + n5, rec));
+  // the file's grid must be this context's uniform grid
+  double dev = 0.0;
+  for (size_t i = 0; i < xg.size(); i++) dev = std::max(dev, std::fabs(rec[i] - xg[i]));
+  for (size_t k = 0; k < zg.size(); k++) dev = std::max(dev, std::fabs(rec[xg.size() + k] - zg[k]));
+  if (!(dev <= 1e-12 * std::max(1.0, t.aspect))) {
+    set_error("Trestart grid coordinates differ from this context's grid by " + std::to_string(dev));
+    return PT_ERR_INVALID;
+  }
+  PT_TRY(read_record(join(dir, "restart_tracers"), 4 * ntr, tr));
+  const double* T3 = rec.data() + xg.size() + zg.size();
+  const double* C5 = T3 + n3;
+  const double* SF3 = C5 + n5;
+  const double *r = tr.data(), *s = r + ntr, *Cq = s + ntr, *Tq = Cq + ntr;
+  return restart_state(c, T3, C5, SF3, Tbot, Ttop, r, s, Tq, Cq);
+}
+
+}  // namespace pt

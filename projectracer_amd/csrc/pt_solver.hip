@@ -846,6 +846,18 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_coarse_apply_wg(const double*
   coarse_apply_wg(G, rhs, err, nxc, nzc, pitch, sz, ca_sm);
 }
 
+// enforceBCs_stream (PS/iterative_solver.f90:1616-1664) on a whole level-1
+// field: walls (with their ghost extensions) zero, ghosts from the interior
+__global__ void k_stream_bcs(double* S, int n, int m, int p, double sz) {
+  const int i = -3 + (int)(blockIdx.x * blockDim.x + threadIdx.x), k = -3 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i > n + 3 || k > m + 3) return;
+  if (i == 0 || i == n || k == 0 || k == m) {
+    S[IDX(i, k, p)] = 0.0;
+    return;
+  }
+  if (i >= 1 && i <= n - 1 && k >= 1 && k <= m - 1) store_with_ghosts<double>(S, i, k, n, m, p, S[IDX(i, k, p)], sz);
+}
+
 // compute_RHS(1): RaT*d(Tbuoy)/dx - RaC*d(Cbuoy)/dx (5-point D1)
 __global__ void k_rhs1(const double* Tb, const double* Cb, double* rhs, int nx, int nz, int p3, int p5, double dr,
                        double RaT, double RaC, double d0, double d1, double d2, double d3, double d4) {
@@ -1777,6 +1789,13 @@ int multigrid(Context& c) {
   c.last_q = q;
   c.residual1 = resmag1;
   c.solves++;
+  return PT_OK;
+}
+
+int stream_bcs(Context& c, double* S) {
+  dim3 b16(64, 4);
+  hipLaunchKernelGGL(k_stream_bcs, grid2(c.T.nx + 7, c.T.nz + 7, b16), b16, 0, c.stream, S, c.T.nx, c.T.nz,
+                     c.T.nx + 7, c.P.Vbc == 0 ? -1.0 : 1.0);
   return PT_OK;
 }
 

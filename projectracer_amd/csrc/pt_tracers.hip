@@ -853,10 +853,55 @@ int tracers_in_id_order(Context& c, double** r, double** s, double** Tt, double*
   return PT_OK;
 }
 
+// tracers (reference ID order) onto the device, projected onto the nodes, in cell order
+static int place_tracers(Context& c, const double* rtr, const double* str, const double* Ttr, const double* Ctr) {
+  size_t tb = sizeof(double) * (size_t)c.T.ntr;
+  PT_HIP(hipMemcpyAsync(c.r, rtr, tb, hipMemcpyHostToDevice, c.stream));
+  PT_HIP(hipMemcpyAsync(c.s, str, tb, hipMemcpyHostToDevice, c.stream));
+  PT_HIP(hipMemcpyAsync(c.Tt, Ttr, tb, hipMemcpyHostToDevice, c.stream));
+  PT_HIP(hipMemcpyAsync(c.Ct, Ctr, tb, hipMemcpyHostToDevice, c.stream));
+  PT_HIP(hipMemcpyAsync(c.tid, c.iota, sizeof(int32_t) * (size_t)c.T.ntr, hipMemcpyDeviceToDevice, c.stream));
+  PT_TRY(tracers_to_corners(c, c.r, c.s, c.Tt));
+  return reorder_tracers(c);
+}
+
+static int clear_coarse_errors(Context& c) {
+  for (int l = 2; l <= c.T.grids; l++) {
+    PT_HIP(hipMemsetAsync(c.L[l].err, 0, sizeof(double) * c.L[l].nelem, c.stream));
+    if (c.L[l].ferr) PT_HIP(hipMemsetAsync(c.L[l].ferr, 0, sizeof(float) * c.L[l].nelem, c.stream));
+  }
+  return PT_OK;
+}
+
+// restart=1 start (PS/grid.f90:55-103, PS/main.f90:204-209, :226-229): T,
+// Cnew and SF as print_restart wrote them (whole arrays with halos), the
+// tracers in ID order.  T and Cnew are the values kept at nodes without
+// tracers; the module T becomes the re-projected field.  The walls take
+// Tbot/Ttop: the reference leaves Tbot_iso/Ttop_iso at 0 on this path.
+int restart_state(Context& c, const double* T3, const double* C5, const double* SF3, double Tbot, double Ttop,
+                  const double* rtr, const double* str, const double* Ttr, const double* Ctr) {
+  const int nx = c.T.nx, nz = c.T.nz;
+  c.Tbot_iso = Tbot; c.Ttop_iso = Ttop; c.time = 0.0; c.first_solve = true;
+  c.t_initial = true; c.mass_set = false;
+  const size_t b3 = sizeof(double) * (size_t)(nx + 7) * (nz + 7), b5 = sizeof(double) * (size_t)(nx + 11) * (nz + 11);
+  PT_HIP(hipMemcpyAsync(c.T0, T3, b3, hipMemcpyHostToDevice, c.stream));
+  hipLaunchKernelGGL(k_t_walls, dim3((nx + 7 + 255) / 256), dim3(256), 0, c.stream, c.T0, nx, nz, 3, nx + 7, 0, Tbot,
+                     Ttop);
+  t_bcs(c, c.T0);
+  PT_HIP(hipMemcpyAsync(c.Cnew, C5, b5, hipMemcpyHostToDevice, c.stream));
+  c_bcs(c, c.Cnew);
+  PT_TRY(place_tracers(c, rtr, str, Ttr, Ctr));
+  PT_HIP(hipMemcpyAsync(c.T0, c.Tratio, b3, hipMemcpyDeviceToDevice, c.stream));
+  PT_HIP(hipMemcpyAsync(c.SF, SF3, b3, hipMemcpyHostToDevice, c.stream));
+  PT_TRY(stream_bcs(c, c.SF));
+  PT_TRY(clear_coarse_errors(c));
+  PT_HIP(hipStreamSynchronize(c.stream));
+  return PT_OK;
+}
+
 int load_state(Context& c, const double* T_nodes, double Tbot, double Ttop, const double* rtr, const double* str,
                const double* Ttr, const double* Ctr) {
   const int nx = c.T.nx, nz = c.T.nz;
-  const int64_t n = c.T.ntr;
   c.Tbot_iso = Tbot; c.Ttop_iso = Ttop; c.time = 0.0; c.first_solve = true;
   c.t_initial = true; c.mass_set = false;
   size_t nb = sizeof(double) * (size_t)(nx + 1) * (nz + 1);
@@ -868,20 +913,10 @@ int load_state(Context& c, const double* T_nodes, double Tbot, double Ttop, cons
   hipLaunchKernelGGL(k_t_walls, dim3((nx + 7 + 255) / 256), dim3(256), 0, c.stream, c.T0, nx, nz, 3, nx + 7, 0, Tbot,
                      Ttop);
   t_bcs(c, c.T0);
-  size_t tb = sizeof(double) * (size_t)n;
-  PT_HIP(hipMemcpyAsync(c.r, rtr, tb, hipMemcpyHostToDevice, c.stream));
-  PT_HIP(hipMemcpyAsync(c.s, str, tb, hipMemcpyHostToDevice, c.stream));
-  PT_HIP(hipMemcpyAsync(c.Tt, Ttr, tb, hipMemcpyHostToDevice, c.stream));
-  PT_HIP(hipMemcpyAsync(c.Ct, Ctr, tb, hipMemcpyHostToDevice, c.stream));
-  PT_HIP(hipMemcpyAsync(c.tid, c.iota, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, c.stream));
   PT_HIP(hipMemsetAsync(c.Cnew, 0, sizeof(double) * (size_t)(nx + 11) * (nz + 11), c.stream));
-  PT_TRY(tracers_to_corners(c, c.r, c.s, c.Tt));
-  PT_TRY(reorder_tracers(c));
+  PT_TRY(place_tracers(c, rtr, str, Ttr, Ctr));
   PT_HIP(hipMemsetAsync(c.SF, 0, sizeof(double) * (size_t)(nx + 7) * (nz + 7), c.stream));
-  for (int l = 2; l <= c.T.grids; l++) {
-    PT_HIP(hipMemsetAsync(c.L[l].err, 0, sizeof(double) * c.L[l].nelem, c.stream));
-    if (c.L[l].ferr) PT_HIP(hipMemsetAsync(c.L[l].ferr, 0, sizeof(float) * c.L[l].nelem, c.stream));
-  }
+  PT_TRY(clear_coarse_errors(c));
   PT_HIP(hipStreamSynchronize(c.stream));
   return PT_OK;
 }

@@ -162,6 +162,17 @@ module projectracer_gpu
    type(c_ptr), value :: ctx
    type(pt_stats_row), intent(out) :: row
   end function
+  integer(c_int) function pt_write_restart(ctx, dir) bind(C, name="pt_write_restart")
+   import :: c_int, c_ptr, c_char
+   type(c_ptr), value :: ctx
+   character(kind=c_char), intent(in) :: dir(*)
+  end function
+  integer(c_int) function pt_read_restart(ctx, dir, Tbot_iso, Ttop_iso) bind(C, name="pt_read_restart")
+   import :: c_int, c_ptr, c_char, c_double
+   type(c_ptr), value :: ctx
+   character(kind=c_char), intent(in) :: dir(*)
+   real(c_double), value :: Tbot_iso, Ttop_iso
+  end function
   integer(c_int) function pt_synchronize(ctx) bind(C, name="pt_synchronize")
    import :: c_int, c_ptr
    type(c_ptr), value :: ctx

@@ -133,6 +133,18 @@ class Simulation:
         self._check(self.lib.pt_stats(self.ctx, C.byref(row)), "stats")
         return {name: getattr(row, name) for name, _ in _abi.PtStatsRow._fields_}
 
+    # -- checkpoint / resume ---------------------------------------------------
+    def print_restart(self, directory):
+        """print_restart (PS/main.f90:457-496): Trestart and restart_tracers
+        in `directory`, in the reference's unformatted layout."""
+        self._check(self.lib.pt_write_restart(self.ctx, str(directory).encode()), "print_restart")
+
+    def read_restart(self, directory, Tbot_iso: float, Ttop_iso: float):
+        """The restart=1 start (PS/grid.f90:55-103) from the files in
+        `directory`; the wall temperatures come from the caller."""
+        self._check(self.lib.pt_read_restart(self.ctx, str(directory).encode(), Tbot_iso, Ttop_iso),
+                    "read_restart")
+
     # -- timing ----------------------------------------------------------------
     def enable_kernel_timing(self, on: bool = True):
         self._check(self.lib.pt_enable_kernel_timing(self.ctx, 1 if on else 0), "enable_kernel_timing")

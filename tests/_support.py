@@ -61,6 +61,7 @@ def oracle_lib():
             "pto_band_lu_solve": (C.c_int, [C.c_int, C.c_int, C.c_int, dp, ip, dp]),
             "pto_mzran": (None, [C.c_int64, dp]),
             "pto_initial_positions": (None, [vp, C.c_int, dp, dp]),
+            "pto_grid_coords": (None, [C.c_int, C.c_int, C.c_double, dp, dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)

@@ -8,6 +8,11 @@ Two sources:
       T10000  -- initial snapshot of the shipped 100x100, tpc=60, 2-thread run
                  (PS/main.f90:377-408 writes xg, zg, T, SF, u, w, stress, vis,
                  Tvis, nempty, Cnew, Cvis per node); columns 9-12 kept.
+      Trestart -- print_restart's record after the run's last step
+                 (PS/main.f90:457-470): xg, zg, T, Cnew, SF as exact FP64,
+                 kept whole with the record length;
+      T10001  -- the snapshot of the same step: T, SF and Cnew columns, to
+                 cross-check which array is which;
       stats.dat, stdout -- scalar values quoted as constants in the tests.
   * the reference program compiled from its Fortran sources by
     oracle/Makefile (`make -C oracle ref` -> oracle/_ref/ref_driver), run on
@@ -82,6 +87,27 @@ def shipped_snapshot():
     print("wrote shipped_T10000.npz")
 
 
+def shipped_restart():
+    raw = (REF_DIR / "Trestart").read_bytes()
+    head = int.from_bytes(raw[:4], "little")
+    nx = nz = 100
+    h, h3, h5 = 6, 3, 5
+    nxg, n3, n5 = nx + 1 + 2 * h, (nx + 1 + 2 * h3) * (nz + 1 + 2 * h3), (nx + 1 + 2 * h5) * (nz + 1 + 2 * h5)
+    d = np.frombuffer(raw[4:-4], dtype="<f8")
+    assert head == d.nbytes == 8 * (2 * nxg + 2 * n3 + n5)
+    xg, zg = d[:nxg], d[nxg:2 * nxg]
+    T = d[2 * nxg:2 * nxg + n3]
+    C = d[2 * nxg + n3:2 * nxg + n3 + n5]
+    SF = d[2 * nxg + n3 + n5:]
+    rows = [line.split()[:12] for line in (REF_DIR / "T10001").read_text().splitlines() if len(line.split()) >= 12]
+    a = np.array(rows)
+    def col(c):
+        return a[:, c].astype(np.float64).reshape(nx + 1, nz + 1).T.copy()
+    np.savez_compressed(HERE / "shipped_Trestart.npz", record_bytes=np.int64(head), xg=xg, zg=zg, T=T, Cnew=C, SF=SF,
+                        snap_T=col(2), snap_SF=col(3), snap_Cnew=col(10))
+    print("wrote shipped_Trestart.npz")
+
+
 def reference_runs():
     from projectracer_amd import cases
     from _support import run_reference, REF_BIN
@@ -129,4 +155,5 @@ def reference_runs():
 if __name__ == "__main__":
     if REF_DIR.exists():
         shipped_snapshot()
+        shipped_restart()
     reference_runs()

@@ -441,3 +441,37 @@ def test_stats_of_the_shipped_initial_state():
             assert got == pytest.approx(ref, rel=1e-8), k
         else:
             assert got == pytest.approx(ref, rel=5e-14, abs=5e-15), k
+
+
+def test_restart_record_of_the_shipped_Trestart():
+    """print_restart's Trestart (PS/main.f90:457-470) of the shipped 100x100
+    run: one record of xg(-6:106), zg(-6:106), T(-3:103)^2, Cnew(1,-5:105)^2,
+    SF(-3:103)^2.  The oracle's build_grid coordinates equal the file's bit
+    for bit; T, SF and Cnew at the nodes are the same step's T10001 columns
+    (printed to 5 significant digits), which fixes which array is which."""
+    g = np.load(GOLDEN / "shipped_Trestart.npz")
+    nx = nz = 100
+    n3, n5 = (nx + 7) * (nz + 7), (nx + 11) * (nz + 11)
+    assert int(g["record_bytes"]) == 8 * (2 * (nx + 13) + 2 * n3 + n5) == 283560
+    xg, zg = np.empty(nx + 13), np.empty(nz + 13)
+    oracle_lib().pto_grid_coords(nx, nz, 1.0, _dp(xg), _dp(zg))
+    assert np.array_equal(xg, g["xg"]) and np.array_equal(zg, g["zg"])
+    # a wrong halo count or aspect moves the coordinates
+    xg2 = np.empty(nx + 13)
+    oracle_lib().pto_grid_coords(nx, nz, 1.0 + 1e-15, _dp(xg2), _dp(zg))
+    assert not np.array_equal(xg2, g["xg"])
+
+    def nodes(a, h):
+        n = nx + 1 + 2 * h
+        return a.reshape(n, n)[h:h + nz + 1, h:h + nx + 1]
+
+    def close5(a, b):
+        return np.all(np.abs(a - b) <= 6e-5 * np.abs(b) + 1e-14)
+
+    T, SF, C = nodes(g["T"], 3), nodes(g["SF"], 3), nodes(g["Cnew"], 5)
+    assert close5(T, g["snap_T"]) and close5(SF, g["snap_SF"]) and close5(C, g["snap_Cnew"])
+    assert not close5(SF, g["snap_T"])
+    # enforceBCs_stream held when it was written: walls zero, antisymmetric ghosts
+    S = g["SF"].reshape(nz + 7, nx + 7)
+    assert np.all(S[:, 3] == 0.0) and np.all(S[:, nx + 3] == 0.0) and np.all(S[3, :] == 0.0)
+    assert np.array_equal(S[4:nz + 3, 2], -S[4:nz + 3, 4]) and np.array_equal(S[2, 4:nx + 3], -S[4, 4:nx + 3])
