@@ -202,6 +202,13 @@ int pt_stats(void* ctx, pt_stats_row* row);
  * this context's grid and tracer count, or another grid.  Records above
  * 2^31-1 bytes are PT_ERR_UNSUPPORTED. */
 int pt_write_restart(void* ctx, const char* dir);
+/* snapshots (PS/main.f90:377-424) into directory dir: T<10000+fcount> with,
+ * per node (i outer, k inner), xg, zg, T, SF, u, w, 2*vis*strain, vis, Tvis
+ * (g14.5E3), nempty (i6), Cnew, Cvis (g14.5E3) and a blank record after each
+ * i; P<10000+fcount> with zg and the lateral means of T, strain and
+ * 2*vis*strain per row.  strain is compute_strain_rate_invariant_direct
+ * (PS/momentum.f90:73-93) of the current psi.  fcount 0..89999. */
+int pt_write_snapshot(void* ctx, const char* dir, int32_t fcount);
 int pt_read_restart(void* ctx, const char* dir, double Tbot_iso, double Ttop_iso);
 int pt_synchronize(void* ctx);
 

@@ -14,7 +14,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libprojectracer_gpu.so"
-SOURCES = ["pt_setup.cpp", "pt_solver.hip", "pt_tracers.hip", "pt_sort.hip", "pt_stats.hip", "pt_restart.cpp",
+SOURCES = ["pt_setup.cpp", "pt_solver.hip", "pt_tracers.hip", "pt_sort.hip", "pt_stats.hip", "pt_files.cpp",
            "pt_api.hip"]
 
 # gfx950 only.  -ffp-contract=off keeps every FP64 expression in the

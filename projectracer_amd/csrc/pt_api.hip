@@ -327,6 +327,12 @@ int pt_write_restart(void* p, const char* dir) {
   return write_restart(*c, dir);
 }
 
+int pt_write_snapshot(void* p, const char* dir, int32_t fcount) {
+  Context* c = ctx_of(p);
+  if (!c || !dir) return PT_ERR_INVALID;
+  return write_snapshot(*c, dir, fcount);
+}
+
 int pt_read_restart(void* p, const char* dir, double Tbot_iso, double Ttop_iso) {
   Context* c = ctx_of(p);
   if (!c || !dir) return PT_ERR_INVALID;

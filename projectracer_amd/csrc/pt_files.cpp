@@ -1,4 +1,5 @@
-// pt_restart.cpp -- checkpoint and resume in the reference's own files.
+// pt_files.cpp -- the reference's on-disk formats: checkpoint / resume and
+// the ASCII snapshots.
 //
 // print_restart (PS/main.f90:457-496) writes, for RaT != 0, comp = 1,
 // tracer = 1 and iC_continuous = 1:
@@ -10,10 +11,19 @@
 // restart=1 branch of build_grid (PS/grid.f90:55-103) reads them back.
 // Records longer than 2^31-1 bytes (beyond 67 M tracers) would need the
 // compilers' sub-record framing and are refused.
+//
+// snapshots (PS/main.f90:377-424) writes T1xxxx (per node, i outer, k inner:
+// xg, zg, T, SF, u, w, 2*vis*strain, vis, Tvis as g14.5E3, nempty as i6,
+// Cnew, Cvis as g14.5E3; a list-directed blank record after each i) and
+// P1xxxx (per k: zg and the lateral means of T, strain and 2*vis*strain).
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
+#include "fortran_format.hpp"
 #include "pt_internal.hpp"
 
 namespace pt {
@@ -105,7 +115,94 @@ void grid_coords(const Tables& t, std::vector<double>& xg, std::vector<double>& 
   X(t.nx) = t.aspect; Z(t.nz) = 1.0;
 }
 
+// integrate1Dx (PS/basics.f90:494-515), xr = 1 on a uniform grid
+double integrate1Dx(const double* f, int nx, double dr) {
+  double fint = 0.0;
+  for (int i = 0; i < nx; i++) fint = fint + 0.5 * (f[i] * 1.0 + f[i + 1] * 1.0) * dr;
+  return fint;
+}
+
 }  // namespace
+
+int write_snapshot(Context& c, const char* dir, int fcount) {
+  const Tables& t = c.T;
+  const int nx = t.nx, nz = t.nz;
+  if (fcount < 0 || fcount > 89999) {
+    set_error("snapshot number must be 0..89999 (file names T10000..T99999)");
+    return PT_ERR_INVALID;
+  }
+  const size_t n3 = (size_t)(nx + 7) * (nz + 7), n5 = (size_t)(nx + 11) * (nz + 11), nn = (size_t)(nx + 1) * (nz + 1);
+  std::vector<double> T(n3), SF(n3), u(n3), w(n3), vis(n3), Tvis(n3), C(n5), Cv(n5), st(nn);
+  std::vector<int32_t> ne(nn);
+  if (!c.strain_buf) PT_HIP(hipMalloc((void**)&c.strain_buf, sizeof(double) * nn));
+  PT_TRY(strain_field(c, c.strain_buf));
+  const double* Tdev = c.t_initial ? c.T0 : c.Tratio;   // the module T
+  const std::pair<double*, const double*> f3[] = {{T.data(), Tdev}, {SF.data(), c.SF}, {u.data(), c.u},
+                                                  {w.data(), c.w}, {vis.data(), c.L[1].v.vis}, {Tvis.data(), c.Tvis}};
+  for (auto& q : f3) PT_HIP(hipMemcpyAsync(q.first, q.second, sizeof(double) * n3, hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipMemcpyAsync(C.data(), c.Cnew, sizeof(double) * n5, hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipMemcpyAsync(Cv.data(), c.Cvis, sizeof(double) * n5, hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipMemcpyAsync(st.data(), c.strain_buf, sizeof(double) * nn, hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipMemcpyAsync(ne.data(), c.nempty, sizeof(int32_t) * nn, hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipStreamSynchronize(c.stream));
+  std::vector<double> xg, zg;
+  grid_coords(t, xg, zg);
+  auto I3 = [&](int i, int k) { return (size_t)(k + 3) * (nx + 7) + (i + 3); };
+  auto I5 = [&](int i, int k) { return (size_t)(k + 5) * (nx + 11) + (i + 5); };
+  auto IN = [&](int i, int k) { return (size_t)k * (nx + 1) + i; };
+  char name[8];
+  std::snprintf(name, sizeof name, "T%5d", 10000 + fcount);
+  std::string body;
+  body.reserve((size_t)(nx + 1) * ((nz + 1) * 161 + 3));
+  for (int i = 0; i <= nx; i++) {
+    for (int k = 0; k <= nz; k++) {
+      const double stress = 2.0 * vis[I3(i, k)] * st[IN(i, k)];
+      const double v[9] = {xg[i + 6], zg[k + 6], T[I3(i, k)], SF[I3(i, k)], u[I3(i, k)], w[I3(i, k)],
+                           stress, vis[I3(i, k)], Tvis[I3(i, k)]};
+      for (double x : v) g14_5e3(x, body);
+      char nb[16];
+      std::snprintf(nb, sizeof nb, "%6d", ne[IN(i, k)]);
+      body += nb;
+      g14_5e3(C[I5(i, k)], body);
+      g14_5e3(Cv[I5(i, k)], body);
+      body += '\n';
+    }
+    body += "  \n";   // write(101,*) " "
+  }
+  std::string prof;
+  std::vector<double> rt(nx + 1), rs(nx + 1), rq(nx + 1);
+  for (int k = 0; k <= nz; k++) {
+    for (int i = 0; i <= nx; i++) {
+      rt[i] = T[I3(i, k)];
+      rs[i] = st[IN(i, k)];
+      rq[i] = 2.0 * vis[I3(i, k)] * st[IN(i, k)];
+    }
+    g14_5e3(zg[k + 6], prof);
+    g14_5e3(integrate1Dx(rt.data(), nx, t.dr) / t.aspect, prof);
+    g14_5e3(integrate1Dx(rs.data(), nx, t.dr) / t.aspect, prof);
+    g14_5e3(integrate1Dx(rq.data(), nx, t.dr) / t.aspect, prof);
+    prof += '\n';
+  }
+  for (int pass = 0; pass < 2; pass++) {
+    std::string path = join(dir, name);
+    if (pass == 1) path[path.size() - 6] = 'P';
+    const std::string& text = pass == 0 ? body : prof;
+    File fh;
+    fh.f = std::fopen(path.c_str(), "w");
+    if (!fh.f) {
+      set_error("cannot open " + path + " for writing");
+      return PT_ERR_INVALID;
+    }
+    bool ok = std::fwrite(text.data(), 1, text.size(), fh.f) == text.size();
+    ok = (std::fclose(fh.f) == 0) && ok;
+    fh.f = nullptr;
+    if (!ok) {
+      set_error("write to " + path + " failed");
+      return PT_ERR_INVALID;
+    }
+  }
+  return PT_OK;
+}
 
 int write_restart(Context& c, const char* dir) {
   const Tables& t = c.T;

@@ -128,6 +128,7 @@ struct Context {
   bool t_initial = true;
   double mass = 0.0; bool mass_set = false;
   double* stats_rows = nullptr;
+  double* strain_buf = nullptr;   // strain-rate invariant at the nodes (snapshots)
   // timing
   bool timing = false;
   KTimer kt[PT_K_COUNT];
@@ -167,6 +168,8 @@ int tracers_in_id_order(Context& c, double** r, double** s, double** Tt, double*
 int restart_state(Context& c, const double* T3, const double* C5, const double* SF3, double Tbot, double Ttop,
                   const double* rtr, const double* str, const double* Ttr, const double* Ctr);
 int stream_bcs(Context& c, double* S);
+int strain_field(Context& c, double* out);
+int write_snapshot(Context& c, const char* dir, int fcount);
 int write_restart(Context& c, const char* dir);
 int read_restart(Context& c, const char* dir, double Tbot, double Ttop);
 // ---- diagnostics (pt_stats.hip) -------------------------------------------------

@@ -1879,7 +1879,8 @@ int solver_free(Context& c) {
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
-  void* ps[] = {c.Mc, c.RowB, c.Bc, c.ipiv, c.d_info, c.red, c.red_out, c.dres64, c.Gc, c.Gwork, c.stats_rows};
+  void* ps[] = {c.Mc, c.RowB, c.Bc, c.ipiv, c.d_info, c.red, c.red_out, c.dres64, c.Gc, c.Gwork, c.stats_rows,
+                c.strain_buf};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c.h_red) (void)hipHostFree(c.h_red);

@@ -112,6 +112,19 @@ struct EdgeArgs {
   double dr, ds, dr2, ds2, drds;
 };
 
+// compute_strain_rate_invariant_direct (PS/momentum.f90:73-93, uniform grid)
+// at node (i, k) from psi
+__device__ __forceinline__ double strain_at(const double* SF, int i, int k, int p3, const EdgeArgs& a) {
+  double sxx = 0.0, szz = 0.0, sxz = 0.0;
+  for (int j = -2; j <= 2; j++) sxx = sxx + a.D2[j + 2] * SF[S3(i + j, k, p3)];
+  for (int j = -2; j <= 2; j++) szz = szz + a.D2[j + 2] * SF[S3(i, k + j, p3)];
+  for (int b = -2; b <= 2; b++)
+    for (int q = -2; q <= 2; q++) sxz = sxz + a.Drs[b + 2][q + 2] * SF[S3(i + q, k + b, p3)];
+  const double Sxx = sxx / a.dr2, Szz = szz / a.ds2, Sxz = sxz / a.drds;
+  const double d = Szz - Sxx;
+  return sqrt(2.0 * (Sxz * Sxz) + 0.5 * (d * d));
+}
+
 // D1 . T(i, k-2..k+2) / ds / zs(k), zs = 1 on a uniform grid
 __device__ __forceinline__ double ddz(const double* T, int i, int k, int p3, const EdgeArgs& a) {
   double s = 0.0;
@@ -136,16 +149,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_stats_edges(const double* T, con
     bot = bot + 0.5 * (fb0 * 1.0 + fb1 * 1.0) * a.dr;
     tb = tb + 0.5 * (T[S3(i, 0, p3)] * 1.0 + T[S3(i + 1, 0, p3)] * 1.0) * a.dr;
   }
-  for (int i = threadIdx.x; i <= nx; i += ST_THREADS) {
-    double sxx = 0.0, szz = 0.0, sxz = 0.0;
-    for (int j = -2; j <= 2; j++) sxx = sxx + a.D2[j + 2] * SF[S3(i + j, nz, p3)];
-    for (int j = -2; j <= 2; j++) szz = szz + a.D2[j + 2] * SF[S3(i, nz + j, p3)];
-    for (int b = -2; b <= 2; b++)
-      for (int q = -2; q <= 2; q++) sxz = sxz + a.Drs[b + 2][q + 2] * SF[S3(i + q, nz + b, p3)];
-    const double Sxx = sxx / a.dr2, Szz = szz / a.ds2, Sxz = sxz / a.drds;
-    const double d = Szz - Sxx;
-    smax = fmax(smax, sqrt(2.0 * (Sxz * Sxz) + 0.5 * (d * d)));
-  }
+  for (int i = threadIdx.x; i <= nx; i += ST_THREADS) smax = fmax(smax, strain_at(SF, i, nz, p3, a));
   block_sum(sh, top);
   if (threadIdx.x == 0) out[0] = sh[0];
   __syncthreads();
@@ -194,6 +198,28 @@ __global__ void k_stats_final(const double* rows, int nz, int kent, double ds, d
   out[6] = vmin; out[7] = vmax; out[8] = e0; out[9] = nmin; out[10] = nmax;
 }
 
+// strain(0:nx, 0:nz) of the current psi, row-major (k, i)
+__global__ void k_strain_field(const double* SF, int nx, int nz, EdgeArgs a, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > nx || k > nz) return;
+  out[(long)k * (nx + 1) + i] = strain_at(SF, i, k, nx + 7, a);
+}
+
+static EdgeArgs edge_args(Context& c) {
+  EdgeArgs a;
+  for (int q = 0; q < 5; q++) { a.D1[q] = c.st.D1[q]; a.D2[q] = c.st.D2[q]; }
+  for (int b = 0; b < 5; b++)
+    for (int q = 0; q < 5; q++) a.Drs[b][q] = c.T.Drs[b + 1][q + 1];
+  a.dr = c.T.dr; a.ds = c.T.ds; a.dr2 = c.T.dr2; a.ds2 = c.T.ds2; a.drds = c.L[1].g.drds;
+  return a;
+}
+
+int strain_field(Context& c, double* out) {
+  dim3 b(64, 4), g((c.T.nx + 1 + 63) / 64, (c.T.nz + 1 + 3) / 4);
+  hipLaunchKernelGGL(k_strain_field, g, b, 0, c.stream, c.SF, c.T.nx, c.T.nz, edge_args(c), out);
+  return PT_OK;
+}
+
 int stats(Context& c, pt_stats_row* row) {
   const int nx = c.T.nx, nz = c.T.nz;
   if (!c.stats_rows) {
@@ -205,11 +231,7 @@ int stats(Context& c, pt_stats_row* row) {
   const double* T = c.t_initial ? c.T0 : c.Tratio;   // the reference's module T
   hipLaunchKernelGGL(k_stats_rows, dim3(nz + 1), dim3(ST_THREADS), 0, c.stream, c.Cnew, c.u, c.w, T, c.cond,
                      c.L[1].v.vis, c.nempty, nx, nz, c.T.dr, c.P.RaT, rows);
-  EdgeArgs a;
-  for (int q = 0; q < 5; q++) { a.D1[q] = c.st.D1[q]; a.D2[q] = c.st.D2[q]; }
-  for (int b = 0; b < 5; b++)
-    for (int q = 0; q < 5; q++) a.Drs[b][q] = c.T.Drs[b + 1][q + 1];
-  a.dr = c.T.dr; a.ds = c.T.ds; a.dr2 = c.T.dr2; a.ds2 = c.T.ds2; a.drds = c.L[1].g.drds;
+  const EdgeArgs a = edge_args(c);
   hipLaunchKernelGGL(k_stats_edges, dim3(1), dim3(ST_THREADS), 0, c.stream, T, c.cond, c.SF, nx, nz, a, edge);
   // kent: last row at or below the entrainment height (entrainment_setup,
   // PS/composition.f90:765-774, uniform grid)

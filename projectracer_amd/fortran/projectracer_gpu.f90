@@ -167,6 +167,12 @@ module projectracer_gpu
    type(c_ptr), value :: ctx
    character(kind=c_char), intent(in) :: dir(*)
   end function
+  integer(c_int) function pt_write_snapshot(ctx, dir, fcount) bind(C, name="pt_write_snapshot")
+   import :: c_int, c_ptr, c_char, c_int32_t
+   type(c_ptr), value :: ctx
+   character(kind=c_char), intent(in) :: dir(*)
+   integer(c_int32_t), value :: fcount
+  end function
   integer(c_int) function pt_read_restart(ctx, dir, Tbot_iso, Ttop_iso) bind(C, name="pt_read_restart")
    import :: c_int, c_ptr, c_char, c_double
    type(c_ptr), value :: ctx

@@ -139,6 +139,11 @@ class Simulation:
         in `directory`, in the reference's unformatted layout."""
         self._check(self.lib.pt_write_restart(self.ctx, str(directory).encode()), "print_restart")
 
+    def snapshots(self, directory, fcount: int = 0):
+        """snapshots (PS/main.f90:377-424): the ASCII files T<10000+fcount>
+        and P<10000+fcount> in `directory`."""
+        self._check(self.lib.pt_write_snapshot(self.ctx, str(directory).encode(), fcount), "snapshots")
+
     def read_restart(self, directory, Tbot_iso: float, Ttop_iso: float):
         """The restart=1 start (PS/grid.f90:55-103) from the files in
         `directory`; the wall temperatures come from the caller."""

@@ -13,6 +13,8 @@ Two sources:
                  kept whole with the record length;
       T10001  -- the snapshot of the same step: T, SF and Cnew columns, to
                  cross-check which array is which;
+      T10000, P10000 text -- every 7th line of T10000 and all of P10000,
+                 verbatim, for the snapshot writer's layout and editing;
       stats.dat, stdout -- scalar values quoted as constants in the tests.
   * the reference program compiled from its Fortran sources by
     oracle/Makefile (`make -C oracle ref` -> oracle/_ref/ref_driver), run on
@@ -108,6 +110,16 @@ def shipped_restart():
     print("wrote shipped_Trestart.npz")
 
 
+def shipped_snapshot_text():
+    t = (REF_DIR / "T10000").read_bytes().split(b"\n")
+    p = (REF_DIR / "P10000").read_bytes().split(b"\n")
+    assert t[-1] == b"" and p[-1] == b""
+    idx = np.arange(0, len(t) - 1, 7)
+    np.savez_compressed(HERE / "shipped_snapshot_text.npz", t_count=np.int64(len(t) - 1), t_idx=idx,
+                        t_lines=np.array([t[i] for i in idx], dtype="S160"), p_lines=np.array(p[:-1], dtype="S56"))
+    print("wrote shipped_snapshot_text.npz")
+
+
 def reference_runs():
     from projectracer_amd import cases
     from _support import run_reference, REF_BIN
@@ -156,4 +168,5 @@ if __name__ == "__main__":
     if REF_DIR.exists():
         shipped_snapshot()
         shipped_restart()
+        shipped_snapshot_text()
     reference_runs()
