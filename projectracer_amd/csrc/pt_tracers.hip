@@ -22,6 +22,7 @@
 //   k_rk1..k_rk4     interpolate (x3) + the RK4 stage updates of energy_time
 //                    (PS/composition.f90:1359-1393, PS/energy.f90:258-537)
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <algorithm>
 #include "pt_internal.hpp"
@@ -80,7 +81,7 @@ __global__ void k_cell_bounds(const int32_t* skey, int64_t n, int ncell, int32_t
 // Fixed summation order, no atomics: the result is deterministic.
 constexpr int PJ_CELLS = 28;     // 28 cells x 9 neighbour nodes = 252 phase-2 threads
 constexpr int PJ_THREADS = 256;
-constexpr int PJ_CHUNK = 1024;
+constexpr int PJ_CHUNK = 512;
 __global__ void __launch_bounds__(PJ_THREADS) k_cell_reduce(const double* __restrict__ r, const double* __restrict__ s,
                                                             const double* __restrict__ Tt, const double* __restrict__ Ct,
                                                             const int32_t* __restrict__ perm,
@@ -90,21 +91,24 @@ __global__ void __launch_bounds__(PJ_THREADS) k_cell_reduce(const double* __rest
                                                             double* __restrict__ cellsum, int nx, int nz, double dr,
                                                             double ds) {
   __shared__ double hx[3][PJ_CHUNK], hz[3][PJ_CHUNK], sT[PJ_CHUNK], sC[PJ_CHUNK];
-  __shared__ int64_t range[2];
+  __shared__ int32_t range[2];
   const int ncell = (nx + 1) * (nz + 1);
   const int cfirst = blockIdx.x * PJ_CELLS;
   const int clast = min(cfirst + PJ_CELLS, ncell);   // exclusive
   const int tid = threadIdx.x;
-  if (tid < 2) {
-    // lower bound of the block's first / one-past-last cell in the sorted keys
-    const int target = tid == 0 ? cfirst : clast;
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-      const int64_t mid = lo + (hi - lo) / 2;
-      if (skey[mid] < target) lo = mid + 1;
-      else hi = mid;
+  // the block's tracers: from the first non-empty cell's segment start to
+  // the last one's end (consecutive cells are consecutive in the sorted order)
+  if (tid == 0) {
+    range[0] = INT32_MAX;
+    range[1] = 0;
+  }
+  __syncthreads();
+  if (tid < clast - cfirst) {
+    const int32_t b0 = cstart[cfirst + tid], b1 = cend[cfirst + tid];
+    if (b1 > b0) {
+      atomicMin(&range[0], b0);
+      atomicMax(&range[1], b1);
     }
-    range[tid] = lo;
   }
   // phase-2 role: cell c, neighbour node (a, b) of the 3x3 around it
   const int c = cfirst + tid / 9, ab = tid % 9, na = ab / 3, nb = ab % 3;
@@ -116,7 +120,7 @@ __global__ void __launch_bounds__(PJ_THREADS) k_cell_reduce(const double* __rest
   }
   double txs = 0.0, cxs = 0.0, ws = 0.0;
   __syncthreads();
-  const int64_t t0 = range[0], t1 = range[1];
+  const int64_t t0 = range[0] == INT32_MAX ? 0 : range[0], t1 = range[0] == INT32_MAX ? 0 : range[1];
   for (int64_t base = t0; base < t1; base += PJ_CHUNK) {
     const int cnt = (int)min((int64_t)PJ_CHUNK, t1 - base);
     for (int q = tid; q < cnt; q += PJ_THREADS) {
@@ -136,9 +140,25 @@ __global__ void __launch_bounds__(PJ_THREADS) k_cell_reduce(const double* __rest
     }
     __syncthreads();
     if (owner) {
-      const int64_t lo = max(b0, base), hi = min(b1, base + cnt);
-      for (int64_t t = lo; t < hi; t++) {
-        const int q = (int)(t - base);
+      const int q0 = (int)(max(b0, base) - base), q1 = (int)(min(b1, base + cnt) - base);
+      int q = q0;
+      // four tracers' LDS operands in flight; the sums stay in slot order
+      for (; q + 4 <= q1; q += 4) {
+        double w[4], tv[4], cv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          w[u] = hx[na][q + u] * hz[nb][q + u];
+          tv[u] = sT[q + u];
+          cv[u] = sC[q + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          txs = txs + tv[u] * w[u];
+          cxs = cxs + cv[u] * w[u];
+          ws = ws + w[u];
+        }
+      }
+      for (; q < q1; q++) {
         const double w = hx[na][q] * hz[nb][q];
         txs = txs + sT[q] * w;
         cxs = cxs + sC[q] * w;
@@ -603,54 +623,60 @@ struct RkArgs {
   InterpConst C;
 };
 
+// The tracer streams are read and written once per stage kernel:
+// nontemporal loads and stores keep them from displacing the coefficient
+// table in the caches.
+#define NT_LD(x) __builtin_nontemporal_load(&(x))
+#define NT_ST(x, v) __builtin_nontemporal_store((v), &(x))
+
 // stage 1: PS/energy.f90:258-301
 __global__ void k_rk1(RkArgs a) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= a.n) return;
-  double rv = a.r[j], sv = a.s[j], Tv = a.T[j];
-  a.r0[j] = rv; a.s0[j] = sv; a.T0[j] = Tv;
+  double rv = NT_LD(a.r[j]), sv = NT_LD(a.s[j]), Tv = NT_LD(a.T[j]);
+  NT_ST(a.r0[j], rv); NT_ST(a.s0[j], sv); NT_ST(a.T0[j], Tv);
   double o[3];
   interp3(a.tab, a.C, rv, sv, o);
-  a.r[j] = o[0]; a.s[j] = o[1]; a.T[j] = o[2];
-  a.r1[j] = rv + a.dtstage * o[0];
-  a.s1[j] = sv + a.dtstage * o[1];
-  a.T1[j] = Tv + a.dtstage * o[2];
+  NT_ST(a.r[j], o[0]); NT_ST(a.s[j], o[1]); NT_ST(a.T[j], o[2]);
+  NT_ST(a.r1[j], rv + a.dtstage * o[0]);
+  NT_ST(a.s1[j], sv + a.dtstage * o[1]);
+  NT_ST(a.T1[j], Tv + a.dtstage * o[2]);
 }
 // stage 2: PS/energy.f90:350-402
 __global__ void k_rk2(RkArgs a) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= a.n) return;
   double o[3];
-  interp3(a.tab, a.C, a.r1[j], a.s1[j], o);
-  a.r[j] = a.r[j] + 2.0 * o[0];
-  a.s[j] = a.s[j] + 2.0 * o[1];
-  a.T[j] = a.T[j] + 2.0 * o[2];
-  a.r2[j] = a.r0[j] + a.dtstage * o[0];
-  a.s2[j] = a.s0[j] + a.dtstage * o[1];
-  a.T2[j] = a.T0[j] + a.dtstage * o[2];
+  interp3(a.tab, a.C, NT_LD(a.r1[j]), NT_LD(a.s1[j]), o);
+  NT_ST(a.r[j], NT_LD(a.r[j]) + 2.0 * o[0]);
+  NT_ST(a.s[j], NT_LD(a.s[j]) + 2.0 * o[1]);
+  NT_ST(a.T[j], NT_LD(a.T[j]) + 2.0 * o[2]);
+  NT_ST(a.r2[j], NT_LD(a.r0[j]) + a.dtstage * o[0]);
+  NT_ST(a.s2[j], NT_LD(a.s0[j]) + a.dtstage * o[1]);
+  NT_ST(a.T2[j], NT_LD(a.T0[j]) + a.dtstage * o[2]);
 }
 // stage 3: PS/energy.f90:450-473
 __global__ void k_rk3(RkArgs a) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= a.n) return;
   double o[3];
-  interp3(a.tab, a.C, a.r2[j], a.s2[j], o);
-  a.r[j] = a.r[j] + 2.0 * o[0];
-  a.r1[j] = a.r0[j] + a.dtstage * o[0];
-  a.s[j] = a.s[j] + 2.0 * o[1];
-  a.s1[j] = a.s0[j] + a.dtstage * o[1];
-  a.T[j] = a.T[j] + 2.0 * o[2];
-  a.T1[j] = a.T0[j] + a.dtstage * o[2];
+  interp3(a.tab, a.C, NT_LD(a.r2[j]), NT_LD(a.s2[j]), o);
+  NT_ST(a.r[j], NT_LD(a.r[j]) + 2.0 * o[0]);
+  NT_ST(a.r1[j], NT_LD(a.r0[j]) + a.dtstage * o[0]);
+  NT_ST(a.s[j], NT_LD(a.s[j]) + 2.0 * o[1]);
+  NT_ST(a.s1[j], NT_LD(a.s0[j]) + a.dtstage * o[1]);
+  NT_ST(a.T[j], NT_LD(a.T[j]) + 2.0 * o[2]);
+  NT_ST(a.T1[j], NT_LD(a.T0[j]) + a.dtstage * o[2]);
 }
 // stage 4: PS/energy.f90:517-537 (dtstage = dt/6)
 __global__ void k_rk4(RkArgs a) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= a.n) return;
   double o[3];
-  interp3(a.tab, a.C, a.r1[j], a.s1[j], o);
-  a.r[j] = a.r0[j] + a.dtstage * (a.r[j] + o[0]);
-  a.s[j] = a.s0[j] + a.dtstage * (a.s[j] + o[1]);
-  a.T[j] = a.T0[j] + a.dtstage * (a.T[j] + o[2]);
+  interp3(a.tab, a.C, NT_LD(a.r1[j]), NT_LD(a.s1[j]), o);
+  NT_ST(a.r[j], NT_LD(a.r0[j]) + a.dtstage * (NT_LD(a.r[j]) + o[0]));
+  NT_ST(a.s[j], NT_LD(a.s0[j]) + a.dtstage * (NT_LD(a.s[j]) + o[1]));
+  NT_ST(a.T[j], NT_LD(a.T0[j]) + a.dtstage * (NT_LD(a.T[j]) + o[2]));
 }
 
 static int source_and_tables(Context& c, bool velocity_too) {
