@@ -24,6 +24,11 @@ Two sources:
       ref_thermo96.npz  -- 96x32 T/C-dependent viscosity, tpc=4, one step
       ref_cloud.npz     -- initial tracer cloud drawn by the reference's own
                            initialize_tracers with 2 and 3 OpenMP threads
+      ref_config1_100.npz -- the 64x64 isoviscous Ra_T=1e5 configuration
+                           (tpc=20, RK4, courant 0.5) after 100 time steps:
+                           psi, u, w, T at the nodes, per-node tracer counts,
+                           every tracer's cell, a tracer subsample, q of every
+                           solve
 Usage: python tests/golden/make_golden.py
 """
 from __future__ import annotations
@@ -162,6 +167,33 @@ def reference_runs():
         out[f"t{threads}_str"] = st["str"].copy()
     np.savez_compressed(HERE / "ref_cloud.npz", **out)
     print("wrote ref_cloud.npz")
+    reference_config1(tmp)
+
+
+def reference_config1(tmp, tolerance=1e-8, name="ref_config1_100.npz"):
+    from projectracer_amd import cases
+    from _support import run_reference
+    iso = cases.isoviscous(nx=64, nz=64, tpc=20, tolerance=tolerance)
+    states, solves, _, _ = run_reference(iso, nrun=100, dump_every=100, threads=1,
+                                         workdir=tmp / f"config1_{tolerance:g}")
+    st = states[100]
+    out = {f"s100_{n}": _node(st, n) for n in ("SF", "U", "W", "T")}
+    out["s100_nempty"] = st["nempty"].astype(np.int32)
+    nx, nz = st["nx"], st["nz"]
+    dr, ds = iso.params.aspect / nx, 1.0 / nz
+    # the tracer's cell: the node nearest to it (nint, PS/composition.f90:1005-1006)
+    i = np.floor(np.abs(st["rtr"] / dr) + 0.5) * np.sign(st["rtr"])
+    k = np.floor(np.abs(st["str"] / ds) + 0.5) * np.sign(st["str"])
+    out["s100_cell"] = (k * (nx + 1) + i).astype(np.int32)
+    sel = np.arange(0, st["ntr"], 29)
+    out["s100_sel"] = sel
+    for n in ("rtr", "str", "Ttr"):
+        out[f"s100_{n}"] = st[n][sel].copy()
+    out["time"] = np.float64(st["time"])
+    out["solves_tq"], out["solves_res"] = _solves(solves)
+    out["tolerance"] = np.float64(tolerance)
+    np.savez_compressed(HERE / name, **out)
+    print("wrote", name, len(solves), "solves")
 
 
 if __name__ == "__main__":
