@@ -123,6 +123,9 @@ typedef struct pt_solve_info {
   int32_t n_temp;         /* T filter passes of the last projection */
   int32_t n_comp;         /* C filter passes of the last projection */
   double  eig_fine;       /* ratio_min of the last level-1 smoother_vis */
+  int32_t cycle_graphs;   /* multigrid cycles captured as HIP graphs and replayed
+                             (one per starting buffer arrangement; 0: launched
+                             one kernel at a time) */
 } pt_solve_info;
 
 /* ---- per-step diagnostics (PS/basics.f90:363-462) ---------------------

@@ -77,7 +77,7 @@ class PtSolveInfo(C.Structure):
         ("q", C.c_int32), ("residual", C.c_double), ("res_ref", C.c_double),
         ("dt", C.c_double), ("time", C.c_double), ("solves", C.c_int64),
         ("cycles", C.c_int64), ("n_eta", C.c_int32), ("n_temp", C.c_int32),
-        ("n_comp", C.c_int32), ("eig_fine", C.c_double),
+        ("n_comp", C.c_int32), ("eig_fine", C.c_double), ("cycle_graphs", C.c_int32),
     ]
 
 

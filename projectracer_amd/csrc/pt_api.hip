@@ -1,4 +1,5 @@
 // pt_api.hip -- C ABI entry points (include/projectracer_gpu.h).
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -120,6 +121,8 @@ int pt_create(const pt_params* p, void** out) {
   std::memset(&c->L, 0, sizeof c->L);
   c->P = *p;
   c->device = dev;
+  const char* graph_env = std::getenv("PT_GRAPH");
+  c->use_graph = !(graph_env && graph_env[0] == '0');
   int rc = build_tables(c->P, c->T);
   if (rc != PT_OK) { delete c; return rc; }
   c->P.nx = c->T.nx; c->P.nz = c->T.nz; c->P.grids = c->T.grids;
@@ -312,6 +315,7 @@ int pt_get_info(void* p, pt_solve_info* info) {
   info->q = c->last_q; info->residual = c->residual1; info->res_ref = c->res_ref1;
   info->dt = c->dt; info->time = c->time; info->solves = c->solves; info->cycles = c->cycles;
   info->n_eta = c->n_eta; info->n_temp = c->n_temp; info->n_comp = c->n_comp; info->eig_fine = c->eig_fine;
+  info->cycle_graphs = c->graph_state == 2 ? (int32_t)c->cycle_graphs.size() : 0;
   return PT_OK;
 }
 

@@ -121,6 +121,19 @@ struct Context {
   int last_q = 0; int64_t solves = 0, cycles = 0;
   int n_eta = 0, n_temp = 0, n_comp = 0; double eig_fine = 0;
   bool first_solve = true;
+  // the multigrid cycle as captured HIP graphs, one per arrangement of the
+  // rotating level buffers it starts from (`before`; it leaves them as
+  // `after`): graph_state 0 = not yet run, 1 = run eagerly once, 2 =
+  // capturing/replaying, -1 = eager only; use_graph is false when PT_GRAPH=0
+  // at pt_create
+  struct CycleGraph {
+    std::vector<void*> before, after;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  bool use_graph = true;
+  int graph_state = 0;
+  std::vector<CycleGraph> cycle_graphs;
   // mixed precision work
   double* dres64;           // FP64 residual for defect correction
   // stats: the reference's module T is the initial field until the first

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <algorithm>
+#include <vector>
 #include "pt_internal.hpp"
 
 namespace pt {
@@ -1417,7 +1418,7 @@ static int launch_coarse_apply_wg(Context& c, const float* rhs, float* err, int 
 // (PS/iterative_solver.f90:1418-1614, 415-448).  `s0zero` runs the smoother
 // on a correction that starts from zero (mixed precision).
 template <typename R>
-static int iterate_stream_t(Context& c, int l, int p, bool fp32, double* resmag) {
+static int iterate_stream_t(Context& c, int l, int p, bool fp32, bool l1_sum) {
   Level& L = c.L[l];
   const LevelGeom& g = L.g;
   const int s = c.P.nstageS;
@@ -1431,7 +1432,7 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, double* resmag)
   if constexpr (sizeof(R) == 4) {
     // FP32 isoviscous levels too large for one workgroup: strip sweeps
     const StripCfg sc = strip_cfg(g);
-    if (l >= 2 && c.P.ivis == 0 && ppt == 0 && sc.cw > 0 && resmag == nullptr) {
+    if (l >= 2 && c.P.ivis == 0 && ppt == 0 && sc.cw > 0 && !l1_sum) {
       RkcCoef co;
       std::memcpy(co.mewt, c.T.mewt, sizeof co.mewt); std::memcpy(co.mew, c.T.mew, sizeof co.mew);
       std::memcpy(co.vv, c.T.vv, sizeof co.vv); std::memcpy(co.cc, c.T.cc, sizeof co.cc);
@@ -1447,7 +1448,7 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, double* resmag)
       return PT_OK;
     }
   }
-  if (l >= 2 && ppt > 0 && lds <= SMALL_LDS_BYTES && resmag == nullptr) {
+  if (l >= 2 && ppt > 0 && lds <= SMALL_LDS_BYTES && !l1_sum) {
     // whole sweep + residual in one workgroup, buffers in LDS
     RkcCoef co;
     std::memcpy(co.mewt, c.T.mewt, sizeof co.mewt); std::memcpy(co.mew, c.T.mew, sizeof co.mew);
@@ -1514,11 +1515,10 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, double* resmag)
   R* res = fp32 ? (R*)L.fB : (R*)L.res;   // FP32 residual reuses the B buffer
   int nb = grd.x * grd.y;
   launch_residual<R>(c, grd, blk, a, res, c.red);
-  if (resmag) {
+  if (l1_sum) {
+    // the L1 norm of the residual to pinned host memory (read after a sync)
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, nb, (int)RED_SUM_ABS, c.red_out);
     PT_HIP(hipMemcpyAsync(c.h_red, c.red_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    PT_HIP(hipStreamSynchronize(c.stream));
-    *resmag = c.h_red[0] / (double)(g.n - 2) / (double)(g.m - 2);
   }
   (void)p;
   return PT_OK;
@@ -1677,14 +1677,180 @@ static int to_f32(Context& c, const double* a, float* b, size_t n) {
   return PT_OK;
 }
 
+__global__ void k_fill_f32(float* v, long n) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) v[t] = 0.0f;
+}
+
+// schedule entry p of the cycle (PS/iterative_solver.f90:143-340): the
+// restriction into its level when the schedule descends, the visit (RKC1
+// smoothing + residual, or the coarse solve), and the prolongation when it
+// ascends next.  Launches only: a level-1 visit leaves the residual's L1 norm
+// in pinned host memory, read by level1_residual after a sync.
+static int visit(Context& c, int p, bool mixed) {
+  const Tables& t = c.T;
+  dim3 b16(64, 4);
+  const double sz = c.P.Vbc == 0 ? -1.0 : 1.0;
+  const int l = t.multi[p];
+  Level& L = c.L[l];
+  if (t.multi[p - 1] < t.multi[p] && l >= 2) {
+    // compute_RHS(l) = restrict(residual(l-1))
+    Level& F = c.L[l - 1];
+    if (!mixed)
+      hipLaunchKernelGGL((k_restrict<double, double>), grid2(L.g.n - 1, L.g.m - 1, b16), b16, 0, c.stream, F.res,
+                         F.g.pitch, L.rhs, L.g.n, L.g.m, L.g.pitch);
+    else if (l == 2)
+      hipLaunchKernelGGL((k_restrict<double, float>), grid2(L.g.n - 1, L.g.m - 1, b16), b16, 0, c.stream, F.res,
+                         F.g.pitch, L.frhs, L.g.n, L.g.m, L.g.pitch);
+    else
+      hipLaunchKernelGGL((k_restrict<float, float>), grid2(L.g.n - 1, L.g.m - 1, b16), b16, 0, c.stream, F.fB,
+                         F.g.pitch, L.frhs, L.g.n, L.g.m, L.g.pitch);
+  }
+  if (l < t.grids) {
+    if (!mixed) {
+      PT_TRY(iterate_stream_t<double>(c, l, p, false, l == 1));
+    } else if (l == 1) {
+      // FP64 defect -> FP32 RKC1 smoothing of the correction -> FP64 update
+      StageArgs<double> a = stage_base<double>(c, 1, false);
+      a.Sx = L.err;
+      dim3 blk(TX, TZ), grd((L.g.n - 1 + TX - 1) / TX, (L.g.m - 1 + TZ * PZ - 1) / (TZ * PZ));
+      launch_residual<double>(c, grd, blk, a, c.dres64, c.red);
+      PT_TRY(to_f32(c, c.dres64, L.frhs, L.nelem));
+      hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((L.nelem + 255) / 256)), dim3(256), 0, c.stream, L.ferr,
+                         (long)L.nelem);
+      PT_TRY(iterate_stream_t<float>(c, 1, p, true, false));
+      hipLaunchKernelGGL(k_add_f32_to_f64, grid2(L.g.n - 1, L.g.m - 1, b16), b16, 0, c.stream, L.err, L.ferr,
+                         L.g.n, L.g.m, L.g.pitch, sz);
+      launch_residual<double>(c, grd, blk, a, L.res, c.red);
+      int nb = grd.x * grd.y;
+      hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, nb, (int)RED_SUM_ABS, c.red_out);
+      PT_HIP(hipMemcpyAsync(c.h_red, c.red_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    } else {
+      PT_TRY(iterate_stream_t<float>(c, l, p, true, false));
+    }
+  } else {
+    // Compute_coarse_vector + DGBTRS + Stream_grid_coarse + enforceBCs_stream
+    // as one dense matvec with the operator built from the LU factors
+    dim3 gb((c.nci + 255) / 256);
+    size_t sh = sizeof(double) * c.nci;
+    if (!mixed)
+      hipLaunchKernelGGL((k_coarse_apply<double, double>), gb, dim3(256), sh, c.stream, c.Gc, L.rhs, L.err, L.g.n,
+                         L.g.m, L.g.pitch, sz);
+    else
+      PT_TRY(launch_coarse_apply_wg(c, L.frhs, L.ferr, L.g.n, L.g.m, L.g.pitch, (float)sz));
+  }
+  if (l > 1 && t.multi[p + 1] < t.multi[p]) {
+    Level& F = c.L[l - 1];
+    dim3 gp = grid2(F.g.n - 1, F.g.m - 1, b16);
+    if (!mixed)
+      hipLaunchKernelGGL((k_prolong<double, double>), gp, b16, 0, c.stream, L.err, L.g.pitch, F.err, F.g.n, F.g.m,
+                         F.g.pitch, sz);
+    else if (l - 1 == 1)
+      hipLaunchKernelGGL((k_prolong<float, double>), gp, b16, 0, c.stream, L.ferr, L.g.pitch, F.err, F.g.n,
+                         F.g.m, F.g.pitch, sz);
+    else
+      hipLaunchKernelGGL((k_prolong<float, float>), gp, b16, 0, c.stream, L.ferr, L.g.pitch, F.ferr, F.g.n,
+                         F.g.m, F.g.pitch, (float)sz);
+  }
+  return PT_OK;
+}
+
+// residual_mag(1) of the level-1 visit just enqueued, and the abnormal-
+// residual stop (PS/iterative_solver.f90:438-447)
+static int level1_residual(Context& c, double* resmag1) {
+  PT_HIP(hipStreamSynchronize(c.stream));
+  const LevelGeom& g = c.L[1].g;
+  *resmag1 = c.h_red[0] / (double)(g.n - 2) / (double)(g.m - 2);
+  if (*resmag1 > 1.0e10 * c.res_ref1) {
+    set_error("residual " + std::to_string(*resmag1) + " > 1e10 * res_ref");
+    return PT_ERR_HIGH_RESIDUAL;
+  }
+  return PT_OK;
+}
+
+// The level buffers whose roles rotate: a strip sweep swaps its level's
+// error buffer with a work buffer, so a cycle may start from one of a few
+// arrangements, and a replay must leave the pointers as the eager launches
+// would have.
+static void buffer_roles(Context& c, std::vector<void*>& out) {
+  out.clear();
+  for (int l = 1; l <= c.T.grids; l++) {
+    Level& L = c.L[l];
+    out.insert(out.end(), {(void*)L.ferr, (void*)L.fB, (void*)L.fC, (void*)L.fD, (void*)L.fE});
+  }
+}
+static void restore_roles(Context& c, const std::vector<void*>& v) {
+  size_t j = 0;
+  for (int l = 1; l <= c.T.grids; l++) {
+    Level& L = c.L[l];
+    float** f[5] = {&L.ferr, &L.fB, &L.fC, &L.fD, &L.fE};
+    for (float** q : f) *q = (float*)v[j++];
+  }
+}
+
+// visits 2..nsteps of a cycle and visit 1 of the next.  After one eager run
+// the sequence is captured as a HIP graph (no memset nodes) per starting
+// arrangement of the rotating buffers (at most MAX_CYCLE_GRAPHS) and
+// replayed; the graphs' kernel arguments (buffers, level constants) do not
+// change between cycles or solves of a context.  PT_GRAPH=0 at pt_create,
+// per-kernel timing, a failed capture or more arrangements keep the launches
+// eager; the results are the same either way.
+static constexpr size_t MAX_CYCLE_GRAPHS = 4;
+static int rest_and_next_level1(Context& c, bool mixed) {
+  const Tables& t = c.T;
+  auto enqueue = [&]() -> int {
+    for (int p = 2; p <= t.nsteps; p++) PT_TRY(visit(c, p, mixed));
+    return visit(c, 1, mixed);
+  };
+  if (!c.use_graph || c.timing || c.graph_state < 0) return enqueue();
+  if (c.graph_state == 0) {
+    c.graph_state = 1;   // the first run is eager (sets the kernels' attributes)
+    return enqueue();
+  }
+  c.graph_state = 2;
+  std::vector<void*> before;
+  buffer_roles(c, before);
+  for (const Context::CycleGraph& cg : c.cycle_graphs) {
+    if (cg.before != before) continue;
+    PT_HIP(hipGraphLaunch(cg.exec, c.stream));
+    restore_roles(c, cg.after);
+    return PT_OK;
+  }
+  if (c.cycle_graphs.size() >= MAX_CYCLE_GRAPHS) {
+    c.graph_state = -1;
+    return enqueue();
+  }
+  // capture: the launches are recorded, not run, while the host-side pointer
+  // swaps take effect; the replay below then runs them
+  Context::CycleGraph cg;
+  cg.before = before;
+  PT_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue();
+  hipError_t e = hipStreamEndCapture(c.stream, &cg.graph);
+  if (rc != PT_OK || e != hipSuccess) {
+    if (cg.graph) (void)hipGraphDestroy(cg.graph);
+    restore_roles(c, before);
+    c.graph_state = -1;
+    if (rc != PT_OK) return rc;
+    return enqueue();
+  }
+  buffer_roles(c, cg.after);
+  e = hipGraphInstantiate(&cg.exec, cg.graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(cg.graph);
+    PT_HIP(e);
+  }
+  c.cycle_graphs.push_back(std::move(cg));
+  PT_HIP(hipGraphLaunch(c.cycle_graphs.back().exec, c.stream));
+  return PT_OK;
+}
+
 // multigrid: PS/iterative_solver.f90:124-341
 int multigrid(Context& c) {
   const Tables& t = c.T;
   const bool mixed = c.P.precision == PT_PREC_MIXED;
   Level& L1 = c.L[1];
   size_t b1 = sizeof(double) * L1.nelem;
-  dim3 b16(64, 4);
-  double sz = c.P.Vbc == 0 ? -1.0 : 1.0;
   PT_HIP(hipMemcpyAsync(L1.err, c.SF, b1, hipMemcpyDeviceToDevice, c.stream));
   PT_TRY(compute_rhs1(c));
   double rr;
@@ -1706,85 +1872,23 @@ int multigrid(Context& c) {
   }
   int q = 1;
   double resmag1 = 0.0;
-  bool done = false;
-  for (;;) {
-    for (int p = 1; p <= t.nsteps; p++) {
-      int l = t.multi[p];
-      Level& L = c.L[l];
-      if (t.multi[p - 1] < t.multi[p] && l >= 2) {
-        // compute_RHS(l) = restrict(residual(l-1))
-        Level& F = c.L[l - 1];
-        if (!mixed)
-          hipLaunchKernelGGL((k_restrict<double, double>), grid2(L.g.n - 1, L.g.m - 1, b16), b16, 0, c.stream, F.res,
-                             F.g.pitch, L.rhs, L.g.n, L.g.m, L.g.pitch);
-        else if (l == 2)
-          hipLaunchKernelGGL((k_restrict<double, float>), grid2(L.g.n - 1, L.g.m - 1, b16), b16, 0, c.stream, F.res,
-                             F.g.pitch, L.frhs, L.g.n, L.g.m, L.g.pitch);
-        else
-          hipLaunchKernelGGL((k_restrict<float, float>), grid2(L.g.n - 1, L.g.m - 1, b16), b16, 0, c.stream, F.fB,
-                             F.g.pitch, L.frhs, L.g.n, L.g.m, L.g.pitch);
-      }
-      if (l < t.grids) {
-        if (!mixed) {
-          PT_TRY(iterate_stream_t<double>(c, l, p, false, l == 1 ? &resmag1 : nullptr));
-        } else if (l == 1) {
-          // FP64 defect -> FP32 RKC1 smoothing of the correction -> FP64 update
-          StageArgs<double> a = stage_base<double>(c, 1, false);
-          a.Sx = L.err;
-          dim3 blk(TX, TZ), grd((L.g.n - 1 + TX - 1) / TX, (L.g.m - 1 + TZ * PZ - 1) / (TZ * PZ));
-          launch_residual<double>(c, grd, blk, a, c.dres64, c.red);
-          PT_TRY(to_f32(c, c.dres64, L.frhs, L.nelem));
-          PT_HIP(hipMemsetAsync(L.ferr, 0, sizeof(float) * L.nelem, c.stream));
-          PT_TRY(iterate_stream_t<float>(c, 1, p, true, nullptr));
-          hipLaunchKernelGGL(k_add_f32_to_f64, grid2(L.g.n - 1, L.g.m - 1, b16), b16, 0, c.stream, L.err, L.ferr,
-                             L.g.n, L.g.m, L.g.pitch, sz);
-          launch_residual<double>(c, grd, blk, a, L.res, c.red);
-          int nb = grd.x * grd.y;
-          hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, nb, (int)RED_SUM_ABS, c.red_out);
-          PT_HIP(hipMemcpyAsync(c.h_red, c.red_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
-          PT_HIP(hipStreamSynchronize(c.stream));
-          resmag1 = c.h_red[0] / (double)(L.g.n - 2) / (double)(L.g.m - 2);
-        } else {
-          PT_TRY(iterate_stream_t<float>(c, l, p, true, nullptr));
-        }
-        if (l == 1 && resmag1 > 1.0e10 * c.res_ref1) {
-          set_error("residual " + std::to_string(resmag1) + " > 1e10 * res_ref");
-          return PT_ERR_HIGH_RESIDUAL;
-        }
-      } else {
-        // Compute_coarse_vector + DGBTRS + Stream_grid_coarse + enforceBCs_stream
-        // as one dense matvec with the operator built from the LU factors
-        dim3 gb((c.nci + 255) / 256);
-        size_t sh = sizeof(double) * c.nci;
-        if (!mixed)
-          hipLaunchKernelGGL((k_coarse_apply<double, double>), gb, dim3(256), sh, c.stream, c.Gc, L.rhs, L.err, L.g.n,
-                             L.g.m, L.g.pitch, sz);
-        else
-          PT_TRY(launch_coarse_apply_wg(c, L.frhs, L.ferr, L.g.n, L.g.m, L.g.pitch, (float)sz));
-      }
-      if (l == 1 && (resmag1 / c.res_ref1) <= c.P.tolerance) { done = true; break; }
-      if (t.multi[p + 1] < t.multi[p]) {
-        Level& F = c.L[l - 1];
-        dim3 gp = grid2(F.g.n - 1, F.g.m - 1, b16);
-        if (!mixed)
-          hipLaunchKernelGGL((k_prolong<double, double>), gp, b16, 0, c.stream, L.err, L.g.pitch, F.err, F.g.n, F.g.m,
-                             F.g.pitch, sz);
-        else if (l - 1 == 1)
-          hipLaunchKernelGGL((k_prolong<float, double>), gp, b16, 0, c.stream, L.ferr, L.g.pitch, F.err, F.g.n,
-                             F.g.m, F.g.pitch, sz);
-        else
-          hipLaunchKernelGGL((k_prolong<float, float>), gp, b16, 0, c.stream, L.ferr, L.g.pitch, F.ferr, F.g.n,
-                             F.g.m, F.g.pitch, (float)sz);
-      }
-    }
-    c.cycles++;
-    if (done || (resmag1 / c.res_ref1) <= c.P.tolerance) break;
-    q++;
-    if (q > c.P.max_cycles) {
+  // cycle q: visit 1 (level 1, ends with its residual's L1 norm on the host),
+  // then visits 2..nsteps; the stop test follows every level-1 visit
+  PT_TRY(visit(c, 1, mixed));
+  PT_TRY(level1_residual(c, &resmag1));
+  while (!((resmag1 / c.res_ref1) <= c.P.tolerance)) {
+    if (q + 1 > c.P.max_cycles) {
+      for (int p = 2; p <= t.nsteps; p++) PT_TRY(visit(c, p, mixed));
+      c.cycles++;
       set_error("q > max_cycles = " + std::to_string(c.P.max_cycles));
       return PT_ERR_MAX_CYCLES;
     }
+    PT_TRY(rest_and_next_level1(c, mixed));
+    c.cycles++;
+    q++;
+    PT_TRY(level1_residual(c, &resmag1));
   }
+  c.cycles++;
   PT_HIP(hipMemcpyAsync(c.SF, L1.err, b1, hipMemcpyDeviceToDevice, c.stream));
   c.last_q = q;
   c.residual1 = resmag1;
@@ -1884,6 +1988,11 @@ int solver_free(Context& c) {
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c.h_red) (void)hipHostFree(c.h_red);
+  for (Context::CycleGraph& cg : c.cycle_graphs) {
+    if (cg.exec) (void)hipGraphExecDestroy(cg.exec);
+    if (cg.graph) (void)hipGraphDestroy(cg.graph);
+  }
+  c.cycle_graphs.clear();
   return PT_OK;
 }
 

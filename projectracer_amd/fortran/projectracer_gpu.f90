@@ -64,6 +64,7 @@ module projectracer_gpu
   integer(c_int64_t) :: solves, cycles
   integer(c_int32_t) :: n_eta, n_temp, n_comp
   real(c_double) :: eig_fine
+  integer(c_int32_t) :: cycle_graphs
  end type pt_solve_info
 
  type(c_ptr), save :: pt_ctx = c_null_ptr
