@@ -87,6 +87,10 @@ typedef struct pt_params {
   int32_t guess;          /* PT_GUESS_* */
   /* &inputs: diagnostics */
   double  dent;           /* reference height of the entrainment integral */
+  /* &inputs: stream-function solver */
+  int32_t iterate;        /* 0 direct banded LU (PS/momentum.f90:172-653), 1 multigrid */
+  int32_t Database;       /* direct solver unknown order (PS/momentum.f90:238-253):
+                             1 p = (nx+1) k + i + 1, 2 p = (nz+1) i + k + 1 */
 } pt_params;
 
 /* Fill *p with the reference defaults used by this build (inputs.nml values
@@ -164,9 +168,21 @@ int pt_load_state(void* ctx, const double* T_nodes, double Tbot_iso, double Ttop
                   const double* Ctr);
 
 /* ---- hot path: drop-in replacements for the reference subroutines ------ */
-int pt_multigrid(void* ctx);            /* PS/iterative_solver.f90:124-341 */
+int pt_multigrid(void* ctx);            /* PS/iterative_solver.f90:124-341 (iterate=1) */
 int pt_compute_velocities(void* ctx);   /* PS/momentum.f90:655-710 + velocityBCs */
-int pt_update_velocities(void* ctx);    /* PS/momentum.f90:173-189 */
+/* PS/momentum.f90:173-189: the direct solve when the context was created with
+ * iterate=0 (building the matrix on its first call, and on every call for
+ * ivis >= 1, as PS/main.f90:247-251 does), else multigrid; then velocities */
+int pt_update_velocities(void* ctx);
+/* Direct solver (iterate=0 contexts), PS/momentum.f90:319-653:
+ * build_SF_matrix assembles the whole-grid banded operator (Database 1 or 2
+ * ordering), scales its rows by DGBEQU's factors and factors it (band LU with
+ * partial pivoting, DGBTF2's column order); solve_SF_equation forms the
+ * buoyancy vector from Tbuoy/Cbuoy, runs DGBTRS and writes psi with its ghost
+ * images (Stream_grid).  pt_solve_sf_equation before any build returns
+ * PT_ERR_INVALID; a zero pivot returns PT_ERR_SINGULAR_COARSE. */
+int pt_build_sf_matrix(void* ctx);      /* PS/momentum.f90:321-488 */
+int pt_solve_sf_equation(void* ctx);    /* PS/momentum.f90:621-653 */
 int pt_energy_time(void* ctx);          /* PS/energy.f90:181-556 (RK4, 3 solves, 4 projections) */
 int pt_tracers_to_corners(void* ctx);   /* PS/composition.f90:923-1087 on (rtr,str,Ttr) */
 /* One iteration of the time loop at PS/main.f90:245-299 (without stats and

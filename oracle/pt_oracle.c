@@ -102,6 +102,8 @@ struct pto_case {
   A2 S0[MAXL + 1], S1[MAXL + 1], S2[MAXL + 1];   /* iterate_stream automatic arrays */
   double res_ref[MAXL + 1], residual_mag[52];
   int kl, ku, ngc; double *Mc, *RowB, *Bc; int32_t* ipiv;
+  /* direct solver (iterate=0): band widths, equilibrated LU factors, right-hand side */
+  int subD, supD, ngrid, direct_built; double *MB, *RowD, *Bvec; int32_t* ipivD;
   /* tracers */
   int64_t ntr;
   double *rtr, *str, *Ttr, *Ctr, *rtr0, *rtr1, *rtr2, *str0, *str1, *str2, *Ttr0, *Ttr1, *Ttr2;
@@ -150,7 +152,10 @@ static void multigrid_setup(pto_case* c) {
   static const int w5[] = {1, 2, 3, 4, 5, 4, 5, 4, 3, 4, 5, 4, 3, 2, 3, 4, 5, 4, 3, 4, 5, 4, 5, 4, 3, 2};
   static const int w6[] = {1, 2, 3, 4, 5, 6, 5, 6, 5, 4, 5, 6, 5, 4, 3, 4, 5, 6, 5, 4, 3,
                            2, 3, 4, 5, 6, 5, 4, 3, 4, 5, 6, 5, 4, 5, 6, 5, 6, 5, 4, 3, 2};
-  optimize_grid(&c->nx, &c->nz, &c->grids, c->P.mingrid);
+  /* the direct path (iterate=0) keeps nx, nz (PS/main.f90:170); two levels
+   * are kept for the shared level-1 buffers */
+  if (c->P.iterate != 0) optimize_grid(&c->nx, &c->nz, &c->grids, c->P.mingrid);
+  else c->grids = 2;
   const int* m = NULL; int n = 0;
   switch (c->grids) {
     case 2: m = w2; n = 2; break;  case 3: m = w3; n = 6; break;
@@ -877,6 +882,35 @@ static void coarse_bandwidth(pto_case* c) {
   c->kl = kl; c->ku = ku;
 }
 
+/* DGBEQU row scale factors R(i) = 1/max_j |A(i,j)| (LAPACK dgbequ; the column
+ * factors are not used) and the row scaling that follows the call
+ * (PS/iterative_solver.f90:1832-1839, PS/momentum.f90:462-467).  AB has
+ * LDAB = 2*kl+ku+1 with A(p,q) at row kl+ku+1+p-q. */
+static void equilibrate_rows(int n, int kl, int ku, double* ab, double* R) {
+  int ld = 2 * kl + ku + 1, kk1 = kl + ku + 1;
+#define MQ(r, q) ab[(long)((q) - 1) * ld + ((r) - 1)]
+  for (int i = 1; i <= n; i++) R[i - 1] = 0.0;
+  for (int q = 1; q <= n; q++) {
+    int lo = q - ku > 1 ? q - ku : 1, hi = q + kl < n ? q + kl : n;
+    for (int p = lo; p <= hi; p++) {
+      double a = fabs(MQ(kk1 + p - q, q));
+      if (a > R[p - 1]) R[p - 1] = a;
+    }
+  }
+  double smlnum = DBL_MIN, bignum = 1.0 / smlnum;
+  for (int i = 1; i <= n; i++) {
+    double r = R[i - 1];
+    if (r < smlnum) r = smlnum;
+    if (r > bignum) r = bignum;
+    R[i - 1] = 1.0 / r;
+  }
+  for (int q = 1; q <= n; q++) {
+    int lo = q - kl > 1 ? q - kl : 1, hi = q + ku < n ? q + ku : n;
+    for (int p = lo; p <= hi; p++) MQ(kk1 + p - q, q) = R[p - 1] * MQ(kk1 + p - q, q);
+  }
+#undef MQ
+}
+
 /* Compute_Matrix_coarse analysis=0 + DGBEQU row scaling: PS/iterative_solver.f90:1747-1839 */
 static void compute_matrix_coarse(pto_case* c) {
   int nxc = c->nxg[c->grids], nzc = c->nzg[c->grids];
@@ -914,26 +948,7 @@ static void compute_matrix_coarse(pto_case* c) {
   }
   for (int kp = 0; kp <= nzc; kp++) { MC(kk1, pf_coarse(c, 0, kp)) = 1.0; MC(kk1, pf_coarse(c, nxc, kp)) = 1.0; }
   for (int ip = 0; ip <= nxc; ip++) { MC(kk1, pf_coarse(c, ip, 0)) = 1.0; MC(kk1, pf_coarse(c, ip, nzc)) = 1.0; }
-  /* DGBEQU row scale factors: R(i) = 1/max_j |A(i,j)| (LAPACK dgbequ) */
-  for (int i = 1; i <= n; i++) c->RowB[i - 1] = 0.0;
-  for (int q = 1; q <= n; q++) {
-    int lo = q - ku > 1 ? q - ku : 1, hi = q + kl < n ? q + kl : n;
-    for (int p = lo; p <= hi; p++) {
-      double a = fabs(MC(kk1 + p - q, q));
-      if (a > c->RowB[p - 1]) c->RowB[p - 1] = a;
-    }
-  }
-  double smlnum = DBL_MIN, bignum = 1.0 / smlnum;
-  for (int i = 1; i <= n; i++) {
-    double r = c->RowB[i - 1];
-    if (r < smlnum) r = smlnum;
-    if (r > bignum) r = bignum;
-    c->RowB[i - 1] = 1.0 / r;
-  }
-  for (int q = 1; q <= n; q++) {
-    int lo = q - kl > 1 ? q - kl : 1, hi = q + ku < n ? q + ku : n;
-    for (int p = lo; p <= hi; p++) MC(kk1 + p - q, q) = c->RowB[p - 1] * MC(kk1 + p - q, q);
-  }
+  equilibrate_rows(n, kl, ku, c->Mc, c->RowB);
 #undef MC
 }
 
@@ -1001,6 +1016,220 @@ int pto_band_lu_solve(int n, int kl, int ku, double* ab, int32_t* ipiv, double* 
   int info = band_lu(n, kl, ku, ab, ipiv);
   if (info == 0 && b) band_solve(n, kl, ku, ab, ipiv, b);
   return info;
+}
+
+/* ---- direct solver (iterate=0): PS/momentum.f90:172-653 ------------------- */
+/* pf and indices for Database 1 and 2: PS/momentum.f90:221-253 */
+static inline int pf_full(pto_case* c, int ip, int kp) {
+  return c->P.Database == 1 ? (c->nx + 1) * kp + ip + 1 : (c->nz + 1) * ip + kp + 1;
+}
+static inline void indices_full(pto_case* c, int p, int* ip, int* kp) {
+  if (c->P.Database == 1) {
+    int N = c->nx + 1;
+    *ip = p - N * ((p - 1) / N) - 1; *kp = (p - 1) / N;
+  } else {
+    int N = c->nz + 1;
+    *kp = p - N * ((p - 1) / N) - 1; *ip = (p - 1) / N;
+  }
+}
+
+/* viscosity terms of one interior row (PS/momentum.f90:369-381, second-order
+ * stencils; uniform grid metrics xr = zs = 1, xrr = zss = 0) */
+typedef struct { double vis, visx, visz, visxx, viszz, visxz; } vis_terms;
+
+/* Matrix_element: PS/momentum.f90:507-606.  mode 0 widens subD/supD
+ * (bandwidth, PS/momentum.f90:608-619), mode 1 adds the entry into MB. */
+static void matrix_element(pto_case* c, int i, int k, int ip, int kp, int p, int q, int rsym, int ssym,
+                           const vis_terms* v, int mode) {
+  const double xr = 1.0, xrr = 0.0, xrrr = 0.0, xrrrr = 0.0, zs = 1.0, zss = 0.0, zsss = 0.0, zssss = 0.0;
+  double Ss, Sss, Ssss, Sssss, Sr, Srr, Srrr, Srrrr;
+  if (i == 0) {
+    Ss = c->D1[k + 3] / c->ds; Sss = c->D2[k + 3] / c->ds2; Ssss = c->D3[k + 3] / c->ds3; Sssss = c->D4[k + 3] / c->ds4;
+  } else {
+    Ss = 0.0; Sss = 0.0; Ssss = 0.0; Sssss = 0.0;
+  }
+  if (ssym == 1 && c->P.Vbc == 0) { Ss = -Ss; Sss = -Sss; Ssss = -Ssss; Sssss = -Sssss; }
+  if (k == 0) {
+    Sr = c->D1[i + 3] / c->dr; Srr = c->D2[i + 3] / c->dr2; Srrr = c->D3[i + 3] / c->dr3; Srrrr = c->D4[i + 3] / c->dr4;
+  } else {
+    Sr = 0.0; Srr = 0.0; Srrr = 0.0; Srrrr = 0.0;
+  }
+  if (rsym == 1) { Sr = -Sr; Srr = -Srr; Srrr = -Srrr; Srrrr = -Srrrr; }
+  double Sx = Sr / xr;
+  double Sxx = (Srr - Sx * xrr) / pow(xr, 2.0);
+  double Sxxx = (Srrr - 3.0 * Sxx * xr * xrr - Sx * xrrr) / pow(xr, 3.0);
+  double Sxxxx = (Srrrr - 6.0 * Sxxx * xrr * pow(xr, 2.0) - (3.0 * pow(xrr, 2.0) + 4.0 * xr * xrrr) * Sxx - Sx * xrrrr) /
+                 pow(xr, 4.0);
+  double Sz = Ss / zs;
+  double Szz = (Sss - Sz * zss) / pow(zs, 2.0);
+  double Szzz = (Ssss - 3.0 * Szz * zs * zss - Sz * zsss) / pow(zs, 3.0);
+  double Szzzz = (Sssss - 6.0 * Szzz * zss * pow(zs, 2.0) - (3.0 * pow(zss, 2.0) + 4.0 * zs * zsss) * Szz - Sz * zssss) /
+                 pow(zs, 4.0);
+  double Srs = c->Drs[k + 3][i + 3] / c->drds, Srss = c->Drss[k + 3][i + 3] / c->drds2;
+  double Srrs = c->Drrs[k + 3][i + 3] / c->dr2ds, Srrss = c->Drrss[k + 3][i + 3] / c->dr2ds2;
+  int flip = (c->P.Vbc == 0) ? ((rsym == 1) != (ssym == 1)) : (rsym == 1);
+  if (flip) { Srs = -Srs; Srss = -Srss; Srrs = -Srrs; Srrss = -Srrss; }
+  double Sxz = Srs / xr / zs;
+  double Sxxz = (Srrs - Sxz * zs * xrr) / zs / pow(xr, 2.0);
+  double Sxzz = (Srss - Sxz * xr * zss) / xr / pow(zs, 2.0);
+  double Sxxzz = (Srrss - Sxzz * xrr * pow(zs, 2.0) - Sxxz * zss * pow(xr, 2.0) - Sxz * xrr * zss) / pow(xr * zs, 2.0);
+  const int kk1 = c->subD + c->supD + 1, ld = 2 * c->subD + c->supD + 1;
+  double* m = mode ? &c->MB[(long)(q - 1) * ld + (kk1 + p - q - 1)] : NULL;
+  int wall_x = (ip == 0 || ip == c->nx), wall_z = (kp == 0 || kp == c->nz);
+  if (wall_x || (wall_z && c->P.Vbc == 0)) {
+    if (mode == 0) {
+      int diff = q - p;
+      if (diff < c->subD) c->subD = diff;
+      else if (diff > c->supD) c->supD = diff;
+    } else {
+      *m = *m + (wall_x ? Sxx : Szz);
+    }
+  } else if (wall_z) {
+    /* rigid top/bottom rows are identity rows (set by the caller) */
+  } else if (mode == 0) {
+    int diff = q - p;
+    if (diff < c->subD) c->subD = diff;
+    else if (diff > c->supD) c->supD = diff;
+  } else {
+    double term1 = (v->visxx - v->viszz) * (Sxx - Szz);
+    double term2 = v->vis * (Sxxxx + Szzzz);
+    double term3 = 2.0 * v->vis * Sxxzz;
+    double term4 = 4.0 * v->visxz * Sxz;
+    double term5 = 2.0 * v->visx * (Sxxx + Sxzz);
+    double term6 = 2.0 * v->visz * (Szzz + Sxxz);
+    *m = *m + term1 + term2 + term3 + term4 + term5 + term6;
+  }
+}
+
+/* the stencil point (i,k) of row (ip,kp), folded into the domain with the
+ * (anti)symmetry flags: PS/momentum.f90:383-474 */
+static void stencil_point(pto_case* c, int i, int k, int ip, int kp, int p, const vis_terms* v, int mode) {
+  int rsym = 0, ssym = 0, iq = ip + i, kq = kp + k;
+  if (iq < 0) { iq = -iq; rsym = 1; }
+  else if (iq > c->nx) { iq = c->nx - (iq - c->nx); rsym = 1; }
+  if (kq < 0) { kq = -kq; ssym = 1; }
+  else if (kq > c->nz) { kq = c->nz - (kq - c->nz); ssym = 1; }
+  matrix_element(c, i, k, ip, kp, p, pf_full(c, iq, kq), rsym, ssym, v, mode);
+}
+
+/* the row loop of build_SF_matrix (PS/momentum.f90:362-478): mode 0 is the
+ * tstep=-1 call that finds the band widths, mode 1 adds the entries into MB
+ * (zeroed by the caller) from the viscosity in vis_grid(:,:,1) */
+static void sf_rows(pto_case* c, int mode) {
+  const int nx = c->nx, nz = c->nz, n = c->ngrid;
+  const A2* vis = &c->vis_grid[1];
+  const int kk1 = c->subD + c->supD + 1, ld = 2 * c->subD + c->supD + 1;
+  for (int p = 1; p <= n; p++) {
+    int ip, kp; indices_full(c, p, &ip, &kp);
+    int corner = (ip == 0 || ip == nx) && (kp == 0 || kp == nz);
+    if (corner || (c->P.Vbc == 1 && (kp == 0 || kp == nz))) {
+      if (mode) c->MB[(long)(p - 1) * ld + (kk1 - 1)] = 1.0;
+      continue;
+    }
+    vis_terms v = {0};
+    if (mode && ip > 0 && ip < nx && kp > 0 && kp < nz) {
+      /* the reference evaluates these on the wall rows too, where they are unused */
+      double visr = 0.0, visrr = 0.0, viss = 0.0, visss = 0.0, visrs = 0.0;
+      for (int j = -1; j <= 1; j++) visr = visr + c->D1_so[j + 1] * AT(*vis, ip + j, kp);
+      for (int j = -1; j <= 1; j++) visrr = visrr + c->D2_so[j + 1] * AT(*vis, ip + j, kp);
+      for (int j = -1; j <= 1; j++) viss = viss + c->D1_so[j + 1] * AT(*vis, ip, kp + j);
+      for (int j = -1; j <= 1; j++) visss = visss + c->D2_so[j + 1] * AT(*vis, ip, kp + j);
+      for (int b = -1; b <= 1; b++)
+        for (int a = -1; a <= 1; a++) visrs = visrs + c->Drs_so[b + 1][a + 1] * AT(*vis, ip + a, kp + b);
+      visr = visr / c->dr; visrr = visrr / c->dr2; viss = viss / c->ds; visss = visss / c->ds2; visrs = visrs / c->drds;
+      v.vis = AT(*vis, ip, kp);
+      v.visx = visr / 1.0;
+      v.visz = viss / 1.0;
+      v.visxx = (visrr - v.visx * 0.0) / pow(1.0, 2.0);
+      v.viszz = (visss - v.visz * 0.0) / pow(1.0, 2.0);
+      v.visxz = visrs / 1.0 / 1.0;
+    }
+    for (int i = -3; i <= 3; i += 6) stencil_point(c, i, 0, ip, kp, p, &v, mode);
+    for (int k = -3; k <= 3; k += 6) stencil_point(c, 0, k, ip, kp, p, &v, mode);
+    for (int i = -2; i <= 2; i++)
+      for (int k = -2; k <= 2; k++) stencil_point(c, i, k, ip, kp, p, &v, mode);
+  }
+}
+
+/* build_SF_matrix's assembly with tstep >= 0 (PS/momentum.f90:342-478): MatrixB
+ * from zero, vis = visf at the nodes (isoviscous: 1; the smoother_vis filter of
+ * ivis >= 1 is not part of this build) */
+static void assemble_sf(pto_case* c) {
+  memset(c->MB, 0, sizeof(double) * (size_t)(2 * c->subD + c->supD + 1) * c->ngrid);
+  for (int k = 0; k <= c->nz; k++)
+    for (int i = 0; i <= c->nx; i++) AT(c->vis_grid[1], i, k) = visf(c, i, k);
+  sf_rows(c, 1);
+}
+
+/* build_SF_matrix: assembly, DGBEQU row scaling and the band LU
+ * (PS/momentum.f90:321-488) */
+static int build_sf_matrix(pto_case* c) {
+  assemble_sf(c);
+  equilibrate_rows(c->ngrid, c->subD, c->supD, c->MB, c->RowD);
+  return band_lu(c->ngrid, c->subD, c->supD, c->MB, c->ipivD) == 0 ? PT_OK : PT_ERR_SINGULAR_COARSE;
+}
+
+/* buoyancy + DGBTRS + Stream_grid: PS/momentum.f90:255-319, 621-653 */
+static void solve_sf_equation(pto_case* c) {
+  const int nx = c->nx, nz = c->nz, n = c->ngrid;
+  for (int p = 1; p <= n; p++) {
+    int ip, kp; indices_full(c, p, &ip, &kp);
+    c->Bvec[p - 1] = 0.0;
+    if (ip > 0 && ip < nx && kp > 0 && kp < nz) {
+      double Tx = (dot5x(c->D1, &c->Tbuoy, ip, kp) / c->dr) / 1.0;
+      double Cx = (dot5x(c->D1, &c->Cbuoy, ip, kp) / c->dr) / 1.0;
+      double rc = 0.0 + c->P.RaC * Cx;
+      c->Bvec[p - 1] = (c->P.RaT * Tx - rc) * c->RowD[p - 1];
+    }
+  }
+  band_solve(n, c->subD, c->supD, c->MB, c->ipivD, c->Bvec);
+  for (int p = 1; p <= n; p++) {
+    int ip, kp; indices_full(c, p, &ip, &kp);
+    AT(c->SF, ip, kp) = c->Bvec[p - 1];
+  }
+  for (int kp = 0; kp <= nz; kp++)
+    for (int ip = 1; ip <= c->span; ip++) {
+      AT(c->SF, -ip, kp) = -AT(c->SF, ip, kp);
+      AT(c->SF, nx + ip, kp) = -AT(c->SF, nx - ip, kp);
+    }
+  for (int kp = 1; kp <= c->span; kp++)
+    for (int ip = -c->span; ip <= nx + c->span; ip++) {
+      if (c->P.Vbc == 0) {
+        AT(c->SF, ip, -kp) = -AT(c->SF, ip, kp);
+        AT(c->SF, ip, nz + kp) = -AT(c->SF, ip, nz - kp);
+      } else {
+        AT(c->SF, ip, -kp) = AT(c->SF, ip, kp);
+        AT(c->SF, ip, nz + kp) = AT(c->SF, ip, nz - kp);
+      }
+    }
+}
+
+/* update_velocities' iterate=0 branch (PS/momentum.f90:175-179) with the
+ * time loop's tstep=0 build (PS/main.f90:247-251): the matrix is built on the
+ * first solve and, for ivis >= 1, on every solve */
+int pto_solve_direct(pto_case* c) {
+  if (c->P.iterate != 0) return PT_ERR_INVALID;
+  if (!c->direct_built || c->P.ivis >= 1) {
+    int rc = build_sf_matrix(c);
+    if (rc) return rc;
+    c->direct_built = 1;
+  }
+  solve_sf_equation(c);
+  c->solves++;
+  return PT_OK;
+}
+
+/* the assembled band matrix (LDAB = 2*kl+ku+1) before factoring and, when
+ * row is given, after the DGBEQU row scaling; for tests of the assembly */
+int pto_direct_matrix(pto_case* c, double* ab, double* row, int32_t* kl, int32_t* ku) {
+  if (c->P.iterate != 0) return PT_ERR_INVALID;
+  double* keep = c->MB;
+  c->MB = ab;
+  assemble_sf(c);
+  if (row) equilibrate_rows(c->ngrid, c->subD, c->supD, ab, row);
+  c->MB = keep;
+  *kl = c->subD; *ku = c->supD;
+  return PT_OK;
 }
 
 /* ---- smoother: iterate_stream (RKC1, Nmulti iterations) --------------------- */
@@ -1097,8 +1326,9 @@ void pto_compute_velocities(pto_case* c) {
   velocity_bcs(c);
 }
 
+/* update_velocities: PS/momentum.f90:172-189 */
 int pto_update_velocities(pto_case* c) {
-  int rc = pto_multigrid(c);
+  int rc = c->P.iterate == 0 ? pto_solve_direct(c) : pto_multigrid(c);
   if (rc) return rc;
   pto_compute_velocities(c);
   return PT_OK;
@@ -1422,9 +1652,8 @@ int pto_energy_time(pto_case* c) {
 
 /* one iteration of the time loop: PS/main.f90:246-299 without stats/output */
 int pto_time_step(pto_case* c) {
-  int rc = pto_multigrid(c);
+  int rc = pto_update_velocities(c);
   if (rc) return rc;
-  pto_compute_velocities(c);
   rc = pto_energy_time(c);
   if (rc) return rc;
   c->time = c->time + c->dt;
@@ -1437,6 +1666,8 @@ static int check_supported(const pt_params* p) {
   if (p->nstageS < 2 || p->nstageS > 60 || p->Nmulti < 1) return PT_ERR_INVALID;
   if (p->ivis < 0 || p->ivis > 2) return PT_ERR_UNSUPPORTED;
   if (p->RaT == 0.0) return PT_ERR_UNSUPPORTED;
+  if (p->iterate != 0 && p->iterate != 1) return PT_ERR_INVALID;
+  if (p->iterate == 0 && (p->ivis != 0 || (p->Database != 1 && p->Database != 2))) return PT_ERR_UNSUPPORTED;
   return PT_OK;
 }
 
@@ -1509,6 +1740,15 @@ int pto_create(const pt_params* p, pto_case** out) {
   c->Mc = (double*)calloc((size_t)(2 * c->kl + c->ku + 1) * c->ngc, sizeof(double));
   c->RowB = (double*)calloc(c->ngc, sizeof(double)); c->Bc = (double*)calloc(c->ngc, sizeof(double));
   c->ipiv = (int32_t*)calloc(c->ngc, sizeof(int32_t));
+  if (p->iterate == 0) {
+    /* build_SF_matrix with tstep=-1 (PS/main.f90:224-226) sizes MatrixB */
+    c->ngrid = (nx + 1) * (nz + 1);
+    sf_rows(c, 0);
+    c->subD = -c->subD;
+    c->MB = (double*)calloc((size_t)(2 * c->subD + c->supD + 1) * c->ngrid, sizeof(double));
+    c->RowD = (double*)calloc(c->ngrid, sizeof(double)); c->Bvec = (double*)calloc(c->ngrid, sizeof(double));
+    c->ipivD = (int32_t*)calloc(c->ngrid, sizeof(int32_t));
+  }
   int64_t ntr = c->ntr;
   double** tarr[] = {&c->rtr, &c->str, &c->Ttr, &c->Ctr, &c->rtr0, &c->rtr1, &c->rtr2, &c->str0, &c->str1,
                      &c->str2, &c->Ttr0, &c->Ttr1, &c->Ttr2};
@@ -1567,6 +1807,7 @@ void pto_destroy(pto_case* c) {
     for (unsigned j = 0; j < sizeof ls / sizeof ls[0]; j++) a2_free(ls[j]);
   }
   free(c->DERr_u); free(c->DERr_w); free(c->DERr_tracer); free(c->Mc); free(c->RowB); free(c->Bc); free(c->ipiv);
+  free(c->MB); free(c->RowD); free(c->Bvec); free(c->ipivD);
   double* tarr[] = {c->rtr, c->str, c->Ttr, c->Ctr, c->rtr0, c->rtr1, c->rtr2, c->str0, c->str1, c->str2, c->Ttr0, c->Ttr1, c->Ttr2};
   for (int j = 0; j < 13; j++) free(tarr[j]);
   free(c->nempty); free(c->TxS); free(c->CxS); free(c->shape_sum);
@@ -1575,6 +1816,9 @@ void pto_destroy(pto_case* c) {
 
 void pto_dims(pto_case* c, int32_t* nx, int32_t* nz, int32_t* grids, int64_t* ntr) {
   *nx = c->nx; *nz = c->nz; *grids = c->grids; *ntr = c->ntr;
+}
+void pto_direct_dims(pto_case* c, int32_t* ngrid, int32_t* kl, int32_t* ku) {
+  *ngrid = c->ngrid; *kl = c->subD; *ku = c->supD;
 }
 
 /* tail of initial_temperature + initialize_tracers: PS/basics.f90:352-359,
@@ -1681,6 +1925,14 @@ static double strain_node(pto_case* c, const A2* e, int i, int k) {
   double Sxx = sxx / c->dr2, Szz = szz / c->ds2, Sxz = sxz / c->drds;
   return pow(2.0 * pow(Sxz, 2.0) + 0.5 * pow(Szz - Sxx, 2.0), 0.5);
 }
+/* strain-rate invariant at node (i,k) from the velocities, the direct path's
+ * compute_strain_rate_invariant (PS/momentum.f90:1-23; uniform grid) */
+static double strain_vel_node(pto_case* c, int i, int k) {
+  double ur = dot5x(c->D1, &c->u, i, k) / c->dr, us = dot5z(c->D1, &c->u, i, k) / c->ds;
+  double wr = dot5x(c->D1, &c->w, i, k) / c->dr, ws = dot5z(c->D1, &c->w, i, k) / c->ds;
+  double ux = ur / 1.0, uz = us / 1.0, wx = wr / 1.0, wz = ws / 1.0;
+  return pow(0.5 * (2.0 * (pow(ux, 2.0) + pow(wz, 2.0)) + pow(uz + wx, 2.0)), 0.5);
+}
 
 int pto_stats(pto_case* c, pt_stats_row* row) {
   const int nx = c->nx, nz = c->nz;
@@ -1691,7 +1943,7 @@ int pto_stats(pto_case* c, pt_stats_row* row) {
   /* strain_max = maxval(strain(0:nx,nz,1)) from error(:,:,1) = psi */
   double smax = -DBL_MAX;
   for (int i = 0; i <= nx; i++) {
-    double sv = strain_node(c, &c->SF, i, nz);
+    double sv = c->P.iterate == 0 ? strain_vel_node(c, i, nz) : strain_node(c, &c->SF, i, nz);
     if (sv > smax) smax = sv;
   }
   /* Cavg and entrainment (kent from entrainment_setup, PS/composition.f90:765-774) */

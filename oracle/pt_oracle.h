@@ -29,6 +29,9 @@ int  pto_load_state(pto_case* c, const double* T_nodes, double Tbot_iso, double 
 int  pto_multigrid(pto_case* c);
 void pto_compute_velocities(pto_case* c);
 int  pto_update_velocities(pto_case* c);
+int  pto_solve_direct(pto_case* c);
+int  pto_direct_matrix(pto_case* c, double* ab, double* row, int32_t* kl, int32_t* ku);
+void pto_direct_dims(pto_case* c, int32_t* ngrid, int32_t* kl, int32_t* ku);
 int  pto_energy_time(pto_case* c);
 int  pto_tracers_to_corners(pto_case* c);
 int  pto_time_step(pto_case* c);

@@ -12,6 +12,8 @@
 !!   * the state is written in full FP64 to state_NNNNN.bin (stream access)
 !!     instead of the reference's snapshots (dump_every<0: no state files;
 !!     each loop iteration's wall time is printed for timing runs).
+!! With iterate=0 the set-up and the velocity solve follow main.f90's direct
+!! branches (PS/main.f90:189-203, 224-226, 247-251).
 !! Nothing in the reference sources is modified.
 program ref_driver
  use basics
@@ -46,11 +48,17 @@ program ref_driver
  call build_grid
  call derivative_eigenvalues
  call stability_RK4_smoother
- call eigenvalue_prep
- call stream_space_matrix_prep
- do level=1,grids
+ if (iterate.eq.1) then
+  call eigenvalue_prep
+  call stream_space_matrix_prep
+  do level=1,grids
+   call stability_vis_smoother(level)
+  end do
+ else
+  grids=1
+  level=1
   call stability_vis_smoother(level)
- end do
+ end if
  call write_init
 
  if (inject.eq.0) then
@@ -69,8 +77,13 @@ program ref_driver
   call enforceBCs(T)
   call tracers_to_corners(rtr,str,Ttr,T)
  end if
- SF=0.d0
- error(:,:,2:grids)=0.d0
+ tstep=-1
+ if (iterate.eq.0) then
+  call build_SF_matrix
+ else
+  SF=0.d0
+  error(:,:,2:grids)=0.d0
+ end if
  time=0.d0
  write(*,*) "driver: initialised nx,nz,grids,ntr=",nx,nz,grids,ntr
  t_solve0=0.d0
@@ -78,7 +91,12 @@ program ref_driver
 !$ t1=omp_get_wtime_d()
  do tstep=0,nrun
   t_it0=omp_get_wtime_d()
-  call multigrid
+  if (iterate.eq.0) then
+   if ((ivis.ge.1).or.(tstep.eq.0)) call build_SF_matrix
+   call solve_SF_equation
+  else
+   call multigrid
+  end if
   call compute_velocities
   step_dump=0
   if ((tstep.eq.nrun).and.(dump_every.ge.0)) step_dump=1
@@ -123,7 +141,11 @@ contains
   write(603) dt,time,Tbot_iso,Ttop_iso
   write(603) SF,Tratio,u,w,Tvis,Tbuoy
   write(603) Cnew(1,:,:),Cvis(1,:,:),conduct
-  write(603) vis_grid(:,:,1)
+  if (iterate.eq.0) then
+   write(603) vis
+  else
+   write(603) vis_grid(:,:,1)
+  end if
   write(603) nempty(0:nx,0:nz)
   write(603) rtr,str,Ttr,Ctr
   close(603)

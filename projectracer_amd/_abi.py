@@ -68,7 +68,7 @@ class PtParams(C.Structure):
         ("visP", C.c_double), ("visT", C.c_double), ("delta_visT", C.c_double),
         ("eig_ratio_min", C.c_double), ("visC", C.c_double), ("delta_visC", C.c_double),
         ("precision", C.c_int32), ("guess", C.c_int32),
-        ("dent", C.c_double),
+        ("dent", C.c_double), ("iterate", C.c_int32), ("Database", C.c_int32),
     ]
 
 
@@ -105,7 +105,8 @@ def default_params(**overrides) -> PtParams:
         courant=0.5, courantT=0.99, courantC=0.99, courant_stream=0.99,
         tpc=20, Tbot=0, Vbc=0, RaT=1e5, RaC=0.0, H=0.0, conduct_factor=1.0, Htype=0.0,
         ivis=0, visP=1.0, visT=1e3, delta_visT=2.0, eig_ratio_min=2.0, visC=10.0,
-        delta_visC=2.0, precision=PT_PREC_FP64, guess=PT_GUESS_WARM, dent=0.5)
+        delta_visC=2.0, precision=PT_PREC_FP64, guess=PT_GUESS_WARM, dent=0.5,
+        iterate=1, Database=2)
     for k, v in overrides.items():
         setattr(p, k, v)
     return p
@@ -141,6 +142,8 @@ def load_library(path: os.PathLike | None = None):
         "pt_multigrid": (C.c_int, [vp]),
         "pt_compute_velocities": (C.c_int, [vp]),
         "pt_update_velocities": (C.c_int, [vp]),
+        "pt_build_sf_matrix": (C.c_int, [vp]),
+        "pt_solve_sf_equation": (C.c_int, [vp]),
         "pt_energy_time": (C.c_int, [vp]),
         "pt_tracers_to_corners": (C.c_int, [vp]),
         "pt_time_step": (C.c_int, [vp]),
@@ -171,7 +174,8 @@ def load_library(path: os.PathLike | None = None):
 
 EXPORTED_SYMBOLS = [
     "pt_default_params", "pt_create", "pt_destroy", "pt_get_dims", "pt_load_state",
-    "pt_multigrid", "pt_compute_velocities", "pt_update_velocities", "pt_energy_time",
+    "pt_multigrid", "pt_compute_velocities", "pt_update_velocities", "pt_build_sf_matrix",
+    "pt_solve_sf_equation", "pt_energy_time",
     "pt_tracers_to_corners", "pt_time_step", "pt_run_steps", "pt_field_extent",
     "pt_get_field", "pt_set_field", "pt_get_tracers", "pt_get_nempty", "pt_get_info",
     "pt_stats", "pt_write_restart", "pt_write_snapshot", "pt_read_restart", "pt_synchronize", "pt_enable_kernel_timing",

@@ -54,6 +54,14 @@ static int check_supported(const pt_params& p) {
     set_error("only the warm-start initial guess is built");
     return PT_ERR_UNSUPPORTED;
   }
+  if (p.iterate != 0 && p.iterate != 1) {
+    set_error("iterate must be 0 (direct) or 1 (multigrid)");
+    return PT_ERR_INVALID;
+  }
+  if (p.iterate == 0 && (p.ivis != 0 || (p.Database != 1 && p.Database != 2))) {
+    set_error("the direct solver (iterate=0) is built for ivis=0 and Database 1 or 2");
+    return PT_ERR_UNSUPPORTED;
+  }
   return PT_OK;
 }
 
@@ -97,7 +105,7 @@ void pt_default_params(pt_params* p) {
   p->H = 0.0; p->conduct_factor = 1.0; p->Htype = 0.0;
   p->ivis = 0; p->visP = 1.0; p->visT = 1e3; p->delta_visT = 2.0; p->eig_ratio_min = 2.0;
   p->visC = 10.0; p->delta_visC = 2.0; p->precision = PT_PREC_FP64; p->guess = PT_GUESS_WARM;
-  p->dent = 0.5;
+  p->dent = 0.5; p->iterate = 1; p->Database = 2;
 }
 
 int pt_create(const pt_params* p, void** out) {
@@ -138,9 +146,11 @@ int pt_create(const pt_params* p, void** out) {
     }
   rc = solver_alloc(*c);
   if (rc == PT_OK) rc = tracers_alloc(*c);
+  if (rc == PT_OK && c->P.iterate == 0) rc = direct_alloc(*c);
   if (rc != PT_OK) {
     solver_free(*c);
     tracers_free(*c);
+    direct_free(*c);
     delete c;
     return rc;
   }
@@ -154,6 +164,7 @@ int pt_destroy(void* p) {
   (void)hipStreamSynchronize(c->stream);
   solver_free(*c);
   tracers_free(*c);
+  direct_free(*c);
   for (auto& t : c->kt)
     for (int j = 0; j < 1024; j++) {
       (void)hipEventDestroy(t.beg[j]);
@@ -184,6 +195,10 @@ int pt_load_state(void* p, const double* T_nodes, double Tbot_iso, double Ttop_i
 int pt_multigrid(void* p) {
   Context* c = ctx_of(p);
   if (!c) return PT_ERR_INVALID;
+  if (c->P.iterate != 1) {
+    set_error("multigrid needs a context created with iterate=1");
+    return PT_ERR_INVALID;
+  }
   PT_TRY(multigrid(*c));
   PT_HIP(hipStreamSynchronize(c->stream));
   return PT_OK;
@@ -200,8 +215,24 @@ int pt_compute_velocities(void* p) {
 int pt_update_velocities(void* p) {
   Context* c = ctx_of(p);
   if (!c) return PT_ERR_INVALID;
-  PT_TRY(multigrid(*c));
+  PT_TRY(solve_stream(*c));
   PT_TRY(compute_velocities(*c));
+  PT_HIP(hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+int pt_build_sf_matrix(void* p) {
+  Context* c = ctx_of(p);
+  if (!c) return PT_ERR_INVALID;
+  PT_TRY(build_sf_matrix(*c));
+  PT_HIP(hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+int pt_solve_sf_equation(void* p) {
+  Context* c = ctx_of(p);
+  if (!c) return PT_ERR_INVALID;
+  PT_TRY(solve_sf_equation(*c));
   PT_HIP(hipStreamSynchronize(c->stream));
   return PT_OK;
 }
@@ -225,7 +256,7 @@ int pt_tracers_to_corners(void* p) {
 }
 
 static int time_step_async(Context& c) {
-  PT_TRY(multigrid(c));
+  PT_TRY(solve_stream(c));
   PT_TRY(compute_velocities(c));
   PT_TRY(energy_time(c));
   c.time = c.time + c.dt;

@@ -44,6 +44,7 @@ struct Tables {
   double dts;                    // dts_array value (uniform grid => constant)
   double dt_vis[MAXL + 1];       // dt_vis per level (constant per level)
   int kl, ku, ngc;
+  int dn, dkl, dku;               // direct solver (iterate=0): unknowns, band widths
   int64_t ntr;
 };
 int build_tables(const pt_params& p, Tables& t);   // returns PT_* status
@@ -100,6 +101,10 @@ struct Context {
   // restricted to interior unknowns, built from the LU once per factorization
   double *Mc, *RowB, *Bc; int32_t* ipiv; int32_t* d_info;
   double* Gc; double* Gwork; int nci;
+  // direct solver (iterate=0): equilibrated band LU of the whole grid, row
+  // factors, right-hand side / solution vector
+  double *dAB = nullptr, *dRow = nullptr, *dB = nullptr; int32_t* dipiv = nullptr;
+  bool direct_built = false;
   // tracers, SoA in cell order (re-sorted after every projection of the step's
   // final positions); tid maps a storage slot to the tracer's reference ID
   double *r, *s, *Tt, *Ct, *r0, *s0, *T0t, *r1, *s1, *T1t, *r2, *s2, *T2t;
@@ -169,6 +174,15 @@ int solver_free(Context& c);
 int multigrid(Context& c);
 int compute_velocities(Context& c);
 int compute_rhs1(Context& c);
+int launch_equilibrate(Context& c, double* ab, double* row, int n, int kl, int ku);
+// ---- direct solver (pt_direct.hip) --------------------------------------------
+int direct_alloc(Context& c);
+int direct_free(Context& c);
+int build_sf_matrix(Context& c);
+int solve_sf_equation(Context& c);
+int solve_direct(Context& c);
+// the solve of update_velocities: direct (iterate=0) or multigrid
+inline int solve_stream(Context& c) { return c.P.iterate == 0 ? solve_direct(c) : multigrid(c); }
 // ---- tracers (pt_tracers.hip) ------------------------------------------------
 int tracers_alloc(Context& c);
 int tracers_free(Context& c);

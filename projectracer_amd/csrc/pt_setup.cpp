@@ -318,6 +318,36 @@ static void coarse_bandwidth(Tables& t) {
   t.kl = kl; t.ku = ku;
 }
 
+// build_SF_matrix with tstep=-1 (PS/momentum.f90:341-483, 608-619): the band
+// widths of the whole-grid operator, min/max of q-p over the folded stencil
+// points (i = +-3 at k = 0, k = +-3 at i = 0, the 5x5 block) of every row that
+// is not an identity row (corners; the top and bottom rows when rigid)
+static void direct_bandwidth(Tables& t, int Vbc, int Database) {
+  const int nx = t.nx, nz = t.nz;
+  auto pf = [&](int ip, int kp) { return Database == 1 ? (nx + 1) * kp + ip + 1 : (nz + 1) * ip + kp + 1; };
+  int sub = 0, sup = 0;
+  auto visit = [&](int ip, int kp, int i, int k) {
+    int iq = ip + i, kq = kp + k;
+    if (iq < 0) iq = -iq; else if (iq > nx) iq = 2 * nx - iq;
+    if (kq < 0) kq = -kq; else if (kq > nz) kq = 2 * nz - kq;
+    const int d = pf(iq, kq) - pf(ip, kp);
+    if (d < sub) sub = d;
+    else if (d > sup) sup = d;
+  };
+  for (int ip = 0; ip <= nx; ip++)
+    for (int kp = 0; kp <= nz; kp++) {
+      const bool corner = (ip == 0 || ip == nx) && (kp == 0 || kp == nz);
+      if (corner || (Vbc == 1 && (kp == 0 || kp == nz))) continue;
+      for (int i = -3; i <= 3; i += 6) visit(ip, kp, i, 0);
+      for (int k = -3; k <= 3; k += 6) visit(ip, kp, 0, k);
+      for (int i = -2; i <= 2; i++)
+        for (int k = -2; k <= 2; k++) visit(ip, kp, i, k);
+    }
+  t.dn = (nx + 1) * (nz + 1);
+  t.dkl = -sub;
+  t.dku = sup;
+}
+
 int build_tables(const pt_params& P, Tables& t) {
   std::memset(&t, 0, sizeof t);
   t.nx = P.nx; t.nz = P.nz; t.grids = P.grids; t.aspect = P.aspect;
@@ -327,7 +357,10 @@ int build_tables(const pt_params& P, Tables& t) {
   t.delta_C = 2.0;
   if (P.conduct_factor != 1.0) t.delta_C = std::fabs(std::log(P.delta_visC) / std::log(1.0 / P.conduct_factor));
   if (P.ivis >= 2 && P.visC != 1.0) t.delta_C = std::fabs(std::log(P.delta_visC) / std::log(P.visC));
-  optimize_grid(t.nx, t.nz, t.grids, P.mingrid);
+  // the direct path (iterate=0) keeps nx, nz (PS/main.f90:170); two levels are
+  // kept for the shared level-1 buffers
+  if (P.iterate != 0) optimize_grid(t.nx, t.nz, t.grids, P.mingrid);
+  else t.grids = 2;
   t.nsteps = cycle_schedule(t.grids, t.multi);
   if (t.nsteps < 0) return PT_ERR_UNSUPPORTED;
   // compute_basics: PS/basics.f90:204-300
@@ -350,6 +383,7 @@ int build_tables(const pt_params& P, Tables& t) {
     t.dr2dsg[j] = t.dr2g[j] * t.dsg[j]; t.dr2ds2g[j] = t.dr2g[j] * t.ds2g[j];
   }
   coarse_bandwidth(t);
+  if (P.iterate == 0) direct_bandwidth(t, P.Vbc, P.Database);
   rkc1_tables(t, P.nstageS, P.dampingS);
   fd_tables(t);
   eigen_bases(t);

@@ -1623,6 +1623,11 @@ static int compute_viscosity(Context& c, int l) {
   return smoother_vis(c, l, L.v, 0);
 }
 
+int launch_equilibrate(Context& c, double* ab, double* row, int n, int kl, int ku) {
+  hipLaunchKernelGGL(k_coarse_equilibrate, dim3((n + 127) / 128), dim3(128), 0, c.stream, ab, row, n, kl, ku);
+  return PT_OK;
+}
+
 static int coarse_factor(Context& c) {
   const Tables& t = c.T;
   Level& G = c.L[t.grids];
@@ -1948,16 +1953,19 @@ int solver_alloc(Context& c) {
     }
   }
   if (c.P.precision == PT_PREC_MIXED) DALLOC(c.dres64, sizeof(double) * c.L[1].nelem);
-  size_t mb = sizeof(double) * (size_t)(2 * t.kl + t.ku + 1) * t.ngc;
-  if (t.ngc > 4096) {
-    set_error("coarsest grid has more than 4096 unknowns");
-    return PT_ERR_UNSUPPORTED;
+  DALLOC(c.d_info, sizeof(int32_t));
+  if (c.P.iterate == 1) {   // the coarsest-grid operator of the multigrid path
+    size_t mb = sizeof(double) * (size_t)(2 * t.kl + t.ku + 1) * t.ngc;
+    if (t.ngc > 4096) {
+      set_error("coarsest grid has more than 4096 unknowns");
+      return PT_ERR_UNSUPPORTED;
+    }
+    DALLOC(c.Mc, mb); DALLOC(c.RowB, sizeof(double) * t.ngc); DALLOC(c.Bc, sizeof(double) * t.ngc);
+    c.nci = (t.nxg[t.grids] - 1) * (t.nzg[t.grids] - 1);
+    DALLOC(c.Gc, sizeof(double) * (size_t)c.nci * c.nci);
+    DALLOC(c.Gwork, sizeof(double) * (size_t)c.nci * t.ngc);
+    DALLOC(c.ipiv, sizeof(int32_t) * t.ngc);
   }
-  DALLOC(c.Mc, mb); DALLOC(c.RowB, sizeof(double) * t.ngc); DALLOC(c.Bc, sizeof(double) * t.ngc);
-  c.nci = (t.nxg[t.grids] - 1) * (t.nzg[t.grids] - 1);
-  DALLOC(c.Gc, sizeof(double) * (size_t)c.nci * c.nci);
-  DALLOC(c.Gwork, sizeof(double) * (size_t)c.nci * t.ngc);
-  DALLOC(c.ipiv, sizeof(int32_t) * t.ngc); DALLOC(c.d_info, sizeof(int32_t));
   c.red_cap = 1 << 20;
   DALLOC(c.red, sizeof(double) * c.red_cap);
   DALLOC(c.red_out, sizeof(double) * 16);

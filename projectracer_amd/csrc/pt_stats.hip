@@ -110,6 +110,8 @@ __global__ void __launch_bounds__(ST_THREADS) k_stats_rows(const double* Cnew, c
 struct EdgeArgs {
   double D1[5], D2[5], Drs[5][5];   // offsets -2..2; Drs[b+2][a+2] = D1(a) D1(b)
   double dr, ds, dr2, ds2, drds;
+  const double *u, *w;              // velocities (halo 3)
+  int from_velocity;                // 1: the direct path's strain from u, w
 };
 
 // compute_strain_rate_invariant_direct (PS/momentum.f90:73-93, uniform grid)
@@ -123,6 +125,22 @@ __device__ __forceinline__ double strain_at(const double* SF, int i, int k, int 
   const double Sxx = sxx / a.dr2, Szz = szz / a.ds2, Sxz = sxz / a.drds;
   const double d = Szz - Sxx;
   return sqrt(2.0 * (Sxz * Sxz) + 0.5 * (d * d));
+}
+
+// compute_strain_rate_invariant (PS/momentum.f90:1-23, uniform grid), the
+// direct path's strain: from the velocities' first derivatives
+__device__ __forceinline__ double strain_vel_at(int i, int k, int p3, const EdgeArgs& a) {
+  double ur = 0.0, us = 0.0, wr = 0.0, ws = 0.0;
+  for (int j = -2; j <= 2; j++) ur = ur + a.D1[j + 2] * a.u[S3(i + j, k, p3)];
+  for (int j = -2; j <= 2; j++) us = us + a.D1[j + 2] * a.u[S3(i, k + j, p3)];
+  for (int j = -2; j <= 2; j++) wr = wr + a.D1[j + 2] * a.w[S3(i + j, k, p3)];
+  for (int j = -2; j <= 2; j++) ws = ws + a.D1[j + 2] * a.w[S3(i, k + j, p3)];
+  const double ux = (ur / a.dr) / 1.0, uz = (us / a.ds) / 1.0, wx = (wr / a.dr) / 1.0, wz = (ws / a.ds) / 1.0;
+  const double e = uz + wx;
+  return sqrt(0.5 * (2.0 * (ux * ux + wz * wz) + e * e));
+}
+__device__ __forceinline__ double strain_node(const double* SF, int i, int k, int p3, const EdgeArgs& a) {
+  return a.from_velocity ? strain_vel_at(i, k, p3, a) : strain_at(SF, i, k, p3, a);
 }
 
 // D1 . T(i, k-2..k+2) / ds / zs(k), zs = 1 on a uniform grid
@@ -149,7 +167,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_stats_edges(const double* T, con
     bot = bot + 0.5 * (fb0 * 1.0 + fb1 * 1.0) * a.dr;
     tb = tb + 0.5 * (T[S3(i, 0, p3)] * 1.0 + T[S3(i + 1, 0, p3)] * 1.0) * a.dr;
   }
-  for (int i = threadIdx.x; i <= nx; i += ST_THREADS) smax = fmax(smax, strain_at(SF, i, nz, p3, a));
+  for (int i = threadIdx.x; i <= nx; i += ST_THREADS) smax = fmax(smax, strain_node(SF, i, nz, p3, a));
   block_sum(sh, top);
   if (threadIdx.x == 0) out[0] = sh[0];
   __syncthreads();
@@ -202,7 +220,7 @@ __global__ void k_stats_final(const double* rows, int nz, int kent, double ds, d
 __global__ void k_strain_field(const double* SF, int nx, int nz, EdgeArgs a, double* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
   if (i > nx || k > nz) return;
-  out[(long)k * (nx + 1) + i] = strain_at(SF, i, k, nx + 7, a);
+  out[(long)k * (nx + 1) + i] = strain_node(SF, i, k, nx + 7, a);
 }
 
 static EdgeArgs edge_args(Context& c) {
@@ -211,6 +229,8 @@ static EdgeArgs edge_args(Context& c) {
   for (int b = 0; b < 5; b++)
     for (int q = 0; q < 5; q++) a.Drs[b][q] = c.T.Drs[b + 1][q + 1];
   a.dr = c.T.dr; a.ds = c.T.ds; a.dr2 = c.T.dr2; a.ds2 = c.T.ds2; a.drds = c.L[1].g.drds;
+  a.u = c.u; a.w = c.w;
+  a.from_velocity = c.P.iterate == 0;   // stats: PS/basics.f90:371-376
   return a;
 }
 

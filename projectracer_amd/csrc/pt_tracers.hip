@@ -766,7 +766,7 @@ static int stability_rk4(Context& c) {
 }
 
 static int update_velocities(Context& c) {
-  PT_TRY(multigrid(c));
+  PT_TRY(solve_stream(c));
   return compute_velocities(c);
 }
 
@@ -919,7 +919,7 @@ int restart_state(Context& c, const double* T3, const double* C5, const double* 
   PT_TRY(place_tracers(c, rtr, str, Ttr, Ctr));
   PT_HIP(hipMemcpyAsync(c.T0, c.Tratio, b3, hipMemcpyDeviceToDevice, c.stream));
   PT_HIP(hipMemcpyAsync(c.SF, SF3, b3, hipMemcpyHostToDevice, c.stream));
-  PT_TRY(stream_bcs(c, c.SF));
+  if (c.P.iterate == 1) PT_TRY(stream_bcs(c, c.SF));   // PS/grid.f90:68
   PT_TRY(clear_coarse_errors(c));
   PT_HIP(hipStreamSynchronize(c.stream));
   return PT_OK;

@@ -46,6 +46,7 @@ module projectracer_gpu
   real(c_double) :: visP, visT, delta_visT, eig_ratio_min, visC, delta_visC
   integer(c_int32_t) :: precision, guess
   real(c_double) :: dent
+  integer(c_int32_t) :: iterate, Database
  end type pt_params
 
  !! stats.dat / viscosity_smoother.dat values (PS/basics.f90:363-462)
@@ -106,6 +107,14 @@ module projectracer_gpu
    type(c_ptr), value :: ctx
   end function
   integer(c_int) function pt_update_velocities(ctx) bind(C, name="pt_update_velocities")
+   import :: c_int, c_ptr
+   type(c_ptr), value :: ctx
+  end function
+  integer(c_int) function pt_build_sf_matrix(ctx) bind(C, name="pt_build_sf_matrix")
+   import :: c_int, c_ptr
+   type(c_ptr), value :: ctx
+  end function
+  integer(c_int) function pt_solve_sf_equation(ctx) bind(C, name="pt_solve_sf_equation")
    import :: c_int, c_ptr
    type(c_ptr), value :: ctx
   end function

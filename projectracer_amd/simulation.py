@@ -75,6 +75,14 @@ class Simulation:
     def update_velocities(self):
         self._check(self.lib.pt_update_velocities(self.ctx), "update_velocities")
 
+    def build_sf_matrix(self):
+        """build_SF_matrix (PS/momentum.f90:321-488), iterate=0 contexts."""
+        self._check(self.lib.pt_build_sf_matrix(self.ctx), "build_sf_matrix")
+
+    def solve_sf_equation(self):
+        """solve_SF_equation (PS/momentum.f90:621-653), iterate=0 contexts."""
+        self._check(self.lib.pt_solve_sf_equation(self.ctx), "solve_sf_equation")
+
     def energy_time(self):
         self._check(self.lib.pt_energy_time(self.ctx), "energy_time")
 

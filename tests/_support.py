@@ -43,6 +43,9 @@ def oracle_lib():
             "pto_multigrid": (C.c_int, [vp]),
             "pto_compute_velocities": (None, [vp]),
             "pto_update_velocities": (C.c_int, [vp]),
+            "pto_solve_direct": (C.c_int, [vp]),
+            "pto_direct_matrix": (C.c_int, [vp, dp, dp, ip, ip]),
+            "pto_direct_dims": (None, [vp, ip, ip, ip]),
             "pto_energy_time": (C.c_int, [vp]),
             "pto_tracers_to_corners": (C.c_int, [vp]),
             "pto_time_step": (C.c_int, [vp]),
@@ -115,6 +118,28 @@ class OracleSim:
     def update_velocities(self):
         self._chk(self.lib.pto_update_velocities(self.ctx), "update_velocities")
 
+    def solve_direct(self):
+        """The direct path's solve (iterate=0): build_SF_matrix on the first call,
+        then solve_SF_equation."""
+        self._chk(self.lib.pto_solve_direct(self.ctx), "solve_direct")
+
+    def direct_dims(self):
+        n, kl, ku = C.c_int32(), C.c_int32(), C.c_int32()
+        self.lib.pto_direct_dims(self.ctx, C.byref(n), C.byref(kl), C.byref(ku))
+        return n.value, kl.value, ku.value
+
+    def direct_matrix(self, equilibrate=False):
+        """The assembled band matrix (LDAB = 2 kl + ku + 1 rows, column-major
+        as numpy [column, row]) and, with equilibrate, the DGBEQU row factors
+        it was scaled by."""
+        n, kl, ku = self.direct_dims()
+        ab = np.zeros((n, 2 * kl + ku + 1))
+        row = np.zeros(n) if equilibrate else None
+        klo, kuo = C.c_int32(), C.c_int32()
+        self._chk(self.lib.pto_direct_matrix(self.ctx, _dp(ab), _dp(row) if equilibrate else None,
+                                             C.byref(klo), C.byref(kuo)), "direct_matrix")
+        return ab, row, kl, ku
+
     def energy_time(self):
         self._chk(self.lib.pto_energy_time(self.ctx), "energy_time")
 
@@ -176,9 +201,9 @@ def write_inputs_nml(path: Path, p: _abi.PtParams):
     d = _abi.params_dict(p)
     f = _nml_value
     txt = f"""&inputs
-aspect={f(d['aspect'])}, nx={d['nx']}, nz={d['nz']}, Database=2, order=4, gridX=0, gridZ=0,
+aspect={f(d['aspect'])}, nx={d['nx']}, nz={d['nz']}, Database={d['Database']}, order=4, gridX=0, gridZ=0,
 thickness=0.1d0, nbound=8, spaceX=1.d0, spaceZ=1.d0,
-iterate=1, MG=1, non_Newtonian=0, tolerance={f(d['tolerance'])}, Nmulti={d['Nmulti']},
+iterate={d['iterate']}, MG=1, non_Newtonian=0, tolerance={f(d['tolerance'])}, Nmulti={d['Nmulti']},
 mingrid={d['mingrid']}, grids={d['grids']}, eig_ratio={f(d['eig_ratio'])}, vis_iter={d['vis_iter']},
 max_cycles={d['max_cycles']}, tol_coarse=1.d-3, fp_max=100, splitting=0, RKC1=1,
 nstageS={d['nstageS']}, dampingS={f(d['dampingS'])}, c_2={f(d['c_2'])}, c_1={f(d['c_1'])},

@@ -29,6 +29,12 @@ Two sources:
                            psi, u, w, T at the nodes, per-node tracer counts,
                            every tracer's cell, a tracer subsample, q of every
                            solve
+      ref_direct64.npz  -- the direct solver (iterate=0, Database=2) on the
+                           64x64 isoviscous state, tpc=4: states after the
+                           solves of steps 0, 1 and 2
+      ref_direct_db1.npz -- the direct solver with Database=1 on a 48x32
+                           thermochemical state (Ra_C=5e4, isoviscous), tpc=4:
+                           states after the solves of steps 0 and 1
 Usage: python tests/golden/make_golden.py
 """
 from __future__ import annotations
@@ -168,6 +174,31 @@ def reference_runs():
     np.savez_compressed(HERE / "ref_cloud.npz", **out)
     print("wrote ref_cloud.npz")
     reference_config1(tmp)
+    reference_direct(tmp)
+
+
+def reference_direct(tmp):
+    from projectracer_amd import cases
+    from _support import run_reference
+    iso = cases.isoviscous(nx=64, nz=64, tpc=4, iterate=0, Database=2)
+    states, _, _, _ = run_reference(iso, nrun=2, dump_every=1, threads=1, workdir=tmp / "direct64")
+    out = {}
+    for step in (0, 1, 2):
+        for n in ("SF", "U", "W", "T"):
+            out[f"s{step}_{n}"] = _node(states[step], n)
+        out[f"s{step}_nempty"] = states[step]["nempty"].astype(np.int32)
+        out[f"s{step}_dt"] = np.float64(states[step]["dt"])
+    np.savez_compressed(HERE / "ref_direct64.npz", **out)
+    print("wrote ref_direct64.npz")
+    th = cases.thermochem(nx=48, nz=32, tpc=4, aspect=1.5, RaT=1e5, RaC=5e4, ivis=0, iterate=0, Database=1)
+    states, _, _, _ = run_reference(th, nrun=1, dump_every=1, threads=1, workdir=tmp / "direct_db1")
+    out = {}
+    for step in (0, 1):
+        for n in ("SF", "U", "W", "T"):
+            out[f"s{step}_{n}"] = _node(states[step], n)
+        out[f"s{step}_nempty"] = states[step]["nempty"].astype(np.int32)
+    np.savez_compressed(HERE / "ref_direct_db1.npz", **out)
+    print("wrote ref_direct_db1.npz")
 
 
 def reference_config1(tmp, tolerance=1e-8, name="ref_config1_100.npz"):
@@ -197,6 +228,9 @@ def reference_config1(tmp, tolerance=1e-8, name="ref_config1_100.npz"):
 
 
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "direct":
+        reference_direct(Path(tempfile.mkdtemp(prefix="golden_")))
+        raise SystemExit(0)
     if REF_DIR.exists():
         shipped_snapshot()
         shipped_restart()
