@@ -235,97 +235,187 @@ __global__ void __launch_bounds__(LU_THREADS) k_band_lu_wg(double* ab, int32_t* 
 }
 
 // ---------------------------------------------------------------------------
-// DGBTRS('N', one right-hand side) with one workgroup.  The band window of b
-// lives in an LDS ring (slot = (position-1) mod RING), refilled from global
-// memory in chunks; finished positions are written back in place.  One
-// barrier per column step:
+// DGBTRS('N', one right-hand side): the forward and the backward substitution
+// each run as one workgroup sized to its band (one band row per thread up to
+// 1024 threads), so no wave of the block idles through the steps.  The band
+// window of b (and, forward, the pivot indices) lives in LDS rings (slot =
+// (position-1) mod RING), refilled from global memory in chunks; finished
+// positions are written back in place.  The band columns stream from HBM
+// through a register queue DEPTH steps ahead of the step that uses them, with
+// unconditional loads from clamped addresses so that no branch separates the
+// queue slots, and every LDS read of a step is issued before any is waited
+// on.  One barrier per column step:
 //   forward (L with the interchanges): the pivot swap of step j is read from
 //     two broadcast slots filled during step j-1 (the values at positions j
 //     and ipiv(j) after that step), so no thread reads a ring entry that
 //     another thread writes in the same phase;
 //   backward (DTBSV, upper, bandwidth kl+ku): x(j) = b(j)/U(j,j) unless b(j)
 //     is zero, then b(j-1 .. j-kl-ku) -= x(j) U(., j).
+// Thread t owns the band rows t, t+blockDim.x, ... (E of them).
 // ---------------------------------------------------------------------------
-constexpr int SOLVE_THREADS = 512;
+constexpr int SOLVE_MAX_THREADS = 1024;
 constexpr int RING = 4096;
 constexpr int CHUNK = 512;
+constexpr int DEPTH = 8;
 
-__global__ void __launch_bounds__(SOLVE_THREADS) k_band_solve(const double* ab, const int32_t* ipiv, double* b,
-                                                              int n, int kl, int ku) {
+// Workgroup barrier that orders LDS only: the per-step handoffs of the solve
+// go through LDS, and a full __syncthreads() would also wait for the band
+// columns still in flight from HBM.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+__device__ __forceinline__ int ring_slot(int pos) { return (pos - 1) & (RING - 1); }
+
+// forward: L y = P b, in place on b
+template <int E>
+__global__ void __launch_bounds__(SOLVE_MAX_THREADS) k_band_forward(const double* __restrict__ ab,
+                                                                    const int32_t* __restrict__ ipiv, double* b,
+                                                                    int n, int kl, int ku) {
   __shared__ double R[RING];
+  __shared__ int32_t IP[RING];
   __shared__ double bx[2], by[2];
-  const int tid = threadIdx.x;
-  const int ld = 2 * kl + ku + 1, kd = ku + kl + 1;
-  auto slot = [](int pos) { return (pos - 1) & (RING - 1); };
-#define AB(r, q) ab[(long)((q) - 1) * ld + ((r) - 1)]
-  // ---- forward: L y = P b ----
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const long ld = 2 * kl + ku + 1;
+  const int kd = ku + kl + 1;
+  if (kl == 0 || n < 2) return;
   int hi = min(n, kl + CHUNK);
-  for (int pos = 1 + tid; pos <= hi; pos += SOLVE_THREADS) R[slot(pos)] = b[pos - 1];
+  for (int pos = 1 + tid; pos <= hi; pos += NT) {
+    R[ring_slot(pos)] = b[pos - 1];
+    IP[ring_slot(pos)] = ipiv[pos - 1];
+  }
   __syncthreads();
-  if (kl > 0 && n > 1) {
-    if (tid == 0) {
-      bx[1] = R[slot(ipiv[0])];
-      by[1] = R[slot(1)];
-    }
-    __syncthreads();
-    for (int j = 1; j <= n - 1; j++) {
+  // L(t, j) = AB(kd+1+t, j), t = 0..lm-1 (0-based rows kd+t of column j);
+  // entries past the band or the last column are loaded but never used
+  double q[DEPTH][E];
+  auto fetch_l = [&](double* dst, int j) {
+    const long col = (long)(min(j, n - 1) - 1) * ld + kd;
+#pragma unroll
+    for (int e = 0; e < E; e++) dst[e] = ab[col + min(tid + e * NT, kl - 1)];
+  };
+#pragma unroll
+  for (int u = 0; u < DEPTH; u++) fetch_l(q[u], 1 + u);
+  if (tid == 0) {
+    bx[1] = R[ring_slot(IP[ring_slot(1)])];
+    by[1] = R[ring_slot(1)];
+  }
+  __syncthreads();
+  for (int j0 = 1; j0 <= n - 1; j0 += DEPTH) {
+    int pv[DEPTH + 1];   // pivot rows of this block's steps (and the next one's first)
+#pragma unroll
+    for (int u = 0; u < DEPTH; u++) {
+      const int j = j0 + u;
+      if (j > n - 1) break;
       if (j + kl + 1 > hi && hi < n) {
         // refill (up to j+kl+1, which the next pivot may name): positions
         // hi+1 .. hi+CHUNK reuse the slots of finished positions
         const int top = min(n, hi + CHUNK);
-        for (int pos = hi + 1 + tid; pos <= top; pos += SOLVE_THREADS) R[slot(pos)] = b[pos - 1];
+        for (int pos = hi + 1 + tid; pos <= top; pos += NT) {
+          R[ring_slot(pos)] = b[pos - 1];
+          IP[ring_slot(pos)] = ipiv[pos - 1];
+        }
         hi = top;
-        __syncthreads();
+        lds_barrier();
       }
+      if (u == 0) {
+        // positions up to j0+kl+1 (kl >= DEPTH) are in the ring
+#pragma unroll
+        for (int v = 0; v <= DEPTH; v++) pv[v] = IP[ring_slot(min(j0 + v, n - 1))];
+      }
+      double Lv[E];
+#pragma unroll
+      for (int e = 0; e < E; e++) Lv[e] = q[u][e];
+      fetch_l(q[u], j + DEPTH);
       const int lm = min(kl, n - j);
-      const int l = ipiv[j - 1];
+      const int l = pv[u];
+      const int ln = (j + 1 <= n - 1) ? pv[u + 1] : 0;   // next step's pivot row
+      // the ring slots of positions past the update range are valid and never
+      // written in this phase, so all reads are issued at once
+      double rv[E];
+#pragma unroll
+      for (int e = 0; e < E; e++) rv[e] = R[ring_slot(j + 1 + tid + e * NT)];
       const double old_j = by[j & 1], old_l = bx[j & 1];
       const double bj = (l != j) ? old_l : old_j;   // b(j) after the interchange
       const double temp = -1.0 * bj;
       if (tid == 0) b[j - 1] = bj;
-      const int ln = (j + 1 <= n - 1) ? ipiv[j] : 0;   // next step's pivot row
-      for (int t = tid; t < lm; t += SOLVE_THREADS) {
-        const int pos = j + 1 + t;
-        double v = (pos == l && l != j) ? old_j : R[slot(pos)];
-        v = v + AB(kd + 1 + t, j) * temp;
-        R[slot(pos)] = v;
-        if (pos == j + 1) by[(j + 1) & 1] = v;
-        if (pos == ln) bx[(j + 1) & 1] = v;
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        const int t = tid + e * NT;
+        if (t < lm) {
+          const int pos = j + 1 + t;
+          double v = (pos == l && l != j) ? old_j : rv[e];
+          v = v + Lv[e] * temp;
+          R[ring_slot(pos)] = v;
+          if (pos == j + 1) by[(j + 1) & 1] = v;
+          if (pos == ln) bx[(j + 1) & 1] = v;
+        }
       }
       // the next pivot row lies outside this step's update range: its value is unchanged
-      if (tid == 0 && ln > j + lm) bx[(j + 1) & 1] = R[slot(ln)];
-      __syncthreads();
+      if (tid == 0 && ln > j + lm) bx[(j + 1) & 1] = R[ring_slot(ln)];
+      lds_barrier();
     }
-    if (tid == 0) b[n - 1] = by[n & 1];
   }
-  __syncthreads();
-  // ---- backward: U x = y ----
-  const int kk = kl + ku, kp1 = kk + 1;
+  if (tid == 0) b[n - 1] = by[n & 1];
+}
+
+// backward: U x = y (DTBSV, upper, non-unit, bandwidth kl+ku), in place on b
+template <int E>
+__global__ void __launch_bounds__(SOLVE_MAX_THREADS) k_band_backward(const double* __restrict__ ab, double* b,
+                                                                     int n, int kl, int ku) {
+  __shared__ double R[RING];
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const long ld = 2 * kl + ku + 1;
+  const int kk = kl + ku;
   int lo = max(1, n - kk - CHUNK + 1);
-  for (int pos = lo + tid; pos <= n; pos += SOLVE_THREADS) R[slot(pos)] = b[pos - 1];
+  for (int pos = lo + tid; pos <= n; pos += NT) R[ring_slot(pos)] = b[pos - 1];
+  // U(j-1-t, j) = AB(kk-t, j) (0-based row kk-1-t), diagonal AB(kk+1, j)
+  double q[DEPTH][E], qd[DEPTH];
+  auto fetch_u = [&](double* dst, double& dg, int j) {
+    const long col = (long)(max(j, 1) - 1) * ld;
+#pragma unroll
+    for (int e = 0; e < E; e++) dst[e] = ab[col + (kk - 1 - min(tid + e * NT, kk - 1))];
+    dg = ab[col + kk];
+  };
+#pragma unroll
+  for (int u = 0; u < DEPTH; u++) fetch_u(q[u], qd[u], n - u);
   __syncthreads();
-  for (int j = n; j >= 1; j--) {
-    if (j - kk < lo && lo > 1) {
-      const int bot = max(1, lo - CHUNK);
-      for (int pos = bot + tid; pos < lo; pos += SOLVE_THREADS) R[slot(pos)] = b[pos - 1];
-      lo = bot;
-      __syncthreads();
-    }
-    const double bj = R[slot(j)];
-    if (bj != 0.0) {
-      const double x = bj / AB(kp1, j);
-      if (tid == 0) b[j - 1] = x;
-      const int m = min(kk, j - 1), l = kp1 - j;
-      for (int t = tid; t < m; t += SOLVE_THREADS) {
-        const int i = j - 1 - t;
-        R[slot(i)] = R[slot(i)] - x * AB(l + i, j);
+  for (int j0 = n; j0 >= 1; j0 -= DEPTH) {
+#pragma unroll
+    for (int u = 0; u < DEPTH; u++) {
+      const int j = j0 - u;
+      if (j < 1) break;
+      if (j - kk < lo && lo > 1) {
+        const int bot = max(1, lo - CHUNK);
+        for (int pos = bot + tid; pos < lo; pos += NT) R[ring_slot(pos)] = b[pos - 1];
+        lo = bot;
+        lds_barrier();
       }
-    } else if (tid == 0) {
-      b[j - 1] = bj;
+      double Uv[E];
+#pragma unroll
+      for (int e = 0; e < E; e++) Uv[e] = q[u][e];
+      const double dg = qd[u];
+      fetch_u(q[u], qd[u], j - DEPTH);
+      double rv[E];   // issued together with the read of b(j)
+#pragma unroll
+      for (int e = 0; e < E; e++) rv[e] = R[ring_slot(j - 1 - tid - e * NT)];
+      const double bj = R[ring_slot(j)];
+      if (bj != 0.0) {
+        const double x = bj / dg;
+        if (tid == 0) b[j - 1] = x;
+        const int m = min(kk, j - 1);
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const int t = tid + e * NT;
+          if (t < m) R[ring_slot(j - 1 - t)] = rv[e] - x * Uv[e];
+        }
+      } else if (tid == 0) {
+        b[j - 1] = bj;
+      }
+      lds_barrier();
     }
-    __syncthreads();
   }
-#undef AB
 }
 
 // ---------------------------------------------------------------------------
@@ -391,6 +481,10 @@ static SfArgs sf_args(const Context& c) {
 
 int direct_alloc(Context& c) {
   const Tables& t = c.T;
+  if (t.dkl < DEPTH) {   // the solve reads a block's pivots from positions up to j0+kl+1
+    set_error("direct solver: lower band width below " + std::to_string(DEPTH));
+    return PT_ERR_UNSUPPORTED;
+  }
   if (t.dkl + t.dku + 1 > RING - CHUNK) {
     set_error("direct solver: band width " + std::to_string(t.dkl + t.dku) + " exceeds " +
               std::to_string(RING - CHUNK - 1));
@@ -454,8 +548,31 @@ int solve_sf_equation(Context& c) {
   const SfArgs a = sf_args(c);
   hipLaunchKernelGGL(k_sf_buoyancy, dim3((t.dn + 255) / 256), dim3(256), 0, c.stream, a, c.Tbuoy, c.Cbuoy, c.dRow,
                      c.P.RaT, c.P.RaC, c.dB);
-  hipLaunchKernelGGL(k_band_solve, dim3(1), dim3(SOLVE_THREADS), 0, c.stream, c.dAB, c.dipiv, c.dB, t.dn, t.dkl,
-                     t.dku);
+  // one workgroup per triangular solve, sized to its band: 64-thread waves,
+  // at most 1024 threads, E band rows per thread
+  auto shape = [](int band, int& nt, int& e) {
+    nt = std::min(SOLVE_MAX_THREADS, ((band + 63) / 64) * 64);
+    e = (band + nt - 1) / nt;
+  };
+  int ntf, ef, ntb, eb;
+  shape(t.dkl, ntf, ef);
+  shape(t.dkl + t.dku, ntb, eb);
+  if (ef > 4 || eb > 4) {
+    set_error("direct solver: band too wide for the solve kernels");
+    return PT_ERR_UNSUPPORTED;
+  }
+  auto fwd = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(1), dim3(ntf), 0, c.stream, c.dAB, c.dipiv, c.dB, t.dn, t.dkl, t.dku);
+  };
+  auto bwd = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(1), dim3(ntb), 0, c.stream, c.dAB, c.dB, t.dn, t.dkl, t.dku);
+  };
+  if (ef == 1) fwd(k_band_forward<1>);
+  else if (ef == 2) fwd(k_band_forward<2>);
+  else fwd(k_band_forward<4>);
+  if (eb == 1) bwd(k_band_backward<1>);
+  else if (eb == 2) bwd(k_band_backward<2>);
+  else bwd(k_band_backward<4>);
   dim3 b16(64, 4), g((t.nx + 7 + 63) / 64, (t.nz + 7 + 3) / 4);
   hipLaunchKernelGGL(k_sf_scatter, g, b16, 0, c.stream, a, c.dB, c.SF);
   c.solves++;
