@@ -33,9 +33,14 @@
  *    shipped T10000 snapshot (2-thread cloud projected and filtered).
  *  - build_grid coordinates (print_restart's xg, zg): the shipped Trestart
  *    record, bit for bit.
- *  - parity unpinned: the insulating-bottom (Tbot=1) and rigid top/bottom
- *    (Vbc=1) boundary branches and the ivis=1 viscosity law are restated
- *    but no fixture or closed form covers them.
+ *  - the insulating-bottom (Tbot=1) and rigid top/bottom (Vbc=1) boundary
+ *    branches, through multigrid and the direct solver, and the ivis=1
+ *    viscosity law: reference program runs on injected 48x32 states
+ *    (tests/golden/ref_variants.npz), cycle counts of every solve exact.
+ *  - direct solver (iterate=0, build_SF_matrix / solve_SF_equation): the
+ *    shipped stdout band widths and stats.dat row 1, reference program runs
+ *    with Database 1 and 2, the operator's symbol on sine modes, LAPACK's
+ *    dgbtrf pivots.
  */
 #include "pt_oracle.h"
 #include <math.h>

@@ -35,6 +35,12 @@ Two sources:
       ref_direct_db1.npz -- the direct solver with Database=1 on a 48x32
                            thermochemical state (Ra_C=5e4, isoviscous), tpc=4:
                            states after the solves of steps 0 and 1
+      ref_variants.npz  -- 48x32 (aspect 1.5), tpc=4, two steps, for the branches
+                           no shipped file covers: rigid top/bottom with an
+                           insulating bottom (Vbc=1, Tbot=1) through multigrid
+                           and through the direct solver, and the T- and
+                           depth-dependent viscosity law ivis=1; q of every
+                           solve, states after the solves of steps 0 and 2
 Usage: python tests/golden/make_golden.py
 """
 from __future__ import annotations
@@ -175,6 +181,7 @@ def reference_runs():
     print("wrote ref_cloud.npz")
     reference_config1(tmp)
     reference_direct(tmp)
+    reference_variants(tmp)
 
 
 def reference_direct(tmp):
@@ -199,6 +206,22 @@ def reference_direct(tmp):
         out[f"s{step}_nempty"] = states[step]["nempty"].astype(np.int32)
     np.savez_compressed(HERE / "ref_direct_db1.npz", **out)
     print("wrote ref_direct_db1.npz")
+
+
+def reference_variants(tmp):
+    from _support import run_reference, VARIANTS, variant_case
+    out = {}
+    for name in VARIANTS:
+        states, solves, _, _ = run_reference(variant_case(name), nrun=2, dump_every=1, threads=1,
+                                             workdir=tmp / f"variant_{name}")
+        for step in (0, 1, 2):
+            out[f"{name}_s{step}_nempty"] = states[step]["nempty"].astype(np.int32)
+        for step in (0, 2):
+            for n in ("SF", "U", "W", "T"):
+                out[f"{name}_s{step}_{n}"] = _node(states[step], n)
+        out[f"{name}_solves_tq"], _ = _solves(solves)
+    np.savez_compressed(HERE / "ref_variants.npz", **out)
+    print("wrote ref_variants.npz")
 
 
 def reference_config1(tmp, tolerance=1e-8, name="ref_config1_100.npz"):
@@ -230,6 +253,9 @@ def reference_config1(tmp, tolerance=1e-8, name="ref_config1_100.npz"):
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "direct":
         reference_direct(Path(tempfile.mkdtemp(prefix="golden_")))
+        raise SystemExit(0)
+    if len(sys.argv) > 1 and sys.argv[1] == "variants":
+        reference_variants(Path(tempfile.mkdtemp(prefix="golden_")))
         raise SystemExit(0)
     if REF_DIR.exists():
         shipped_snapshot()
