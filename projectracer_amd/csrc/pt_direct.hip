@@ -235,186 +235,283 @@ __global__ void __launch_bounds__(LU_THREADS) k_band_lu_wg(double* ab, int32_t* 
 }
 
 // ---------------------------------------------------------------------------
-// DGBTRS('N', one right-hand side): the forward and the backward substitution
-// each run as one workgroup sized to its band (one band row per thread up to
-// 1024 threads), so no wave of the block idles through the steps.  The band
-// window of b (and, forward, the pivot indices) lives in LDS rings (slot =
-// (position-1) mod RING), refilled from global memory in chunks; finished
-// positions are written back in place.  The band columns stream from HBM
-// through a register queue DEPTH steps ahead of the step that uses them, with
-// unconditional loads from clamped addresses so that no branch separates the
-// queue slots, and every LDS read of a step is issued before any is waited
-// on.  One barrier per column step:
-//   forward (L with the interchanges): the pivot swap of step j is read from
-//     two broadcast slots filled during step j-1 (the values at positions j
-//     and ipiv(j) after that step), so no thread reads a ring entry that
-//     another thread writes in the same phase;
+// DGBTRS('N', one right-hand side), blocked.  The forward and the backward
+// substitution each run as one workgroup; b's band window lives in an LDS
+// ring (slot = (position-1) mod RING) refilled from global memory in chunks,
+// and finished positions are written back in place.  The column steps are
+// taken NBH (one wavefront) at a time:
+//   head: one wave runs the block's steps on the block's NBH rows, one row
+//     per lane, with the step's value broadcast by v_readlane and the
+//     block's triangle of L (or U) staged in LDS: no barrier per step;
+//   tail: every thread then applies the block's steps to one of the kl (or
+//     kl+ku) rows below (above) the head, in step order.
+// Each element sees the same operations in the same order as DGBTRS's
+// column-by-column loops, so the results are identical.  Forward blocks
+// end early where a pivot row would leave the head (the block's first step
+// may pivot anywhere: its swap is made in the ring while every row is
+// current); the pivots of this operator are rare and adjacent.
+//   forward (L with the interchanges): b(j) <-> b(ipiv(j)), then
+//     b(j+1 .. j+kl) += L(., j) (-b(j));
 //   backward (DTBSV, upper, bandwidth kl+ku): x(j) = b(j)/U(j,j) unless b(j)
 //     is zero, then b(j-1 .. j-kl-ku) -= x(j) U(., j).
-// Thread t owns the band rows t, t+blockDim.x, ... (E of them).
 // ---------------------------------------------------------------------------
 constexpr int SOLVE_MAX_THREADS = 1024;
 constexpr int RING = 4096;
 constexpr int CHUNK = 512;
-constexpr int DEPTH = 8;
-
-// Workgroup barrier that orders LDS only: the per-step handoffs of the solve
-// go through LDS, and a full __syncthreads() would also wait for the band
-// columns still in flight from HBM.
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
+constexpr int NBH = 64;   // head rows per block: one wavefront
 
 __device__ __forceinline__ int ring_slot(int pos) { return (pos - 1) & (RING - 1); }
 
+// the value of v in lane `lane` (uniform), as a uniform double
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const long long x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(x & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(x >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
 // forward: L y = P b, in place on b
-template <int E>
+template <int E>   // tail rows per thread
 __global__ void __launch_bounds__(SOLVE_MAX_THREADS) k_band_forward(const double* __restrict__ ab,
                                                                     const int32_t* __restrict__ ipiv, double* b,
                                                                     int n, int kl, int ku) {
   __shared__ double R[RING];
-  __shared__ int32_t IP[RING];
-  __shared__ double bx[2], by[2];
-  const int tid = threadIdx.x, NT = blockDim.x;
+  __shared__ double Lh[NBH][NBH];   // Lh[s][r] = L(j0+r, j0+s)
+  __shared__ double T[NBH];         // -b(j) of the block's steps
+  const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63;
   const long ld = 2 * kl + ku + 1;
-  const int kd = ku + kl + 1;
+  const int kd = ku + kl + 1;   // L(j+t, j) = AB(kd+t, j), t = 1..kl (1-based)
   if (kl == 0 || n < 2) return;
-  int hi = min(n, kl + CHUNK);
-  for (int pos = 1 + tid; pos <= hi; pos += NT) {
-    R[ring_slot(pos)] = b[pos - 1];
-    IP[ring_slot(pos)] = ipiv[pos - 1];
-  }
+  auto Lg = [&](int r, int j) { return ab[(long)(j - 1) * ld + (kd - 1 + (r - j))]; };
+  int hi = min(n, NBH + kl + CHUNK);
+  for (int pos = 1 + tid; pos <= hi; pos += NT) R[ring_slot(pos)] = b[pos - 1];
   __syncthreads();
-  // L(t, j) = AB(kd+1+t, j), t = 0..lm-1 (0-based rows kd+t of column j);
-  // entries past the band or the last column are loaded but never used
-  double q[DEPTH][E];
-  auto fetch_l = [&](double* dst, int j) {
-    const long col = (long)(min(j, n - 1) - 1) * ld + kd;
-#pragma unroll
-    for (int e = 0; e < E; e++) dst[e] = ab[col + min(tid + e * NT, kl - 1)];
-  };
-#pragma unroll
-  for (int u = 0; u < DEPTH; u++) fetch_l(q[u], 1 + u);
-  if (tid == 0) {
-    bx[1] = R[ring_slot(IP[ring_slot(1)])];
-    by[1] = R[ring_slot(1)];
-  }
-  __syncthreads();
-  for (int j0 = 1; j0 <= n - 1; j0 += DEPTH) {
-    int pv[DEPTH + 1];   // pivot rows of this block's steps (and the next one's first)
-#pragma unroll
-    for (int u = 0; u < DEPTH; u++) {
-      const int j = j0 + u;
-      if (j > n - 1) break;
-      if (j + kl + 1 > hi && hi < n) {
-        // refill (up to j+kl+1, which the next pivot may name): positions
-        // hi+1 .. hi+CHUNK reuse the slots of finished positions
-        const int top = min(n, hi + CHUNK);
-        for (int pos = hi + 1 + tid; pos <= top; pos += NT) {
-          R[ring_slot(pos)] = b[pos - 1];
-          IP[ring_slot(pos)] = ipiv[pos - 1];
-        }
-        hi = top;
-        lds_barrier();
-      }
-      if (u == 0) {
-        // positions up to j0+kl+1 (kl >= DEPTH) are in the ring
-#pragma unroll
-        for (int v = 0; v <= DEPTH; v++) pv[v] = IP[ring_slot(min(j0 + v, n - 1))];
-      }
-      double Lv[E];
-#pragma unroll
-      for (int e = 0; e < E; e++) Lv[e] = q[u][e];
-      fetch_l(q[u], j + DEPTH);
-      const int lm = min(kl, n - j);
-      const int l = pv[u];
-      const int ln = (j + 1 <= n - 1) ? pv[u + 1] : 0;   // next step's pivot row
-      // the ring slots of positions past the update range are valid and never
-      // written in this phase, so all reads are issued at once
-      double rv[E];
-#pragma unroll
-      for (int e = 0; e < E; e++) rv[e] = R[ring_slot(j + 1 + tid + e * NT)];
-      const double old_j = by[j & 1], old_l = bx[j & 1];
-      const double bj = (l != j) ? old_l : old_j;   // b(j) after the interchange
-      const double temp = -1.0 * bj;
-      if (tid == 0) b[j - 1] = bj;
-#pragma unroll
-      for (int e = 0; e < E; e++) {
-        const int t = tid + e * NT;
-        if (t < lm) {
-          const int pos = j + 1 + t;
-          double v = (pos == l && l != j) ? old_j : rv[e];
-          v = v + Lv[e] * temp;
-          R[ring_slot(pos)] = v;
-          if (pos == j + 1) by[(j + 1) & 1] = v;
-          if (pos == ln) bx[(j + 1) & 1] = v;
-        }
-      }
-      // the next pivot row lies outside this step's update range: its value is unchanged
-      if (tid == 0 && ln > j + lm) bx[(j + 1) & 1] = R[ring_slot(ln)];
-      lds_barrier();
+  int j0 = 1;
+  while (j0 <= n - 1) {
+    if (j0 + NBH - 1 + kl > hi && hi < n) {
+      // rows up to j0+NBH-1+kl; the slots reused belong to rows below j0
+      const int top = min(n, j0 + NBH - 1 + kl + CHUNK);
+      for (int pos = hi + 1 + tid; pos <= top; pos += NT) R[ring_slot(pos)] = b[pos - 1];
+      hi = top;
+      __syncthreads();
     }
+    // the block: steps j0 .. j1, the pivots of steps j0+1 .. j1 inside the head
+    const int jp = j0 + lane;
+    const bool out = lane >= 1 && jp <= n - 1 && ipiv[jp - 1] > j0 + NBH - 1;
+    const unsigned long long m = __ballot(out);
+    const int f = m ? __ffsll((long long)m) - 1 : NBH;
+    const int j1 = min(min(j0 + f - 1, j0 + NBH - 1), n - 1);
+    const int nb = j1 - j0 + 1;
+    if (tid == 0) {
+      const int l0 = ipiv[j0 - 1];
+      if (l0 != j0) {
+        const double t0 = R[ring_slot(j0)];
+        R[ring_slot(j0)] = R[ring_slot(l0)];
+        R[ring_slot(l0)] = t0;
+      }
+    }
+    {
+      // wave w stages columns w, w+nw, ...: up to 16 loads in flight per lane,
+      // from clamped (valid) band entries, then the masked stores
+      const int nw = NT >> 6, wv = tid >> 6;
+      for (int c0 = 0; c0 < NBH; c0 += 16 * nw) {
+        double tmp[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+          const int j = min(j0 + min(c0 + q * nw + wv, nb - 1), n - 1);
+          tmp[q] = Lg(min(max(j0 + lane, j + 1), j + kl), j);
+        }
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+          const int sc = c0 + q * nw + wv, j = j0 + sc, row = j0 + lane;
+          if (sc < NBH) Lh[sc][lane] = (sc < nb && lane > sc && row <= min(n, j + kl)) ? tmp[q] : 0.0;
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < 64) {
+      double v = (j0 + lane <= n) ? R[ring_slot(j0 + lane)] : 0.0;
+      // the block's pivot rows, one per lane, and each step's multiplier kept
+      // in the step's lane: no memory operation inside the step loop
+      const int pl = ipiv[min(j0 + lane, n - 1) - 1];
+      double tv = 0.0;
+      // a full block with no interchange after its first step and a band at
+      // least a head wide (the common case): each step is one lane read and
+      // one masked multiply-add
+      const bool swaps = __ballot(lane >= 1 && lane < nb && pl != j0 + lane) != 0;
+      const bool fast = nb == NBH && !swaps && kl >= NBH && j0 + NBH - 1 <= n;
+      if (fast) {
+#pragma unroll
+        for (int half = 0; half < 2; half++) {
+          double lc[NBH / 2];
+#pragma unroll
+          for (int q = 0; q < NBH / 2; q++) lc[q] = Lh[half * (NBH / 2) + q][lane];
+#pragma unroll
+          for (int q = 0; q < NBH / 2; q++) {
+            const int sc = half * (NBH / 2) + q;
+            const double temp = -1.0 * readlane_f64(v, sc);
+            if (lane == sc) tv = temp;
+            if (lane > sc) v = v + lc[q] * temp;
+          }
+        }
+      }
+      // the steps in two halves, each with its column of the triangle in
+      // registers and constant lane indices
+#pragma unroll
+      for (int half = 0; half < 2 && !fast; half++) {
+        double lc[NBH / 2];
+#pragma unroll
+        for (int q = 0; q < NBH / 2; q++) lc[q] = Lh[half * (NBH / 2) + q][lane];
+#pragma unroll
+        for (int q = 0; q < NBH / 2; q++) {
+          const int sc = half * (NBH / 2) + q;
+          if (sc < nb) {
+            const int j = j0 + sc;
+            if (sc > 0) {
+              const int l = __builtin_amdgcn_readlane(pl, sc);
+              if (l != j) {
+                const double a = readlane_f64(v, sc), c = readlane_f64(v, l - j0);
+                if (lane == sc) v = c;
+                if (lane == l - j0) v = a;
+              }
+            }
+            const double temp = -1.0 * readlane_f64(v, sc);
+            if (lane == sc) tv = temp;
+            if (lane > sc && j0 + lane <= min(n, j + kl)) v = v + lc[q] * temp;
+          }
+        }
+      }
+      T[lane] = tv;
+      if (lane < nb) b[j0 + lane - 1] = v;                    // y(j0 .. j1): final
+      else if (j0 + lane <= n) R[ring_slot(j0 + lane)] = v;   // head rows of the next block
+    }
+    __syncthreads();
+    // tail rows j0+NBH .. j1+kl
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int t = tid + e * NT, r = j0 + NBH + t;
+      const bool act = t < kl && r <= min(n, j1 + kl);
+      double v = act ? R[ring_slot(r)] : 0.0;
+#pragma unroll
+      for (int half = 0; half < 2; half++) {
+        // unconditional loads from clamped (valid) band entries, all in flight at once
+        double lt[NBH / 2];
+#pragma unroll
+        for (int q = 0; q < NBH / 2; q++) {
+          const int j = min(j0 + half * (NBH / 2) + q, j1);
+          lt[q] = Lg(min(max(r, j + 1), j + kl), j);
+        }
+#pragma unroll
+        for (int q = 0; q < NBH / 2; q++) {
+          const int sc = half * (NBH / 2) + q, j = j0 + sc;
+          if (sc < nb && r <= j + kl) v = v + lt[q] * T[sc];
+        }
+      }
+      if (act) R[ring_slot(r)] = v;
+    }
+    __syncthreads();
+    j0 = j1 + 1;
   }
-  if (tid == 0) b[n - 1] = by[n & 1];
+  if (tid == 0) b[n - 1] = R[ring_slot(n)];
 }
 
 // backward: U x = y (DTBSV, upper, non-unit, bandwidth kl+ku), in place on b
-template <int E>
+template <int E>   // tail rows per thread
 __global__ void __launch_bounds__(SOLVE_MAX_THREADS) k_band_backward(const double* __restrict__ ab, double* b,
                                                                      int n, int kl, int ku) {
   __shared__ double R[RING];
-  const int tid = threadIdx.x, NT = blockDim.x;
+  __shared__ double Uh[NBH][NBH];   // Uh[s][r] = U(j0+r, j0+s), r < s
+  __shared__ double X[NBH];         // x(j) of the block's steps
+  __shared__ int F[NBH];            // step taken (b(j) != 0)
+  const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63;
   const long ld = 2 * kl + ku + 1;
-  const int kk = kl + ku;
-  int lo = max(1, n - kk - CHUNK + 1);
+  const int kk = kl + ku;   // U(i, j) = AB(kk+1+i-j, j)
+  auto Ug = [&](int i, int j) { return ab[(long)(j - 1) * ld + (kk + i - j)]; };
+  int lo = max(1, n - NBH - kk - CHUNK + 1);
   for (int pos = lo + tid; pos <= n; pos += NT) R[ring_slot(pos)] = b[pos - 1];
-  // U(j-1-t, j) = AB(kk-t, j) (0-based row kk-1-t), diagonal AB(kk+1, j)
-  double q[DEPTH][E], qd[DEPTH];
-  auto fetch_u = [&](double* dst, double& dg, int j) {
-    const long col = (long)(max(j, 1) - 1) * ld;
-#pragma unroll
-    for (int e = 0; e < E; e++) dst[e] = ab[col + (kk - 1 - min(tid + e * NT, kk - 1))];
-    dg = ab[col + kk];
-  };
-#pragma unroll
-  for (int u = 0; u < DEPTH; u++) fetch_u(q[u], qd[u], n - u);
   __syncthreads();
-  for (int j0 = n; j0 >= 1; j0 -= DEPTH) {
-#pragma unroll
-    for (int u = 0; u < DEPTH; u++) {
-      const int j = j0 - u;
-      if (j < 1) break;
-      if (j - kk < lo && lo > 1) {
-        const int bot = max(1, lo - CHUNK);
-        for (int pos = bot + tid; pos < lo; pos += NT) R[ring_slot(pos)] = b[pos - 1];
-        lo = bot;
-        lds_barrier();
-      }
-      double Uv[E];
-#pragma unroll
-      for (int e = 0; e < E; e++) Uv[e] = q[u][e];
-      const double dg = qd[u];
-      fetch_u(q[u], qd[u], j - DEPTH);
-      double rv[E];   // issued together with the read of b(j)
-#pragma unroll
-      for (int e = 0; e < E; e++) rv[e] = R[ring_slot(j - 1 - tid - e * NT)];
-      const double bj = R[ring_slot(j)];
-      if (bj != 0.0) {
-        const double x = bj / dg;
-        if (tid == 0) b[j - 1] = x;
-        const int m = min(kk, j - 1);
-#pragma unroll
-        for (int e = 0; e < E; e++) {
-          const int t = tid + e * NT;
-          if (t < m) R[ring_slot(j - 1 - t)] = rv[e] - x * Uv[e];
-        }
-      } else if (tid == 0) {
-        b[j - 1] = bj;
-      }
-      lds_barrier();
+  int j1 = n;
+  while (j1 >= 1) {
+    const int j0 = max(1, j1 - NBH + 1), nb = j1 - j0 + 1;
+    if (j0 - kk < lo && lo > 1) {
+      const int bot = max(1, j0 - kk - CHUNK);
+      for (int pos = bot + tid; pos < lo; pos += NT) R[ring_slot(pos)] = b[pos - 1];
+      lo = bot;
+      __syncthreads();
     }
+    {
+      // as in the forward pass; r == sc is the diagonal U(j,j)
+      const int nw = NT >> 6, wv = tid >> 6;
+      for (int c0 = 0; c0 < NBH; c0 += 16 * nw) {
+        double tmp[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+          const int j = j0 + min(c0 + q * nw + wv, nb - 1);
+          tmp[q] = Ug(max(min(j0 + lane, j), j - kk), j);
+        }
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+          const int sc = c0 + q * nw + wv, j = j0 + sc, row = j0 + lane;
+          if (sc < NBH) Uh[sc][lane] = (sc < nb && lane <= sc && row >= j - kk) ? tmp[q] : 0.0;
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < 64) {
+      double v = (lane < nb) ? R[ring_slot(j0 + lane)] : 0.0;
+      double xv = 0.0;   // x(j) and whether step j was taken, in the step's lane
+      int fv = 0;
+      // the band reaches every head row when kl+ku >= NBH
+      const bool wide = kk >= NBH;
+#pragma unroll
+      for (int half = 1; half >= 0; half--) {
+        double uc[NBH / 2];
+#pragma unroll
+        for (int q = 0; q < NBH / 2; q++) uc[q] = Uh[half * (NBH / 2) + q][lane];
+#pragma unroll
+        for (int q = NBH / 2 - 1; q >= 0; q--) {
+          const int sc = half * (NBH / 2) + q;
+          if (sc < nb) {
+            const int j = j0 + sc;
+            const double bj = readlane_f64(v, sc);
+            if (bj != 0.0) {
+              const double x = bj / readlane_f64(uc[q], sc);   // U(j,j)
+              if (lane == sc) v = x;
+              if (lane < sc && (wide || j0 + lane >= j - kk)) v = v - x * uc[q];
+              if (lane == sc) { xv = x; fv = 1; }
+            }
+          }
+        }
+      }
+      X[lane] = xv;
+      F[lane] = fv;
+      if (lane < nb) b[j0 + lane - 1] = v;   // x(j0 .. j1): final
+    }
+    __syncthreads();
+    // tail rows j0-1 down to j0-kk
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int t = tid + e * NT, i = j0 - 1 - t;
+      const bool act = t < kk && i >= 1;
+      double v = act ? R[ring_slot(i)] : 0.0;
+#pragma unroll
+      for (int half = 1; half >= 0; half--) {
+        double ut[NBH / 2];
+#pragma unroll
+        for (int q = 0; q < NBH / 2; q++) {
+          const int j = min(j0 + half * (NBH / 2) + q, j1);
+          ut[q] = Ug(min(max(i, j - kk), j - 1 < 1 ? j : j - 1), j);
+        }
+#pragma unroll
+        for (int q = NBH / 2 - 1; q >= 0; q--) {
+          const int sc = half * (NBH / 2) + q, j = j0 + sc;
+          if (sc < nb && F[sc] && i >= j - kk) v = v - X[sc] * ut[q];
+        }
+      }
+      if (act) R[ring_slot(i)] = v;
+    }
+    __syncthreads();
+    j1 = j0 - 1;
   }
 }
 
@@ -481,13 +578,9 @@ static SfArgs sf_args(const Context& c) {
 
 int direct_alloc(Context& c) {
   const Tables& t = c.T;
-  if (t.dkl < DEPTH) {   // the solve reads a block's pivots from positions up to j0+kl+1
-    set_error("direct solver: lower band width below " + std::to_string(DEPTH));
-    return PT_ERR_UNSUPPORTED;
-  }
-  if (t.dkl + t.dku + 1 > RING - CHUNK) {
+  if (t.dkl + t.dku + NBH + CHUNK >= RING) {   // the solves' LDS window
     set_error("direct solver: band width " + std::to_string(t.dkl + t.dku) + " exceeds " +
-              std::to_string(RING - CHUNK - 1));
+              std::to_string(RING - NBH - CHUNK - 1));
     return PT_ERR_UNSUPPORTED;
   }
   const size_t ld = (size_t)(2 * t.dkl + t.dku + 1);
