@@ -153,34 +153,51 @@ __global__ void k_sf_assemble(SfArgs a, const double* vis, double* ab) {
 }
 
 // ---------------------------------------------------------------------------
-// DGBTF2 with one workgroup: pivot search as a block-wide first-maximum
-// reduction, then the row swap, the column scaling and the rank-1 update of
-// the (ju-j) x km window spread over all threads
+// DGBTF2 (band LU with partial pivoting, LAPACK's unblocked column order) in
+// panels of nb columns.
+//   k_lu_panel: one workgroup factors columns p0 .. p0+nb-1 held in LDS
+//     (rows p0 .. p0+nb-1+kl): per column the first-maximum pivot search, the
+//     row interchange, the scaling and the rank-1 update, each restricted to
+//     the panel's columns.  It records ju after every step (the last column
+//     DGBTF2's step touches; -1 for a zero pivot, which DGBTF2 skips).
+//   k_lu_trailing: one wave per column beyond the panel replays the panel's
+//     steps on that column in order -- the interchange, then
+//     x = x + L(t,j) (-A(j,c)) -- while the column is within the step's ju.
+// Every element sees the operations of DGBTF2 in the same order, so the
+// factors are DGBTF2's bit for bit.  AB's fill rows start zero (memset) and
+// are first written by the step that DGBTF2 zeroes them at, so that zeroing
+// is not repeated.
 // ---------------------------------------------------------------------------
-constexpr int LU_THREADS = 1024;
+constexpr int LU_PANEL_THREADS = 256;
+constexpr int LU_LDS_LIMIT = 160 * 1024 - 1024;   // dynamic LDS per workgroup (gfx950: 160 KB)
+constexpr int LU_TRAIL_WAVES = 8;   // columns per trailing workgroup
 
-__global__ void __launch_bounds__(LU_THREADS) k_band_lu_wg(double* ab, int32_t* ipiv, int n, int kl, int ku,
-                                                           int32_t* info) {
-  __shared__ double s_best[LU_THREADS / 64];
-  __shared__ int s_idx[LU_THREADS / 64];
+__global__ void __launch_bounds__(LU_PANEL_THREADS) k_lu_panel(double* ab, int32_t* ipiv, int32_t* ju_rec,
+                                                               int32_t* state, int n, int kl, int ku, int p0,
+                                                               int nb) {
+  extern __shared__ double P[];   // P[(c-p0)*W + (r-p0)], W = kl + nb
+  __shared__ double s_best[LU_PANEL_THREADS / 64];
+  __shared__ int s_idx[LU_PANEL_THREADS / 64];
   __shared__ int s_jp;
-  const int ld = 2 * kl + ku + 1, kv = ku + kl;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-#define AB(r, q) ab[(long)((q) - 1) * ld + ((r) - 1)]
-  // zero the fill-in rows of the first columns (DGBTF2's set-up)
-  for (int j = ku + 2 + tid; j <= min(kv, n); j += LU_THREADS)
-    for (int i = kv - j + 2; i <= kl; i++) AB(i, j) = 0.0;
+  const long ld = 2 * kl + ku + 1;
+  const int kv = ku + kl, W = kl + nb, pe = min(p0 + nb - 1, n), np = pe - p0 + 1;
+  // A(r, c) = AB(kv+1+r-c, c): rows above c+kl are stored for c >= p0
+  for (int e = tid; e < np * W; e += LU_PANEL_THREADS) {
+    const int cc = e / W, r = e - cc * W, col = p0 + cc, row = p0 + r;
+    P[e] = (row <= min(n, col + kl)) ? ab[(long)(col - 1) * ld + (kv + row - col)] : 0.0;
+  }
+  // ju and the first zero pivot carried between panels
+  int ju = (p0 == 1) ? 1 : state[0];
+  int inf = (p0 == 1) ? 0 : state[1];
   __syncthreads();
-  int ju = 1, inf = 0;
-  for (int j = 1; j <= n; j++) {
-    if (j + kv <= n)
-      for (int i = 1 + tid; i <= kl; i += LU_THREADS) AB(i, j + kv) = 0.0;
+  auto A = [&](int r, int col) -> double& { return P[(col - p0) * W + (r - p0)]; };
+  for (int j = p0; j <= pe; j++) {
     const int km = min(kl, n - j);
-    // IDAMAX over AB(kv+1 .. kv+1+km, j): the first index of the largest |.|
     double best = -1.0;
     int bi = 0x7fffffff;
-    for (int t = tid; t <= km; t += LU_THREADS) {
-      const double x = fabs(AB(kv + 1 + t, j));
+    for (int t = tid; t <= km; t += LU_PANEL_THREADS) {
+      const double x = fabs(A(j + t, j));
       if (x > best) { best = x; bi = t; }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -193,45 +210,98 @@ __global__ void __launch_bounds__(LU_THREADS) k_band_lu_wg(double* ab, int32_t* 
     if (tid == 0) {
       double b = s_best[0];
       int x = s_idx[0];
-      for (int w = 1; w < LU_THREADS / 64; w++)
+      for (int w = 1; w < LU_PANEL_THREADS / 64; w++)
         if (s_best[w] > b || (s_best[w] == b && s_idx[w] < x)) { b = s_best[w]; x = s_idx[w]; }
       s_jp = x + 1;
       ipiv[j - 1] = x + j;
     }
     __syncthreads();
     const int jp = s_jp;
-    const double piv = AB(kv + jp, j);
-    if (piv != 0.0) {
+    if (A(j + jp - 1, j) != 0.0) {
       ju = max(ju, min(j + ku + jp - 1, n));
+      const int ce = min(ju, pe);
       if (jp != 1)
-        for (int t = tid; t <= ju - j; t += LU_THREADS) {
-          double* pa = &ab[(long)(j - 1 + t) * ld + (kv + jp - 1 - t)];
-          double* pb = &ab[(long)(j - 1 + t) * ld + (kv - t)];
-          const double tmp = *pa;
-          *pa = *pb;
-          *pb = tmp;
+        for (int col = j + tid; col <= ce; col += LU_PANEL_THREADS) {
+          const double tmp = A(j + jp - 1, col);
+          A(j + jp - 1, col) = A(j, col);
+          A(j, col) = tmp;
         }
       __syncthreads();
       if (km > 0) {
-        const double rcp = 1.0 / AB(kv + 1, j);
-        for (int t = 1 + tid; t <= km; t += LU_THREADS) AB(kv + 1 + t, j) = rcp * AB(kv + 1 + t, j);
+        const double rcp = 1.0 / A(j, j);
+        for (int t = 1 + tid; t <= km; t += LU_PANEL_THREADS) A(j + t, j) = rcp * A(j + t, j);
         __syncthreads();
-        // DGER: A(t, jj) += x(t) * (-y(jj)); the pivot row y is not updated
-        const int total = (ju - j) * km;
-        for (int idx = tid; idx < total; idx += LU_THREADS) {
+        const int total = (ce - j) * km;
+        for (int idx = tid; idx < total; idx += LU_PANEL_THREADS) {
           const int jj = 1 + idx / km, t = 1 + idx % km;
-          const double temp = -1.0 * ab[(long)(j - 1 + jj) * ld + (kv - jj)];
-          double& x = ab[(long)(j - 1 + jj) * ld + (kv - jj + t)];
-          x = x + AB(kv + 1 + t, j) * temp;
+          const double temp = -1.0 * A(j, j + jj);
+          double& x = A(j + t, j + jj);
+          x = x + A(j + t, j) * temp;
         }
       }
-    } else if (inf == 0) {
-      inf = j;
+      if (tid == 0) ju_rec[j - 1] = ju;
+    } else {
+      if (inf == 0) inf = j;
+      if (tid == 0) ju_rec[j - 1] = -1;
     }
     __syncthreads();
   }
-  if (tid == 0) *info = inf;
-#undef AB
+  for (int e = tid; e < np * W; e += LU_PANEL_THREADS) {
+    const int cc = e / W, r = e - cc * W, col = p0 + cc, row = p0 + r;
+    if (row <= min(n, col + kl)) ab[(long)(col - 1) * ld + (kv + row - col)] = P[e];
+  }
+  if (tid == 0) { state[0] = ju; state[1] = inf; }
+}
+
+// wave-level ordering of LDS accesses between lanes of one wave
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void __launch_bounds__(LU_TRAIL_WAVES * 64) k_lu_trailing(double* ab, const int32_t* ipiv,
+                                                                     const int32_t* ju_rec, int n, int kl, int ku,
+                                                                     int p0, int nb) {
+  extern __shared__ double sh[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long ld = 2 * kl + ku + 1;
+  const int kv = ku + kl, W = kl + nb, pe = min(p0 + nb - 1, n), np = pe - p0 + 1;
+  double* Ls = sh;                                   // Ls[s*kl + t-1] = L(p0+s+t, p0+s)
+  double* Cb = sh + (size_t)nb * kl + (size_t)wave * W;   // this wave's column, rows p0 .. p0+W-1
+  for (int e = tid; e < np * kl; e += LU_TRAIL_WAVES * 64) {
+    const int sc = e / kl, t = 1 + (e - sc * kl), j = p0 + sc;
+    Ls[e] = (t <= min(kl, n - j)) ? ab[(long)(j - 1) * ld + (kv + t)] : 0.0;
+  }
+  __syncthreads();
+  const int c = pe + 1 + blockIdx.x * LU_TRAIL_WAVES + wave;
+  if (c > min(n, pe + kv)) return;
+  for (int r = lane; r < W; r += 64) {
+    const int row = p0 + r;
+    Cb[r] = (row >= c - kv && row <= min(n, c + kl)) ? ab[(long)(c - 1) * ld + (kv + row - c)] : 0.0;
+  }
+  wave_lds_sync();
+  for (int sc = 0; sc < np; sc++) {
+    const int j = p0 + sc, jut = ju_rec[j - 1];
+    if (jut < 0 || c > jut) continue;
+    const int l = ipiv[j - 1];
+    if (l != j) {
+      if (lane == 0) {
+        const double tmp = Cb[l - p0];
+        Cb[l - p0] = Cb[sc];
+        Cb[sc] = tmp;
+      }
+      wave_lds_sync();
+    }
+    const double temp = -1.0 * Cb[sc];
+    const int km = min(kl, n - j);
+    for (int t = 1 + lane; t <= km; t += 64) Cb[sc + t] = Cb[sc + t] + Ls[sc * kl + t - 1] * temp;
+    wave_lds_sync();
+  }
+  for (int r = lane; r < W; r += 64) {
+    const int row = p0 + r;
+    if (row >= c - kv && row <= min(n, c + kl)) ab[(long)(c - 1) * ld + (kv + row - c)] = Cb[r];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -587,7 +657,9 @@ int direct_alloc(Context& c) {
   if (hipMalloc((void**)&c.dAB, sizeof(double) * ld * (size_t)t.dn) != hipSuccess ||
       hipMalloc((void**)&c.dRow, sizeof(double) * (size_t)t.dn) != hipSuccess ||
       hipMalloc((void**)&c.dB, sizeof(double) * (size_t)t.dn) != hipSuccess ||
-      hipMalloc((void**)&c.dipiv, sizeof(int32_t) * (size_t)t.dn) != hipSuccess) {
+      hipMalloc((void**)&c.dipiv, sizeof(int32_t) * (size_t)t.dn) != hipSuccess ||
+      hipMalloc((void**)&c.dju, sizeof(int32_t) * (size_t)t.dn) != hipSuccess ||
+      hipMalloc((void**)&c.d_lustate, sizeof(int32_t) * 2) != hipSuccess) {
     set_error("direct solver: out of device memory for the band matrix");
     return PT_ERR_HIP;
   }
@@ -595,10 +667,10 @@ int direct_alloc(Context& c) {
 }
 
 int direct_free(Context& c) {
-  void* ps[] = {c.dAB, c.dRow, c.dB, c.dipiv};
+  void* ps[] = {c.dAB, c.dRow, c.dB, c.dipiv, c.dju, c.d_lustate};
   for (void* p : ps)
     if (p) (void)hipFree(p);
-  c.dAB = nullptr; c.dRow = nullptr; c.dB = nullptr; c.dipiv = nullptr;
+  c.dAB = nullptr; c.dRow = nullptr; c.dB = nullptr; c.dipiv = nullptr; c.dju = nullptr; c.d_lustate = nullptr;
   return PT_OK;
 }
 
@@ -618,10 +690,36 @@ int build_sf_matrix(Context& c) {
   PT_HIP(hipMemsetAsync(c.dAB, 0, sizeof(double) * ld * (size_t)t.dn, c.stream));
   hipLaunchKernelGGL(k_sf_assemble, dim3((t.dn + 127) / 128), dim3(128), 0, c.stream, a, L1.v.vis, c.dAB);
   PT_TRY(launch_equilibrate(c, c.dAB, c.dRow, t.dn, t.dkl, t.dku));
-  hipLaunchKernelGGL(k_band_lu_wg, dim3(1), dim3(LU_THREADS), 0, c.stream, c.dAB, c.dipiv, t.dn, t.dkl, t.dku,
-                     c.d_info);
+  // panels of nb columns: the panel in LDS (W = kl+nb rows), the trailing
+  // workgroups hold the panel's L columns and one column per wave
+  int nb = 32;
+  auto panel_lds = [&](int b) { return sizeof(double) * (size_t)(t.dkl + b) * b; };
+  auto trail_lds = [&](int b) { return sizeof(double) * ((size_t)b * t.dkl + (size_t)LU_TRAIL_WAVES * (t.dkl + b)); };
+  while (nb > 4 && (panel_lds(nb) > LU_LDS_LIMIT || trail_lds(nb) > LU_LDS_LIMIT)) nb /= 2;
+  if (panel_lds(nb) > LU_LDS_LIMIT || trail_lds(nb) > LU_LDS_LIMIT) {
+    set_error("direct solver: lower band width too large for the LU panels");
+    return PT_ERR_UNSUPPORTED;
+  }
+  static bool attr_set = false;
+  if (!attr_set) {
+    PT_HIP(hipFuncSetAttribute((const void*)k_lu_panel, hipFuncAttributeMaxDynamicSharedMemorySize, LU_LDS_LIMIT));
+    PT_HIP(hipFuncSetAttribute((const void*)k_lu_trailing, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               LU_LDS_LIMIT));
+    attr_set = true;
+  }
+  const int kv = t.dkl + t.dku;
+  for (int p0 = 1; p0 <= t.dn; p0 += nb) {
+    const int pe = std::min(p0 + nb - 1, t.dn);
+    hipLaunchKernelGGL(k_lu_panel, dim3(1), dim3(LU_PANEL_THREADS), panel_lds(nb), c.stream, c.dAB, c.dipiv, c.dju,
+                       c.d_lustate, t.dn, t.dkl, t.dku, p0, nb);
+    const int ncols = std::min(t.dn, pe + kv) - pe;
+    if (ncols > 0)
+      hipLaunchKernelGGL(k_lu_trailing, dim3((ncols + LU_TRAIL_WAVES - 1) / LU_TRAIL_WAVES),
+                         dim3(LU_TRAIL_WAVES * 64), trail_lds(nb), c.stream, c.dAB, c.dipiv, c.dju, t.dn, t.dkl,
+                         t.dku, p0, nb);
+  }
   int32_t info = 0;
-  PT_HIP(hipMemcpyAsync(&info, c.d_info, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipMemcpyAsync(&info, c.d_lustate + 1, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
   PT_HIP(hipStreamSynchronize(c.stream));
   if (info != 0) {
     set_error("direct solver band LU: zero pivot at column " + std::to_string(info));

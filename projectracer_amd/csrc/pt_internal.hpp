@@ -104,6 +104,7 @@ struct Context {
   // direct solver (iterate=0): equilibrated band LU of the whole grid, row
   // factors, right-hand side / solution vector
   double *dAB = nullptr, *dRow = nullptr, *dB = nullptr; int32_t* dipiv = nullptr;
+  int32_t *dju = nullptr, *d_lustate = nullptr;   // LU panels: ju per step; carried ju, info
   bool direct_built = false;
   // tracers, SoA in cell order (re-sorted after every projection of the step's
   // final positions); tid maps a storage slot to the tracer's reference ID
