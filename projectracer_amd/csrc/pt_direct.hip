@@ -168,7 +168,7 @@ __global__ void k_sf_assemble(SfArgs a, const double* vis, double* ab) {
 // are first written by the step that DGBTF2 zeroes them at, so that zeroing
 // is not repeated.
 // ---------------------------------------------------------------------------
-constexpr int LU_PANEL_THREADS = 256;
+constexpr int LU_PANEL_THREADS = 1024;   // upper bound; launched with roundup64(kl+1) threads
 constexpr int LU_LDS_LIMIT = 160 * 1024 - 1024;   // dynamic LDS per workgroup (gfx950: 160 KB)
 constexpr int LU_TRAIL_WAVES = 8;   // columns per trailing workgroup
 
@@ -179,11 +179,11 @@ __global__ void __launch_bounds__(LU_PANEL_THREADS) k_lu_panel(double* ab, int32
   __shared__ double s_best[LU_PANEL_THREADS / 64];
   __shared__ int s_idx[LU_PANEL_THREADS / 64];
   __shared__ int s_jp;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, NT = blockDim.x, nwv = NT >> 6;
   const long ld = 2 * kl + ku + 1;
   const int kv = ku + kl, W = kl + nb, pe = min(p0 + nb - 1, n), np = pe - p0 + 1;
   // A(r, c) = AB(kv+1+r-c, c): rows above c+kl are stored for c >= p0
-  for (int e = tid; e < np * W; e += LU_PANEL_THREADS) {
+  for (int e = tid; e < np * W; e += NT) {
     const int cc = e / W, r = e - cc * W, col = p0 + cc, row = p0 + r;
     P[e] = (row <= min(n, col + kl)) ? ab[(long)(col - 1) * ld + (kv + row - col)] : 0.0;
   }
@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(LU_PANEL_THREADS) k_lu_panel(double* ab, int32
     const int km = min(kl, n - j);
     double best = -1.0;
     int bi = 0x7fffffff;
-    for (int t = tid; t <= km; t += LU_PANEL_THREADS) {
+    for (int t = tid; t <= km; t += NT) {
       const double x = fabs(A(j + t, j));
       if (x > best) { best = x; bi = t; }
     }
@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(LU_PANEL_THREADS) k_lu_panel(double* ab, int32
     if (tid == 0) {
       double b = s_best[0];
       int x = s_idx[0];
-      for (int w = 1; w < LU_PANEL_THREADS / 64; w++)
+      for (int w = 1; w < nwv; w++)
         if (s_best[w] > b || (s_best[w] == b && s_idx[w] < x)) { b = s_best[w]; x = s_idx[w]; }
       s_jp = x + 1;
       ipiv[j - 1] = x + j;
@@ -221,22 +221,26 @@ __global__ void __launch_bounds__(LU_PANEL_THREADS) k_lu_panel(double* ab, int32
       ju = max(ju, min(j + ku + jp - 1, n));
       const int ce = min(ju, pe);
       if (jp != 1)
-        for (int col = j + tid; col <= ce; col += LU_PANEL_THREADS) {
+        for (int col = j + tid; col <= ce; col += NT) {
           const double tmp = A(j + jp - 1, col);
           A(j + jp - 1, col) = A(j, col);
           A(j, col) = tmp;
         }
       __syncthreads();
       if (km > 0) {
+        // thread t owns row j+1+t of the panel (km <= kl < blockDim): the
+        // scaled multiplier stays in a register for the rank-1 update
         const double rcp = 1.0 / A(j, j);
-        for (int t = 1 + tid; t <= km; t += LU_PANEL_THREADS) A(j + t, j) = rcp * A(j + t, j);
-        __syncthreads();
-        const int total = (ce - j) * km;
-        for (int idx = tid; idx < total; idx += LU_PANEL_THREADS) {
-          const int jj = 1 + idx / km, t = 1 + idx % km;
-          const double temp = -1.0 * A(j, j + jj);
-          double& x = A(j + t, j + jj);
-          x = x + A(j + t, j) * temp;
+        const int t = 1 + tid;
+        double lt = 0.0;
+        if (t <= km) {
+          lt = rcp * A(j + t, j);
+          A(j + t, j) = lt;
+        }
+#pragma unroll 8
+        for (int col = j + 1; col <= ce; col++) {
+          const double temp = -1.0 * A(j, col);
+          if (t <= km) A(j + t, col) = A(j + t, col) + lt * temp;
         }
       }
       if (tid == 0) ju_rec[j - 1] = ju;
@@ -246,7 +250,7 @@ __global__ void __launch_bounds__(LU_PANEL_THREADS) k_lu_panel(double* ab, int32
     }
     __syncthreads();
   }
-  for (int e = tid; e < np * W; e += LU_PANEL_THREADS) {
+  for (int e = tid; e < np * W; e += NT) {
     const int cc = e / W, r = e - cc * W, col = p0 + cc, row = p0 + r;
     if (row <= min(n, col + kl)) ab[(long)(col - 1) * ld + (kv + row - col)] = P[e];
   }
@@ -708,9 +712,16 @@ int build_sf_matrix(Context& c) {
     attr_set = true;
   }
   const int kv = t.dkl + t.dku;
+  // one thread per subdiagonal row of the pivot column (the panel keeps each
+  // row's multiplier in a register)
+  const int panel_threads = ((t.dkl + 1 + 63) / 64) * 64;
+  if (panel_threads > LU_PANEL_THREADS) {
+    set_error("direct solver: lower band width above " + std::to_string(LU_PANEL_THREADS - 1));
+    return PT_ERR_UNSUPPORTED;
+  }
   for (int p0 = 1; p0 <= t.dn; p0 += nb) {
     const int pe = std::min(p0 + nb - 1, t.dn);
-    hipLaunchKernelGGL(k_lu_panel, dim3(1), dim3(LU_PANEL_THREADS), panel_lds(nb), c.stream, c.dAB, c.dipiv, c.dju,
+    hipLaunchKernelGGL(k_lu_panel, dim3(1), dim3(panel_threads), panel_lds(nb), c.stream, c.dAB, c.dipiv, c.dju,
                        c.d_lustate, t.dn, t.dkl, t.dku, p0, nb);
     const int ncols = std::min(t.dn, pe + kv) - pe;
     if (ncols > 0)
