@@ -16,6 +16,10 @@ Prints one JSON line.  Paths timed:
   * plain FP64 GPU path (same algorithm as the reference, bit-faithful kernels)
   * cpu_baseline: the reference program (oracle/_ref, flang -O2 -fopenmp) on
     the host cores, one warm time step of the same workload.
+Beside them, `direct_solver`: the direct banded solver (iterate=0,
+PS/momentum.f90:319-653) at the reference's shipped 100x100 size -- the band
+LU's build once, then solve_SF_equation per solve -- with the reference's
+own timers for its shipped run quoted for context.
 """
 from __future__ import annotations
 
@@ -63,6 +67,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--paths", default="mixed,fp64",
                     help="GPU paths to time (profiling runs may select one)")
+    ap.add_argument("--direct-only", action="store_true",
+                    help="time only the direct solver (100x100) and print its object")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB read + 256 MiB write elementwise kernel, the "
                          "reference for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -131,6 +137,48 @@ def time_gpu(args, precision, restart_io=False):
     return out
 
 
+# the reference's own timers for its shipped 100x100 run (2 OpenMP threads,
+# CPU not recorded): build_SF_matrix once, solve_SF_equation over its 5 solves
+# (stdout:17-18)
+REF_SHIPPED_BUILD_S = 0.3331035
+REF_SHIPPED_SOLVE_S = 0.0437131 / 5
+
+
+def time_direct(nx=100, nz=100, solves=20):
+    """The direct solver on the isoviscous nx x nz state: the first
+    update_velocities builds the band LU (assembly, DGBEQU scaling, DGBTF2 in
+    panels) and solves; then `solves` more solve_SF_equation calls."""
+    from projectracer_amd import Simulation, cases
+    case = cases.isoviscous(nx=nx, nz=nz, tpc=1, iterate=0, Database=2)
+    sim = Simulation(case.params)
+    sim.load_state(case)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.build_sf_matrix()
+    t_build = time.perf_counter() - t0
+    sim.solve_sf_equation()
+    ts = []
+    for _ in range(solves):
+        t0 = time.perf_counter()
+        sim.solve_sf_equation()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    solve_s = ts[len(ts) // 2]
+    out = {
+        "grid": f"{nx}x{nz} isoviscous, Database 2: {(nx + 1) * (nz + 1)} unknowns, kl = ku = {3 * (nz + 1)}",
+        "build_ms": t_build * 1e3, "solve_ms_median": solve_s * 1e3, "solves_timed": solves,
+        "dtype": "f64",
+    }
+    if (nx, nz) == (100, 100):
+        out["reference_shipped_run"] = {
+            "build_ms": REF_SHIPPED_BUILD_S * 1e3, "solve_ms": REF_SHIPPED_SOLVE_S * 1e3,
+            "source": "the reference's own timers for its shipped 100x100 run (2 OpenMP threads, CPU not "
+                      "recorded): build_SF_matrix once, solve_SF_equation averaged over 5 solves",
+        }
+    sim.close()
+    return out
+
+
 def cpu_baseline(args):
     """The reference (flang build) on the host: inject the same state, run
     two loop iterations, report the second (warm) one."""
@@ -183,6 +231,9 @@ def main():
         torch.mul(x, 2.0, out=y)
         torch.cuda.synchronize()
         del x, y
+    if args.direct_only:
+        print(json.dumps({"direct_solver": time_direct()}), flush=True)
+        return
     paths = set(args.paths.split(","))
     if paths != {"mixed", "fp64"}:
         # profiling run: time the selected path only, no JSON line
@@ -277,6 +328,7 @@ def main():
         # print_restart + restart=1 read of the whole state (Trestart, restart_tracers) to a
         # temporary directory on the host
         result["restart_io"] = mixed["restart"]
+    result["direct_solver"] = time_direct()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(result), flush=True)
