@@ -83,6 +83,28 @@ def test_direct_path_against_the_reference_program(name):
             sim.energy_time()
 
 
+def test_wide_band_matches_the_oracle_bit_for_bit():
+    """200x48 with Database 1: kl = ku = 603, so the LU runs in 16-column
+    panels (the 32-column panel exceeds LDS) and the backward substitution
+    gives each thread two band rows; psi, u, w still equal the oracle's
+    exactly on the same buoyancy fields."""
+    print("[direct] 200x48 Database 1 (kl = 603): GPU vs oracle", flush=True)
+    case = cases.isoviscous(nx=200, nz=48, tpc=1, aspect=4.0, iterate=0, Database=1)
+    gpu = Simulation(case.params)
+    gpu.load_state(case)
+    ora = OracleSim(case.params)
+    ora.load_state(case)
+    assert ora.direct_dims() == (201 * 49, 603, 603)
+    for n in ("TBUOY", "CBUOY"):
+        gpu.set_field(n, ora.field(n))
+    gpu.update_velocities()
+    ora.update_velocities()
+    for n in ("SF", "U", "W"):
+        a, b = _node(gpu, n), _node(ora, n)
+        print(f"[direct] {n}: max |gpu - oracle| = {np.max(np.abs(a - b)):.3e}", flush=True)
+        assert np.array_equal(a, b), n
+
+
 def test_explicit_build_and_solve_entry_points():
     """pt_solve_sf_equation before a build is an error; build + solve equal
     update_velocities' solve; a rebuild reproduces the same factors."""
