@@ -559,28 +559,52 @@ struct InterpConstTab {
 // 4+1 short sums instead of 7 rows of Taylor sums and three derivative sums
 // with their divisions.  Same polynomial; the rounding differs from the
 // reference's order at the level of the last bit.
-// ctab[node*48 + f*16 + q*4 + p], node = k*(nx+1) + i, f in {u, w, source}
+// ctab[node*48 + f*16 + q*4 + p], node = k*(nx+1) + i, f in {u, w, source}.
+// One thread per coefficient, in ctab order, so a wave's stores are one
+// contiguous run; the 7-row sums keep the order jj = 0..6.
 __global__ void k_ctab(const double* __restrict__ tab, double* __restrict__ ctab, int nx, int nz, InterpConstTab C) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i > nx || k > nz) return;
-  const double* rows = tab + ((long)i * (nz + 7) + k) * 12;   // row kk = k-3 at offset 0
-  double* out = ctab + ((long)k * (nx + 1) + i) * 48;
-  const double dsp[3] = {C.dsp1, C.dsp2, C.dsp3};
-  for (int f = 0; f < 3; f++) {
-    for (int p = 0; p < 4; p++) {
-      out[f * 16 + p] = rows[3 * 12 + f * 4 + p];
-      for (int q = 1; q <= 3; q++) {
-        double acc = 0.0;
-        for (int jj = 0; jj < 7; jj++) acc = acc + C.Dm[q - 1][jj] * rows[jj * 12 + f * 4 + p];
-        out[f * 16 + q * 4 + p] = acc / dsp[q - 1];
-      }
-    }
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nnode = (long)(nx + 1) * (nz + 1);
+  if (t >= nnode * 48) return;
+  const int node = (int)(t / 48), e = (int)(t - (long)node * 48);
+  const int k = node / (nx + 1), i = node - k * (nx + 1);
+  const int f = e >> 4, q = (e >> 2) & 3, p = e & 3;
+  const double* col = tab + ((long)i * (nz + 7) + k) * 12 + f * 4 + p;   // row kk = k-3 at offset 0
+  double v;
+  if (q == 0) {
+    v = col[3 * 12];
+  } else {
+    const double* D = C.Dm[q - 1];
+    double acc = 0.0;
+#pragma unroll
+    for (int jj = 0; jj < 7; jj++) acc = acc + D[jj] * col[jj * 12];
+    v = acc / (q == 1 ? C.dsp1 : (q == 2 ? C.dsp2 : C.dsp3));
   }
+  ctab[t] = v;
 }
 
-// interpolate for the three fields at one position from the node's C_pq
-__device__ __forceinline__ void interp3(const double* __restrict__ ctab, const InterpConst& C, double r, double s,
-                                        double out[3]) {
+// ---- interpolation at the tracers ------------------------------------------
+// A workgroup holds RK_THREADS consecutive tracers of the cell-ordered SoA.
+// Sorted by nearest node at the step's start, they reach a small box of
+// nodes (at 1024x256 and 50 tracers per node: 6-7 nodes at stage 1, 14-24
+// at the drifted stage positions).  The workgroup reduces the box, copies
+// its coefficients from ctab into LDS with contiguous loads, and every lane
+// reads its node's 48 coefficients there: the per-lane 384 B gather no
+// longer shares the vector-memory path with the tracer streams.  A box
+// larger than RK_BOX nodes (a workgroup that wraps a grid row, or positions
+// spread far) reads ctab directly.  Same coefficients in the same order:
+// the results do not depend on the path.
+constexpr int RK_THREADS = 256;
+constexpr int RK_BOX = 42;     // staged nodes: 42 * 24 double2 fit 4 loads per thread
+constexpr int RK_NODE = 25;    // double2 per staged node: 24 + 1 pad spreads nodes over LDS banks
+static_assert(RK_BOX * 24 <= 4 * RK_THREADS, "box copy is 4 double2 per thread");
+
+struct Located {
+  int i, k;                      // nearest node
+  double a1, a2, a3, b1, b2, b3; // hr^p/p!, hs^q/q!
+};
+__device__ __forceinline__ Located locate(const InterpConst& C, double r, double s) {
+  Located p;
   int i = (int)round(r / C.dr), k = (int)round(s / C.ds);
   // positions reaching this point passed the projection's domain check;
   // the clamp only guards the table reads
@@ -589,10 +613,15 @@ __device__ __forceinline__ void interp3(const double* __restrict__ ctab, const I
   // factorial(1) = 1 and factorial(2) = 2 exactly, so h/factorial(1) = h and
   // h^2/factorial(2) = h^2 * 0.5 are the reference's quotients bit for bit
   const double hr = r - (double)i * C.dr;
-  const double a1 = hr, a2 = (hr * hr) * 0.5, a3 = cube_cr(hr) / C.f3;
+  p.a1 = hr; p.a2 = (hr * hr) * 0.5; p.a3 = cube_cr(hr) / C.f3;
   const double hs = s - (double)k * C.ds;
-  const double b1 = hs, b2 = (hs * hs) * 0.5, b3 = cube_cr(hs) / C.f3;
-  const double2* cf = reinterpret_cast<const double2*>(ctab + ((long)k * (C.nx + 1) + i) * 48);
+  p.b1 = hs; p.b2 = (hs * hs) * 0.5; p.b3 = cube_cr(hs) / C.f3;
+  p.i = i; p.k = k;
+  return p;
+}
+// the bicubic from one node's 24 double2 of C_pq (LDS or global pointer)
+template <typename P>
+__device__ __forceinline__ void eval3(P cf, const Located& p, double out[3]) {
 #pragma unroll
   for (int f = 0; f < 3; f++) {
     double g[4];
@@ -600,17 +629,83 @@ __device__ __forceinline__ void interp3(const double* __restrict__ ctab, const I
     for (int q = 0; q < 4; q++) {
       const double2 lo = cf[f * 8 + q * 2], hi = cf[f * 8 + q * 2 + 1];
       double v = lo.x;
-      v = v + lo.y * a1;
-      v = v + hi.x * a2;
-      v = v + hi.y * a3;
+      v = v + lo.y * p.a1;
+      v = v + hi.x * p.a2;
+      v = v + hi.y * p.a3;
       g[q] = v;
     }
     double fi = g[0];
-    fi = fi + g[1] * b1;
-    fi = fi + g[2] * b2;
-    fi = fi + g[3] * b3;
+    fi = fi + g[1] * p.b1;
+    fi = fi + g[2] * p.b2;
+    fi = fi + g[3] * p.b3;
     out[f] = fi;
   }
+}
+// workgroup barrier that orders LDS only: global loads issued before it stay
+// in flight
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+// interpolate for the three fields at (r, s).  Every thread of the workgroup
+// calls it (threads past the last tracer pass that tracer's position), since
+// it holds two workgroup barriers.
+__device__ __forceinline__ void interp3(const double* __restrict__ ctab, const InterpConst& C, double r, double s,
+                                        double out[3]) {
+  __shared__ double2 box[RK_BOX * RK_NODE];
+  __shared__ int bb[RK_THREADS / 64][4];
+  const Located p = locate(C, r, s);
+  const int i0w = wave_min(p.i), i1w = wave_max(p.i), k0w = wave_min(p.k), k1w = wave_max(p.k);
+  if ((threadIdx.x & 63) == 0) {
+    const int wv = threadIdx.x >> 6;
+    bb[wv][0] = i0w; bb[wv][1] = i1w; bb[wv][2] = k0w; bb[wv][3] = k1w;
+  }
+  lds_sync();
+  int i0 = bb[0][0], i1 = bb[0][1], k0 = bb[0][2], k1 = bb[0][3];
+#pragma unroll
+  for (int q = 1; q < RK_THREADS / 64; q++) {
+    i0 = min(i0, bb[q][0]); i1 = max(i1, bb[q][1]); k0 = min(k0, bb[q][2]); k1 = max(k1, bb[q][3]);
+  }
+  // the box [i0,i1] x [k0,k1] holds every thread's node; `staged` is the
+  // same in all threads
+  const int w = i1 - i0 + 1, h = k1 - k0 + 1;
+  const bool staged = w * h <= RK_BOX;
+  const double2* g2 = reinterpret_cast<const double2*>(ctab);
+  if (staged) {
+    // box rows are contiguous runs of w nodes in ctab; loads are clamped to
+    // the box so they can all be issued before the first store
+    const int total = w * h * 24, per = w * 24;
+    double2 v[4];
+    int dst[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int t = threadIdx.x + u * RK_THREADS;
+      const int tc = t < total ? t : total - 1;
+      const int row = tc / per, rem = tc - row * per;
+      const int nd = rem / 24, q = rem - nd * 24;
+      v[u] = g2[((long)(k0 + row) * (C.nx + 1) + i0) * 24 + rem];
+      dst[u] = t < total ? (row * w + nd) * RK_NODE + q : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (dst[u] >= 0) box[dst[u]] = v[u];
+  }
+  lds_sync();
+  if (staged)
+    eval3(box + ((p.k - k0) * w + (p.i - i0)) * RK_NODE, p, out);
+  else
+    eval3(g2 + ((long)p.k * (C.nx + 1) + p.i) * 24, p, out);
 }
 
 struct RkArgs {
@@ -625,58 +720,66 @@ struct RkArgs {
 
 // The tracer streams are read and written once per stage kernel:
 // nontemporal loads and stores keep them from displacing the coefficient
-// table in the caches.
+// table in the caches.  Each kernel issues all of its stream loads before
+// the interpolation's barriers; threads past the last tracer load that
+// tracer (j clamped) and store nothing.
 #define NT_LD(x) __builtin_nontemporal_load(&(x))
 #define NT_ST(x, v) __builtin_nontemporal_store((v), &(x))
 
 // stage 1: PS/energy.f90:258-301
-__global__ void k_rk1(RkArgs a) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.n) return;
-  double rv = NT_LD(a.r[j]), sv = NT_LD(a.s[j]), Tv = NT_LD(a.T[j]);
-  NT_ST(a.r0[j], rv); NT_ST(a.s0[j], sv); NT_ST(a.T0[j], Tv);
+__global__ void __launch_bounds__(RK_THREADS) k_rk1(RkArgs a) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, jc = j < a.n ? j : a.n - 1;
+  const double rv = NT_LD(a.r[jc]), sv = NT_LD(a.s[jc]), Tv = NT_LD(a.T[jc]);
   double o[3];
   interp3(a.tab, a.C, rv, sv, o);
+  if (j >= a.n) return;
+  NT_ST(a.r0[j], rv); NT_ST(a.s0[j], sv); NT_ST(a.T0[j], Tv);
   NT_ST(a.r[j], o[0]); NT_ST(a.s[j], o[1]); NT_ST(a.T[j], o[2]);
   NT_ST(a.r1[j], rv + a.dtstage * o[0]);
   NT_ST(a.s1[j], sv + a.dtstage * o[1]);
   NT_ST(a.T1[j], Tv + a.dtstage * o[2]);
 }
 // stage 2: PS/energy.f90:350-402
-__global__ void k_rk2(RkArgs a) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.n) return;
+__global__ void __launch_bounds__(RK_THREADS) k_rk2(RkArgs a) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, jc = j < a.n ? j : a.n - 1;
+  const double rv = NT_LD(a.r[jc]), sv = NT_LD(a.s[jc]), Tv = NT_LD(a.T[jc]);
+  const double r0 = NT_LD(a.r0[jc]), s0 = NT_LD(a.s0[jc]), T0 = NT_LD(a.T0[jc]);
   double o[3];
-  interp3(a.tab, a.C, NT_LD(a.r1[j]), NT_LD(a.s1[j]), o);
-  NT_ST(a.r[j], NT_LD(a.r[j]) + 2.0 * o[0]);
-  NT_ST(a.s[j], NT_LD(a.s[j]) + 2.0 * o[1]);
-  NT_ST(a.T[j], NT_LD(a.T[j]) + 2.0 * o[2]);
-  NT_ST(a.r2[j], NT_LD(a.r0[j]) + a.dtstage * o[0]);
-  NT_ST(a.s2[j], NT_LD(a.s0[j]) + a.dtstage * o[1]);
-  NT_ST(a.T2[j], NT_LD(a.T0[j]) + a.dtstage * o[2]);
+  interp3(a.tab, a.C, NT_LD(a.r1[jc]), NT_LD(a.s1[jc]), o);
+  if (j >= a.n) return;
+  NT_ST(a.r[j], rv + 2.0 * o[0]);
+  NT_ST(a.s[j], sv + 2.0 * o[1]);
+  NT_ST(a.T[j], Tv + 2.0 * o[2]);
+  NT_ST(a.r2[j], r0 + a.dtstage * o[0]);
+  NT_ST(a.s2[j], s0 + a.dtstage * o[1]);
+  NT_ST(a.T2[j], T0 + a.dtstage * o[2]);
 }
 // stage 3: PS/energy.f90:450-473
-__global__ void k_rk3(RkArgs a) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.n) return;
+__global__ void __launch_bounds__(RK_THREADS) k_rk3(RkArgs a) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, jc = j < a.n ? j : a.n - 1;
+  const double rv = NT_LD(a.r[jc]), sv = NT_LD(a.s[jc]), Tv = NT_LD(a.T[jc]);
+  const double r0 = NT_LD(a.r0[jc]), s0 = NT_LD(a.s0[jc]), T0 = NT_LD(a.T0[jc]);
   double o[3];
-  interp3(a.tab, a.C, NT_LD(a.r2[j]), NT_LD(a.s2[j]), o);
-  NT_ST(a.r[j], NT_LD(a.r[j]) + 2.0 * o[0]);
-  NT_ST(a.r1[j], NT_LD(a.r0[j]) + a.dtstage * o[0]);
-  NT_ST(a.s[j], NT_LD(a.s[j]) + 2.0 * o[1]);
-  NT_ST(a.s1[j], NT_LD(a.s0[j]) + a.dtstage * o[1]);
-  NT_ST(a.T[j], NT_LD(a.T[j]) + 2.0 * o[2]);
-  NT_ST(a.T1[j], NT_LD(a.T0[j]) + a.dtstage * o[2]);
+  interp3(a.tab, a.C, NT_LD(a.r2[jc]), NT_LD(a.s2[jc]), o);
+  if (j >= a.n) return;
+  NT_ST(a.r[j], rv + 2.0 * o[0]);
+  NT_ST(a.r1[j], r0 + a.dtstage * o[0]);
+  NT_ST(a.s[j], sv + 2.0 * o[1]);
+  NT_ST(a.s1[j], s0 + a.dtstage * o[1]);
+  NT_ST(a.T[j], Tv + 2.0 * o[2]);
+  NT_ST(a.T1[j], T0 + a.dtstage * o[2]);
 }
 // stage 4: PS/energy.f90:517-537 (dtstage = dt/6)
-__global__ void k_rk4(RkArgs a) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.n) return;
+__global__ void __launch_bounds__(RK_THREADS) k_rk4(RkArgs a) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, jc = j < a.n ? j : a.n - 1;
+  const double rv = NT_LD(a.r[jc]), sv = NT_LD(a.s[jc]), Tv = NT_LD(a.T[jc]);
+  const double r0 = NT_LD(a.r0[jc]), s0 = NT_LD(a.s0[jc]), T0 = NT_LD(a.T0[jc]);
   double o[3];
-  interp3(a.tab, a.C, NT_LD(a.r1[j]), NT_LD(a.s1[j]), o);
-  NT_ST(a.r[j], NT_LD(a.r0[j]) + a.dtstage * (NT_LD(a.r[j]) + o[0]));
-  NT_ST(a.s[j], NT_LD(a.s0[j]) + a.dtstage * (NT_LD(a.s[j]) + o[1]));
-  NT_ST(a.T[j], NT_LD(a.T0[j]) + a.dtstage * (NT_LD(a.T[j]) + o[2]));
+  interp3(a.tab, a.C, NT_LD(a.r1[jc]), NT_LD(a.s1[jc]), o);
+  if (j >= a.n) return;
+  NT_ST(a.r[j], r0 + a.dtstage * (rv + o[0]));
+  NT_ST(a.s[j], s0 + a.dtstage * (sv + o[1]));
+  NT_ST(a.T[j], T0 + a.dtstage * (Tv + o[2]));
 }
 
 static int source_and_tables(Context& c, bool velocity_too) {
@@ -700,7 +803,8 @@ static int build_itab(Context& c) {
   C.dsp1 = c.T.ds_power[1]; C.dsp2 = c.T.ds_power[2]; C.dsp3 = c.T.ds_power[3];
   for (int m = 0; m < 3; m++)
     for (int q = 0; q < 7; q++) C.Dm[m][q] = c.T.D[m + 1][q];
-  hipLaunchKernelGGL(k_ctab, grid2(nx + 1, nz + 1, b), b, 0, c.stream, c.itab, c.ctab, nx, nz, C);
+  const long ncoef = (long)(nx + 1) * (nz + 1) * 48;
+  hipLaunchKernelGGL(k_ctab, dim3((unsigned)((ncoef + 255) / 256)), dim3(256), 0, c.stream, c.itab, c.ctab, nx, nz, C);
   return PT_OK;
 }
 
@@ -773,10 +877,10 @@ static int update_velocities(Context& c) {
 // `which` names the stage's own timer (PT_K_TRACER_RK2 for stage 2, else -1);
 // every stage also counts towards PT_K_TRACER_STAGE
 static void rk_launch(Context& c, void (*kern)(RkArgs), RkArgs a, int which = -1) {
-  unsigned nb = (unsigned)((a.n + 255) / 256);
+  unsigned nb = (unsigned)((a.n + RK_THREADS - 1) / RK_THREADS);
   if (c.timing) timer_begin(c, PT_K_TRACER_STAGE);
   if (c.timing && which >= 0) timer_begin(c, which);
-  hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, c.stream, a);
+  hipLaunchKernelGGL(kern, dim3(nb), dim3(RK_THREADS), 0, c.stream, a);
   if (c.timing && which >= 0) timer_end(c, which);
   if (c.timing) timer_end(c, PT_K_TRACER_STAGE);
 }
