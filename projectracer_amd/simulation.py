@@ -60,8 +60,10 @@ class Simulation:
     def load_state(self, case):
         T = np.ascontiguousarray(case.T_nodes, dtype=np.float64)
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (case.rtr, case.str_, case.Ttr, case.Ctr)]
-        if arrs[0].size != self.ntr:
-            raise ValueError(f"case has {arrs[0].size} tracers, context expects {self.ntr}")
+        if any(a.size != self.ntr for a in arrs):
+            raise ValueError(f"case has {[a.size for a in arrs]} tracer values, context expects {self.ntr} each")
+        if T.size != (self.nx + 1) * (self.nz + 1):
+            raise ValueError(f"case has {T.size} node temperatures, context expects {(self.nx + 1) * (self.nz + 1)}")
         self._check(self.lib.pt_load_state(self.ctx, _dp(T), case.Tbot_iso, case.Ttop_iso,
                                            *[_dp(a) for a in arrs]), "pt_load_state")
 

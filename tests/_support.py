@@ -106,6 +106,10 @@ class OracleSim:
     def load_state(self, case):
         T = np.ascontiguousarray(case.T_nodes, dtype=np.float64)
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (case.rtr, case.str_, case.Ttr, case.Ctr)]
+        # the C side reads ntr values from each array and the (nx+1)(nz+1) nodes
+        if any(a.size != self.ntr for a in arrs) or T.size != (self.nx + 1) * (self.nz + 1):
+            raise ValueError(f"case sizes {T.size}, {[a.size for a in arrs]} do not fit "
+                             f"{self.nx}x{self.nz} with {self.ntr} tracers")
         self._chk(self.lib.pto_load_state(self.ctx, _dp(T), case.Tbot_iso, case.Ttop_iso,
                                           *[_dp(a) for a in arrs]), "load_state")
 

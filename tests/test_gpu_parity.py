@@ -94,6 +94,36 @@ def test_two_steps_isoviscous(iso):
     assert rel_err(gr, orr) <= 1e-12 and rel_err(gs, os_) <= 1e-12 and rel_err(gT, oT) <= 1e-12
 
 
+@pytest.mark.parametrize("nx,nz,tpc", [(64, 64, 20), (40, 24, 1), (32, 16, 37)],
+                         ids=["staged+row-wrap", "all-direct", "staged-dense"])
+def test_rk_stage_coefficient_paths(nx, nz, tpc):
+    """One energy_time (four RK4 stages) against the oracle at the tracer bar.
+    A workgroup of the stage kernels holds 256 cell-ordered tracers and
+    stages the coefficients of the node box they reach in LDS when the box
+    has at most 42 nodes, else reads them from the table:
+      * 20 per node on 64x64: boxes of ~15 nodes staged, workgroups that wrap
+        a grid row read directly;
+      * 1 per node: a workgroup's tracers span several grid rows, every box
+        is read directly;
+      * 37 per node on 32x16: small staged boxes.
+    The tracer counts (84500, 1025, 20757) leave partial last workgroups of
+    20, 1 and 21 threads."""
+    case = cases.isoviscous(nx=nx, nz=nz, tpc=tpc, aspect=nx / nz)
+    g, o = _pair(case)
+    print(f"[rk paths] {nx}x{nz} tpc={tpc}: {g.ntr} tracers, last workgroup {g.ntr % 256 or 256} of 256",
+          flush=True)
+    for sim in (g, o):
+        sim.multigrid()
+        sim.compute_velocities()
+        sim.energy_time()
+    gr, gs, gT, _ = g.tracers()
+    orr, os_, oT, _ = o.tracers()
+    errs = (rel_err(gr, orr), rel_err(gs, os_), rel_err(gT, oT))
+    print(f"[rk paths] r {errs[0]:.2e} s {errs[1]:.2e} T {errs[2]:.2e}", flush=True)
+    assert max(errs) <= 1e-12
+    assert np.array_equal(g.nempty(), o.nempty())
+
+
 def test_thermochem_variable_viscosity(thermo):
     g, o = _pair(thermo)
     g.multigrid()

@@ -475,3 +475,18 @@ def test_restart_record_of_the_shipped_Trestart():
     S = g["SF"].reshape(nz + 7, nx + 7)
     assert np.all(S[:, 3] == 0.0) and np.all(S[:, nx + 3] == 0.0) and np.all(S[3, :] == 0.0)
     assert np.array_equal(S[4:nz + 3, 2], -S[4:nz + 3, 4]) and np.array_equal(S[2, 4:nx + 3], -S[4, 4:nx + 3])
+
+
+def test_load_state_refuses_sizes_that_do_not_fit_the_grid():
+    """optimize_grid may widen the grid (30x20 becomes 32x20): a case built
+    for the requested grid no longer fits the context, and loading it is
+    refused instead of reading past the case's arrays."""
+    case = cases.isoviscous(nx=30, nz=20, tpc=3, aspect=1.5)
+    sim = OracleSim(case.params)
+    assert (sim.nx, sim.nz) == (32, 20)
+    with pytest.raises(ValueError):
+        sim.load_state(case)
+    ok = cases.isoviscous(nx=32, nz=20, tpc=3, aspect=1.6)
+    fit = OracleSim(ok.params)
+    fit.load_state(ok)
+    assert fit.ntr == ok.ntr
