@@ -1336,6 +1336,10 @@ static int launch_cb(Context& c, int rb, size_t lds, const StageArgs<float>& a, 
 // stages per segment (cw == 0: not eligible).  The widest stage region (core
 // + 6 columns per further stage) spans at most the 64 column lanes; the
 // three strip buffers must fit LDS; rows are covered by 16 blocks of 4.
+// The most fused stages come first (9 stages on a 16-column core: 4
+// launches per 35-stage sweep): a workgroup's stage takes about the same
+// time for any core width, since its 16 waves wait on LDS and the barrier
+// rather than on issue, and each launch pays a strip load and store.
 struct StripCfg {
   int cw = 0, ns = 0, W = 0;
 };
@@ -1343,7 +1347,7 @@ static StripCfg strip_cfg(const LevelGeom& g) {
   StripCfg sc;
   if (g.m - 1 > STRIP_TY * STRIP_RB) return sc;
   const size_t rows = (size_t)g.m + 7;
-  for (int ns = 5; ns >= 2; ns--) {
+  for (int ns = (STRIP_TX + 6 - 16) / 6; ns >= 2; ns--) {
     for (int cw = STRIP_TX + 6 - 6 * ns; cw >= 16; cw -= 4) {
       const int W = std::min(cw + 6 * ns, g.n + 7);
       if (3 * (size_t)W * rows * sizeof(float) <= SMALL_LDS_BYTES) {
