@@ -2,9 +2,9 @@
  * pt_oracle.h -- CPU oracle for the ProjecTracer hot path.
  *
  * TEST INFRASTRUCTURE ONLY.  This is a single-threaded plain-C restatement
- * of the reference Fortran (seantrim/ProjecTracer) used as the checker for
- * the HIP implementation.  Only tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg may load it, and only as the checker or the
+ * of the reference Fortran (seantrim/ProjecTracer): the oracle the HIP
+ * implementation is compared with.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it, and only as that oracle or the
  * timed CPU baseline -- never as a product code path.
  *
  * Parameters and field layouts are the ones of include/projectracer_gpu.h

@@ -1,5 +1,5 @@
-"""Pinning the CPU oracle (oracle/liboracle.so) before it is trusted as the
-checker of the HIP path.  CPU only.
+"""Pinning the CPU oracle (oracle/liboracle.so) before the HIP path is held
+to it.  CPU only.
 
 Each test names what fixes the expected values:
   * the reference's shipped output files (stdout, stats.dat, T10000 of the
