@@ -264,6 +264,23 @@ __global__ void __launch_bounds__(TX* TZ) k_residual(StageArgs<R> a, R* res, dou
 struct RkcCoef {
   double mewt[64], mew[64], vv[64], cc[64];
 };
+// the FP32 sweep's stage coefficients, rounded once on the host: c0 = cc,
+// c1 = mew, c2 = vv, cf = mewt * cpar (the product in FP64, as the device
+// would form it); read at a uniform stage index they are scalar loads of
+// the kernel arguments
+struct RkcCoefF {
+  float c0[64], c1[64], c2[64], cf[64];
+};
+static RkcCoefF rkc_f32(const RkcCoef& co, double cpar) {
+  RkcCoefF f;
+  for (int j = 0; j < 64; j++) {
+    f.c0[j] = (float)co.cc[j];
+    f.c1[j] = (float)co.mew[j];
+    f.c2[j] = (float)co.vv[j];
+    f.cf[j] = (float)(co.mewt[j] * cpar);
+  }
+  return f;
+}
 constexpr int SMALL_THREADS = 1024;
 
 // RKC1 coefficients of stages 0..63 held one stage per lane of every
@@ -424,8 +441,8 @@ __device__ __forceinline__ void eval_cb(const float* S, int W, int lo, int i, in
 // Factorised sums round differently from the reference's 5x5 product
 // weights, so only the FP32 path uses this.
 template <int RB>
-__device__ __forceinline__ void sweep_cb(const StageArgs<float>& a, float* err, float* res, int nstage, double cpar,
-                                         const RkcCoef& co, float* buf0) {
+__device__ __forceinline__ void sweep_cb(const StageArgs<float>& a, float* err, float* res, int nstage,
+                                         const RkcCoefF& co, float* buf0) {
   const int n = a.n, m = a.m, p = a.pitch;
   const int ne = p * (m + 7);
   float* buf1 = buf0 + ne;
@@ -460,10 +477,9 @@ __device__ __forceinline__ void sweep_cb(const StageArgs<float>& a, float* err, 
     }
   }
   const int rmax = m + 3;   // last stored row
-  const LaneRkc lk = lane_rkc(co);
   __syncthreads();
   {
-    const float cf = (float)(lane_at(lk.mewt, 1) * cpar);
+    const float cf = co.cf[1];
     float F[RB];
     eval_cb<RB>(buf0, p, -3, i, k0, rmax, a, rhsv, visv, F);
 #pragma unroll
@@ -476,8 +492,7 @@ __device__ __forceinline__ void sweep_cb(const StageArgs<float>& a, float* err, 
   float* spare = buf2;
   for (int j = 2; j <= nstage; j++) {
     float* out = (prev == buf0) ? spare : prev;
-    const float c0 = (float)lane_at(lk.cc, j), c1 = (float)lane_at(lk.mew, j), c2 = (float)lane_at(lk.vv, j);
-    const float cf = (float)(lane_at(lk.mewt, j) * cpar);
+    const float c0 = co.c0[j], c1 = co.c1[j], c2 = co.c2[j], cf = co.cf[j];
     float F[RB];
     eval_cb<RB>(cur, p, -3, i, k0, rmax, a, rhsv, visv, F);
 #pragma unroll
@@ -503,9 +518,9 @@ __device__ __forceinline__ void sweep_cb(const StageArgs<float>& a, float* err, 
 
 template <int RB>
 __global__ void __launch_bounds__(SMALL_THREADS) k_smooth_cb(StageArgs<float> a, float* err, float* res, int nstage,
-                                                            double cpar, RkcCoef co) {
+                                                            RkcCoefF co) {
   extern __shared__ unsigned char smem_raw[];
-  sweep_cb<RB>(a, err, res, nstage, cpar, co, reinterpret_cast<float*>(smem_raw));
+  sweep_cb<RB>(a, err, res, nstage, co, reinterpret_cast<float*>(smem_raw));
 }
 
 // ---------------------------------------------------------------------------
@@ -527,11 +542,10 @@ struct StripArgs {
   const float* Gprev; // stage j0-2, read when j0 >= 3
   float* Ocur;        // stage j1 on the core columns (+ ghost images)
   float* Oprev;       // stage j1-1 on the core columns; null on the last segment
-  double cpar;        // Courant factor of the sweep
   int j0, j1;         // stages of this segment (1-based, j1 > j0)
   int cw;             // core columns per workgroup
   int W;              // LDS strip width in columns
-  RkcCoef co;
+  RkcCoefF co;        // stage coefficients, cf with the sweep's Courant factor
 };
 
 __global__ void __launch_bounds__(STRIP_TX* STRIP_TY) k_strip_cb(StageArgs<float> a, StripArgs s) {
@@ -587,8 +601,7 @@ __global__ void __launch_bounds__(STRIP_TX* STRIP_TY) k_strip_cb(StageArgs<float
     const float* cur = buf + ci * bsz;
     const float* prev = buf + pi * bsz;
     float* out = buf + oi * bsz;
-    const float cc0 = (float)s.co.cc[j], cc1 = (float)s.co.mew[j], cc2 = (float)s.co.vv[j];
-    const float cf = (float)(s.co.mewt[j] * s.cpar);
+    const float cc0 = s.co.c0[j], cc1 = s.co.c1[j], cc2 = s.co.c2[j], cf = s.co.cf[j];
     if (ic >= rlo && ic <= rhi && k0 <= m - 1) {
       float F[RB];
       eval_cb<RB>(cur, W, lo, ic, k0, rmax, a, rhsv, visv, F);
@@ -1323,7 +1336,7 @@ static int launch_cb_t(Context& c, size_t lds, const StageArgs<float>& a, float*
                                (int)SMALL_LDS_BYTES));
     attr_set = true;
   }
-  hipLaunchKernelGGL(k_smooth_cb<RB>, dim3(1), dim3(SMALL_THREADS), lds, c.stream, a, err, res, s, cc, co);
+  hipLaunchKernelGGL(k_smooth_cb<RB>, dim3(1), dim3(SMALL_THREADS), lds, c.stream, a, err, res, s, rkc_f32(co, cc));
   return PT_OK;
 }
 static int launch_cb(Context& c, int rb, size_t lds, const StageArgs<float>& a, float* err, float* res, int s,
@@ -1382,7 +1395,7 @@ static int strip_sweep(Context& c, int l, const StripCfg& sc, double cpar, const
   const size_t lds = 3 * (size_t)sc.W * (g.m + 7) * sizeof(float);
   StageArgs<float> a = stage_base<float>(c, l, true);
   StripArgs sa;
-  sa.S0 = *f[0]; sa.cpar = cpar; sa.cw = sc.cw; sa.W = sc.W; sa.co = co;
+  sa.S0 = *f[0]; sa.cw = sc.cw; sa.W = sc.W; sa.co = rkc_f32(co, cpar);
   int rd = 1, wr = 3;   // read pair f[rd], f[rd+1]; write pair f[wr], f[wr+1]
   int j0 = 1;
   for (int seg = 0; seg < nseg; seg++) {
