@@ -312,7 +312,14 @@ __device__ __forceinline__ void sweep_small(const StageArgs<R>& a, R* err, R* re
     buf1[e] = (R)0;
     buf2[e] = (R)0;
   }
-  const int nxi = n - 1, npts = nxi * (m - 1);
+  // point e of the thread's PPT: row e / tcol, column e % tcol, with rows
+  // of tcol points padded to whole wavefronts when the workgroup has room,
+  // so that no wave spans two rows (lanes a row lower hit other lanes' LDS
+  // banks)
+  const int nxi = n - 1;
+  const int padded = (nxi + 63) & ~63;
+  const int tcol = padded * (m - 1) <= PPT * SMALL_THREADS ? padded : nxi;
+  const int npts = tcol * (m - 1);
   int pi[PPT], pk[PPT], pg[PPT];
   R pdt[PPT];
   PointCoef<R> pc[PPT];
@@ -323,9 +330,9 @@ __device__ __forceinline__ void sweep_small(const StageArgs<R>& a, R* err, R* re
     pk[q] = 0;
     pg[q] = 0;
     pdt[q] = (R)0;
-    if (e < npts) {
-      int k = e / nxi;
-      pi[q] = 1 + (e - k * nxi);
+    const int k = e / tcol, ic = e - k * tcol;
+    if (e < npts && ic < nxi) {
+      pi[q] = 1 + ic;
       pk[q] = 1 + k;
       pg[q] = (int)IDX(pi[q], pk[q], p);
       pdt[q] = a.dts[pg[q]];
@@ -453,14 +460,19 @@ __device__ __forceinline__ void sweep_cb(const StageArgs<float>& a, float* err, 
     buf1[e] = 0.0f;
     buf2[e] = 0.0f;
   }
+  // threads of a row block: the n-1 interior columns, padded to whole
+  // wavefronts when the workgroup has room, so that no wave spans two row
+  // blocks (lanes one block lower hit other lanes' LDS banks)
   const int nxi = n - 1, nkb = (m - 1 + RB - 1) / RB;
-  const bool active = tid < nxi * nkb;
+  const int padded = (nxi + 63) & ~63;
+  const int tcol = padded * nkb <= SMALL_THREADS ? padded : nxi;
+  const int kb = tid / tcol, ic = tid - kb * tcol;
+  const bool active = kb < nkb && ic < nxi;
   int i = 1, k0 = 1;
   float s0v[RB], dtv[RB], rhsv[RB], visv[RB];
   bool on[RB];
   if (active) {
-    int kb = tid / nxi;
-    i = 1 + (tid - kb * nxi);
+    i = 1 + ic;
     k0 = 1 + kb * RB;
   }
 #pragma unroll
