@@ -90,7 +90,9 @@ __global__ void __launch_bounds__(PJ_THREADS) k_cell_reduce(const double* __rest
                                                             const int32_t* __restrict__ cend,
                                                             double* __restrict__ cellsum, int nx, int nz, double dr,
                                                             double ds) {
-  __shared__ double hx[3][PJ_CHUNK], hz[3][PJ_CHUNK], sT[PJ_CHUNK], sC[PJ_CHUNK];
+  // hat-factor rows padded by one double: an owner thread's row na (nb) is
+  // then two LDS banks from its neighbours', not in the same bank
+  __shared__ double hx[3][PJ_CHUNK + 1], hz[3][PJ_CHUNK + 1], sT[PJ_CHUNK], sC[PJ_CHUNK];
   __shared__ int32_t range[2];
   const int ncell = (nx + 1) * (nz + 1);
   const int cfirst = blockIdx.x * PJ_CELLS;
