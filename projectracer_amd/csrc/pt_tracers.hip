@@ -591,8 +591,8 @@ __global__ void k_ctab(const double* __restrict__ tab, double* __restrict__ ctab
 // nodes (at 1024x256 and 50 tracers per node: 6-7 nodes at stage 1, 14-24
 // at the drifted stage positions).  The workgroup reduces the box, copies
 // its coefficients from ctab into LDS with contiguous loads, and every lane
-// reads its node's 48 coefficients there: the per-lane 384 B gather no
-// longer shares the vector-memory path with the tracer streams.  A box
+// reads its node's 48 coefficients (384 B) there, so the vector-memory path
+// carries the tracer streams and one copy of the box per workgroup.  A box
 // larger than RK_BOX nodes (a workgroup that wraps a grid row, or positions
 // spread far) reads ctab directly.  Same coefficients in the same order:
 // the results do not depend on the path.
