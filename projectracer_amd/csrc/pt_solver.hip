@@ -71,6 +71,7 @@ __device__ __forceinline__ float scale_(float s, float d, float rd) { (void)d; r
 constexpr int TX = 64, TZ = 8, PZ = 1;
 constexpr size_t SMALL_LDS_BYTES = 160 * 1024;
 constexpr int TW = TX + 6, TH = TZ * PZ + 6;
+constexpr int TILE_PER = (TH * TW + TX * TZ - 1) / (TX * TZ);   // tile elements per thread
 
 // pointwise operands of stream_space at one point (read once per launch)
 template <typename R>
@@ -150,13 +151,38 @@ __device__ __forceinline__ R stream_space_pt(At at, const StageArgs<R>& a, const
 }
 
 template <typename R>
-__device__ __forceinline__ void load_tile(R (*t)[TW], const R* S, int i0, int k0, int n, int m, int pitch) {
-  for (int idx = threadIdx.y * TX + threadIdx.x; idx < TH * TW; idx += TX * TZ) {
-    int rr = idx / TW, cc = idx - rr * TW;
-    int gi = i0 - 3 + cc, gk = k0 - 3 + rr;
-    R v = 0;
-    if (gi <= n + 3 && gk <= m + 3) v = S[IDX(gi, gk, pitch)];
-    t[rr][cc] = v;
+struct TileRegs {
+  R v[TILE_PER];
+  bool in[TILE_PER];
+};
+// the thread's tile elements, loaded in two phases so that every global
+// load of the launch (tile and pointwise operands) is in flight at once:
+// tile_issue loads from addresses clamped to the stored range (so the loads
+// are unconditional), tile_commit writes LDS, zero outside the level
+template <typename R>
+__device__ __forceinline__ TileRegs<R> tile_issue(const R* S, int i0, int k0, int n, int m, int pitch) {
+  TileRegs<R> r;
+  const int tid = threadIdx.y * TX + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < TILE_PER; u++) {
+    const int idx = min(tid + u * TX * TZ, TH * TW - 1);
+    const int rr = idx / TW, cc = idx - rr * TW;
+    const int gi = i0 - 3 + cc, gk = k0 - 3 + rr;
+    r.in[u] = gi <= n + 3 && gk <= m + 3;
+    r.v[u] = S[IDX(min(gi, n + 3), min(gk, m + 3), pitch)];
+  }
+  return r;
+}
+template <typename R>
+__device__ __forceinline__ void tile_commit(R (*t)[TW], const TileRegs<R>& r) {
+  const int tid = threadIdx.y * TX + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < TILE_PER; u++) {
+    const int idx = tid + u * TX * TZ;
+    if (idx < TH * TW) {
+      const int rr = idx / TW, cc = idx - rr * TW;
+      t[rr][cc] = r.in[u] ? r.v[u] : (R)0;
+    }
   }
 }
 
@@ -186,7 +212,7 @@ template <typename R, bool IV0>
 __global__ void __launch_bounds__(TX* TZ) k_stage(StageArgs<R> a) {
   __shared__ R tile[TH][TW];
   const int i0 = 1 + blockIdx.x * TX, k0 = 1 + blockIdx.y * TZ * PZ;
-  load_tile<R>(tile, a.Sx, i0, k0, a.n, a.m, a.pitch);
+  const TileRegs<R> tr = tile_issue<R>(a.Sx, i0, k0, a.n, a.m, a.pitch);
   R dt[PZ], s0[PZ], s2[PZ];
   PointCoef<R> pc[PZ];
   const int i = i0 + threadIdx.x;
@@ -201,6 +227,7 @@ __global__ void __launch_bounds__(TX* TZ) k_stage(StageArgs<R> a) {
     if (!a.stage1) s2[pz] = a.S2[g];
     pc[pz] = load_pc<R, IV0>(a, g);
   }
+  tile_commit<R>(tile, tr);
   __syncthreads();
 #pragma unroll
   for (int pz = 0; pz < PZ; pz++) {
@@ -225,7 +252,7 @@ __global__ void __launch_bounds__(TX* TZ) k_residual(StageArgs<R> a, R* res, dou
   __shared__ R tile[TH][TW];
   __shared__ double red[TX * TZ];
   const int i0 = 1 + blockIdx.x * TX, k0 = 1 + blockIdx.y * TZ * PZ;
-  load_tile<R>(tile, a.Sx, i0, k0, a.n, a.m, a.pitch);
+  const TileRegs<R> tr = tile_issue<R>(a.Sx, i0, k0, a.n, a.m, a.pitch);
   PointCoef<R> pc[PZ];
   const int i = i0 + threadIdx.x;
 #pragma unroll
@@ -234,6 +261,7 @@ __global__ void __launch_bounds__(TX* TZ) k_residual(StageArgs<R> a, R* res, dou
     if (i > a.n - 1 || k > a.m - 1) continue;
     pc[pz] = load_pc<R, IV0>(a, IDX(i, k, a.pitch));
   }
+  tile_commit<R>(tile, tr);
   __syncthreads();
   double acc = 0.0;
 #pragma unroll
