@@ -327,6 +327,28 @@ __device__ __forceinline__ double lane_at(double v, int j) {
   return __hiloint2double(hi, lo);
 }
 
+// a level array into LDS (and two more LDS arrays zeroed) by a whole
+// workgroup of NT threads, B elements per thread per batch: the batch's
+// loads (indices clamped into the array) are all issued before its first
+// LDS write
+template <typename R, int NT, int B>
+__device__ __forceinline__ void copy_in3(const R* src, R* d0, R* d1, R* d2, int ne) {
+  for (int e0 = threadIdx.x; e0 < ne; e0 += B * NT) {
+    R v[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) v[u] = src[min(e0 + u * NT, ne - 1)];
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      const int e = e0 + u * NT;
+      if (e < ne) {
+        d0[e] = v[u];
+        d1[e] = (R)0;
+        d2[e] = (R)0;
+      }
+    }
+  }
+}
+
 template <typename R, bool IV0, int PPT>
 __device__ __forceinline__ void sweep_small(const StageArgs<R>& a, R* err, R* res, int nstage, double cpar,
                                             const RkcCoef& co, R* buf0) {
@@ -335,11 +357,7 @@ __device__ __forceinline__ void sweep_small(const StageArgs<R>& a, R* err, R* re
   R* buf1 = buf0 + ne;
   R* buf2 = buf1 + ne;
   const int tid = threadIdx.x;
-  for (int e = tid; e < ne; e += SMALL_THREADS) {
-    buf0[e] = err[e];
-    buf1[e] = (R)0;
-    buf2[e] = (R)0;
-  }
+  copy_in3<R, SMALL_THREADS, 8>(err, buf0, buf1, buf2, ne);
   // point e of the thread's PPT: row e / tcol, column e % tcol, with rows
   // of tcol points padded to whole wavefronts when the workgroup has room,
   // so that no wave spans two rows (lanes a row lower hit other lanes' LDS
@@ -483,11 +501,6 @@ __device__ __forceinline__ void sweep_cb(const StageArgs<float>& a, float* err, 
   float* buf1 = buf0 + ne;
   float* buf2 = buf1 + ne;
   const int tid = threadIdx.x;
-  for (int e = tid; e < ne; e += SMALL_THREADS) {
-    buf0[e] = err[e];
-    buf1[e] = 0.0f;
-    buf2[e] = 0.0f;
-  }
   // threads of a row block: the n-1 interior columns, padded to whole
   // wavefronts when the workgroup has room, so that no wave spans two row
   // blocks (lanes one block lower hit other lanes' LDS banks)
@@ -503,19 +516,21 @@ __device__ __forceinline__ void sweep_cb(const StageArgs<float>& a, float* err, 
     i = 1 + ic;
     k0 = 1 + kb * RB;
   }
+  // the pointwise operands from clamped (valid) points, zero where the
+  // thread has no point; issued before the copy-in so that both are in
+  // flight together
 #pragma unroll
   for (int r = 0; r < RB; r++) {
     int k = k0 + r;
     on[r] = active && k <= m - 1;
-    s0v[r] = dtv[r] = rhsv[r] = visv[r] = 0.0f;
-    if (on[r]) {
-      long g = IDX(i, k, p);
-      s0v[r] = err[g];
-      dtv[r] = a.dts[g];
-      rhsv[r] = a.rhs[g];
-      visv[r] = a.vis[g];
-    }
+    const long g = IDX(i, min(k, m - 1), p);
+    const float e0 = err[g], d0 = a.dts[g], r0 = a.rhs[g], v0 = a.vis[g];
+    s0v[r] = on[r] ? e0 : 0.0f;
+    dtv[r] = on[r] ? d0 : 0.0f;
+    rhsv[r] = on[r] ? r0 : 0.0f;
+    visv[r] = on[r] ? v0 : 0.0f;
   }
+  copy_in3<float, SMALL_THREADS, 8>(err, buf0, buf1, buf2, ne);
   const int rmax = m + 3;   // last stored row
   __syncthreads();
   {
@@ -579,7 +594,7 @@ constexpr int STRIP_TX = 64, STRIP_TY = 16, STRIP_RB = 4;
 struct StripArgs {
   const float* S0;    // sweep start (pointwise; stencil source of stage 1)
   const float* Gcur;  // stage j0-1, read when j0 >= 3
-  const float* Gprev; // stage j0-2, read when j0 >= 3
+  const float* Gprev; // stage j0-2, used when j0 >= 3 (an allocated level buffer: always loaded)
   float* Ocur;        // stage j1 on the core columns (+ ghost images)
   float* Oprev;       // stage j1-1 on the core columns; null on the last segment
   int j0, j1;         // stages of this segment (1-based, j1 > j0)
@@ -601,16 +616,8 @@ __global__ void __launch_bounds__(STRIP_TX* STRIP_TY) k_strip_cb(StageArgs<float
   const int tid = threadIdx.y * STRIP_TX + threadIdx.x, nthr = STRIP_TX * STRIP_TY;
   const float* cur_src = (s.j0 == 1) ? s.S0 : s.Gcur;
   const bool has_prev = s.j0 >= 3;
-  for (int e = tid; e < W * rows; e += nthr) {
-    int rr = e / W, lc = e - rr * W;
-    int gc = lo + lc;
-    bool in = gc <= hi;
-    long g = IDX(gc, rr - 3, p);
-    buf[e] = in ? cur_src[g] : 0.0f;
-    buf[bsz + e] = (in && has_prev) ? s.Gprev[g] : 0.0f;
-    buf[2 * bsz + e] = 0.0f;
-  }
-  // this thread's column and row block; pointwise operands in registers
+  // this thread's column and row block; pointwise operands in registers,
+  // loaded from clamped (valid) points and zero where the thread has none
   const int ic = c0 - halo + 3 + threadIdx.x;
   const int k0 = 1 + threadIdx.y * RB;
   float s0v[RB], dtv[RB], rhsv[RB], visv[RB];
@@ -619,13 +626,37 @@ __global__ void __launch_bounds__(STRIP_TX* STRIP_TY) k_strip_cb(StageArgs<float
   for (int r = 0; r < RB; r++) {
     const int k = k0 + r;
     on[r] = ic >= 1 && ic <= n - 1 && k <= m - 1;
-    s0v[r] = dtv[r] = rhsv[r] = visv[r] = 0.0f;
-    if (on[r]) {
-      long g = IDX(ic, k, p);
-      s0v[r] = s.S0[g];
-      dtv[r] = a.dts[g];
-      rhsv[r] = a.rhs[g];
-      visv[r] = a.vis[g];
+    const long g = IDX(min(max(ic, 1), n - 1), min(k, m - 1), p);
+    const float e0 = s.S0[g], d0 = a.dts[g], r0 = a.rhs[g], v0 = a.vis[g];
+    s0v[r] = on[r] ? e0 : 0.0f;
+    dtv[r] = on[r] ? d0 : 0.0f;
+    rhsv[r] = on[r] ? r0 : 0.0f;
+    visv[r] = on[r] ? v0 : 0.0f;
+  }
+  // the strips: each batch's loads (columns clamped to the strip, both
+  // source buffers always allocated) issued before its LDS writes
+  constexpr int SB = 8;
+  const int nfill = W * rows;
+  for (int e0 = tid; e0 < nfill; e0 += SB * nthr) {
+    float vc[SB], vp[SB];
+#pragma unroll
+    for (int u = 0; u < SB; u++) {
+      const int e = min(e0 + u * nthr, nfill - 1);
+      const int rr = e / W, lc = e - rr * W;
+      const long g = IDX(min(lo + lc, hi), rr - 3, p);
+      vc[u] = cur_src[g];
+      vp[u] = s.Gprev[g];
+    }
+#pragma unroll
+    for (int u = 0; u < SB; u++) {
+      const int e = e0 + u * nthr;
+      if (e < nfill) {
+        const int rr = e / W, lc = e - rr * W;
+        const bool in = lo + lc <= hi;
+        buf[e] = in ? vc[u] : 0.0f;
+        buf[bsz + e] = (in && has_prev) ? vp[u] : 0.0f;
+        buf[2 * bsz + e] = 0.0f;
+      }
     }
   }
   const int rmax = m + 3;
