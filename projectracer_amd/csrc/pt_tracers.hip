@@ -82,6 +82,7 @@ __global__ void k_cell_bounds(const int32_t* skey, int64_t n, int ncell, int32_t
 constexpr int PJ_CELLS = 28;     // 28 cells x 9 neighbour nodes = 252 phase-2 threads
 constexpr int PJ_THREADS = 256;
 constexpr int PJ_CHUNK = 512;
+static_assert(PJ_CHUNK % PJ_THREADS == 0, "a chunk is a whole number of tracers per thread");
 __global__ void __launch_bounds__(PJ_THREADS) k_cell_reduce(const double* __restrict__ r, const double* __restrict__ s,
                                                             const double* __restrict__ Tt, const double* __restrict__ Ct,
                                                             const int32_t* __restrict__ perm,
@@ -125,20 +126,40 @@ __global__ void __launch_bounds__(PJ_THREADS) k_cell_reduce(const double* __rest
   const int64_t t0 = range[0] == INT32_MAX ? 0 : range[0], t1 = range[0] == INT32_MAX ? 0 : range[1];
   for (int64_t base = t0; base < t1; base += PJ_CHUNK) {
     const int cnt = (int)min((int64_t)PJ_CHUNK, t1 - base);
-    for (int q = tid; q < cnt; q += PJ_THREADS) {
-      const int32_t j = perm[base + q];
-      const double rv = r[j], sv = s[j];
-      const int key = skey[base + q];
+    // the thread's PJ_Q tracers of the chunk: slot and permutation loads
+    // (clamped into the chunk) for all of them, then their r, s, T, C, so
+    // the chunk waits for two memory round trips
+    constexpr int PJ_Q = PJ_CHUNK / PJ_THREADS;
+    int32_t jq[PJ_Q], kq[PJ_Q];
+#pragma unroll
+    for (int u = 0; u < PJ_Q; u++) {
+      const int q = min(tid + u * PJ_THREADS, cnt - 1);
+      jq[u] = perm[base + q];
+      kq[u] = skey[base + q];
+    }
+    double rq[PJ_Q], sq[PJ_Q], tq[PJ_Q], cq[PJ_Q];
+#pragma unroll
+    for (int u = 0; u < PJ_Q; u++) {
+      rq[u] = r[jq[u]];
+      sq[u] = s[jq[u]];
+      tq[u] = Tt[jq[u]];
+      cq[u] = Ct[jq[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < PJ_Q; u++) {
+      const int q = tid + u * PJ_THREADS;
+      if (q >= cnt) continue;
+      const int key = kq[u];
       const int i = key % (nx + 1), k = key / (nx + 1);
 #pragma unroll
       for (int e = 0; e < 3; e++) {
-        const double tx = 1.0 - fabs(rv - (double)(i + e - 1) * dr) / dr;
+        const double tx = 1.0 - fabs(rq[u] - (double)(i + e - 1) * dr) / dr;
         hx[e][q] = tx < 0.0 ? 0.0 : tx;
-        const double tz = 1.0 - fabs(sv - (double)(k + e - 1) * ds) / ds;
+        const double tz = 1.0 - fabs(sq[u] - (double)(k + e - 1) * ds) / ds;
         hz[e][q] = tz < 0.0 ? 0.0 : tz;
       }
-      sT[q] = Tt[j];
-      sC[q] = Ct[j];
+      sT[q] = tq[u];
+      sC[q] = cq[u];
     }
     __syncthreads();
     if (owner) {
