@@ -65,6 +65,7 @@ def parse():
     ap.add_argument("--no-restart-io", action="store_true",
                     help="skip timing pt_write_restart / pt_read_restart after the headline run")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--workload", default="law", choices=sorted(WORKLOADS))
     ap.add_argument("--paths", default="mixed,fp64",
                     help="GPU paths to time (profiling runs may select one)")
     ap.add_argument("--direct-only", action="store_true",
@@ -75,9 +76,24 @@ def parse():
     return ap.parse_args()
 
 
-def make_case(args, precision):
-    from projectracer_amd import cases, _abi
-    case = cases.thermochem_large(nx=args.nx, nz=args.nz, tpc=args.tpc, ivis=0, tolerance=1e-7)
+# Viscosity laws of the 1024x256 workload.  "law": the strongest T- and
+# C-dependent law the reference's multigrid solves at this size,
+# eta = exp(-ln(30) T + ln(2) C) with filter thresholds 1.2 (the third
+# configuration's own eta = exp(-ln(1e5) T + ln(30) C) makes the reference stop
+# with "Abnormally High Residual" in its first solve), at courant 0.25 (at
+# 0.5 the flow this law drives carries tracers out of the domain within three
+# steps); "isoviscous": eta = 1, courant 0.5.  Both at an L1 relative
+# residual of 1e-7: the reference's FP64 residual does not reach 1e-8 at
+# 1024x256.
+WORKLOADS = {
+    "law": dict(visT=30.0, visC=0.5, delta_visT=1.2, delta_visC=1.2, courant=0.25, tolerance=1e-7),
+    "isoviscous": dict(ivis=0, tolerance=1e-7),
+}
+
+
+def make_case(args, precision, workload=None):
+    from projectracer_amd import cases
+    case = cases.thermochem_large(nx=args.nx, nz=args.nz, tpc=args.tpc, **WORKLOADS[workload or args.workload])
     case.params.precision = precision
     return case
 

@@ -131,9 +131,24 @@ int pt_create(const pt_params* p, void** out) {
   c->device = dev;
   const char* graph_env = std::getenv("PT_GRAPH");
   c->use_graph = !(graph_env && graph_env[0] == '0');
+  const char* filter_env = std::getenv("PT_FILTER_HOST");
+  c->filter_on_device = !(filter_env && filter_env[0] == '1');
   int rc = build_tables(c->P, c->T);
   if (rc != PT_OK) { delete c; return rc; }
   c->P.nx = c->T.nx; c->P.nz = c->T.nz; c->P.grids = c->T.grids;
+  std::memcpy(c->cyc, c->T.multi, sizeof c->cyc);
+  c->cyc_n = c->T.nsteps;
+  if (c->P.precision == PT_PREC_MIXED) {
+    // the mixed path's cycle: a V-cycle whose coarse corrections start from
+    // zero at each descent (PT_MIXED_CYCLE=W|F|V and PT_MIXED_COARSE_ZERO=0|1
+    // select others, e.g. the reference's W-cycle with warm-started coarse
+    // levels)
+    const char* ce = std::getenv("PT_MIXED_CYCLE");
+    const char kind = (ce && (ce[0] == 'W' || ce[0] == 'F' || ce[0] == 'V')) ? ce[0] : 'V';
+    c->cyc_n = cycle_schedule_kind(c->T.grids, kind, c->cyc);
+    const char* cz = std::getenv("PT_MIXED_COARSE_ZERO");
+    c->coarse_zero = !(cz && cz[0] == '0');
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");
     delete c;

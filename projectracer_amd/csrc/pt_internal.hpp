@@ -48,6 +48,7 @@ struct Tables {
   int64_t ntr;
 };
 int build_tables(const pt_params& p, Tables& t);   // returns PT_* status
+int cycle_schedule_kind(int grids, char kind, int* multi);   // 'W', 'V' or 'F'; returns nsteps
 
 // ---- stencil constants passed to kernels by value ----------------------
 struct Stencil {
@@ -116,6 +117,14 @@ struct Context {
   double* cellsum;          // 27 per cell: (T*w, C*w, w) for the 3x3 neighbour nodes
   double *TxS, *CxS;        // halo-5 scratch fields for the filters
   int32_t* d_flags;         // [0]=tracer outside domain
+  // device-driven filter loops: control (done flag, pass count), its pinned
+  // host mirror, and the ring of per-check smoothness maxima (FP64 bit
+  // patterns); filter_on_device is false when PT_FILTER_HOST=1 at pt_create
+  int32_t* d_fctl = nullptr;
+  int32_t* h_fctl = nullptr;
+  unsigned long long* d_fsm = nullptr;
+  bool filter_on_device = true;
+  int hint_comp = 0, hint_temp = 0;   // pass counts of the previous C / T filter loops
   // reductions
   double* red;              // block partials
   double* red_out;          // device scalars (pinned mirror below)
@@ -139,6 +148,14 @@ struct Context {
   };
   bool use_graph = true;
   int graph_state = 0;
+  // cycle schedule the solve follows: the reference's W-cycle in the FP64
+  // path; the mixed path may take a cheaper one (V or F, the reference's own
+  // alternatives in PS/multigrid_cycle.f90), since it is judged by the same
+  // FP64 stop test.  coarse_zero: the mixed path starts each coarse-level
+  // correction from zero when the cycle descends to it.
+  int cyc[52] = {0};
+  int cyc_n = 0;
+  bool coarse_zero = false;
   std::vector<CycleGraph> cycle_graphs;
   // mixed precision work
   double* dres64;           // FP64 residual for defect correction

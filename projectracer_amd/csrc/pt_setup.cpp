@@ -58,6 +58,30 @@ static int cycle_schedule(int grids, int* multi) {
   return n;
 }
 
+// The cycle the mixed path may follow instead: 'V' the standard V-cycle, 'F'
+// the standard F-cycle (both listed, commented out, beside the W-cycle at
+// PS/multigrid_cycle.f90:43-96), 'W' the W-cycle.  Where a kind has no
+// distinct form for `grids` levels it is the W-cycle.
+int cycle_schedule_kind(int grids, char kind, int* multi) {
+  static const int f4[] = {1, 2, 3, 4, 3, 4, 3, 2, 3, 4, 3, 2};
+  static const int f5[] = {1, 2, 3, 4, 5, 4, 5, 4, 3, 4, 5, 4, 3, 2, 3, 4, 5, 4, 3, 2};
+  static const int f6[] = {1, 2, 3, 4, 5, 6, 5, 6, 5, 4, 5, 6, 5, 4, 3, 4, 5, 6, 5, 4, 3, 2, 3, 4, 5, 6, 5, 4, 3, 2};
+  int n = 0;
+  if (kind == 'V' && grids >= 3) {
+    for (int l = 1; l <= grids; l++) multi[++n] = l;
+    for (int l = grids - 1; l >= 2; l--) multi[++n] = l;
+  } else if (kind == 'F' && grids >= 4) {
+    const int* m = grids == 4 ? f4 : grids == 5 ? f5 : f6;
+    n = grids == 4 ? 12 : grids == 5 ? 20 : 30;
+    for (int p = 1; p <= n; p++) multi[p] = m[p - 1];
+  } else {
+    return cycle_schedule(grids, multi);
+  }
+  multi[0] = multi[n];
+  multi[n + 1] = multi[1];
+  return n;
+}
+
 // FD_coefficients: PS/main.f90:499-618 (Fornberg's recursion for nodes
 // 0,1,-1,2,-2,3,-3 and derivative orders up to 4)
 static void fd_tables(Tables& t) {

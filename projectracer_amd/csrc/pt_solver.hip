@@ -1604,9 +1604,9 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, bool l1_sum) {
   int nb = grd.x * grd.y;
   launch_residual<R>(c, grd, blk, a, res, c.red);
   if (l1_sum) {
-    // the L1 norm of the residual to pinned host memory (read after a sync)
+    // the L1 norm of the residual into device memory; level1_residual copies
+    // it to the host after the cycle's launches (or graph replay)
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, nb, (int)RED_SUM_ABS, c.red_out);
-    PT_HIP(hipMemcpyAsync(c.h_red, c.red_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   }
   (void)p;
   return PT_OK;
@@ -1778,15 +1778,20 @@ __global__ void k_fill_f32(float* v, long n) {
 // schedule entry p of the cycle (PS/iterative_solver.f90:143-340): the
 // restriction into its level when the schedule descends, the visit (RKC1
 // smoothing + residual, or the coarse solve), and the prolongation when it
-// ascends next.  Launches only: a level-1 visit leaves the residual's L1 norm
-// in pinned host memory, read by level1_residual after a sync.
+// ascends next.  Kernel launches only (so a captured cycle holds kernel nodes
+// and nothing else): a level-1 visit leaves the residual's L1 norm in
+// device memory (red_out), which level1_residual copies to the host.
 static int visit(Context& c, int p, bool mixed) {
   const Tables& t = c.T;
+  const int* multi = c.cyc;
   dim3 b16(64, 4);
   const double sz = c.P.Vbc == 0 ? -1.0 : 1.0;
-  const int l = t.multi[p];
+  const int l = multi[p];
   Level& L = c.L[l];
-  if (t.multi[p - 1] < t.multi[p] && l >= 2) {
+  if (multi[p - 1] < multi[p] && l >= 2 && mixed && c.coarse_zero && l < t.grids)
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((L.nelem + 255) / 256)), dim3(256), 0, c.stream, L.ferr,
+                       (long)L.nelem);
+  if (multi[p - 1] < multi[p] && l >= 2) {
     // compute_RHS(l) = restrict(residual(l-1))
     Level& F = c.L[l - 1];
     if (!mixed)
@@ -1817,7 +1822,6 @@ static int visit(Context& c, int p, bool mixed) {
       launch_residual<double>(c, grd, blk, a, L.res, c.red);
       int nb = grd.x * grd.y;
       hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, nb, (int)RED_SUM_ABS, c.red_out);
-      PT_HIP(hipMemcpyAsync(c.h_red, c.red_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     } else {
       PT_TRY(iterate_stream_t<float>(c, l, p, true, false));
     }
@@ -1832,7 +1836,7 @@ static int visit(Context& c, int p, bool mixed) {
     else
       PT_TRY(launch_coarse_apply_wg(c, L.frhs, L.ferr, L.g.n, L.g.m, L.g.pitch, (float)sz));
   }
-  if (l > 1 && t.multi[p + 1] < t.multi[p]) {
+  if (l > 1 && multi[p + 1] < multi[p]) {
     Level& F = c.L[l - 1];
     dim3 gp = grid2(F.g.n - 1, F.g.m - 1, b16);
     if (!mixed)
@@ -1851,6 +1855,7 @@ static int visit(Context& c, int p, bool mixed) {
 // residual_mag(1) of the level-1 visit just enqueued, and the abnormal-
 // residual stop (PS/iterative_solver.f90:438-447)
 static int level1_residual(Context& c, double* resmag1) {
+  PT_HIP(hipMemcpyAsync(c.h_red, c.red_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   PT_HIP(hipStreamSynchronize(c.stream));
   const LevelGeom& g = c.L[1].g;
   *resmag1 = c.h_red[0] / (double)(g.n - 2) / (double)(g.m - 2);
@@ -1882,7 +1887,7 @@ static void restore_roles(Context& c, const std::vector<void*>& v) {
 }
 
 // visits 2..nsteps of a cycle and visit 1 of the next.  After one eager run
-// the sequence is captured as a HIP graph (no memset nodes) per starting
+// the sequence is captured as a HIP graph (kernel nodes only) per starting
 // arrangement of the rotating buffers (at most MAX_CYCLE_GRAPHS) and
 // replayed; the graphs' kernel arguments (buffers, level constants) do not
 // change between cycles or solves of a context.  PT_GRAPH=0 at pt_create,
@@ -1892,7 +1897,7 @@ static constexpr size_t MAX_CYCLE_GRAPHS = 4;
 static int rest_and_next_level1(Context& c, bool mixed) {
   const Tables& t = c.T;
   auto enqueue = [&]() -> int {
-    for (int p = 2; p <= t.nsteps; p++) PT_TRY(visit(c, p, mixed));
+    for (int p = 2; p <= c.cyc_n; p++) PT_TRY(visit(c, p, mixed));
     return visit(c, 1, mixed);
   };
   if (!c.use_graph || c.timing || c.graph_state < 0) return enqueue();
@@ -1930,8 +1935,11 @@ static int rest_and_next_level1(Context& c, bool mixed) {
   buffer_roles(c, cg.after);
   e = hipGraphInstantiate(&cg.exec, cg.graph, nullptr, nullptr, 0);
   if (e != hipSuccess) {
+    // nothing of the capture ran: back to the starting arrangement, eager from now on
     (void)hipGraphDestroy(cg.graph);
-    PT_HIP(e);
+    restore_roles(c, before);
+    c.graph_state = -1;
+    return enqueue();
   }
   c.cycle_graphs.push_back(std::move(cg));
   PT_HIP(hipGraphLaunch(c.cycle_graphs.back().exec, c.stream));
@@ -1960,7 +1968,10 @@ int multigrid(Context& c) {
       PT_TRY(to_f32(c, L.v.vis, L.fvis, L.nelem)); PT_TRY(to_f32(c, L.v.vxxmzz, L.fvxm, L.nelem));
       PT_TRY(to_f32(c, L.v.vxz, L.fvxz, L.nelem)); PT_TRY(to_f32(c, L.v.vx, L.fvx, L.nelem));
       PT_TRY(to_f32(c, L.v.vz, L.fvz, L.nelem)); PT_TRY(to_f32(c, L.dts, L.fdts, L.nelem));
-      if (l >= 2) PT_TRY(to_f32(c, L.err, L.ferr, L.nelem));
+      // coarse corrections of the mixed path start each solve from zero
+      if (l >= 2)
+        hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((L.nelem + 255) / 256)), dim3(256), 0, c.stream, L.ferr,
+                           (long)L.nelem);
     }
   }
   int q = 1;
@@ -1971,7 +1982,7 @@ int multigrid(Context& c) {
   PT_TRY(level1_residual(c, &resmag1));
   while (!((resmag1 / c.res_ref1) <= c.P.tolerance)) {
     if (q + 1 > c.P.max_cycles) {
-      for (int p = 2; p <= t.nsteps; p++) PT_TRY(visit(c, p, mixed));
+      for (int p = 2; p <= c.cyc_n; p++) PT_TRY(visit(c, p, mixed));
       c.cycles++;
       set_error("q > max_cycles = " + std::to_string(c.P.max_cycles));
       return PT_ERR_MAX_CYCLES;

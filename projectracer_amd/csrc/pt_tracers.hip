@@ -381,11 +381,221 @@ static void filter_pass(Context& c, const double* T0, double* out, int h, double
                      D1[5], D2[1], D2[2], D2[3], D2[4], D2[5]);
 }
 
+// ---------------------------------------------------------------------------
+// Filter loops driven on the device.  The reference checks the field's
+// smoothness before every pass and stops at the first check below the
+// threshold (PS/smoother.f90:582-672).  Here the host enqueues checks and
+// passes in batches without waiting: a check folds its block maxima into one
+// slot with an integer atomicMax on the bit patterns (order-free and exact:
+// every |jump| is non-negative), and the pass after it reads the slot, and
+// either records the pass count and raises the `done` flag, or filters.
+// Once `done` is set every later check and pass of the batch returns at
+// once, so the field and the count are those of the reference's loop.
+// Passes alternate between two buffers; each writes its field's ghost
+// images itself (the reference's enforceBCs / enforceBCs_comp after the pass).
+// ---------------------------------------------------------------------------
+constexpr int FSLOTS = 64;   // ring of smoothness slots (a pass zeroes the next check's)
+
+__global__ void k_filter_init(int32_t* ctl, unsigned long long* sm) {
+  const int t = threadIdx.x;
+  if (t < 4) ctl[t] = 0;
+  if (t < FSLOTS) sm[t] = 0ull;
+}
+
+// the check before pass j: max 5-neighbour jump of f viewed with halo h and
+// pitch p (smoothness, PS/smoother.f90:75-95), folded into sm[slot]
+__global__ void k_smooth_check(const double* f, int nx, int nz, int h, int p, const int32_t* ctl,
+                               unsigned long long* sm, int slot) {
+  if (ctl[0]) return;
+  __shared__ double red[256];
+  int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x, k = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  double d = 0.0;
+  if (i <= nx - 1 && k <= nz - 1) {
+    auto F = [&](int a, int b) { return f[(long)(b + h) * p + (a + h)]; };
+    double t = F(i, k);
+    double a1 = fabs(F(i, k - 1) - t), a2 = fabs(F(i + 1, k - 1) - t), a3 = fabs(F(i + 1, k) - t),
+           a4 = fabs(F(i, k + 1) - t), a5 = fabs(F(i + 1, k + 1) - t);
+    d = fmax(fmax(fmax(fmax(a1, a2), a3), a4), a5);
+  }
+  red[tid] = d;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) red[tid] = fmax(red[tid], red[tid + st]);
+    __syncthreads();
+  }
+  if (tid == 0) atomicMax(&sm[slot], (unsigned long long)__double_as_longlong(red[0]));
+}
+
+struct FilterArgs {
+  int nx, nz, h, p;
+  double dr, ds, dr2, ds2;
+  double D1[5], D2[5];
+  int clamp01;          // C: clamp to [0,1], symmetric ghosts (enforceBCs_comp)
+  // T (clamp01 == 0): enforceBCs with the isothermal / insulating walls
+  int Tbot;
+  double Tb, Tt, two_over_d20, H, Htype;
+  const double* Cvis;   // halo 5, for the heating term of the T wall ghosts
+};
+
+// pass j of the loop: in -> out (one Richardson pass, smoother_space at every
+// node 0..nx x 0..nz, then the ghost images), unless the check before it
+// succeeded
+__global__ void k_filter_step(const double* in, double* out, FilterArgs a, int32_t* ctl, unsigned long long* sm,
+                              int slot, int next_slot, int pass, double cdts, double delta) {
+  if (ctl[0]) return;
+  const double diff = __longlong_as_double((long long)sm[slot]);
+  const bool first = blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0;
+  if (diff < delta) {
+    if (first) {
+      ctl[1] = pass - 1;   // passes made before the successful check
+      ctl[0] = 1;
+    }
+    return;
+  }
+  if (first) sm[next_slot] = 0ull;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  const int nx = a.nx, nz = a.nz, h = a.h, p = a.p;
+  if (i > nx || k > nz) return;
+  auto F = [&](int u, int v) { return in[(long)(v + h) * p + (u + h)]; };
+  auto W = [&](int u, int v, double val) { out[(long)(v + h) * p + (u + h)] = val; };
+  double sr = 0.0, ss = 0.0, srr = 0.0, sss = 0.0;
+  for (int q = -2; q <= 2; q++) sr = sr + a.D1[q + 2] * F(i + q, k);
+  for (int q = -2; q <= 2; q++) ss = ss + a.D1[q + 2] * F(i, k + q);
+  for (int q = -2; q <= 2; q++) srr = srr + a.D2[q + 2] * F(i + q, k);
+  for (int q = -2; q <= 2; q++) sss = sss + a.D2[q + 2] * F(i, k + q);
+  double Tr = sr / a.dr, Ts = ss / a.ds, Trr = srr / a.dr2, Tss = sss / a.ds2;
+  double Tx = Tr / 1.0, Tz = Ts / 1.0;
+  double Txx = (Trr - Tx * 0.0) / 1.0, Tzz = (Tss - Tz * 0.0) / 1.0;
+  double temp = Txx + Tzz;
+  double y = F(i, k) + cdts * temp;
+  // the columns whose ghost images equal this node's column: itself, and
+  // its mirror beyond a side wall when it lies within h of that wall
+  int cols[2] = {i, i};
+  int ncol = 1;
+  if (i >= 1 && i <= h) cols[ncol++] = -i;
+  else if (i >= nx - h && i <= nx - 1) cols[ncol++] = 2 * nx - i;
+  if (a.clamp01) {
+    if (y < 0.0) y = 0.0;
+    if (y > 1.0) y = 1.0;
+    for (int c = 0; c < ncol; c++) {
+      W(cols[c], k, y);
+      if (k >= 1 && k <= h) W(cols[c], -k, y);
+      if (k >= nz - h && k <= nz - 1) W(cols[c], 2 * nz - k, y);
+    }
+    return;
+  }
+  // T: the wall rows hold the isothermal values (the insulating bottom keeps
+  // its filtered value); ghost rows are 2 T_wall + (2/D2(0)) H ds^2 - mirror
+  // (isothermal) or the mirror (insulating bottom)
+  if (k == 0 && a.Tbot == 0) y = a.Tb;
+  if (k == nz) y = a.Tt;
+  for (int c = 0; c < ncol; c++) {
+    const int col = cols[c];
+    W(col, k, y);
+    if (k >= 1 && k <= h) {
+      const double cb = a.Cvis[(long)5 * (nx + 11) + (col + 5)];
+      const double hb = a.Htype * cb + a.H * (1.0 - (0.0 + cb));
+      W(col, -k, a.Tbot == 0 ? 2.0 * a.Tb + a.two_over_d20 * hb * a.ds2 - y : y);
+    }
+    if (k >= nz - h && k <= nz - 1) {
+      const double ct = a.Cvis[(long)(nz + 5) * (nx + 11) + (col + 5)];
+      const double ht = a.Htype * ct + a.H * (1.0 - (0.0 + ct));
+      W(col, 2 * nz - k, 2.0 * a.Tt + a.two_over_d20 * ht * a.ds2 - y);
+    }
+  }
+}
+
+// one device-driven filter loop on x0 (x1: the second buffer).  cdts_of(j)
+// is pass j's c*dts; on return x0 holds the filtered field with its ghosts
+// and *passes the number of passes made before the successful check
+static int filter_loop(Context& c, double* x0, double* x1, const FilterArgs& fa, int check_h, int check_p,
+                       double delta, double (*cdts_of)(const Context&, int), int* passes, int* hint) {
+  dim3 b16(64, 4);
+  dim3 gchk = grid2(c.T.nx, c.T.nz - 1, b16);
+  dim3 gflt = grid2(c.T.nx + 1, c.T.nz + 1, b16);
+  unsigned long long* sm = c.d_fsm;
+  hipLaunchKernelGGL(k_filter_init, dim3(1), dim3(64), 0, c.stream, c.d_fctl, sm);
+  const int cap = 100000;
+  // the first batch covers the previous call's pass count (consecutive
+  // projections need similar counts), later ones double
+  int j = 0, batch = std::min(std::max(*hint + 2, 8), 1024);
+  for (;;) {
+    for (int b = 0; b < batch; b++) {
+      j++;
+      const double* in = (j & 1) ? x0 : x1;
+      double* out = (j & 1) ? x1 : x0;
+      const int slot = (j - 1) % FSLOTS, nslot = j % FSLOTS;
+      hipLaunchKernelGGL(k_smooth_check, gchk, b16, 0, c.stream, in, c.T.nx, c.T.nz, check_h, check_p, c.d_fctl, sm,
+                         slot);
+      hipLaunchKernelGGL(k_filter_step, gflt, b16, 0, c.stream, in, out, fa, c.d_fctl, sm, slot, nslot, j,
+                         cdts_of(c, j), delta);
+    }
+    PT_HIP(hipMemcpyAsync(c.h_fctl, c.d_fctl, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+    PT_HIP(hipStreamSynchronize(c.stream));
+    if (c.h_fctl[0]) break;
+    if (j > cap) {
+      set_error("filter did not reach its smoothness threshold in 100000 passes");
+      return PT_ERR_INVALID;
+    }
+    batch = std::min(2 * batch, 256);
+  }
+  *passes = c.h_fctl[1];
+  *hint = *passes;
+  if (*passes & 1) {
+    // the last pass wrote the second buffer
+    const size_t bytes = sizeof(double) * (size_t)(c.T.nx + 1 + 2 * fa.h) * (c.T.nz + 1 + 2 * fa.h);
+    PT_HIP(hipMemcpyAsync(x0, x1, bytes, hipMemcpyDeviceToDevice, c.stream));
+  }
+  return PT_OK;
+}
+
+static FilterArgs filter_args(const Context& c, int h, int clamp01) {
+  FilterArgs a;
+  a.nx = c.T.nx; a.nz = c.T.nz; a.h = h; a.p = c.T.nx + 1 + 2 * h;
+  a.dr = c.T.dr; a.ds = c.T.ds; a.dr2 = c.T.dr2; a.ds2 = c.T.ds2;
+  for (int q = 0; q < 5; q++) { a.D1[q] = c.T.D1[q + 1]; a.D2[q] = c.T.D2[q + 1]; }
+  a.clamp01 = clamp01;
+  a.Tbot = c.P.Tbot; a.Tb = c.Tbot_iso; a.Tt = c.Ttop_iso; a.two_over_d20 = 2.0 / c.T.D2[3];
+  a.H = c.P.H; a.Htype = c.P.Htype; a.Cvis = c.Cvis;
+  return a;
+}
+
+// C passes: the first ten with the Chebyshev-ordered steps that damp the
+// unresolved frequencies first, then c = 1 (PS/smoother.f90:596-617)
+static double cdts_comp(const Context& c, int j) {
+  double cc = 1.0;
+  if (j <= 10) {
+    const int Niter = 10, ntemp = Niter - j + 1;
+    const double lb = -5.0, la = -1.0;
+    cc = 2.0 / (-lb - la + (lb - la) * std::cos((double)(2 * ntemp - 1) * 3.1415926535897932 / (double)(2 * Niter)));
+  }
+  return cc * (c.T.dts * c.P.courantC);
+}
+static double cdts_temp(const Context& c, int) { return c.T.dts * c.P.courantT; }
+
+static int smoother_time_host(Context& c);
+
 // smoother_time (C): PS/smoother.f90:582-636
 static int smoother_time(Context& c) {
   size_t b5 = sizeof(double) * (size_t)(c.T.nx + 11) * (c.T.nz + 11);
   PT_HIP(hipMemcpyAsync(c.Cvis, c.Cnew, b5, hipMemcpyDeviceToDevice, c.stream));
   PT_HIP(hipMemcpyAsync(c.Cbuoy, c.Cnew, b5, hipMemcpyDeviceToDevice, c.stream));
+  if (!c.filter_on_device) return smoother_time_host(c);
+  // the reference passes the halo-5 C array to smoothness, whose dummy has
+  // the halo-3 shape (PS/smoother.f90:79, :602): sequence association makes
+  // it read the C storage with the halo-3 row length
+  int P = 0;
+  PT_TRY(filter_loop(c, c.Cvis, c.TxS, filter_args(c, 5, 1), 3, c.T.nx + 7, c.T.delta_C, cdts_comp, &P, &c.hint_comp));
+  // n_comp counts the outer iterations after the first ten-pass one
+  c.n_comp = P < 10 ? 0 : P - 9;
+  return PT_OK;
+}
+
+// smoother_time with a host round trip per check, the loop as the reference
+// writes it (kept as the cross-check of the device-driven loop)
+static int smoother_time_host(Context& c) {
+  size_t b5 = sizeof(double) * (size_t)(c.T.nx + 11) * (c.T.nz + 11);
   int Niter = 10, nn = 0;
   double lb = -5.0, la = -1.0;
   bool trigger = false;
@@ -427,6 +637,12 @@ static int smoother_time_T(Context& c) {
   PT_HIP(hipMemcpyAsync(c.Tvis, c.Tratio, b3, hipMemcpyDeviceToDevice, c.stream));
   PT_HIP(hipMemcpyAsync(c.Tbuoy, c.Tratio, b3, hipMemcpyDeviceToDevice, c.stream));
   if (c.P.RaT == 0.0 || c.P.ivis == 0) return PT_OK;
+  if (c.filter_on_device) {
+    int P = 0;
+    PT_TRY(filter_loop(c, c.Tvis, c.CxS, filter_args(c, 3, 0), 3, c.T.nx + 7, c.T.delta_T, cdts_temp, &P, &c.hint_temp));
+    c.n_temp = P;
+    return PT_OK;
+  }
   int n = 0;
   for (;;) {
     if (n > 100000) {
@@ -1112,6 +1328,9 @@ int tracers_alloc(Context& c) {
   DALLOC(c.nempty, sizeof(int32_t) * ncell);
   DALLOC(c.cellsum, sizeof(double) * 27 * (size_t)ncell);
   DALLOC(c.d_flags, sizeof(int32_t) * 4);
+  DALLOC(c.d_fctl, sizeof(int32_t) * 4);
+  DALLOC(c.d_fsm, sizeof(unsigned long long) * FSLOTS);
+  PT_HIP(hipHostMalloc((void**)&c.h_fctl, sizeof(int32_t) * 4, hipHostMallocDefault));
   hipLaunchKernelGGL(k_iota, dim3((unsigned)((c.T.ntr + 255) / 256)), dim3(256), 0, 0, c.iota, c.T.ntr);
   c.sort_bits = 1;
   while ((1 << c.sort_bits) <= ncell) c.sort_bits++;
@@ -1125,9 +1344,10 @@ int tracers_free(Context& c) {
   void* ps[] = {c.SF, c.Tratio, c.T0, c.Tvis, c.Tbuoy, c.u, c.w, c.src, c.cond_r, c.cond_s, c.Cnew, c.C0, c.Cvis,
                 c.Cbuoy, c.cond, c.Textend, c.TxS, c.CxS, c.itab, c.ctab, c.dtab, c.r, c.s, c.Tt, c.Ct, c.r0, c.s0, c.T0t,
                 c.r1, c.s1, c.T1t, c.r2, c.s2, c.T2t, c.cellkey, c.skey, c.iota, c.perm, c.tid, c.tid_tmp, c.cstart, c.cend,
-                c.nempty, c.cellsum, c.d_flags, c.sort_tmp};
+                c.nempty, c.cellsum, c.d_flags, c.sort_tmp, c.d_fctl, c.d_fsm};
   for (void* p : ps)
     if (p) (void)hipFree(p);
+  if (c.h_fctl) (void)hipHostFree(c.h_fctl);
   return PT_OK;
 }
 
