@@ -101,7 +101,7 @@ struct Context {
   // coarse grid: band LU (DGBTF2) and the dense operator G = A^-1 diag(RowB)
   // restricted to interior unknowns, built from the LU once per factorization
   double *Mc, *RowB, *Bc; int32_t* ipiv; int32_t* d_info;
-  double* Gc; double* Gwork; int nci;
+  double* Gc; int nci;
   // direct solver (iterate=0): equilibrated band LU of the whole grid, row
   // factors, right-hand side / solution vector
   double *dAB = nullptr, *dRow = nullptr, *dB = nullptr; int32_t* dipiv = nullptr;

@@ -718,45 +718,56 @@ __global__ void __launch_bounds__(STRIP_TX* STRIP_TY) k_strip_cb(StageArgs<float
 
 // ---------------------------------------------------------------------------
 // coarsest grid as a dense operator: G(:,q) = A^-1 (RowB(q) e_q) from the band
-// LU factors, interior unknowns only; one thread per column q, work vectors
-// stored row-interleaved so the threads' accesses coalesce
+// LU factors (DGBTRS's forward and backward sweeps on a unit vector),
+// interior unknowns only.  One wavefront per column q, the work vector in
+// LDS: each sweep step's band updates are independent across rows and
+// spread over the lanes; every element sees the same operations in the same
+// order as the sequential sweep.
 // ---------------------------------------------------------------------------
-__global__ void k_coarse_inverse(const double* ab, const int32_t* ipiv, const double* RowB, double* Gwork, double* G,
-                                 int n, int kl, int ku, int nxc, int nzc) {
+__global__ void __launch_bounds__(64) k_coarse_inverse(const double* ab, const int32_t* ipiv, const double* RowB,
+                                                       double* G, int n, int kl, int ku, int nxc, int nzc) {
+  extern __shared__ double Y[];   // n entries
   const int nci = (nxc - 1) * (nzc - 1);
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nci) return;
+  const int t = blockIdx.x, lane = threadIdx.x;
   const int ld = 2 * kl + ku + 1, kd = ku + kl + 1, N = nzc + 1;
-  int iq = 1 + t % (nxc - 1), kq = 1 + t / (nxc - 1);
-  int q = N * iq + kq + 1;   // 1-based unknown index
+  const int iq = 1 + t % (nxc - 1), kq = 1 + t / (nxc - 1);
+  const int q = N * iq + kq + 1;   // 1-based unknown index
 #define AB(r, c) ab[(long)((c) - 1) * ld + ((r) - 1)]
-#define Y(pp) Gwork[(long)((pp) - 1) * nci + t]
-  for (int pp = 1; pp <= n; pp++) Y(pp) = 0.0;
-  Y(q) = RowB[q - 1];
+  for (int pp = lane; pp < n; pp += 64) Y[pp] = 0.0;
+  __syncthreads();
+  if (lane == 0) Y[q - 1] = RowB[q - 1];
+  __syncthreads();
   for (int j = 1; j <= n - 1; j++) {
-    int lm = min(kl, n - j);
-    int l = ipiv[j - 1];
-    if (l != j) { double tmp = Y(l); Y(l) = Y(j); Y(j) = tmp; }
-    double temp = -1.0 * Y(j);
-    for (int u = 1; u <= lm; u++) Y(j + u) = Y(j + u) + AB(kd + u, j) * temp;
+    const int lm = min(kl, n - j);
+    const int l = ipiv[j - 1];
+    if (l != j && lane == 0) {
+      double tmp = Y[l - 1];
+      Y[l - 1] = Y[j - 1];
+      Y[j - 1] = tmp;
+    }
+    __syncthreads();
+    const double temp = -1.0 * Y[j - 1];
+    for (int u = 1 + lane; u <= lm; u += 64) Y[j + u - 1] = Y[j + u - 1] + AB(kd + u, j) * temp;
+    __syncthreads();
   }
   const int kk = kl + ku, kp1 = kk + 1;
   for (int j = n; j >= 1; j--) {
-    double bj = Y(j);
+    const double bj = Y[j - 1];
     if (bj != 0.0) {
-      int l = kp1 - j;
-      double x = bj / AB(kp1, j);
-      Y(j) = x;
-      int lo = max(1, j - kk);
-      for (int i = j - 1; i >= lo; i--) Y(i) = Y(i) - x * AB(l + i, j);
+      const int l = kp1 - j;
+      const double x = bj / AB(kp1, j);
+      const int lo = max(1, j - kk);
+      __syncthreads();   // every lane has read Y(j) before it is replaced
+      if (lane == 0) Y[j - 1] = x;
+      for (int i = j - 1 - lane; i >= lo; i -= 64) Y[i - 1] = Y[i - 1] - x * AB(l + i, j);
     }
+    __syncthreads();
   }
-  for (int pi = 0; pi < nci; pi++) {
-    int ip = 1 + pi % (nxc - 1), kp = 1 + pi / (nxc - 1);
-    G[(long)t * nci + pi] = Y(N * ip + kp + 1);
+  for (int pi = lane; pi < nci; pi += 64) {
+    const int ip = 1 + pi % (nxc - 1), kp = 1 + pi / (nxc - 1);
+    G[(long)t * nci + pi] = Y[N * ip + kp];
   }
 #undef AB
-#undef Y
 }
 
 // coarse correction: err(p) = sum_q G(p,q) rhs(q), fixed order in q
@@ -1357,8 +1368,10 @@ static void launch_residual(Context& c, dim3 grd, dim3 blk, const StageArgs<R>& 
 }
 
 // points per thread of the single-workgroup smoother, 0 when the level is
-// too large for it (beyond 4 points, 2 with the full variable-viscosity
-// operator, the per-thread state no longer fits in 128 VGPRs)
+// too large for it: beyond 4 points (isoviscous) the per-thread state no
+// longer fits in 128 VGPRs; with the full variable-viscosity operator one
+// workgroup is slower than a launch per stage beyond 2 points (FP32 level
+// 127x31: 248 us per 35-stage sweep against 35 x 4.9 us)
 static int small_ppt(int n, int m, bool iv0) {
   const int npts = (n - 1) * (m - 1);
   const int pmax = iv0 ? 4 : 2;
@@ -1739,8 +1752,8 @@ static int coarse_factor(Context& c) {
     set_error("coarse-grid band LU: zero pivot at column " + std::to_string(info));
     return PT_ERR_SINGULAR_COARSE;
   }
-  hipLaunchKernelGGL(k_coarse_inverse, dim3((c.nci + 63) / 64), dim3(64), 0, c.stream, c.Mc, c.ipiv, c.RowB,
-                     c.Gwork, c.Gc, t.ngc, t.kl, t.ku, G.g.n, G.g.m);
+  hipLaunchKernelGGL(k_coarse_inverse, dim3(c.nci), dim3(64), sizeof(double) * t.ngc, c.stream, c.Mc, c.ipiv,
+                     c.RowB, c.Gc, t.ngc, t.kl, t.ku, G.g.n, G.g.m);
   return PT_OK;
 }
 
@@ -2062,7 +2075,6 @@ int solver_alloc(Context& c) {
     DALLOC(c.Mc, mb); DALLOC(c.RowB, sizeof(double) * t.ngc); DALLOC(c.Bc, sizeof(double) * t.ngc);
     c.nci = (t.nxg[t.grids] - 1) * (t.nzg[t.grids] - 1);
     DALLOC(c.Gc, sizeof(double) * (size_t)c.nci * c.nci);
-    DALLOC(c.Gwork, sizeof(double) * (size_t)c.nci * t.ngc);
     DALLOC(c.ipiv, sizeof(int32_t) * t.ngc);
   }
   c.red_cap = 1 << 20;
@@ -2090,7 +2102,7 @@ int solver_free(Context& c) {
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
-  void* ps[] = {c.Mc, c.RowB, c.Bc, c.ipiv, c.d_info, c.red, c.red_out, c.dres64, c.Gc, c.Gwork, c.stats_rows,
+  void* ps[] = {c.Mc, c.RowB, c.Bc, c.ipiv, c.d_info, c.red, c.red_out, c.dres64, c.Gc, c.stats_rows,
                 c.strain_buf};
   for (void* p : ps)
     if (p) (void)hipFree(p);

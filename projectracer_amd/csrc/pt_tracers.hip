@@ -403,28 +403,34 @@ __global__ void k_filter_init(int32_t* ctl, unsigned long long* sm) {
 }
 
 // the check before pass j: max 5-neighbour jump of f viewed with halo h and
-// pitch p (smoothness, PS/smoother.f90:75-95), folded into sm[slot]
-__global__ void k_smooth_check(const double* f, int nx, int nz, int h, int p, const int32_t* ctl,
-                               unsigned long long* sm, int slot) {
+// pitch p (smoothness, PS/smoother.f90:75-95) over i = 0..nx-1, k = 1..nz-1,
+// folded into sm[slot].  A fixed grid of SC_BLOCKS workgroups strides over
+// the points, so one launch makes SC_BLOCKS atomics on the slot (one per
+// workgroup launched per 256 points made the atomics the kernel's cost).
+constexpr int SC_BLOCKS = 256, SC_THREADS = 256;
+__global__ void __launch_bounds__(SC_THREADS) k_smooth_check(const double* f, int nx, int nz, int h, int p,
+                                                             const int32_t* ctl, unsigned long long* sm, int slot) {
   if (ctl[0]) return;
-  __shared__ double red[256];
-  int tid = threadIdx.y * blockDim.x + threadIdx.x;
-  int i = blockIdx.x * blockDim.x + threadIdx.x, k = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  __shared__ double red[SC_THREADS / 64];
+  const long npts = (long)nx * (nz - 1);
+  auto F = [&](int a, int b) { return f[(long)(b + h) * p + (a + h)]; };
   double d = 0.0;
-  if (i <= nx - 1 && k <= nz - 1) {
-    auto F = [&](int a, int b) { return f[(long)(b + h) * p + (a + h)]; };
-    double t = F(i, k);
-    double a1 = fabs(F(i, k - 1) - t), a2 = fabs(F(i + 1, k - 1) - t), a3 = fabs(F(i + 1, k) - t),
-           a4 = fabs(F(i, k + 1) - t), a5 = fabs(F(i + 1, k + 1) - t);
-    d = fmax(fmax(fmax(fmax(a1, a2), a3), a4), a5);
+  for (long e = (long)blockIdx.x * SC_THREADS + threadIdx.x; e < npts; e += (long)SC_BLOCKS * SC_THREADS) {
+    const int k = 1 + (int)(e / nx), i = (int)(e - (long)(k - 1) * nx);
+    const double t = F(i, k);
+    const double a1 = fabs(F(i, k - 1) - t), a2 = fabs(F(i + 1, k - 1) - t), a3 = fabs(F(i + 1, k) - t),
+                 a4 = fabs(F(i, k + 1) - t), a5 = fabs(F(i + 1, k + 1) - t);
+    d = fmax(d, fmax(fmax(fmax(fmax(a1, a2), a3), a4), a5));
   }
-  red[tid] = d;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) d = fmax(d, __shfl_xor(d, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = d;
   __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (tid < st) red[tid] = fmax(red[tid], red[tid + st]);
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < SC_THREADS / 64; w++) m = fmax(m, red[w]);
+    atomicMax(&sm[slot], (unsigned long long)__double_as_longlong(m));
   }
-  if (tid == 0) atomicMax(&sm[slot], (unsigned long long)__double_as_longlong(red[0]));
 }
 
 struct FilterArgs {
@@ -512,7 +518,6 @@ __global__ void k_filter_step(const double* in, double* out, FilterArgs a, int32
 static int filter_loop(Context& c, double* x0, double* x1, const FilterArgs& fa, int check_h, int check_p,
                        double delta, double (*cdts_of)(const Context&, int), int* passes, int* hint) {
   dim3 b16(64, 4);
-  dim3 gchk = grid2(c.T.nx, c.T.nz - 1, b16);
   dim3 gflt = grid2(c.T.nx + 1, c.T.nz + 1, b16);
   unsigned long long* sm = c.d_fsm;
   hipLaunchKernelGGL(k_filter_init, dim3(1), dim3(64), 0, c.stream, c.d_fctl, sm);
@@ -526,8 +531,8 @@ static int filter_loop(Context& c, double* x0, double* x1, const FilterArgs& fa,
       const double* in = (j & 1) ? x0 : x1;
       double* out = (j & 1) ? x1 : x0;
       const int slot = (j - 1) % FSLOTS, nslot = j % FSLOTS;
-      hipLaunchKernelGGL(k_smooth_check, gchk, b16, 0, c.stream, in, c.T.nx, c.T.nz, check_h, check_p, c.d_fctl, sm,
-                         slot);
+      hipLaunchKernelGGL(k_smooth_check, dim3(SC_BLOCKS), dim3(SC_THREADS), 0, c.stream, in, c.T.nx, c.T.nz, check_h,
+                         check_p, c.d_fctl, sm, slot);
       hipLaunchKernelGGL(k_filter_step, gflt, b16, 0, c.stream, in, out, fa, c.d_fctl, sm, slot, nslot, j,
                          cdts_of(c, j), delta);
     }
