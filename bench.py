@@ -3,23 +3,28 @@
 
 A step is one iteration of the reference's time loop (PS/main.f90:245-299):
 one multigrid solve + velocities, then energy_time (RK4: three more solves,
-four tracer sweeps, five projections).  The workload is the 1024x256 case
-(lambda=4, Ra_T=1e7, Ra_C=3e6, two-layer C at z=0.3, 50 tracers per node =
-13,171,250 tracers) with the isoviscous rheology and an L1 relative residual
-tolerance of 1e-7: the reference's own multigrid stalls above 1e-8 at this
-resolution and diverges with the T/C-dependent viscosity (flang build of
-the reference, same inputs), so this is the largest 1024x256 case the
-reference itself solves.
+four tracer sweeps, five projections with their filters).
+
+Workload (headline): the 1024x256 thermochemical case (lambda=4, Ra_T=1e7,
+Ra_C=3e6, two-layer C at z=0.3, T perturbation seed 2, 50 tracers per node
+= 13,171,250 tracers) with the T- and C-dependent viscosity
+eta = exp(-ln(30) T + ln(2) C), filter thresholds delta_visT = delta_visC =
+1.2, courant 0.25 and an L1 relative residual tolerance of 1e-7.  This is
+the strongest eta(T,C) law the reference's own multigrid solves at this
+size: with the third configuration's eta = exp(-ln(1e5) T + ln(30) C) it
+stops with "Abnormally High Residual" in its first solve
+(tests/golden/ref_divergence1024.npz), and its FP64 residual does not reach
+1e-8 at 1024x256.  The isoviscous case (eta = 1, courant 0.5) is timed
+beside it.
 
 Prints one JSON line.  Paths timed:
-  * headline: PT_PREC_MIXED (FP32 RKC1 smoothing inside FP64 defect correction)
-  * plain FP64 GPU path (same algorithm as the reference, bit-faithful kernels)
-  * cpu_baseline: the reference program (oracle/_ref, flang -O2 -fopenmp) on
-    the host cores, one warm time step of the same workload.
+  * headline: PT_PREC_MIXED -- FP32 RKC1 V-cycles inside FP64 defect
+    correction, stopped by the reference's FP64 L1 test
+  * plain FP64 GPU path (the reference's W-cycle, bit-faithful kernels)
+  * cpu_baseline: the reference program (oracle/_ref, flang -O3 -fopenmp) on
+    the host cores, a window of warm time steps of the same workload.
 Beside them, `direct_solver`: the direct banded solver (iterate=0,
-PS/momentum.f90:319-653) at the reference's shipped 100x100 size -- the band
-LU's build once, then solve_SF_equation per solve -- with the reference's
-own timers for its shipped run quoted for context.
+PS/momentum.f90:319-653) at the reference's shipped 100x100 size.
 """
 from __future__ import annotations
 
@@ -37,18 +42,31 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes of one level-1 RKC1 stage per interior point: read S1
-# (stencil source), S0, S2, eta, RHS, dt_stream and write Y; the general
-# operator also reads eta_xx-eta_zz, eta_xz, eta_x, eta_z (96 B in FP64), the
-# isoviscous one does not (56 B)
-RKC_BYTES_PER_POINT_F64 = {"isoviscous": 56, "general": 96}
-# tracer pipeline, per tracer: the four RK4 stage kernels move 96+112+112+88 B
-# (positions, temperature, step-start copies, stage registers); one
-# projection reads r, s, T, C once (32 B)
+# Algorithmic bytes of one RKC1 stage per interior point with the general
+# (variable-viscosity) operator: read S1 (the stencil source, once), S0, S2,
+# eta, eta_xx-eta_zz, eta_xz, eta_x, eta_z, RHS, dt_stream and write Y: 11
+# words.  FP32 (the mixed path's smoother) 44 B, FP64 88 B.  The isoviscous
+# operator reads 4 words fewer.
+STAGE_WORDS = {"general": 11, "isoviscous": 7}
+# Tracer pipeline per tracer and step: the four RK4 stage kernels move
+# 96+112+112+88 B; each of the five projections reads r, s, T, C (32 B).
 RK_BYTES_PER_TRACER_STEP = 408
-RK2_BYTES_PER_TRACER = 112   # stage 2: read r1, s1, r, s, T, r0, s0, T0; write r, s, T, r2, s2, T2
+RK2_BYTES_PER_TRACER = 112
 PROJECT_BYTES_PER_TRACER = 32
-TRAFFIC_FILE = ROOT / "profiles" / "k_rk2_1024x256_traffic.json"
+PROJECTIONS_PER_STEP = 5
+# One filter pass over the node field (halo included) reads the field once
+# and writes it once; its check reads it once: 24 B per node.
+FILTER_BYTES_PER_NODE = 24
+
+# Viscosity laws of the 1024x256 workload (see the module docstring).
+WORKLOADS = {
+    "law": dict(visT=30.0, visC=0.5, delta_visT=1.2, delta_visC=1.2, courant=0.25, tolerance=1e-7),
+    "isoviscous": dict(ivis=0, tolerance=1e-7),
+}
+LAW_TEXT = {
+    "law": "eta = exp(-ln(30) T + ln(2) C), filter thresholds 1.2, courant 0.25, L1 tol 1e-7",
+    "isoviscous": "eta = 1, courant 0.5, L1 tol 1e-7",
+}
 
 
 def parse():
@@ -59,13 +77,16 @@ def parse():
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--nz", type=int, default=256)
     ap.add_argument("--tpc", type=int, default=50)
-    ap.add_argument("--event-steps", type=int, default=3,
+    ap.add_argument("--event-steps", type=int, default=2,
                     help="steps run after the timed ones with per-kernel HIP events")
+    ap.add_argument("--workload", default="law", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-secondary", action="store_true", help="skip the isoviscous line")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=2,
+                    help="warm reference steps timed after its first loop iteration")
     ap.add_argument("--no-restart-io", action="store_true",
                     help="skip timing pt_write_restart / pt_read_restart after the headline run")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", default="law", choices=sorted(WORKLOADS))
     ap.add_argument("--paths", default="mixed,fp64",
                     help="GPU paths to time (profiling runs may select one)")
     ap.add_argument("--direct-only", action="store_true",
@@ -76,21 +97,6 @@ def parse():
     return ap.parse_args()
 
 
-# Viscosity laws of the 1024x256 workload.  "law": the strongest T- and
-# C-dependent law the reference's multigrid solves at this size,
-# eta = exp(-ln(30) T + ln(2) C) with filter thresholds 1.2 (the third
-# configuration's own eta = exp(-ln(1e5) T + ln(30) C) makes the reference stop
-# with "Abnormally High Residual" in its first solve), at courant 0.25 (at
-# 0.5 the flow this law drives carries tracers out of the domain within three
-# steps); "isoviscous": eta = 1, courant 0.5.  Both at an L1 relative
-# residual of 1e-7: the reference's FP64 residual does not reach 1e-8 at
-# 1024x256.
-WORKLOADS = {
-    "law": dict(visT=30.0, visC=0.5, delta_visT=1.2, delta_visC=1.2, courant=0.25, tolerance=1e-7),
-    "isoviscous": dict(ivis=0, tolerance=1e-7),
-}
-
-
 def make_case(args, precision, workload=None):
     from projectracer_amd import cases
     case = cases.thermochem_large(nx=args.nx, nz=args.nz, tpc=args.tpc, **WORKLOADS[workload or args.workload])
@@ -98,13 +104,19 @@ def make_case(args, precision, workload=None):
     return case
 
 
-def time_gpu(args, precision, restart_io=False):
+def _interior_sf(sim):
+    import numpy as np
+    return np.asarray(sim.field("SF"))[3:3 + sim.nz + 1, 3:3 + sim.nx + 1].copy()
+
+
+def time_gpu(args, precision, workload=None, restart_io=False, events=True):
     from projectracer_amd import Simulation, _abi
-    case = make_case(args, precision)
+    case = make_case(args, precision, workload)
     sim = Simulation(case.params)
     sim.load_state(case)
     sim.multigrid()
     sim.compute_velocities()
+    sf_first = _interior_sf(sim)
     for _ in range(args.warmup):
         sim.energy_time()
         sim.multigrid()
@@ -116,18 +128,29 @@ def time_gpu(args, precision, restart_io=False):
     sim.synchronize()
     t1 = time.perf_counter()
     inf1 = sim.info()
-    # kernel durations from a separate, shorter run with HIP events around the
-    # timed kernels: the events add host work per launch, so they stay out of
-    # the steps/s measurement
-    sim.enable_kernel_timing(True)
-    sim.run_steps(args.event_steps)
-    sim.synchronize()
-    kms, klaunch = sim.kernel_time(_abi.PT_K_RKC_STAGE_L1)
-    tms, tlaunch = sim.kernel_time(_abi.PT_K_TRACER_STAGE)
-    pms, plaunch = sim.kernel_time(_abi.PT_K_PROJECT)
-    r2ms, r2launch = sim.kernel_time(_abi.PT_K_TRACER_RK2)
-    sim.enable_kernel_timing(False)
-    restart = None
+    out = dict(seconds=t1 - t0, ms_per_step=1000.0 * (t1 - t0) / args.steps,
+               solves=inf1["solves"] - inf0["solves"], cycles=inf1["cycles"] - inf0["cycles"],
+               nx=sim.nx, nz=sim.nz, ntr=sim.ntr, last_residual_ratio=inf1["residual"] / inf1["res_ref"],
+               n_comp=inf1["n_comp"], n_temp=inf1["n_temp"], sf_first=sf_first, events=None, restart=None)
+    if events and args.event_steps > 0:
+        # per-kernel durations from a separate run of a few more steps with
+        # HIP events on the library stream around the timed launches: the
+        # events add host work per launch (and keep the cycles eager), so they
+        # stay out of the steps/s measurement
+        i0 = sim.info()
+        sim.enable_kernel_timing(True)
+        sim.run_steps(args.event_steps)
+        sim.synchronize()
+        i1 = sim.info()
+        ev = {"steps": args.event_steps, "cycles": i1["cycles"] - i0["cycles"]}
+        for name, k in (("stage_l1", _abi.PT_K_RKC_STAGE_L1), ("stage_coarse", _abi.PT_K_RKC_STAGE_COARSE),
+                        ("tracer_stages", _abi.PT_K_TRACER_STAGE), ("rk2", _abi.PT_K_TRACER_RK2),
+                        ("projection", _abi.PT_K_PROJECT), ("filter", _abi.PT_K_FILTER),
+                        ("solve", _abi.PT_K_SOLVE)):
+            ms, n = sim.kernel_time(k)
+            ev[name] = dict(ms=ms, launches=n, work=sim.kernel_work(k))
+        sim.enable_kernel_timing(False)
+        out["events"] = ev
     if restart_io:
         # checkpoint / resume of the whole device state in the reference's files
         with tempfile.TemporaryDirectory(prefix="ptrst_") as td:
@@ -137,18 +160,8 @@ def time_gpu(args, precision, restart_io=False):
             nbytes = sum(f.stat().st_size for f in Path(td).iterdir())
             sim.read_restart(td, case.Tbot_iso, case.Ttop_iso)
             t4 = time.perf_counter()
-        restart = dict(write_s=t3 - t2, read_s=t4 - t3, bytes=nbytes,
-                       write_GBs=nbytes / (t3 - t2) / 1e9, read_GBs=nbytes / (t4 - t3) / 1e9)
-    out = dict(
-        seconds=t1 - t0,
-        ms_per_step=1000.0 * (t1 - t0) / args.steps,
-        solves=inf1["solves"] - inf0["solves"],
-        cycles=inf1["cycles"] - inf0["cycles"],
-        rkc_ms=kms, rkc_launches=klaunch, tracer_ms=tms, tracer_launches=tlaunch,
-        project_ms=pms, project_launches=plaunch, rk2_ms=r2ms, rk2_launches=r2launch,
-        nx=sim.nx, nz=sim.nz, ntr=sim.ntr, last_residual_ratio=inf1["residual"] / inf1["res_ref"],
-        restart=restart,
-    )
+        out["restart"] = dict(write_s=t3 - t2, read_s=t4 - t3, bytes=nbytes,
+                              write_GBs=nbytes / (t3 - t2) / 1e9, read_GBs=nbytes / (t4 - t3) / 1e9)
     sim.close()
     return out
 
@@ -195,22 +208,34 @@ def time_direct(nx=100, nz=100, solves=20):
     return out
 
 
+def host_cores():
+    """Cores this process may run on, capped by OMP_NUM_THREADS when set
+    (the share of the host given to one GPU)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
 def cpu_baseline(args):
-    """The reference (flang build) on the host: inject the same state, run
-    two loop iterations, report the second (warm) one."""
-    ref = ROOT / "oracle" / "_ref" / "ref_driver"
+    """The reference (flang -O3 build) on the host: inject the same state,
+    run one loop iteration untimed, then time `--cpu-steps` more with its own
+    per-iteration wall clock (omp_get_wtime)."""
+    ref = ROOT / "oracle" / "_ref" / "fast" / "ref_driver"
     if not ref.exists():
         return None
     sys.path.insert(0, str(ROOT / "tests"))
+    import numpy as np
     from _support import write_inputs_nml
     from projectracer_amd import _abi
     case = make_case(args, _abi.PT_PREC_FP64)
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = args.cpu_threads or host_cores()
+    nrun = 1 + args.cpu_steps
     with tempfile.TemporaryDirectory(prefix="ptcpu_") as td:
         tdp = Path(td)
         write_inputs_nml(tdp / "inputs.nml", case.params)
-        (tdp / "driver.nml").write_text("&driver\nnrun=2, inject=1, dump_every=-1\n/\n")
-        import numpy as np
+        (tdp / "driver.nml").write_text(f"&driver\nnrun={nrun}, inject=1, dump_every=-1\n/\n")
         with open(tdp / "inject.bin", "wb") as fh:
             fh.write(np.array([case.Tbot_iso, case.Ttop_iso], dtype=np.float64).tobytes())
             fh.write(np.ascontiguousarray(case.T_nodes).tobytes())
@@ -220,17 +245,63 @@ def cpu_baseline(args):
                    OMP_PLACES="cores")
         t0 = time.perf_counter()
         proc = subprocess.run(["bash", "-c", f"ulimit -s unlimited && exec {ref}"], cwd=tdp, env=env,
-                              capture_output=True, text=True, timeout=900)
+                              capture_output=True, text=True, timeout=1500)
         wall = time.perf_counter() - t0
     steps = [float(m.group(1)) for m in re.finditer(r"driver: step done\s+\d+\s+\S+\s+\S+\s+(\S+)", proc.stdout)]
-    if proc.returncode != 0 or len(steps) < 2:
+    if proc.returncode != 0 or len(steps) < nrun:
         return {"error": (proc.stdout + proc.stderr)[-400:]}
-    sec = steps[1]
+    window = steps[1:nrun]
+    sec = sum(window) / len(window)
     return dict(value=1.0 / sec, unit="time-steps/s", cores=threads, kind="reference",
-                sample=f"reference program (flang -O2 -fopenmp, {threads} threads) on the same "
-                       f"{case.nx}x{case.nz} state with {case.ntr} tracers: second loop iteration "
-                       f"(one solve + one RK4 energy_time) = {sec:.2f} s; whole run {wall:.0f} s",
+                sample=f"reference program (flang -O3 -fopenmp, {threads} threads) on the same "
+                       f"{case.nx}x{case.nz} state ({LAW_TEXT[args.workload]}) with {case.ntr} tracers: "
+                       f"{len(window)} warm loop iterations after the first (one solve + one RK4 "
+                       f"energy_time each) = {', '.join(f'{s:.2f}' for s in window)} s; whole run {wall:.0f} s",
                 seconds_per_step=sec, tracer_updates_per_s=case.ntr / sec)
+
+
+def stage_roofline(ev, words, bytes_per_word, label):
+    """Roofline of the RKC1 stage kernel from the event steps: all its
+    launches (level 1 and the coarse levels swept a launch per stage), the
+    algorithmic bytes of each launch being its interior points times the
+    stage's words."""
+    l1, co = ev["stage_l1"], ev["stage_coarse"]
+    ms = l1["ms"] + co["ms"]
+    n = l1["launches"] + co["launches"]
+    bpp = words * bytes_per_word
+    total_bytes = bpp * (l1["work"] + co["work"])
+    if n == 0 or ms <= 0:
+        return None
+    avg_us = 1e3 * ms / n
+    achieved = total_bytes / n / (avg_us * 1e-6) / 1e9
+    l1_us = 1e3 * l1["ms"] / max(1, l1["launches"])
+    co_us = 1e3 * co["ms"] / max(1, co["launches"])
+    return {
+        "kernel": label, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "avg_launch_us": avg_us, "launches": n, "bytes_per_launch": total_bytes / n,
+        "bytes_per_point": bpp, "ms_per_step": ms / ev["steps"],
+        "level1": {"launches": l1["launches"], "avg_launch_us": l1_us,
+                   "bytes_per_launch": bpp * l1["work"] / max(1, l1["launches"]),
+                   "achieved_GBs": bpp * l1["work"] / max(1, l1["launches"]) / (l1_us * 1e-6) / 1e9},
+        "coarse_levels": {"launches": co["launches"], "avg_launch_us": co_us,
+                          "bytes_per_launch": bpp * co["work"] / max(1, co["launches"]),
+                          "achieved_GBs": bpp * co["work"] / max(1, co["launches"]) / (co_us * 1e-6) / 1e9
+                          if co["launches"] else None},
+    }
+
+
+def step_bytes(ev, r, words, bytes_per_word):
+    """Algorithmic bytes of one step: every stage launch's points x words,
+    the tracer pipeline, and the filter passes of the last projection
+    (n_comp, plus the first nine C passes when it is nonzero, and n_temp)
+    over the halo-3 node field, for each of the five projections."""
+    solver = words * bytes_per_word * (ev["stage_l1"]["work"] + ev["stage_coarse"]["work"]) / ev["steps"]
+    tracers = (RK_BYTES_PER_TRACER_STEP + PROJECTIONS_PER_STEP * PROJECT_BYTES_PER_TRACER) * r["ntr"]
+    nodes = (r["nx"] + 7) * (r["nz"] + 7)
+    passes = r["n_comp"] + 9 * (r["n_comp"] > 0) + r["n_temp"]
+    filters = PROJECTIONS_PER_STEP * passes * FILTER_BYTES_PER_NODE * nodes
+    return solver + tracers + filters, dict(solver=solver, tracers=tracers, filters=filters)
 
 
 def main():
@@ -255,34 +326,29 @@ def main():
         # profiling run: time the selected path only, no JSON line
         for name in sorted(paths):
             prec = _abi.PT_PREC_MIXED if name == "mixed" else _abi.PT_PREC_FP64
-            r = time_gpu(args, prec)
+            r = time_gpu(args, prec, events=False)
             print(f"{name}: {r['ms_per_step']:.2f} ms/step, {r['cycles']} cycles", flush=True)
         return
+    import numpy as np
     mixed = time_gpu(args, _abi.PT_PREC_MIXED, restart_io=not args.no_restart_io)
     fp64 = time_gpu(args, _abi.PT_PREC_FP64)
     steps_s = args.steps / mixed["seconds"]
     fp64_steps_s = args.steps / fp64["seconds"]
-    interior = (mixed["nx"] - 1) * (mixed["nz"] - 1)
-    # roofline: the dominant HBM-streaming kernel of the step, the RK4 tracer
-    # stage k_rk2 (one launch per step, 13.2 M tracers), timed with HIP events
-    # on the library stream in the instrumented steps of the headline path.  The
-    # solver's levels >= 2 are latency-bound and the level-1 stage kernel is
-    # too short (~5 us) for per-launch events to resolve; it is reported
-    # beside it.
+    # the mixed path's first solve against the FP64 path's first solve from
+    # the same injected state (both stop at the same L1 relative residual)
+    den = float(np.max(np.abs(fp64["sf_first"])))
+    mixed_err = float(np.max(np.abs(mixed["sf_first"] - fp64["sf_first"]))) / den
+    general = args.workload != "isoviscous"
+    words = STAGE_WORDS["general" if general else "isoviscous"]
+    ev = mixed["events"]
+    roof = stage_roofline(ev, words, 4, "k_stage<float, %s> (RKC1 stage of the mixed path's FP32 V-cycle: "
+                          "level 1 and the coarse levels swept one launch per stage)"
+                          % ("general operator" if general else "isoviscous"))
+    sb, sb_parts = step_bytes(ev, mixed, words, 4)
     ntr = mixed["ntr"]
-    rk2_us = 1e3 * mixed["rk2_ms"] / max(1, mixed["rk2_launches"])
-    bytes_per_launch = RK2_BYTES_PER_TRACER * ntr
-    achieved = bytes_per_launch / (rk2_us * 1e-6) / 1e9
-    traffic = None
-    if TRAFFIC_FILE.exists():
-        # HBM bytes per launch of the same kernel from the committed rocprofv3
-        # FETCH_SIZE / WRITE_SIZE runs, scaled by the copy-kernel calibration
-        traffic = json.loads(TRAFFIC_FILE.read_text()).get("hbm_bytes_per_launch")
-    rk = fp64
-    l1_us = 1e3 * rk["rkc_ms"] / max(1, rk["rkc_launches"])
-    l1_bytes = RKC_BYTES_PER_POINT_F64["isoviscous"] * interior
-    rk_ms_step = mixed["tracer_ms"] / args.event_steps
-    pj_ms_call = mixed["project_ms"] / max(1, mixed["project_launches"])
+    rk_ms_step = ev["tracer_stages"]["ms"] / ev["steps"]
+    rk2_us = 1e3 * ev["rk2"]["ms"] / max(1, ev["rk2"]["launches"])
+    pj_ms_call = ev["projection"]["ms"] / max(1, ev["projection"]["launches"])
     result = {
         "metric": "time-steps/s & tracer-updates/s at 1024x256, mixed+learned vs plain FP64 GPU",
         "value": steps_s,
@@ -297,30 +363,41 @@ def main():
         "dtype": "f64+f32 (FP32 smoothing in FP64 defect correction)",
         "data": "synthetic",
         "config": {
-            "workload": f"thermochem_{mixed['nx']}x{mixed['nz']} isoviscous, tol 1e-7",
-            "nx": mixed["nx"], "nz": mixed["nz"], "aspect": 4.0, "tracers": mixed["ntr"],
-            "RaT": 1e7, "RaC": 3e6, "levels": 6, "rkc_stages": 35, "tolerance": 1e-7,
-            "initial_guess": "warm start (learned guess not built)",
+            "workload": f"thermochem_{mixed['nx']}x{mixed['nz']}, {LAW_TEXT[args.workload]}",
+            "nx": mixed["nx"], "nz": mixed["nz"], "aspect": 4.0, "tracers": ntr,
+            "RaT": 1e7, "RaC": 3e6, "levels": 6, "rkc_stages": 35, **WORKLOADS[args.workload],
+            "initial_guess": "warm start from the previous psi",
+            "mixed_cycle": "V-cycle, coarse corrections from zero at each descent",
         },
-        "tracer_updates_per_s": steps_s * mixed["ntr"],
+        "tracer_updates_per_s": steps_s * ntr,
         "vcycles_per_solve": mixed["cycles"] / max(1, mixed["solves"]),
+        "mixed_fp64_oracle_error": {
+            "psi_first_solve_rel_max": mixed_err,
+            "note": "max|psi_mixed - psi_FP64| / max|psi_FP64| after the first solve from the same state; "
+                    "both stop at the same FP64 L1 relative residual",
+            "last_residual_ratio": mixed["last_residual_ratio"],
+        },
         "fp64_gpu": {
             "time_steps_per_s": fp64_steps_s, "ms_per_step": fp64["ms_per_step"],
             "tracer_updates_per_s": fp64_steps_s * fp64["ntr"],
             "vcycles_per_solve": fp64["cycles"] / max(1, fp64["solves"]),
+            "cycle": "W-cycle (the reference's)",
         },
         "speedup_vs_fp64_gpu": steps_s / fp64_steps_s,
-        "roofline": {
-            "kernel": "k_rk2 (RK4 tracer stage 2: interpolation of u, w, source + stage update)",
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "avg_launch_us": rk2_us, "launches": mixed["rk2_launches"],
-            "bytes_per_launch": bytes_per_launch,
+        "roofline": roof,
+        "step_roofline": {
+            "bytes_per_step": sb, "parts": sb_parts, "ms_per_step": mixed["ms_per_step"],
+            "achieved": sb / (mixed["ms_per_step"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": sb / (mixed["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
-        "solver_stage_l1_fp64": {
-            "kernel": "k_stage<double, isoviscous> (level-1 RKC1 stage, FP64 path)",
-            "bytes_per_launch": l1_bytes, "avg_launch_us_events": l1_us, "launches": rk["rkc_launches"],
-            "achieved_GBs_events": l1_bytes / (l1_us * 1e-6) / 1e9,
+        "breakdown_ms_per_step_events": {
+            "solves": ev["solve"]["ms"] / ev["steps"],
+            "stage_launches": (ev["stage_l1"]["ms"] + ev["stage_coarse"]["ms"]) / ev["steps"],
+            "filters": ev["filter"]["ms"] / ev["steps"],
+            "projections": ev["projection"]["ms"] / ev["steps"],
+            "tracer_rk_stages": rk_ms_step,
+            "cycles_per_step": ev["cycles"] / ev["steps"],
+            "note": "HIP events with the cycles launched eagerly; each span includes its waits",
         },
         "tracer_roofline": {
             "rk4_stages": {
@@ -328,22 +405,33 @@ def main():
                 "achieved_GBs": RK_BYTES_PER_TRACER_STEP * ntr / (rk_ms_step * 1e-3) / 1e9,
                 "frac": RK_BYTES_PER_TRACER_STEP * ntr / (rk_ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
             },
+            "k_rk2": {
+                "bytes_per_launch": RK2_BYTES_PER_TRACER * ntr, "avg_launch_us": rk2_us,
+                "achieved_GBs": RK2_BYTES_PER_TRACER * ntr / (rk2_us * 1e-6) / 1e9,
+                "frac": RK2_BYTES_PER_TRACER * ntr / (rk2_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+            },
             "projection": {
                 "bytes_per_tracer_call": PROJECT_BYTES_PER_TRACER, "ms_per_call": pj_ms_call,
                 "achieved_GBs": PROJECT_BYTES_PER_TRACER * ntr / (pj_ms_call * 1e-3) / 1e9,
                 "frac": PROJECT_BYTES_PER_TRACER * ntr / (pj_ms_call * 1e-3) / 1e9 / HBM_PEAK_GBS,
             },
         },
-        "kernel_share_fp64": {
-            "rkc_stage_l1_ms_per_step": fp64["rkc_ms"] / args.event_steps,
-            "tracer_rk_ms_per_step": fp64["tracer_ms"] / args.event_steps,
-            "projection_ms_per_step": fp64["project_ms"] / args.event_steps,
-        },
     }
     if mixed["restart"]:
         # print_restart + restart=1 read of the whole state (Trestart, restart_tracers) to a
         # temporary directory on the host
         result["restart_io"] = mixed["restart"]
+    if not args.no_secondary and args.workload != "isoviscous":
+        im = time_gpu(args, _abi.PT_PREC_MIXED, workload="isoviscous", events=False)
+        i64 = time_gpu(args, _abi.PT_PREC_FP64, workload="isoviscous", events=False)
+        result["isoviscous"] = {
+            "workload": f"thermochem_{im['nx']}x{im['nz']}, {LAW_TEXT['isoviscous']}",
+            "mixed_time_steps_per_s": args.steps / im["seconds"], "mixed_ms_per_step": im["ms_per_step"],
+            "mixed_vcycles_per_solve": im["cycles"] / max(1, im["solves"]),
+            "fp64_time_steps_per_s": args.steps / i64["seconds"], "fp64_ms_per_step": i64["ms_per_step"],
+            "fp64_vcycles_per_solve": i64["cycles"] / max(1, i64["solves"]),
+            "speedup_vs_fp64_gpu": i64["seconds"] / im["seconds"],
+        }
     result["direct_solver"] = time_direct()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args)

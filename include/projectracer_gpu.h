@@ -236,13 +236,22 @@ int pt_synchronize(void* ctx);
 /* Enable per-kernel HIP-event timing of the dominant kernels on the
  * context's stream.  While enabled, pt_kernel_time reports the accumulated
  * device time (ms) and launch count of kernel `which` (PT_K_*). */
-/*   PT_K_RKC_STAGE_L1  level-1 RKC1 stage launches of the solver
- *   PT_K_TRACER_STAGE  the four RK4 tracer stage kernels of energy_time
- *   PT_K_PROJECT       one projection (binning, sort, sums, node combine)
- *   PT_K_TRACER_RK2    the second RK4 tracer stage kernel alone */
-enum { PT_K_RKC_STAGE_L1 = 0, PT_K_TRACER_STAGE = 1, PT_K_PROJECT = 2, PT_K_TRACER_RK2 = 3, PT_K_COUNT = 4 };
+/*   PT_K_RKC_STAGE_L1     level-1 RKC1 stage launches of the solver
+ *   PT_K_TRACER_STAGE     the four RK4 tracer stage kernels of energy_time
+ *   PT_K_PROJECT          one projection (binning, sort, sums, node combine)
+ *   PT_K_TRACER_RK2       the second RK4 tracer stage kernel alone
+ *   PT_K_RKC_STAGE_COARSE RKC1 stage launches on levels >= 2 (the levels
+ *                         swept one launch per stage)
+ *   PT_K_FILTER           one filter loop of a projection (smoother_time or
+ *                         smoother_time_T, its batches and waits included)
+ *   PT_K_SOLVE            one multigrid call (set-up and cycles) */
+enum { PT_K_RKC_STAGE_L1 = 0, PT_K_TRACER_STAGE = 1, PT_K_PROJECT = 2, PT_K_TRACER_RK2 = 3,
+       PT_K_RKC_STAGE_COARSE = 4, PT_K_FILTER = 5, PT_K_SOLVE = 6, PT_K_COUNT = 7 };
 int pt_enable_kernel_timing(void* ctx, int32_t on);
 int pt_kernel_time(void* ctx, int32_t which, double* ms, int64_t* launches);
+/* Units of work of the timed launches of `which`: interior grid points for
+ * the stage launches (PT_K_RKC_STAGE_*), 0 for the others. */
+int pt_kernel_work(void* ctx, int32_t which, int64_t* units);
 
 /* Last error message (thread-local, never NULL). */
 const char* pt_last_error(void);

@@ -38,6 +38,9 @@ PT_K_RKC_STAGE_L1 = 0
 PT_K_TRACER_STAGE = 1
 PT_K_PROJECT = 2
 PT_K_TRACER_RK2 = 3
+PT_K_RKC_STAGE_COARSE = 4
+PT_K_FILTER = 5
+PT_K_SOLVE = 6
 
 ERROR_TEXT = {
     PT_ERR_TRACER_OUTSIDE: "Tracer advected outside of domain - stopping",
@@ -161,6 +164,7 @@ def load_library(path: os.PathLike | None = None):
         "pt_synchronize": (C.c_int, [vp]),
         "pt_enable_kernel_timing": (C.c_int, [vp, C.c_int32]),
         "pt_kernel_time": (C.c_int, [vp, C.c_int32, dp, C.POINTER(C.c_int64)]),
+        "pt_kernel_work": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_int64)]),
         "pt_last_error": (C.c_char_p, []),
     }
     for name, (res, args) in sig.items():

@@ -22,9 +22,10 @@ static void timer_flush(KTimer& t) {
   t.launches += t.n;
   t.n = 0;
 }
-void timer_begin(Context& c, int which) {
+void timer_begin(Context& c, int which, int64_t units) {
   KTimer& t = c.kt[which];
   if (t.n == 1024) timer_flush(t);
+  t.work += units;
   (void)hipEventRecord(t.beg[t.n], c.stream);
 }
 void timer_end(Context& c, int which) {
@@ -400,7 +401,7 @@ int pt_enable_kernel_timing(void* p, int32_t on) {
   Context* c = ctx_of(p);
   if (!c) return PT_ERR_INVALID;
   PT_HIP(hipStreamSynchronize(c->stream));
-  for (auto& t : c->kt) { timer_flush(t); t.ms = 0.0; t.launches = 0; }
+  for (auto& t : c->kt) { timer_flush(t); t.ms = 0.0; t.launches = 0; t.work = 0; }
   c->timing = on != 0;
   return PT_OK;
 }
@@ -412,6 +413,13 @@ int pt_kernel_time(void* p, int32_t which, double* ms, int64_t* launches) {
   timer_flush(c->kt[which]);
   if (ms) *ms = c->kt[which].ms;
   if (launches) *launches = c->kt[which].launches;
+  return PT_OK;
+}
+
+int pt_kernel_work(void* p, int32_t which, int64_t* units) {
+  Context* c = ctx_of(p);
+  if (!c || !units || which < 0 || which >= PT_K_COUNT) return PT_ERR_INVALID;
+  *units = c->kt[which].work;
   return PT_OK;
 }
 

@@ -83,6 +83,7 @@ struct KTimer {
   int n = 0;
   double ms = 0.0;
   int64_t launches = 0;
+  int64_t work = 0;   // units of work of the timed launches (interior points of stage launches)
 };
 
 struct Context {
@@ -229,7 +230,7 @@ int reduce_to_host(Context& c, const double* d, int n, int m, int pitch, int off
 __global__ void k_reduce_final(const double* partial, int nb, int op, double* out);
 
 // timing helpers
-void timer_begin(Context& c, int which);
+void timer_begin(Context& c, int which, int64_t units = 0);
 void timer_end(Context& c, int which);
 
 }  // namespace pt

@@ -1527,7 +1527,8 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, bool l1_sum) {
   R* err = fp32 ? (R*)L.ferr : (R*)L.err;
   R* B = fp32 ? (R*)L.fB : (R*)L.bufB;
   R* Cb = fp32 ? (R*)L.fC : (R*)L.bufC;
-  bool timed = (l == 1) && c.timing;
+  const bool timed = c.timing;
+  const int tk = l == 1 ? PT_K_RKC_STAGE_L1 : PT_K_RKC_STAGE_COARSE;
   const size_t lds = 3 * L.nelem * sizeof(R);
   const int ppt = small_ppt(g.n, g.m, c.P.ivis == 0);
   if constexpr (sizeof(R) == 4) {
@@ -1582,9 +1583,9 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, bool l1_sum) {
     StageArgs<R> a = stage_base<R>(c, l, fp32);
     // stage 1: S1 = S0 + mewt1*c*dt*F(S0) -> B
     a.Sx = err; a.S0 = err; a.S2 = err; a.Y = B; a.stage1 = 1; a.cf = (R)(c.T.mewt[1] * cc);
-    if (timed) timer_begin(c, PT_K_RKC_STAGE_L1);
+    if (timed) timer_begin(c, tk, (int64_t)(g.n - 1) * (g.m - 1));
     launch_stage<R>(c, grd, blk, a);
-    if (timed) timer_end(c, PT_K_RKC_STAGE_L1);
+    if (timed) timer_end(c, tk);
     // rotating buffers: S1 in `cur`, S2 in `prev`; Y goes to the S2 buffer
     // except at stage 2 (S2 aliases S0) and at the last stage (Y -> err)
     R* cur = B;
@@ -1599,9 +1600,9 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, bool l1_sum) {
       else if (prev == err) out = spare;
       else out = prev;
       a.Y = out;
-      if (timed) timer_begin(c, PT_K_RKC_STAGE_L1);
+      if (timed) timer_begin(c, tk, (int64_t)(g.n - 1) * (g.m - 1));
       launch_stage<R>(c, grd, blk, a);
-      if (timed) timer_end(c, PT_K_RKC_STAGE_L1);
+      if (timed) timer_end(c, tk);
       if (j < s) {
         R* np = cur;
         cur = out;
@@ -1960,7 +1961,15 @@ static int rest_and_next_level1(Context& c, bool mixed) {
 }
 
 // multigrid: PS/iterative_solver.f90:124-341
+static int multigrid_body(Context& c);
 int multigrid(Context& c) {
+  if (c.timing) timer_begin(c, PT_K_SOLVE);
+  const int rc = multigrid_body(c);
+  if (c.timing) timer_end(c, PT_K_SOLVE);
+  return rc;
+}
+
+static int multigrid_body(Context& c) {
   const Tables& t = c.T;
   const bool mixed = c.P.precision == PT_PREC_MIXED;
   Level& L1 = c.L[1];

@@ -520,6 +520,7 @@ static int filter_loop(Context& c, double* x0, double* x1, const FilterArgs& fa,
   dim3 b16(64, 4);
   dim3 gflt = grid2(c.T.nx + 1, c.T.nz + 1, b16);
   unsigned long long* sm = c.d_fsm;
+  if (c.timing) timer_begin(c, PT_K_FILTER);
   hipLaunchKernelGGL(k_filter_init, dim3(1), dim3(64), 0, c.stream, c.d_fctl, sm);
   const int cap = 100000;
   // the first batch covers the previous call's pass count (consecutive
@@ -545,6 +546,7 @@ static int filter_loop(Context& c, double* x0, double* x1, const FilterArgs& fa,
     }
     batch = std::min(2 * batch, 256);
   }
+  if (c.timing) timer_end(c, PT_K_FILTER);
   *passes = c.h_fctl[1];
   *hint = *passes;
   if (*passes & 1) {

@@ -27,7 +27,7 @@ module projectracer_gpu
   PT_FIELD_CBUOY = 8, PT_FIELD_CONDUCT = 9, PT_FIELD_VIS = 10, PT_FIELD_RESIDUAL = 11, &
   PT_FIELD_SOURCE = 12
  integer(c_int32_t), parameter :: PT_K_RKC_STAGE_L1 = 0, PT_K_TRACER_STAGE = 1, PT_K_PROJECT = 2, &
-  PT_K_TRACER_RK2 = 3
+  PT_K_TRACER_RK2 = 3, PT_K_RKC_STAGE_COARSE = 4, PT_K_FILTER = 5, PT_K_SOLVE = 6
 
  !! namelist values (PS/basics.f90:54-66); layout of struct pt_params
  type, bind(C) :: pt_params
@@ -204,6 +204,12 @@ module projectracer_gpu
    integer(c_int32_t), value :: which
    real(c_double), intent(out) :: ms
    integer(c_int64_t), intent(out) :: launches
+  end function
+  integer(c_int) function pt_kernel_work(ctx, which, units) bind(C, name="pt_kernel_work")
+   import :: c_int, c_ptr, c_int32_t, c_int64_t
+   type(c_ptr), value :: ctx
+   integer(c_int32_t), value :: which
+   integer(c_int64_t), intent(out) :: units
   end function
   type(c_ptr) function pt_last_error() bind(C, name="pt_last_error")
    import :: c_ptr

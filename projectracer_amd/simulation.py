@@ -164,6 +164,11 @@ class Simulation:
     def enable_kernel_timing(self, on: bool = True):
         self._check(self.lib.pt_enable_kernel_timing(self.ctx, 1 if on else 0), "enable_kernel_timing")
 
+    def kernel_work(self, which: int) -> int:
+        n = C.c_int64()
+        self._check(self.lib.pt_kernel_work(self.ctx, which, C.byref(n)), "kernel_work")
+        return n.value
+
     def kernel_time(self, which: int):
         ms = C.c_double()
         n = C.c_int64()
