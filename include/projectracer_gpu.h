@@ -201,8 +201,18 @@ int pt_get_tracers(void* ctx, double* rtr, double* str, double* Ttr, double* Ctr
 /* nempty(0:nx,0:nz): tracers binned to each node's dual cell (INT32). */
 int pt_get_nempty(void* ctx, int32_t* nempty);
 int pt_get_info(void* ctx, pt_solve_info* info);
-/* stats (PS/basics.f90:363-462) of the current state; see pt_stats_row */
+/* stats (PS/basics.f90:363-462) of the current state; see pt_stats_row.
+ * Its partial-sum buffer is allocated on the first call. */
 int pt_stats(void* ctx, pt_stats_row* row);
+/* pt_stats without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same kernels
+ * and results as pt_stats; the partial sums live in a file-scope device
+ * array of the library (one per GPU, 147,712 bytes, held until the process
+ * ends), so grids are limited to nz <= 2047 (PT_ERR_UNSUPPORTED beyond).
+ * All calls on one GPU share that array: calls that overlap on one GPU
+ * (several host threads, several streams) race. */
+int pt_stats_noalloc(void* ctx, pt_stats_row* row);
 
 /* ---- checkpoint / resume in the reference's files --------------------
  * pt_write_restart: print_restart (PS/main.f90:457-496) into directory dir:

@@ -158,6 +158,7 @@ def load_library(path: os.PathLike | None = None):
         "pt_get_nempty": (C.c_int, [vp, ip]),
         "pt_get_info": (C.c_int, [vp, C.POINTER(PtSolveInfo)]),
         "pt_stats": (C.c_int, [vp, C.POINTER(PtStatsRow)]),
+        "pt_stats_noalloc": (C.c_int, [vp, C.POINTER(PtStatsRow)]),
         "pt_write_restart": (C.c_int, [vp, C.c_char_p]),
         "pt_write_snapshot": (C.c_int, [vp, C.c_char_p, C.c_int32]),
         "pt_read_restart": (C.c_int, [vp, C.c_char_p, C.c_double, C.c_double]),
@@ -182,6 +183,6 @@ EXPORTED_SYMBOLS = [
     "pt_solve_sf_equation", "pt_energy_time",
     "pt_tracers_to_corners", "pt_time_step", "pt_run_steps", "pt_field_extent",
     "pt_get_field", "pt_set_field", "pt_get_tracers", "pt_get_nempty", "pt_get_info",
-    "pt_stats", "pt_write_restart", "pt_write_snapshot", "pt_read_restart", "pt_synchronize", "pt_enable_kernel_timing",
-    "pt_kernel_time", "pt_last_error",
+    "pt_stats", "pt_stats_noalloc", "pt_write_restart", "pt_write_snapshot", "pt_read_restart", "pt_synchronize", "pt_enable_kernel_timing",
+    "pt_kernel_time", "pt_kernel_work", "pt_last_error",
 ]

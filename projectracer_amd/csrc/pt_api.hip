@@ -160,7 +160,8 @@ int pt_create(const pt_params* p, void** out) {
       (void)hipEventCreate(&t.beg[j]);
       (void)hipEventCreate(&t.end[j]);
     }
-  rc = solver_alloc(*c);
+  rc = stats_scratch_setup(*c);
+  if (rc == PT_OK) rc = solver_alloc(*c);
   if (rc == PT_OK) rc = tracers_alloc(*c);
   if (rc == PT_OK && c->P.iterate == 0) rc = direct_alloc(*c);
   if (rc != PT_OK) {
@@ -370,6 +371,12 @@ int pt_stats(void* p, pt_stats_row* row) {
   Context* c = ctx_of(p);
   if (!c || !row) return PT_ERR_INVALID;
   return stats(*c, row);
+}
+
+int pt_stats_noalloc(void* p, pt_stats_row* row) {
+  Context* c = ctx_of(p);
+  if (!c || !row) return PT_ERR_INVALID;
+  return stats_noalloc(*c, row);
 }
 
 int pt_write_restart(void* p, const char* dir) {

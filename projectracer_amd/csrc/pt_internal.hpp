@@ -165,6 +165,7 @@ struct Context {
   bool t_initial = true;
   double mass = 0.0; bool mass_set = false;
   double* stats_rows = nullptr;
+  double* stats_scratch = nullptr;   // this device's `scratch` array (pt_stats_noalloc)
   double* strain_buf = nullptr;   // strain-rate invariant at the nodes (snapshots)
   // timing
   bool timing = false;
@@ -220,6 +221,8 @@ int write_restart(Context& c, const char* dir);
 int read_restart(Context& c, const char* dir, double Tbot, double Ttop);
 // ---- diagnostics (pt_stats.hip) -------------------------------------------------
 int stats(Context& c, pt_stats_row* row);
+int stats_noalloc(Context& c, pt_stats_row* row);   // no allocation: the file-scope scratch array
+int stats_scratch_setup(Context& c);                 // once per device, at pt_create
 // ---- sorting (pt_sort.hip) ---------------------------------------------------
 int sort_pairs_temp_bytes(int64_t n, int bits, size_t* bytes);
 int sort_pairs(void* tmp, size_t bytes, const int32_t* kin, int32_t* kout, const int32_t* vin, int32_t* vout,

@@ -15,6 +15,9 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
 #include "pt_internal.hpp"
 
 namespace pt {
@@ -240,12 +243,57 @@ int strain_field(Context& c, double* out) {
   return PT_OK;
 }
 
-int stats(Context& c, pt_stats_row* row) {
-  const int nx = c.T.nx, nz = c.T.nz;
-  if (!c.stats_rows) {
-    PT_HIP(hipMalloc((void**)&c.stats_rows, sizeof(double) * (size_t)NROW * (nz + 1) + sizeof(double) * 32));
+// Scratch of the no-allocation variant (pt_stats_noalloc): the per-row
+// partial integrals, the wall values and the final scalars of one call, for
+// grids of up to STATS_NOALLOC_MAX_NZ rows, in a file-scope device array.
+// Each device holds its own copy; its address there is resolved once per
+// device at pt_create (stats_scratch_setup) and only that address reaches
+// kernels.  Every call on a device shares the array: calls that overlap on
+// one device race.  Every element a call reads is written by the same call
+// first, so nothing needs clearing between calls.
+constexpr int STATS_NOALLOC_MAX_NZ = 2047;
+constexpr size_t STATS_SCRATCH_DOUBLES = (size_t)NROW * (STATS_NOALLOC_MAX_NZ + 1) + 32;
+static __device__ __align__(256) double scratch[STATS_SCRATCH_DOUBLES];
+
+static std::mutex g_scratch_mu;
+static std::map<int, double*> g_scratch_ptr;   // device -> address of `scratch` there
+
+int stats_scratch_setup(Context& c) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_ptr.find(c.device);
+  if (it == g_scratch_ptr.end()) {
+    void* ptr = nullptr;
+    PT_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    it = g_scratch_ptr.emplace(c.device, static_cast<double*>(ptr)).first;
   }
-  double* rows = c.stats_rows;
+  c.stats_scratch = it->second;
+  return PT_OK;
+}
+
+static int stats_with(Context& c, pt_stats_row* row, double* rows);
+
+int stats(Context& c, pt_stats_row* row) {
+  if (!c.stats_rows) {
+    PT_HIP(hipMalloc((void**)&c.stats_rows, sizeof(double) * (size_t)NROW * (c.T.nz + 1) + sizeof(double) * 32));
+  }
+  return stats_with(c, row, c.stats_rows);
+}
+
+int stats_noalloc(Context& c, pt_stats_row* row) {
+  if (c.T.nz > STATS_NOALLOC_MAX_NZ) {
+    set_error("pt_stats_noalloc: nz = " + std::to_string(c.T.nz) + " exceeds the scratch limit of " +
+              std::to_string(STATS_NOALLOC_MAX_NZ) + " rows; use pt_stats");
+    return PT_ERR_UNSUPPORTED;
+  }
+  if (!c.stats_scratch) {
+    set_error("pt_stats_noalloc: scratch address not resolved at pt_create");
+    return PT_ERR_HIP;
+  }
+  return stats_with(c, row, c.stats_scratch);
+}
+
+static int stats_with(Context& c, pt_stats_row* row, double* rows) {
+  const int nx = c.T.nx, nz = c.T.nz;
   double* edge = rows + (size_t)NROW * (nz + 1);
   double* fin = edge + 8;
   const double* T = c.t_initial ? c.T0 : c.Tratio;   // the reference's module T

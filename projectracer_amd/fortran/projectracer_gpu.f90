@@ -172,6 +172,12 @@ module projectracer_gpu
    type(c_ptr), value :: ctx
    type(pt_stats_row), intent(out) :: row
   end function
+  !! pt_stats without any allocation (scratch in the library; nz <= 2047)
+  integer(c_int) function pt_stats_noalloc(ctx, row) bind(C, name="pt_stats_noalloc")
+   import :: c_int, c_ptr, pt_stats_row
+   type(c_ptr), value :: ctx
+   type(pt_stats_row), intent(out) :: row
+  end function
   integer(c_int) function pt_write_restart(ctx, dir) bind(C, name="pt_write_restart")
    import :: c_int, c_ptr, c_char
    type(c_ptr), value :: ctx
