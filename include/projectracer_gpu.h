@@ -263,6 +263,33 @@ int pt_kernel_time(void* ctx, int32_t which, double* ms, int64_t* launches);
  * the stage launches (PT_K_RKC_STAGE_*), 0 for the others. */
 int pt_kernel_work(void* ctx, int32_t which, int64_t* units);
 
+/* ---- learned initial guess (no reference counterpart) -------------------
+ * A four-level convolutional encoder-decoder (3x3 convolutions, channels
+ * 32-64-128-256, skip connections; BF16 weights on MFMA, FP32 accumulation)
+ * that proposes a correction to the warm start of a solve.  The reference
+ * warm-starts every solve from the previous psi (PS/iterative_solver.f90:134);
+ * with guess = PT_GUESS_LEARNED and a network attached (pt_set_guess_net) the
+ * solve evaluates the network on the level-1 fields and keeps its proposal
+ * only when the proposal's FP64 L1 residual is below the warm start's; the
+ * cycles then run to the same stop test either way.
+ * Network input, one pixel per node (i, k), 0 <= i < nx, 0 <= k < nz
+ * (nx, nz multiples of 8): Tbuoy, Cbuoy, ln(eta), psi / rms(psi) and the warm
+ * start's residual r / rms(r); output: the correction in units of
+ * rms(r) * out_scale. */
+int pt_net_create(int32_t seed, void** net);   /* He-initialised weights from `seed` */
+int pt_net_destroy(void* net);
+int pt_net_num_params(void* net, int64_t* count);
+int pt_net_get_params(void* net, float* host);          /* FP32 master weights, flat */
+int pt_net_set_params(void* net, const float* host);
+/* forward pass on host inputs x (N, H, W, 5) FP32 -> y (N, H, W) FP32 */
+int pt_net_forward(void* net, int32_t N, int32_t H, int32_t W, const float* x, float* y);
+/* masked mean-square loss of the forward pass against t under mask (both
+ * (N, H, W)), and its gradient with respect to the flat parameters */
+int pt_net_loss_grad(void* net, int32_t N, int32_t H, int32_t W, const float* x, const float* t, const float* mask,
+                     double* loss, float* grad);
+/* one AdamW step (beta1 0.9, beta2 0.999, eps 1e-8) with the last gradient */
+int pt_net_adamw_step(void* net, double lr, double weight_decay);
+
 /* Last error message (thread-local, never NULL). */
 const char* pt_last_error(void);
 

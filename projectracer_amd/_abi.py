@@ -167,6 +167,16 @@ def load_library(path: os.PathLike | None = None):
         "pt_kernel_time": (C.c_int, [vp, C.c_int32, dp, C.POINTER(C.c_int64)]),
         "pt_kernel_work": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_int64)]),
         "pt_last_error": (C.c_char_p, []),
+        "pt_net_create": (C.c_int, [C.c_int32, C.POINTER(vp)]),
+        "pt_net_destroy": (C.c_int, [vp]),
+        "pt_net_num_params": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "pt_net_get_params": (C.c_int, [vp, C.POINTER(C.c_float)]),
+        "pt_net_set_params": (C.c_int, [vp, C.POINTER(C.c_float)]),
+        "pt_net_forward": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_float),
+                                     C.POINTER(C.c_float)]),
+        "pt_net_loss_grad": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_float),
+                                       C.POINTER(C.c_float), C.POINTER(C.c_float), dp, C.POINTER(C.c_float)]),
+        "pt_net_adamw_step": (C.c_int, [vp, C.c_double, C.c_double]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -185,4 +195,6 @@ EXPORTED_SYMBOLS = [
     "pt_get_field", "pt_set_field", "pt_get_tracers", "pt_get_nempty", "pt_get_info",
     "pt_stats", "pt_stats_noalloc", "pt_write_restart", "pt_write_snapshot", "pt_read_restart", "pt_synchronize", "pt_enable_kernel_timing",
     "pt_kernel_time", "pt_kernel_work", "pt_last_error",
+    "pt_net_create", "pt_net_destroy", "pt_net_num_params", "pt_net_get_params", "pt_net_set_params",
+    "pt_net_forward", "pt_net_loss_grad", "pt_net_adamw_step",
 ]
