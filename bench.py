@@ -81,6 +81,7 @@ def parse():
                     help="steps run after the timed ones with per-kernel HIP events")
     ap.add_argument("--workload", default="law", choices=sorted(WORKLOADS))
     ap.add_argument("--no-secondary", action="store_true", help="skip the isoviscous line")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-guess measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=2,
@@ -205,6 +206,59 @@ def time_direct(nx=100, nz=100, solves=20):
                       "recorded): build_SF_matrix once, solve_SF_equation averaged over 5 solves",
         }
     sim.close()
+    return out
+
+
+def time_learned(args, capture_steps=3, train_iters=100, batch=4, timed_steps=4):
+    """The mixed path with the learned initial guess on the headline
+    workload.  Two contexts run the same first solve and `capture_steps`
+    steps (bit-identical: the solve is deterministic), one of them storing
+    every solve's (inputs, correction) pair for the network; the network
+    trains on those pairs (`train_iters` AdamW steps, batches of `batch`);
+    then both run `timed_steps` more, one with the network's proposals."""
+    from projectracer_amd import Simulation, _abi
+    from projectracer_amd.learned import GuessNet, attach, guess_counts
+    case = make_case(args, _abi.PT_PREC_MIXED)
+    net = GuessNet(seed=0)
+    out = {}
+    sims = {}
+    for mode in ("warm", "learned"):
+        sim = Simulation(case.params)
+        sim.load_state(case)
+        if mode == "learned":
+            net.dataset(sim.nz, sim.nx, 4 * capture_steps + 4)
+            attach(sim, net, guess=False, capture=True)
+        sim.multigrid()
+        sim.compute_velocities()
+        for _ in range(capture_steps):
+            sim.energy_time()
+            sim.multigrid()
+            sim.compute_velocities()
+        sim.synchronize()
+        sims[mode] = sim
+    t0 = time.perf_counter()
+    loss = net.train(train_iters, batch, lr=1e-3, seed=0)
+    out["train_s"] = time.perf_counter() - t0
+    out["train_loss"] = loss
+    out["pairs"] = net.dataset_size()
+    attach(sims["learned"], net, guess=True, capture=False)
+    for mode, sim in sims.items():
+        i0 = sim.info()
+        t0 = time.perf_counter()
+        sim.run_steps(timed_steps)
+        sim.synchronize()
+        t1 = time.perf_counter()
+        i1 = sim.info()
+        out[mode] = {"ms_per_step": 1e3 * (t1 - t0) / timed_steps,
+                     "vcycles_per_solve": (i1["cycles"] - i0["cycles"]) / max(1, i1["solves"] - i0["solves"])}
+        if mode == "learned":
+            out[mode]["proposals_tried_kept"] = guess_counts(sim)
+        sim.close()
+    out["vcycles_saved_frac"] = 1.0 - out["learned"]["vcycles_per_solve"] / out["warm"]["vcycles_per_solve"]
+    out["note"] = (f"network trained on {out['pairs']} pairs captured from this run's first "
+                   f"{capture_steps} steps; both paths then time {timed_steps} steps from the same state; "
+                   "a proposal is kept only when its FP64 L1 residual is below the warm start's")
+    net.close()
     return out
 
 
@@ -432,6 +486,8 @@ def main():
             "fp64_vcycles_per_solve": i64["cycles"] / max(1, i64["solves"]),
             "speedup_vs_fp64_gpu": i64["seconds"] / im["seconds"],
         }
+    if not args.no_learned:
+        result["learned_guess"] = time_learned(args)
     result["direct_solver"] = time_direct()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args)

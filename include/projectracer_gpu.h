@@ -50,7 +50,8 @@ extern "C" {
                              defect-correction loop; FP64 residual, coarse LU and psi */
 
 #define PT_GUESS_WARM    0  /* warm start from the previous psi (PS/iterative_solver.f90:134) */
-#define PT_GUESS_LEARNED 1  /* learned initial guess for psi (no reference counterpart) */
+#define PT_GUESS_LEARNED 1  /* learned initial guess for psi (no reference counterpart); takes
+                               effect once a network is attached (pt_set_guess_net) */
 
 /* ---- namelist values (PS/basics.f90:54-66, read at PS/main.f90:11-21) --- */
 typedef struct pt_params {
@@ -289,6 +290,35 @@ int pt_net_loss_grad(void* net, int32_t N, int32_t H, int32_t W, const float* x,
                      double* loss, float* grad);
 /* one AdamW step (beta1 0.9, beta2 0.999, eps 1e-8) with the last gradient */
 int pt_net_adamw_step(void* net, double lr, double weight_decay);
+/* Training set on the device: `capacity` (inputs, correction) pairs of
+ * nz x nx solves (H = nz, W = nx); a new call empties it. */
+int pt_net_dataset(void* net, int32_t H, int32_t W, int32_t capacity);
+int pt_net_dataset_size(void* net, int32_t* count);
+/* stored pair `slot`: inputs x (H, W, 5) and target t (H, W) = the solve's
+ * correction / rms(residual) */
+int pt_net_dataset_get(void* net, int32_t slot, float* x, float* t);
+/* `iters` AdamW steps on random batches of the stored pairs (targets
+ * divided by out_scale, the pairs' rms correction, fixed at the first call);
+ * *loss: the masked mean-square loss of the last batch */
+int pt_net_train(void* net, int32_t iters, int32_t batch, double lr, double weight_decay, uint64_t seed,
+                 double* loss);
+int pt_net_out_scale(void* net, double* scale);
+/* weight lam of the loss's curvature term: loss = (sum m e^2 + lam sum
+ * (m lap e)^2) / sum m with e = y - t and lap the 5-point Laplacian on each
+ * image (default 0) */
+int pt_net_set_loss(void* net, double lap_weight);
+/* Attach a network to a context (null detaches): guess = PT_GUESS_LEARNED
+ * makes every solve propose the network's guess (kept only when its FP64
+ * L1 residual is below the warm start's); capture != 0 adds every solve's
+ * (inputs, correction) pair to the network's training set while it has
+ * room.  The network must outlive the attachment. */
+int pt_set_guess_net(void* ctx, void* net, int32_t guess, int32_t capture);
+/* solves that evaluated the network, and those that kept its proposal */
+int pt_guess_counts(void* ctx, int64_t* tried, int64_t* used);
+/* the last proposal: the warm start's and the proposal's L1 residual sums,
+ * and the step along the network's (smoothed) correction that minimises the
+ * proposal's L2 residual */
+int pt_guess_last(void* ctx, double* l1_warm, double* l1_guess, double* alpha);
 
 /* Last error message (thread-local, never NULL). */
 const char* pt_last_error(void);

@@ -177,6 +177,15 @@ def load_library(path: os.PathLike | None = None):
         "pt_net_loss_grad": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_float),
                                        C.POINTER(C.c_float), C.POINTER(C.c_float), dp, C.POINTER(C.c_float)]),
         "pt_net_adamw_step": (C.c_int, [vp, C.c_double, C.c_double]),
+        "pt_net_dataset": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32]),
+        "pt_net_dataset_size": (C.c_int, [vp, C.POINTER(C.c_int32)]),
+        "pt_net_dataset_get": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "pt_net_train": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_uint64, dp]),
+        "pt_net_out_scale": (C.c_int, [vp, dp]),
+        "pt_net_set_loss": (C.c_int, [vp, C.c_double]),
+        "pt_set_guess_net": (C.c_int, [vp, vp, C.c_int32, C.c_int32]),
+        "pt_guess_counts": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "pt_guess_last": (C.c_int, [vp, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -196,5 +205,6 @@ EXPORTED_SYMBOLS = [
     "pt_stats", "pt_stats_noalloc", "pt_write_restart", "pt_write_snapshot", "pt_read_restart", "pt_synchronize", "pt_enable_kernel_timing",
     "pt_kernel_time", "pt_kernel_work", "pt_last_error",
     "pt_net_create", "pt_net_destroy", "pt_net_num_params", "pt_net_get_params", "pt_net_set_params",
-    "pt_net_forward", "pt_net_loss_grad", "pt_net_adamw_step",
+    "pt_net_forward", "pt_net_loss_grad", "pt_net_adamw_step", "pt_net_dataset", "pt_net_dataset_size", "pt_net_dataset_get",
+    "pt_net_train", "pt_net_out_scale", "pt_net_set_loss", "pt_set_guess_net", "pt_guess_counts", "pt_guess_last",
 ]

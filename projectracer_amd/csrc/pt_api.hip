@@ -1,4 +1,5 @@
 // pt_api.hip -- C ABI entry points (include/projectracer_gpu.h).
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -51,9 +52,9 @@ static int check_supported(const pt_params& p) {
     set_error("precision must be PT_PREC_FP64 or PT_PREC_MIXED");
     return PT_ERR_INVALID;
   }
-  if (p.guess != PT_GUESS_WARM) {
-    set_error("only the warm-start initial guess is built");
-    return PT_ERR_UNSUPPORTED;
+  if (p.guess != PT_GUESS_WARM && p.guess != PT_GUESS_LEARNED) {
+    set_error("guess must be PT_GUESS_WARM or PT_GUESS_LEARNED");
+    return PT_ERR_INVALID;
   }
   if (p.iterate != 0 && p.iterate != 1) {
     set_error("iterate must be 0 (direct) or 1 (multigrid)");
@@ -132,6 +133,10 @@ int pt_create(const pt_params* p, void** out) {
   c->device = dev;
   const char* graph_env = std::getenv("PT_GRAPH");
   c->use_graph = !(graph_env && graph_env[0] == '0');
+  const char* gs_env = std::getenv("PT_GUESS_SMOOTH");
+  if (gs_env) c->guess_smooth = std::max(0, std::atoi(gs_env));
+  const char* gj_env = std::getenv("PT_GUESS_JUDGE");
+  if (gj_env) c->guess_judge = std::min(2, std::max(0, std::atoi(gj_env)));
   const char* filter_env = std::getenv("PT_FILTER_HOST");
   c->filter_on_device = !(filter_env && filter_env[0] == '1');
   int rc = build_tables(c->P, c->T);
@@ -420,6 +425,41 @@ int pt_kernel_time(void* p, int32_t which, double* ms, int64_t* launches) {
   timer_flush(c->kt[which]);
   if (ms) *ms = c->kt[which].ms;
   if (launches) *launches = c->kt[which].launches;
+  return PT_OK;
+}
+
+int pt_set_guess_net(void* p, void* net, int32_t guess, int32_t capture) {
+  Context* c = ctx_of(p);
+  if (!c || (guess != PT_GUESS_WARM && guess != PT_GUESS_LEARNED)) return PT_ERR_INVALID;
+  if (c->P.iterate != 1) {
+    set_error("the learned guess starts the multigrid solve (iterate=1)");
+    return PT_ERR_INVALID;
+  }
+  if (!net && (guess == PT_GUESS_LEARNED || capture)) {
+    set_error("a network is needed for the learned guess or for capture");
+    return PT_ERR_INVALID;
+  }
+  c->guess_net = net;
+  c->guess_on = guess == PT_GUESS_LEARNED;
+  c->capture = capture != 0;
+  c->P.guess = guess;
+  return PT_OK;
+}
+
+int pt_guess_counts(void* p, int64_t* tried, int64_t* used) {
+  Context* c = ctx_of(p);
+  if (!c || !tried || !used) return PT_ERR_INVALID;
+  *tried = c->guess_tried;
+  *used = c->guess_used;
+  return PT_OK;
+}
+
+int pt_guess_last(void* p, double* l1_warm, double* l1_guess, double* alpha) {
+  Context* c = ctx_of(p);
+  if (!c || !l1_warm || !l1_guess || !alpha) return PT_ERR_INVALID;
+  *l1_warm = c->lg_l1_warm;
+  *l1_guess = c->lg_l1_guess;
+  *alpha = c->lg_alpha;
   return PT_OK;
 }
 

@@ -167,6 +167,18 @@ struct Context {
   double* stats_rows = nullptr;
   double* stats_scratch = nullptr;   // this device's `scratch` array (pt_stats_noalloc)
   double* strain_buf = nullptr;   // strain-rate invariant at the nodes (snapshots)
+  // learned initial guess: the attached network (pt_set_guess_net), whether
+  // solves propose its guess (guess_on) and/or add (inputs, correction) pairs
+  // to its dataset (capture); level-1 work fields; counters of proposals made
+  // and kept; the dataset slot and residual scale of the solve in flight
+  void* guess_net = nullptr;
+  bool guess_on = false, capture = false;
+  double *lg_res = nullptr, *lg_res2 = nullptr, *lg_psi = nullptr;
+  int64_t guess_tried = 0, guess_used = 0;
+  int lg_slot = -1;
+  int guess_smooth = 2;   // binomial smoothing passes of the proposed correction
+  int guess_judge = 0;    // 0: keep a proposal with a lower L1 residual; 1: step by the tiles' smooth residual
+  double lg_sr = 0.0, lg_l1_warm = 0.0, lg_l1_guess = 0.0, lg_alpha = 0.0;
   // timing
   bool timing = false;
   KTimer kt[PT_K_COUNT];
@@ -228,7 +240,11 @@ int sort_pairs_temp_bytes(int64_t n, int bits, size_t* bytes);
 int sort_pairs(void* tmp, size_t bytes, const int32_t* kin, int32_t* kout, const int32_t* vin, int32_t* vout,
                int64_t n, int bits, hipStream_t s);
 // ---- reductions (pt_solver.hip) ----------------------------------------------
-enum RedOp { RED_SUM_ABS = 0, RED_MAX = 1, RED_MIN = 2 };
+enum RedOp { RED_SUM_ABS = 0, RED_MAX = 1, RED_MIN = 2, RED_SUM_SQ = 3 };
+int level1_residual_of(Context& c, const double* psi, double* res, double* l1);
+// ---- learned initial guess (pt_unet.hip) ----------------------------------------
+int learned_before_solve(Context& c);   // network proposal and/or dataset capture
+int learned_after_solve(Context& c);    // the solve's correction as a training target
 int reduce_to_host(Context& c, const double* d, int n, int m, int pitch, int off, RedOp op, double* out);
 __global__ void k_reduce_final(const double* partial, int nb, int op, double* out);
 

@@ -794,12 +794,14 @@ __global__ void k_coarse_apply(const double* G, const RI* rhs, RO* err, int nxc,
 __global__ void k_reduce2d(const double* d, int n, int m, int pitch, int off, int op, double* partial) {
   __shared__ double red[256];
   int tid = threadIdx.x;
-  double acc = (op == RED_SUM_ABS) ? 0.0 : (op == RED_MAX ? -INFINITY : INFINITY);
+  const bool sum = op == RED_SUM_ABS || op == RED_SUM_SQ;
+  double acc = sum ? 0.0 : (op == RED_MAX ? -INFINITY : INFINITY);
   long total = (long)n * m;
   for (long idx = (long)blockIdx.x * 256 + tid; idx < total; idx += (long)gridDim.x * 256) {
     int i = (int)(idx % n), k = (int)(idx / n);
     double v = d[(long)(k + off) * pitch + (i + off)];
     if (op == RED_SUM_ABS) acc = acc + fabs(v);
+    else if (op == RED_SUM_SQ) acc = acc + v * v;
     else if (op == RED_MAX) acc = fmax(acc, v);
     else acc = fmin(acc, v);
   }
@@ -808,33 +810,32 @@ __global__ void k_reduce2d(const double* d, int n, int m, int pitch, int off, in
   for (int s = 128; s > 0; s >>= 1) {
     if (tid < s) {
       double x = red[tid], y = red[tid + s];
-      red[tid] = (op == RED_SUM_ABS) ? x + y : (op == RED_MAX ? fmax(x, y) : fmin(x, y));
+      red[tid] = sum ? x + y : (op == RED_MAX ? fmax(x, y) : fmin(x, y));
     }
     __syncthreads();
   }
   if (tid == 0) partial[blockIdx.x] = red[0];
 }
-
 __global__ void k_reduce_final(const double* partial, int nb, int op, double* out) {
   __shared__ double red[256];
   int tid = threadIdx.x;
-  double acc = (op == RED_SUM_ABS) ? 0.0 : (op == RED_MAX ? -INFINITY : INFINITY);
+  const bool sum = op == RED_SUM_ABS || op == RED_SUM_SQ;
+  double acc = sum ? 0.0 : (op == RED_MAX ? -INFINITY : INFINITY);
   for (int j = tid; j < nb; j += 256) {
     double v = partial[j];
-    acc = (op == RED_SUM_ABS) ? acc + v : (op == RED_MAX ? fmax(acc, v) : fmin(acc, v));
+    acc = sum ? acc + v : (op == RED_MAX ? fmax(acc, v) : fmin(acc, v));
   }
   red[tid] = acc;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (tid < s) {
       double x = red[tid], y = red[tid + s];
-      red[tid] = (op == RED_SUM_ABS) ? x + y : (op == RED_MAX ? fmax(x, y) : fmin(x, y));
+      red[tid] = sum ? x + y : (op == RED_MAX ? fmax(x, y) : fmin(x, y));
     }
     __syncthreads();
   }
   if (tid == 0) *out = red[0];
 }
-
 int reduce_to_host(Context& c, const double* d, int n, int m, int pitch, int off, RedOp op, double* out) {
   long total = (long)n * m;
   int nb = (int)std::min<long>(1024, (total + 255) / 256);
@@ -1866,6 +1867,22 @@ static int visit(Context& c, int p, bool mixed) {
   return PT_OK;
 }
 
+// FP64 residual RHS - L psi of a level-1 iterate (compute_residual's operator
+// with the level's current viscosity set) into res, and sum |res| on the host
+int level1_residual_of(Context& c, const double* psi, double* res, double* l1) {
+  Level& L = c.L[1];
+  StageArgs<double> a = stage_base<double>(c, 1, false);
+  a.Sx = psi;
+  dim3 blk(TX, TZ), grd((L.g.n - 1 + TX - 1) / TX, (L.g.m - 1 + TZ * PZ - 1) / (TZ * PZ));
+  launch_residual<double>(c, grd, blk, a, res, c.red);
+  const int nb = grd.x * grd.y;
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, nb, (int)RED_SUM_ABS, c.red_out);
+  PT_HIP(hipMemcpyAsync(c.h_red, c.red_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipStreamSynchronize(c.stream));
+  *l1 = c.h_red[0];
+  return PT_OK;
+}
+
 // residual_mag(1) of the level-1 visit just enqueued, and the abnormal-
 // residual stop (PS/iterative_solver.f90:438-447)
 static int level1_residual(Context& c, double* resmag1) {
@@ -1996,12 +2013,14 @@ static int multigrid_body(Context& c) {
                            (long)L.nelem);
     }
   }
+  if (c.guess_net) PT_TRY(learned_before_solve(c));
   int q = 1;
   double resmag1 = 0.0;
   // cycle q: visit 1 (level 1, ends with its residual's L1 norm on the host),
   // then visits 2..nsteps; the stop test follows every level-1 visit
   PT_TRY(visit(c, 1, mixed));
-  PT_TRY(level1_residual(c, &resmag1));
+  int rc = level1_residual(c, &resmag1);
+  if (rc != PT_OK) { c.last_q = q; return rc; }
   while (!((resmag1 / c.res_ref1) <= c.P.tolerance)) {
     if (q + 1 > c.P.max_cycles) {
       for (int p = 2; p <= c.cyc_n; p++) PT_TRY(visit(c, p, mixed));
@@ -2012,9 +2031,11 @@ static int multigrid_body(Context& c) {
     PT_TRY(rest_and_next_level1(c, mixed));
     c.cycles++;
     q++;
-    PT_TRY(level1_residual(c, &resmag1));
+    rc = level1_residual(c, &resmag1);
+    if (rc != PT_OK) { c.last_q = q; return rc; }   // the cycle the reference stops at
   }
   c.cycles++;
+  if (c.guess_net) PT_TRY(learned_after_solve(c));
   PT_HIP(hipMemcpyAsync(c.SF, L1.err, b1, hipMemcpyDeviceToDevice, c.stream));
   c.last_q = q;
   c.residual1 = resmag1;
@@ -2111,7 +2132,7 @@ int solver_free(Context& c) {
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
-  void* ps[] = {c.Mc, c.RowB, c.Bc, c.ipiv, c.d_info, c.red, c.red_out, c.dres64, c.Gc, c.stats_rows,
+  void* ps[] = {c.Mc, c.RowB, c.Bc, c.ipiv, c.d_info, c.red, c.red_out, c.dres64, c.Gc, c.lg_res, c.lg_res2, c.lg_psi, c.stats_rows,
                 c.strain_buf};
   for (void* p : ps)
     if (p) (void)hipFree(p);

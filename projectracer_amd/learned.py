@@ -79,3 +79,60 @@ class GuessNet:
 
     def adamw_step(self, lr: float, weight_decay: float = 0.0):
         self._check(self.lib.pt_net_adamw_step(self.ptr, float(lr), float(weight_decay)), "pt_net_adamw_step")
+
+    # -- training set and training on the device ------------------------------
+    def dataset(self, H: int, W: int, capacity: int):
+        """Room for `capacity` (inputs, correction) pairs of H x W (= nz x nx) solves."""
+        self._check(self.lib.pt_net_dataset(self.ptr, int(H), int(W), int(capacity)), "pt_net_dataset")
+
+    def dataset_size(self) -> int:
+        n = C.c_int32()
+        self._check(self.lib.pt_net_dataset_size(self.ptr, C.byref(n)), "pt_net_dataset_size")
+        return n.value
+
+    def dataset_get(self, slot: int, H: int, W: int):
+        """Stored pair `slot`: inputs (H, W, 5) and target (H, W)."""
+        x = np.empty((H, W, 5), dtype=np.float32)
+        t = np.empty((H, W), dtype=np.float32)
+        self._check(self.lib.pt_net_dataset_get(self.ptr, int(slot), _fp(x), _fp(t)), "pt_net_dataset_get")
+        return x, t
+
+    def train(self, iters: int, batch: int, lr: float = 1e-3, weight_decay: float = 1e-4, seed: int = 0) -> float:
+        """AdamW on random batches of the stored pairs; the last batch's loss."""
+        loss = C.c_double()
+        self._check(self.lib.pt_net_train(self.ptr, int(iters), int(batch), float(lr), float(weight_decay),
+                                          int(seed), C.byref(loss)), "pt_net_train")
+        return loss.value
+
+    def set_loss(self, lap_weight: float):
+        """Weight of the error's Laplacian term in the training loss."""
+        self._check(self.lib.pt_net_set_loss(self.ptr, float(lap_weight)), "pt_net_set_loss")
+
+    def out_scale(self) -> float:
+        s = C.c_double()
+        self._check(self.lib.pt_net_out_scale(self.ptr, C.byref(s)), "pt_net_out_scale")
+        return s.value
+
+
+def attach(sim, net, guess: bool = True, capture: bool = False):
+    """Attach `net` to a Simulation: its solves propose the network's guess
+    (guess=True) and/or add their (inputs, correction) pairs to the
+    network's training set (capture=True).  net=None detaches."""
+    mode = _abi.PT_GUESS_LEARNED if guess else _abi.PT_GUESS_WARM
+    ptr = net.ptr if net is not None else None
+    rc = sim.lib.pt_set_guess_net(sim.ctx, ptr, mode, 1 if capture else 0)
+    sim._check(rc, "pt_set_guess_net")
+
+
+def guess_counts(sim):
+    """(solves that evaluated the network, solves that kept its proposal)."""
+    t, u = C.c_int64(), C.c_int64()
+    sim._check(sim.lib.pt_guess_counts(sim.ctx, C.byref(t), C.byref(u)), "pt_guess_counts")
+    return t.value, u.value
+
+
+def guess_last(sim):
+    """The last proposal: (warm start's L1 residual sum, proposal's, step alpha)."""
+    a, b, al = C.c_double(), C.c_double(), C.c_double()
+    sim._check(sim.lib.pt_guess_last(sim.ctx, C.byref(a), C.byref(b), C.byref(al)), "pt_guess_last")
+    return a.value, b.value, al.value

@@ -140,13 +140,26 @@ class UNetRef:
         self.a = a
         return y
 
-    def loss_grad(self, x5, t, m):
-        """Masked mean-square loss and the flat gradient."""
+    @staticmethod
+    def lap(f):
+        """5-point Laplacian on each (H, W) image, zero outside it."""
+        out = -4.0 * f
+        out[:, :, 1:] += f[:, :, :-1]
+        out[:, :, :-1] += f[:, :, 1:]
+        out[:, 1:, :] += f[:, :-1, :]
+        out[:, :-1, :] += f[:, 1:, :]
+        return out
+
+    def loss_grad(self, x5, t, m, lam=0.0):
+        """Masked loss (sum m e^2 + lam sum (m lap e)^2) / sum m, e = y - t,
+        and the flat gradient."""
         y = self.forward(x5)
         a = self.a
         msum = max(m.sum(), 1.0)
-        loss = float((m * (y - t) ** 2).sum() / msum)
-        dy = 2.0 * m * (y - t) / msum
+        e = y - t
+        le = m * self.lap(e)
+        loss = float(((m * e ** 2).sum() + lam * (le ** 2).sum()) / msum)
+        dy = (2.0 * m * e + 2.0 * lam * self.lap(le)) / msum
         g = np.zeros(self.n)
 
         def put(i, dW, db):
