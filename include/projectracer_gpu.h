@@ -320,6 +320,23 @@ int pt_guess_counts(void* ctx, int64_t* tried, int64_t* used);
  * proposal's L2 residual */
 int pt_guess_last(void* ctx, double* l1_warm, double* l1_guess, double* alpha);
 
+/* ---- ensembles of independent cases (no reference counterpart: the
+ * reference runs one case per process, PS/main.f90:245-300) --------------
+ * n cases, each a full context with its own stream (params: an array of n,
+ * grids and parameters free per case).  pt_ensemble_run_steps advances
+ * every case by nsteps time steps (pt_time_step), the cases handed to a pool
+ * of `workers` host threads so that different cases' kernels overlap on the
+ * GPU; workers <= 0 tunes the pool size on the first steps (1, 2, 4, 8, 16
+ * threads, one step each) and keeps the fastest.  A case's results do not
+ * depend on the pool.  status[i] (optional, n entries) receives each case's
+ * code: a case that fails stops there while the others go on.  The cases'
+ * contexts (pt_ensemble_case) take every other call of this header. */
+int pt_ensemble_create(int32_t n, const pt_params* params, void** ens);
+int pt_ensemble_destroy(void* ens);
+int pt_ensemble_case(void* ens, int32_t i, void** ctx);
+int pt_ensemble_run_steps(void* ens, int32_t nsteps, int32_t workers, int32_t* status);
+int pt_ensemble_workers(void* ens, int32_t* workers);   /* pool size of the last run */
+
 /* Last error message (thread-local, never NULL). */
 const char* pt_last_error(void);
 

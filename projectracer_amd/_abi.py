@@ -167,6 +167,11 @@ def load_library(path: os.PathLike | None = None):
         "pt_kernel_time": (C.c_int, [vp, C.c_int32, dp, C.POINTER(C.c_int64)]),
         "pt_kernel_work": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_int64)]),
         "pt_last_error": (C.c_char_p, []),
+        "pt_ensemble_create": (C.c_int, [C.c_int32, C.POINTER(PtParams), C.POINTER(vp)]),
+        "pt_ensemble_destroy": (C.c_int, [vp]),
+        "pt_ensemble_case": (C.c_int, [vp, C.c_int32, C.POINTER(vp)]),
+        "pt_ensemble_run_steps": (C.c_int, [vp, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
+        "pt_ensemble_workers": (C.c_int, [vp, C.POINTER(C.c_int32)]),
         "pt_net_create": (C.c_int, [C.c_int32, C.POINTER(vp)]),
         "pt_net_destroy": (C.c_int, [vp]),
         "pt_net_num_params": (C.c_int, [vp, C.POINTER(C.c_int64)]),
@@ -204,7 +209,8 @@ EXPORTED_SYMBOLS = [
     "pt_get_field", "pt_set_field", "pt_get_tracers", "pt_get_nempty", "pt_get_info",
     "pt_stats", "pt_stats_noalloc", "pt_write_restart", "pt_write_snapshot", "pt_read_restart", "pt_synchronize", "pt_enable_kernel_timing",
     "pt_kernel_time", "pt_kernel_work", "pt_last_error",
-    "pt_net_create", "pt_net_destroy", "pt_net_num_params", "pt_net_get_params", "pt_net_set_params",
+    "pt_ensemble_create", "pt_ensemble_destroy", "pt_ensemble_case", "pt_ensemble_run_steps",
+    "pt_ensemble_workers", "pt_net_create", "pt_net_destroy", "pt_net_num_params", "pt_net_get_params", "pt_net_set_params",
     "pt_net_forward", "pt_net_loss_grad", "pt_net_adamw_step", "pt_net_dataset", "pt_net_dataset_size", "pt_net_dataset_get",
     "pt_net_train", "pt_net_out_scale", "pt_net_set_loss", "pt_set_guess_net", "pt_guess_counts", "pt_guess_last",
 ]

@@ -704,13 +704,13 @@ int build_sf_matrix(Context& c) {
     set_error("direct solver: lower band width too large for the LU panels");
     return PT_ERR_UNSUPPORTED;
   }
-  static bool attr_set = false;
-  if (!attr_set) {
-    PT_HIP(hipFuncSetAttribute((const void*)k_lu_panel, hipFuncAttributeMaxDynamicSharedMemorySize, LU_LDS_LIMIT));
-    PT_HIP(hipFuncSetAttribute((const void*)k_lu_trailing, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               LU_LDS_LIMIT));
-    attr_set = true;
-  }
+  // set once per process (a thread-safe static initialisation)
+  static const hipError_t attr_rc = [] {
+    const hipError_t e = hipFuncSetAttribute((const void*)k_lu_panel, hipFuncAttributeMaxDynamicSharedMemorySize, LU_LDS_LIMIT);
+    return e != hipSuccess ? e : hipFuncSetAttribute((const void*)k_lu_trailing, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               LU_LDS_LIMIT);
+  }();
+  PT_HIP(attr_rc);
   const int kv = t.dkl + t.dku;
   // one thread per subdiagonal row of the pivot column (the panel keeps each
   // row's multiplier in a register)

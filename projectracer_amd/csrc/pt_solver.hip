@@ -1384,12 +1384,10 @@ static int small_ppt(int n, int m, bool iv0) {
 template <typename R, bool IV0, int PPT>
 static int launch_small_t(Context& c, size_t lds, const StageArgs<R>& a, R* err, R* res, int s, double cc,
                           const RkcCoef& co) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    PT_HIP(hipFuncSetAttribute((const void*)k_smooth_small<R, IV0, PPT>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)SMALL_LDS_BYTES));
-    attr_set = true;
-  }
+  // set once per process (a thread-safe static initialisation)
+  static const hipError_t attr_rc = hipFuncSetAttribute((const void*)k_smooth_small<R, IV0, PPT>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)SMALL_LDS_BYTES);
+  PT_HIP(attr_rc);
   hipLaunchKernelGGL((k_smooth_small<R, IV0, PPT>), dim3(1), dim3(SMALL_THREADS), lds, c.stream, a, err, res, s, cc,
                      co);
   return PT_OK;
@@ -1415,12 +1413,10 @@ static int cb_rows(int n, int m) {
 template <int RB>
 static int launch_cb_t(Context& c, size_t lds, const StageArgs<float>& a, float* err, float* res, int s, double cc,
                        const RkcCoef& co) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    PT_HIP(hipFuncSetAttribute((const void*)k_smooth_cb<RB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)SMALL_LDS_BYTES));
-    attr_set = true;
-  }
+  // set once per process (a thread-safe static initialisation)
+  static const hipError_t attr_rc = hipFuncSetAttribute((const void*)k_smooth_cb<RB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)SMALL_LDS_BYTES);
+  PT_HIP(attr_rc);
   hipLaunchKernelGGL(k_smooth_cb<RB>, dim3(1), dim3(SMALL_THREADS), lds, c.stream, a, err, res, s, rkc_f32(co, cc));
   return PT_OK;
 }
@@ -1461,12 +1457,10 @@ static StripCfg strip_cfg(const LevelGeom& g) {
 // buffer holds S0 throughout; segments read one buffer pair and write the
 // other; the sweep result replaces the error buffer by a pointer swap.
 static int strip_sweep(Context& c, int l, const StripCfg& sc, double cpar, const RkcCoef& co) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    PT_HIP(hipFuncSetAttribute((const void*)k_strip_cb, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)SMALL_LDS_BYTES));
-    attr_set = true;
-  }
+  // set once per process (a thread-safe static initialisation)
+  static const hipError_t attr_rc = hipFuncSetAttribute((const void*)k_strip_cb, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)SMALL_LDS_BYTES);
+  PT_HIP(attr_rc);
   Level& L = c.L[l];
   float** f[5] = {&L.ferr, &L.fB, &L.fC, &L.fD, &L.fE};
   const LevelGeom& g = L.g;
@@ -1501,12 +1495,10 @@ static int strip_sweep(Context& c, int l, const StripCfg& sc, double cpar, const
 // kernel arguments, the coarse operator fits one workgroup, and in the mixed
 // path level ls-1 is an FP32 level (ls >= 3)
 static int launch_coarse_apply_wg(Context& c, const float* rhs, float* err, int n, int m, int pitch, float sz) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    PT_HIP(hipFuncSetAttribute((const void*)k_coarse_apply_wg, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)SMALL_LDS_BYTES));
-    attr_set = true;
-  }
+  // set once per process (a thread-safe static initialisation)
+  static const hipError_t attr_rc = hipFuncSetAttribute((const void*)k_coarse_apply_wg, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)SMALL_LDS_BYTES);
+  PT_HIP(attr_rc);
   const size_t lds = sizeof(double) * (1 + CA_SEG) * c.nci;
   if (lds > SMALL_LDS_BYTES) {
     set_error("coarsest grid too large for the single-workgroup coarse operator");

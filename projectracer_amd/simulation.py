@@ -39,6 +39,22 @@ class Simulation:
         self.nx, self.nz, self.grids, self.ntr = nx.value, nz.value, g.value, ntr.value
         self.params = params
 
+    @classmethod
+    def view(cls, ctx: C.c_void_p, params: _abi.PtParams, lib):
+        """A Simulation over a context it does not own (an ensemble's case):
+        close() leaves the context to its owner."""
+        self = cls.__new__(cls)
+        self.lib = lib
+        self.ctx = ctx
+        self._owned = False
+        nx, nz, g = C.c_int32(), C.c_int32(), C.c_int32()
+        ntr = C.c_int64()
+        self._check(self.lib.pt_get_dims(self.ctx, C.byref(nx), C.byref(nz), C.byref(g), C.byref(ntr)),
+                    "pt_get_dims")
+        self.nx, self.nz, self.grids, self.ntr = nx.value, nz.value, g.value, ntr.value
+        self.params = params
+        return self
+
     # -- errors ------------------------------------------------------------
     def _check(self, rc: int, where: str):
         if rc != _abi.PT_OK:
@@ -46,9 +62,9 @@ class Simulation:
             raise PtError(rc, where, detail.decode() if detail else "")
 
     def close(self):
-        if self.ctx:
+        if self.ctx and getattr(self, "_owned", True):
             self.lib.pt_destroy(self.ctx)
-            self.ctx = C.c_void_p()
+        self.ctx = C.c_void_p()
 
     def __del__(self):
         try:

@@ -313,6 +313,33 @@ def reference_law1024(tmp, tpc=4):
     print("wrote ref_law1024.npz", [(t, q) for t, q, _, _ in solves])
 
 
+def reference_law1024_order(tmp, tpc=4):
+    """How much the reference's own first solve on the LAW1024 state depends
+    on the order of its sums: the same run with 4 OpenMP threads instead of 8
+    (psi at every 4th node), and the oracle's single-threaded C restatement
+    (psi at every 4th node and the whole-field sums).  Both differ from the
+    8-thread run by a few 1e-10 at this size."""
+    from projectracer_amd import cases
+    from _support import run_reference, OracleSim, interior
+    case = cases.thermochem_large(tpc=tpc, **LAW1024)
+    states, solves, _, _ = run_reference(case, nrun=0, dump_every=1, threads=4, workdir=tmp / "law1024_t4",
+                                         timeout=7200)
+    out = {}
+    out["t4_s0_SF"], out["t4_s0_SF_sums"] = _sub(states[0], "SF", 4)
+    out["t4_solves_tq"], _ = _solves(solves)
+    o = OracleSim(case.params)
+    o.load_state(case)
+    o.multigrid()
+    o.compute_velocities()
+    for n in ("SF", "U"):
+        f = interior(o.field(n), 3, o.nx, o.nz)
+        out[f"oracle_s0_{n}"] = f[::4, ::4].copy()
+        out[f"oracle_s0_{n}_sums"] = np.array([f.sum(), (f * f).sum()])
+    out["oracle_q"] = np.int32(o.info()["q"])
+    np.savez_compressed(HERE / "ref_law1024_order.npz", **out)
+    print("wrote ref_law1024_order.npz", [(t, q) for t, q, _, _ in solves], "oracle q", o.info()["q"])
+
+
 def reference_divergence(tmp, tpc=4):
     """The third configuration's own law at 1024x256 (eta = exp(-ln(1e5) T +
     ln(30) C), delta_visT = delta_visC = 2), tpc=4: the reference stops in
@@ -339,8 +366,9 @@ def reference_divergence(tmp, tpc=4):
 
 
 if __name__ == "__main__":
-    if len(sys.argv) > 1 and sys.argv[1] in ("config2", "law1024", "divergence"):
-        fn = {"config2": reference_config2, "law1024": reference_law1024, "divergence": reference_divergence}
+    if len(sys.argv) > 1 and sys.argv[1] in ("config2", "law1024", "law1024_order", "divergence"):
+        fn = {"config2": reference_config2, "law1024": reference_law1024, "law1024_order": reference_law1024_order,
+              "divergence": reference_divergence}
         fn[sys.argv[1]](Path(tempfile.mkdtemp(prefix="golden_")))
         raise SystemExit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "direct":
