@@ -82,6 +82,7 @@ def parse():
     ap.add_argument("--workload", default="law", choices=sorted(WORKLOADS))
     ap.add_argument("--no-secondary", action="store_true", help="skip the isoviscous line")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-guess measurement")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=2,
@@ -262,6 +263,78 @@ def time_learned(args, capture_steps=3, train_iters=100, batch=4, timed_steps=4)
     return out
 
 
+def time_ensemble(args, n=64, tune_steps=5, timed_steps=4):
+    """The batched ensemble: n independent cases (cases.ensemble_case, seeds
+    100.., grids 128x64 to 384x128, 30 tracers per node, courant 0.01) on one
+    GPU through pt_ensemble_run_steps, mixed path.
+
+    First, per case, the first solve from the initial state in FP64 and in
+    the mixed path: the mixed psi's max-norm error relative to the FP64 psi
+    (the FP64 path is the one the parity tests pin to the oracle), or the
+    error that stopped either solve.  Then the ensemble: the first steps
+    tune the host thread pool, then `timed_steps` steps of every case are
+    timed with the tuned pool and again with one thread (one case at a
+    time).  The aggregate counts the steps of the cases that ran through."""
+    import numpy as np
+    from projectracer_amd import PtError, Simulation, _abi, cases
+    from projectracer_amd.ensemble import Ensemble
+    t0 = time.perf_counter()
+    cs = [cases.ensemble_case(100 + i) for i in range(n)]
+    setup_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    errs, first_fail = [], {"fp64": {}, "mixed": {}}
+    for i, c in enumerate(cs):
+        sf = {}
+        for label, prec in (("fp64", _abi.PT_PREC_FP64), ("mixed", _abi.PT_PREC_MIXED)):
+            c.params.precision = prec
+            sim = Simulation(c.params)
+            sim.load_state(c)
+            try:
+                sim.multigrid()
+                sf[label] = sim.field("SF").copy()
+            except PtError as e:
+                first_fail[label][i] = e.code
+            sim.close()
+        if len(sf) == 2:
+            errs.append(float(np.max(np.abs(sf["mixed"] - sf["fp64"])) / max(1e-300, np.max(np.abs(sf["fp64"])))))
+    scan_s = time.perf_counter() - t0
+    for c in cs:
+        c.params.precision = _abi.PT_PREC_MIXED
+    ens = Ensemble([c.params for c in cs])
+    for sim, c in zip(ens.cases, cs):
+        sim.load_state(c)
+    ntr = [c.ntr for c in cs]
+    del cs
+    st = ens.run_steps(tune_steps, workers=0)
+    best = ens.workers()
+    out = {"cases": n, "setup_s": setup_s, "tuned_workers": best}
+    for label, w in (("tuned", best), ("one_thread", 1)):
+        alive = st == _abi.PT_OK
+        t0 = time.perf_counter()
+        st = ens.run_steps(timed_steps, workers=w)
+        dt = time.perf_counter() - t0
+        ok = alive & (st == _abi.PT_OK)
+        out[label] = {"seconds": dt, "steps_per_case": timed_steps, "cases_ok": int(ok.sum()),
+                      "aggregate_time_steps_per_s": timed_steps * int(ok.sum()) / dt,
+                      "aggregate_tracer_updates_per_s": timed_steps * sum(t for t, k in zip(ntr, ok) if k) / dt}
+    out["failed_cases"] = {int(i): _abi.ERROR_TEXT.get(int(v), str(int(v))) for i, v in enumerate(st)
+                           if v != _abi.PT_OK}
+    out["concurrency_gain"] = out["tuned"]["aggregate_time_steps_per_s"] / max(
+        1e-30, out["one_thread"]["aggregate_time_steps_per_s"])
+    e = np.array(errs)
+    out["first_solve_mixed_vs_fp64"] = {
+        "cases_compared": int(e.size), "median": float(np.median(e)) if e.size else None,
+        "max": float(e.max()) if e.size else None,
+        "fp64_failed": {int(k): _abi.ERROR_TEXT.get(v, str(v)) for k, v in first_fail["fp64"].items()},
+        "mixed_failed": {int(k): _abi.ERROR_TEXT.get(v, str(v)) for k, v in first_fail["mixed"].items()},
+        "seconds": scan_s}
+    out["note"] = ("mixed path; each case a full context with its own stream, stepped by a pool of host threads "
+                   "(pool size tuned on the first steps); a failing case stops alone. Cases that stop do so in the "
+                   "reference's algorithm too (the FP64 path, pinned to the oracle, stops on them as well)")
+    ens.close()
+    return out
+
+
 def host_cores():
     """Cores this process may run on, capped by OMP_NUM_THREADS when set
     (the share of the host given to one GPU)."""
@@ -358,6 +431,11 @@ def step_bytes(ev, r, words, bytes_per_word):
     return solver + tracers + filters, dict(solver=solver, tracers=tracers, filters=filters)
 
 
+def note(msg):
+    """Progress on stderr (stdout carries the one JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -384,8 +462,11 @@ def main():
             print(f"{name}: {r['ms_per_step']:.2f} ms/step, {r['cycles']} cycles", flush=True)
         return
     import numpy as np
+    note(f"{args.workload}: mixed path, {args.steps} timed steps")
     mixed = time_gpu(args, _abi.PT_PREC_MIXED, restart_io=not args.no_restart_io)
+    note(f"{args.workload}: mixed {mixed['ms_per_step']:.1f} ms/step; FP64 path")
     fp64 = time_gpu(args, _abi.PT_PREC_FP64)
+    note(f"{args.workload}: FP64 {fp64['ms_per_step']:.1f} ms/step")
     steps_s = args.steps / mixed["seconds"]
     fp64_steps_s = args.steps / fp64["seconds"]
     # the mixed path's first solve against the FP64 path's first solve from
@@ -476,6 +557,7 @@ def main():
         # temporary directory on the host
         result["restart_io"] = mixed["restart"]
     if not args.no_secondary and args.workload != "isoviscous":
+        note("isoviscous: mixed and FP64 paths")
         im = time_gpu(args, _abi.PT_PREC_MIXED, workload="isoviscous", events=False)
         i64 = time_gpu(args, _abi.PT_PREC_FP64, workload="isoviscous", events=False)
         result["isoviscous"] = {
@@ -486,10 +568,16 @@ def main():
             "fp64_vcycles_per_solve": i64["cycles"] / max(1, i64["solves"]),
             "speedup_vs_fp64_gpu": i64["seconds"] / im["seconds"],
         }
+    if not args.no_ensemble:
+        note("ensemble of 64 cases")
+        result["ensemble"] = time_ensemble(args)
     if not args.no_learned:
+        note("learned guess")
         result["learned_guess"] = time_learned(args)
+    note("direct solver")
     result["direct_solver"] = time_direct()
     if not args.no_cpu_baseline:
+        note("CPU baseline (the reference program on the host cores)")
         result["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(result), flush=True)
 

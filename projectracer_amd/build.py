@@ -61,5 +61,40 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+FORTRAN_DIR = PKG / "fortran"
+FORTRAN_PROG = LIBDIR / "pt_steps"
+
+
+def fortran_compiler() -> str | None:
+    for cand in (os.environ.get("FC"), "/opt/rocm/llvm/bin/flang", shutil.which("flang"), shutil.which("gfortran")):
+        if cand and Path(cand).exists():
+            return cand
+    return None
+
+
+def build_fortran(verbose: bool = False) -> Path | None:
+    """The Fortran time-loop program (fortran/pt_steps.f90) linked against
+    the library: the interface module used as a reference maintainer would.
+    Returns None when the image has no Fortran compiler."""
+    fc = fortran_compiler()
+    if fc is None:
+        return None
+    objdir = PKG / "_objs"
+    objdir.mkdir(exist_ok=True)
+    mod_o = objdir / "projectracer_gpu_f90.o"
+    tmp = FORTRAN_PROG.with_name(FORTRAN_PROG.name + ".tmp")
+    for cmd in ([fc, "-O2", "-J", str(objdir), "-c", str(FORTRAN_DIR / "projectracer_gpu.f90"), "-o", str(mod_o)],
+                [fc, "-O2", "-J", str(objdir), str(FORTRAN_DIR / "pt_steps.f90"), str(mod_o), f"-L{LIBDIR}",
+                 "-lprojectracer_gpu", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"Fortran build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, FORTRAN_PROG)
+    return FORTRAN_PROG
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))

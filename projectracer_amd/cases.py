@@ -17,6 +17,13 @@ the reference does, PS/basics.f90:238):
                       eta=exp(-ln(1e3)T+ln(10)C), tpc=40, dense layer
                       C=1 for z<0.2+0.01cos(2 pi x/3), T=(1-z)+U(-0.01,0.01)
                       noise (seed 1).
+  * ensemble_case     one case of the batched ensemble: the grid drawn from
+                      {128x64, 192x64, 256x128, 384x128} with lambda = nx/nz,
+                      Ra_T log-uniform in [1e5, 1e7], Ra_C = B Ra_T with B
+                      uniform in [0, 1], layer depth uniform in [0.1, 0.4],
+                      eta = exp(-gamma T) with gamma uniform in [0, ln(1e4)],
+                      30 tracers per node, T perturbed as thermochem_1024,
+                      courant 0.01 (seeds 100..163 for the 64 cases).
   * thermochem_1024   lambda=4, 1024x256, Ra_T=1e7, Ra_C=3e6,
                       eta=exp(-ln(1e5)T+ln(30)C), tpc=50 (13,171,250
                       tracers), layer at z=0.3, T=(1-z)+0.01 sum_k a_k
@@ -79,14 +86,30 @@ def mzran_stream(n: int) -> np.ndarray:
     return out
 
 
-def _positions_mzran(nx, nz, tpc, aspect):
+def _positions_mzran(nx, nz, tpc, aspect, threads=1):
+    """initialize_tracers' positions (PS/composition.f90:786-811) with
+    `threads` OpenMP threads: thread t fills the node columns
+    nx_start(t)..nx_finish(t), (nx+1)/threads of them (the last thread the
+    rest), in ID order, from its own mzran stream.  Thread t's seeds are
+    thread 0's advanced t steps (PS/basics.f90:325-331), so its stream is
+    thread 0's shifted by t."""
     dr, ds = aspect / nx, 1.0 / nz
-    u = mzran_stream(2 * (nx + 1) * (nz + 1) * tpc)
-    pr, ps = u[0::2], u[1::2]
-    i = np.repeat(np.arange(nx + 1), (nz + 1) * tpc)
-    k = np.tile(np.repeat(np.arange(nz + 1), tpc), nx + 1)
-    r = dr * i.astype(np.float64) + (pr - 0.5) * dr
-    s = ds * k.astype(np.float64) + (ps - 0.5) * ds
+    per = (nx + 1) // threads
+    starts = [t * per for t in range(threads)]
+    finish = [s + per - 1 for s in starts[:-1]] + [nx]
+    longest = max(f - s + 1 for s, f in zip(starts, finish))
+    u = mzran_stream(threads - 1 + 2 * longest * (nz + 1) * tpc)
+    rs, ss = [], []
+    for t, (i0, i1) in enumerate(zip(starts, finish)):
+        ncol = i1 - i0 + 1
+        n = ncol * (nz + 1) * tpc
+        ut = u[t:t + 2 * n]
+        pr, ps = ut[0::2], ut[1::2]
+        i = np.repeat(np.arange(i0, i1 + 1), (nz + 1) * tpc)
+        k = np.tile(np.repeat(np.arange(nz + 1), tpc), ncol)
+        rs.append(dr * i.astype(np.float64) + (pr - 0.5) * dr)
+        ss.append(ds * k.astype(np.float64) + (ps - 0.5) * ds)
+    r, s = np.concatenate(rs), np.concatenate(ss)
     return np.clip(r, 0.0, aspect), np.clip(s, 0.0, 1.0)
 
 
@@ -189,3 +212,39 @@ def thermochem_large(nx=1024, nz=256, tpc=50, aspect=4.0, seed=2, **over) -> Cas
     Ttr = temp(r, s)
     Ctr = (s < 0.3 + 0.01 * np.cos(2.0 * PII * r / aspect)).astype(np.float64)
     return _finish(f"thermochem_{nx}x{nz}", p, T, r, s, Ttr, Ctr)
+
+
+ENSEMBLE_GRIDS = ((128, 64), (192, 64), (256, 128), (384, 128))
+
+
+def ensemble_case(seed: int, tpc: int = 30, gamma_max: float = math.log(1e4), **over) -> Case:
+    """One case of the batched ensemble, drawn from `seed` (see the module
+    docstring); ivis=1 with visT = e^gamma, visP = 1 gives eta = exp(-gamma T)."""
+    rng = np.random.default_rng(seed)
+    nx, nz = ENSEMBLE_GRIDS[int(rng.integers(0, len(ENSEMBLE_GRIDS)))]
+    RaT = 10.0 ** rng.uniform(5.0, 7.0)
+    B = rng.uniform(0.0, 1.0)
+    layer = rng.uniform(0.1, 0.4)
+    gam = rng.uniform(0.0, gamma_max)
+    aspect = nx / nz
+    p = default_params(aspect=aspect, nx=nx, nz=nz, tpc=tpc,
+                       **{"RaT": RaT, "RaC": B * RaT, "ivis": 1, "visT": float(math.exp(gam)), "visP": 1.0,
+                          "delta_visT": 2.0, "delta_visC": 2.0, "courant": 0.01, "grids": 6, **over})
+    dr, ds = aspect / nx, 1.0 / nz
+    a = rng.standard_normal(8)
+
+    def temp(xv, zv):
+        acc = np.zeros(np.broadcast(xv, zv).shape)
+        for kk in range(1, 9):
+            acc = acc + a[kk - 1] * np.cos(kk * PII * xv / aspect)
+        return (1.0 - zv) + 0.01 * acc * np.sin(PII * zv)
+
+    xg = dr * np.arange(nx + 1)
+    zg = ds * np.arange(nz + 1)
+    T = temp(xg[None, :], zg[:, None])
+    T[0, :] = T[0, 0]
+    T[nz, :] = T[nz, 0]
+    r, s = _positions_seeded(nx, nz, tpc, aspect, seed + 1000)
+    Ttr = temp(r, s)
+    Ctr = (s < layer + 0.01 * np.cos(2.0 * PII * r / aspect)).astype(np.float64)
+    return _finish(f"ensemble_{seed}_{nx}x{nz}", p, T, r, s, Ttr, Ctr)

@@ -217,6 +217,72 @@ module projectracer_gpu
    integer(c_int32_t), value :: which
    integer(c_int64_t), intent(out) :: units
   end function
+  !! learned initial guess (include/projectracer_gpu.h, "learned initial guess")
+  integer(c_int) function pt_net_create(seed, net) bind(C, name="pt_net_create")
+   import :: c_int, c_int32_t, c_ptr
+   integer(c_int32_t), value :: seed
+   type(c_ptr), intent(out) :: net
+  end function
+  integer(c_int) function pt_net_destroy(net) bind(C, name="pt_net_destroy")
+   import :: c_int, c_ptr
+   type(c_ptr), value :: net
+  end function
+  integer(c_int) function pt_net_dataset(net, H, W, capacity) bind(C, name="pt_net_dataset")
+   import :: c_int, c_int32_t, c_ptr
+   type(c_ptr), value :: net
+   integer(c_int32_t), value :: H, W, capacity
+  end function
+  integer(c_int) function pt_net_dataset_size(net, count) bind(C, name="pt_net_dataset_size")
+   import :: c_int, c_int32_t, c_ptr
+   type(c_ptr), value :: net
+   integer(c_int32_t), intent(out) :: count
+  end function
+  integer(c_int) function pt_net_train(net, iters, batch, lr, weight_decay, seed, loss) bind(C, name="pt_net_train")
+   import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+   type(c_ptr), value :: net
+   integer(c_int32_t), value :: iters, batch
+   real(c_double), value :: lr, weight_decay
+   integer(c_int64_t), value :: seed
+   real(c_double), intent(out) :: loss
+  end function
+  integer(c_int) function pt_set_guess_net(ctx, net, guess, capture) bind(C, name="pt_set_guess_net")
+   import :: c_int, c_int32_t, c_ptr
+   type(c_ptr), value :: ctx, net
+   integer(c_int32_t), value :: guess, capture
+  end function
+  integer(c_int) function pt_guess_counts(ctx, tried, used) bind(C, name="pt_guess_counts")
+   import :: c_int, c_int64_t, c_ptr
+   type(c_ptr), value :: ctx
+   integer(c_int64_t), intent(out) :: tried, used
+  end function
+  !! ensembles of independent cases on one GPU
+  integer(c_int) function pt_ensemble_create(n, params, ens) bind(C, name="pt_ensemble_create")
+   import :: c_int, c_int32_t, c_ptr, pt_params
+   integer(c_int32_t), value :: n
+   type(pt_params), intent(in) :: params(*)
+   type(c_ptr), intent(out) :: ens
+  end function
+  integer(c_int) function pt_ensemble_destroy(ens) bind(C, name="pt_ensemble_destroy")
+   import :: c_int, c_ptr
+   type(c_ptr), value :: ens
+  end function
+  integer(c_int) function pt_ensemble_case(ens, i, ctx) bind(C, name="pt_ensemble_case")
+   import :: c_int, c_int32_t, c_ptr
+   type(c_ptr), value :: ens
+   integer(c_int32_t), value :: i
+   type(c_ptr), intent(out) :: ctx
+  end function
+  integer(c_int) function pt_ensemble_run_steps(ens, nsteps, workers, status) bind(C, name="pt_ensemble_run_steps")
+   import :: c_int, c_int32_t, c_ptr
+   type(c_ptr), value :: ens
+   integer(c_int32_t), value :: nsteps, workers
+   integer(c_int32_t), intent(out) :: status(*)
+  end function
+  integer(c_int) function pt_ensemble_workers(ens, workers) bind(C, name="pt_ensemble_workers")
+   import :: c_int, c_int32_t, c_ptr
+   type(c_ptr), value :: ens
+   integer(c_int32_t), intent(out) :: workers
+  end function
   type(c_ptr) function pt_last_error() bind(C, name="pt_last_error")
    import :: c_ptr
   end function

@@ -121,8 +121,14 @@ def test_full_size_mixed_agrees_with_fp64(big):
     assert rel_err(m["T"], f["T"]) <= 1e-6
 
 
-def test_full_size_determinism():
-    case = cases.thermochem_large(nx=512, nz=128, tpc=20, ivis=0, tolerance=1e-7)
+def test_bench_workload_determinism():
+    """The measured workload itself (1024x256, tpc=50, eta = exp(-ln(30) T +
+    ln(2) C), mixed path): two runs of one step from the same state agree
+    bit for bit (fields, node counts, every tracer)."""
+    case = cases.thermochem_large(visT=30.0, visC=0.5, delta_visT=1.2, delta_visC=1.2, courant=0.25,
+                                  tolerance=1e-7)
+    case.params.precision = _abi.PT_PREC_MIXED
+    print(f"[determinism] {case.name} {case.ntr} tracers, mixed, one step twice", flush=True)
     runs = []
     for _ in range(2):
         sim = Simulation(case.params)

@@ -6,8 +6,9 @@ the same case run alone, and a case that fails must not stop the others."""
 import numpy as np
 import pytest
 
-from projectracer_amd import Simulation, cases, _abi
+from projectracer_amd import Simulation, PtError, cases, _abi
 from projectracer_amd.ensemble import Ensemble
+from _support import OracleSim
 
 pytestmark = pytest.mark.gpu
 
@@ -63,3 +64,40 @@ def test_a_failing_case_leaves_the_others_running():
     assert st[0] == _abi.PT_OK and st[1] == _abi.PT_ERR_MAX_CYCLES
     assert ens.cases[0].info()["solves"] == 8
     ens.close()
+
+
+def _stop(sim, nsteps, first_solve_only=False):
+    """(step, error code) of the first failing call, or (None, PT_OK)."""
+    try:
+        if first_solve_only:
+            sim.multigrid()
+            return None, _abi.PT_OK
+        for step in range(nsteps):
+            sim.time_step()
+    except PtError as e:
+        return (0 if first_solve_only else step), e.code
+    return None, _abi.PT_OK
+
+
+@pytest.mark.parametrize("index,courant,first_solve_only,expect", [
+    (60, 0.05, False, _abi.PT_ERR_TRACER_OUTSIDE),
+    (0, 0.01, True, _abi.PT_ERR_HIGH_RESIDUAL),
+], ids=["case60-tracer-outside", "case0-high-residual"])
+def test_ensemble_failures_match_the_oracle(index, courant, first_solve_only, expect):
+    """Some cases of the ensemble's parameter ranges stop in the reference's
+    own algorithm: case 60 (128x64, Ra_T=1.5e6, gamma=5.8) carries a tracer
+    out of the domain in its second step at courant 0.05, case 0 (384x128,
+    gamma=9.0) stops its first solve with an abnormally high residual.  The
+    FP64 path on the GPU stops at the same step with the same error as the
+    oracle (4 tracers per node)."""
+    c = cases.ensemble_case(100 + index, tpc=4, courant=courant)
+    print(f"[ensemble] {c.name}: FP64 GPU vs oracle until the first error", flush=True)
+    g = Simulation(c.params)
+    g.load_state(c)
+    o = OracleSim(c.params)
+    o.load_state(c)
+    a = _stop(g, 3, first_solve_only)
+    b = _stop(o, 3, first_solve_only)
+    print(f"[ensemble] GPU stop {a}, oracle stop {b}", flush=True)
+    assert a == b and a[1] == expect
+    g.close()

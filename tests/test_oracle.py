@@ -148,6 +148,31 @@ def test_initial_cloud_matches_reference_build(threads):
     np.testing.assert_array_equal(s, g[f"t{threads}_str"])
 
 
+@pytest.mark.parametrize("threads", [1, 2, 3])
+def test_python_mzran_cloud_matches_reference_build(threads):
+    """cases.mzran_stream / cases._positions_mzran (the Python restatement
+    the workload builders use) against the same reference clouds: bit-exact
+    for every thread count, and thread 0's columns are the one-thread
+    stream's prefix."""
+    g = np.load(GOLDEN / "ref_cloud.npz")
+    r1, s1 = cases._positions_mzran(22, 10, 3, 1.0, threads=1)
+    if threads == 1:
+        # one thread: the same columns as the first thread of every split
+        for t in (2, 3):
+            k = (23 // t) * 11 * 3
+            np.testing.assert_array_equal(r1[:k], g[f"t{t}_rtr"][:k])
+            np.testing.assert_array_equal(s1[:k], g[f"t{t}_str"][:k])
+        oracle_r, oracle_s = np.zeros(r1.size), np.zeros(r1.size)
+        sim = OracleSim(cases.isoviscous(nx=22, nz=10, tpc=3).params)
+        oracle_lib().pto_initial_positions(sim.ctx, 1, _dp(oracle_r), _dp(oracle_s))
+        np.testing.assert_array_equal(r1, oracle_r)
+        np.testing.assert_array_equal(s1, oracle_s)
+        return
+    r, s = cases._positions_mzran(22, 10, 3, 1.0, threads=threads)
+    np.testing.assert_array_equal(r, g[f"t{threads}_rtr"])
+    np.testing.assert_array_equal(s, g[f"t{threads}_str"])
+
+
 def test_shipped_initial_projection():
     """The shipped run's t=0 snapshot T10000: tracers from 2 mzran streams,
     T and C from the shipped namelist's custom initial conditions
