@@ -384,53 +384,25 @@ static void filter_pass(Context& c, const double* T0, double* out, int h, double
 // ---------------------------------------------------------------------------
 // Filter loops driven on the device.  The reference checks the field's
 // smoothness before every pass and stops at the first check below the
-// threshold (PS/smoother.f90:582-672).  Here the host enqueues checks and
-// passes in batches without waiting: a check folds its block maxima into one
-// slot with an integer atomicMax on the bit patterns (order-free and exact:
-// every |jump| is non-negative), and the pass after it reads the slot, and
-// either records the pass count and raises the `done` flag, or filters.
-// Once `done` is set every later check and pass of the batch returns at
-// once, so the field and the count are those of the reference's loop.
+// threshold (PS/smoother.f90:582-672).  Here one kernel per pass does both
+// halves of a loop iteration on its input field: the check (its block maxima
+// folded into one slot with an integer atomicMax on the bit patterns,
+// order-free and exact since every |jump| is non-negative) and the pass
+// itself, written to the other buffer.  The pass is speculative: the kernel
+// of the next pass reads the check and, when it is below the threshold,
+// records the pass count and raises the `done` flag instead of going on, so
+// the field the loop returns is the input of the pass whose check succeeded,
+// as in the reference.  The host enqueues pass kernels in batches without
+// waiting; once `done` is set every later kernel returns at once.
 // Passes alternate between two buffers; each writes its field's ghost
 // images itself (the reference's enforceBCs / enforceBCs_comp after the pass).
 // ---------------------------------------------------------------------------
-constexpr int FSLOTS = 64;   // ring of smoothness slots (a pass zeroes the next check's)
+constexpr int FSLOTS = 64;   // ring of smoothness slots (a pass zeroes the slot after its own)
 
 __global__ void k_filter_init(int32_t* ctl, unsigned long long* sm) {
   const int t = threadIdx.x;
   if (t < 4) ctl[t] = 0;
   if (t < FSLOTS) sm[t] = 0ull;
-}
-
-// the check before pass j: max 5-neighbour jump of f viewed with halo h and
-// pitch p (smoothness, PS/smoother.f90:75-95) over i = 0..nx-1, k = 1..nz-1,
-// folded into sm[slot].  A fixed grid of SC_BLOCKS workgroups strides over
-// the points, so one launch makes SC_BLOCKS atomics on the slot (one per
-// workgroup launched per 256 points made the atomics the kernel's cost).
-constexpr int SC_BLOCKS = 256, SC_THREADS = 256;
-__global__ void __launch_bounds__(SC_THREADS) k_smooth_check(const double* f, int nx, int nz, int h, int p,
-                                                             const int32_t* ctl, unsigned long long* sm, int slot) {
-  if (ctl[0]) return;
-  __shared__ double red[SC_THREADS / 64];
-  const long npts = (long)nx * (nz - 1);
-  auto F = [&](int a, int b) { return f[(long)(b + h) * p + (a + h)]; };
-  double d = 0.0;
-  for (long e = (long)blockIdx.x * SC_THREADS + threadIdx.x; e < npts; e += (long)SC_BLOCKS * SC_THREADS) {
-    const int k = 1 + (int)(e / nx), i = (int)(e - (long)(k - 1) * nx);
-    const double t = F(i, k);
-    const double a1 = fabs(F(i, k - 1) - t), a2 = fabs(F(i + 1, k - 1) - t), a3 = fabs(F(i + 1, k) - t),
-                 a4 = fabs(F(i, k + 1) - t), a5 = fabs(F(i + 1, k + 1) - t);
-    d = fmax(d, fmax(fmax(fmax(fmax(a1, a2), a3), a4), a5));
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) d = fmax(d, __shfl_xor(d, o, 64));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = d;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = red[0];
-    for (int w = 1; w < SC_THREADS / 64; w++) m = fmax(m, red[w]);
-    atomicMax(&sm[slot], (unsigned long long)__double_as_longlong(m));
-  }
 }
 
 struct FilterArgs {
@@ -442,26 +414,60 @@ struct FilterArgs {
   int Tbot;
   double Tb, Tt, two_over_d20, H, Htype;
   const double* Cvis;   // halo 5, for the heating term of the T wall ghosts
+  int check_h, check_p; // the check's view of the field: halo and row pitch
 };
 
-// pass j of the loop: in -> out (one Richardson pass, smoother_space at every
-// node 0..nx x 0..nz, then the ghost images), unless the check before it
-// succeeded
-__global__ void k_filter_step(const double* in, double* out, FilterArgs a, int32_t* ctl, unsigned long long* sm,
-                              int slot, int next_slot, int pass, double cdts, double delta) {
+// At most this many workgroups of a pass kernel fold a share of the check,
+// striding over its points, so one launch makes at most this many atomics on
+// the slot (one per 256 points made the atomics the check's cost).
+constexpr int SC_BLOCKS = 256;
+
+// Pass j of the loop, in -> out, on a (64, 4) block grid over the nodes
+// 0..nx x 0..nz.  First the decision on the previous kernel's check (the
+// check before pass j-1, in slot prev): below the threshold, the loop ends
+// with j-2 passes made.  Otherwise: the check before pass j (the max
+// 5-neighbour jump of `in` viewed with halo check_h and pitch check_p,
+// smoothness in PS/smoother.f90:75-95, over i = 0..nx-1, k = 1..nz-1) into
+// slot cur, and the pass (one Richardson pass, smoother_space at every node,
+// then the ghost images) into out.
+__global__ void __launch_bounds__(256) k_filter_pass(const double* in, double* out, FilterArgs a, int32_t* ctl,
+                                                     unsigned long long* sm, int prev, int cur, int next, int pass,
+                                                     double cdts, double delta) {
   if (ctl[0]) return;
-  const double diff = __longlong_as_double((long long)sm[slot]);
-  const bool first = blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0;
-  if (diff < delta) {
-    if (first) {
-      ctl[1] = pass - 1;   // passes made before the successful check
+  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  if (pass > 1 && __longlong_as_double((long long)sm[prev]) < delta) {
+    if (bid == 0 && tid == 0) {
+      ctl[1] = pass - 2;   // passes made before the successful check
       ctl[0] = 1;
     }
     return;
   }
-  if (first) sm[next_slot] = 0ull;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  if (bid == 0 && tid == 0) sm[next] = 0ull;
   const int nx = a.nx, nz = a.nz, h = a.h, p = a.p;
+  const int ncheck = min((int)(gridDim.x * gridDim.y), SC_BLOCKS);
+  if (bid < ncheck) {
+    __shared__ double red[4];
+    const long npts = (long)nx * (nz - 1);
+    auto G = [&](int u, int v) { return in[(long)(v + a.check_h) * a.check_p + (u + a.check_h)]; };
+    double d = 0.0;
+    for (long e = (long)bid * 256 + tid; e < npts; e += (long)ncheck * 256) {
+      const int k = 1 + (int)(e / nx), i = (int)(e - (long)(k - 1) * nx);
+      const double t = G(i, k);
+      const double a1 = fabs(G(i, k - 1) - t), a2 = fabs(G(i + 1, k - 1) - t), a3 = fabs(G(i + 1, k) - t),
+                   a4 = fabs(G(i, k + 1) - t), a5 = fabs(G(i + 1, k + 1) - t);
+      d = fmax(d, fmax(fmax(fmax(fmax(a1, a2), a3), a4), a5));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) d = fmax(d, __shfl_xor(d, o, 64));
+    if ((tid & 63) == 0) red[tid >> 6] = d;
+    __syncthreads();
+    if (tid == 0) {
+      const double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+      atomicMax(&sm[cur], (unsigned long long)__double_as_longlong(m));
+    }
+  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
   if (i > nx || k > nz) return;
   auto F = [&](int u, int v) { return in[(long)(v + h) * p + (u + h)]; };
   auto W = [&](int u, int v, double val) { out[(long)(v + h) * p + (u + h)] = val; };
@@ -515,8 +521,8 @@ __global__ void k_filter_step(const double* in, double* out, FilterArgs a, int32
 // one device-driven filter loop on x0 (x1: the second buffer).  cdts_of(j)
 // is pass j's c*dts; on return x0 holds the filtered field with its ghosts
 // and *passes the number of passes made before the successful check
-static int filter_loop(Context& c, double* x0, double* x1, const FilterArgs& fa, int check_h, int check_p,
-                       double delta, double (*cdts_of)(const Context&, int), int* passes, int* hint) {
+static int filter_loop(Context& c, double* x0, double* x1, const FilterArgs& fa, double delta,
+                       double (*cdts_of)(const Context&, int), int* passes, int* hint) {
   dim3 b16(64, 4);
   dim3 gflt = grid2(c.T.nx + 1, c.T.nz + 1, b16);
   unsigned long long* sm = c.d_fsm;
@@ -524,18 +530,16 @@ static int filter_loop(Context& c, double* x0, double* x1, const FilterArgs& fa,
   hipLaunchKernelGGL(k_filter_init, dim3(1), dim3(64), 0, c.stream, c.d_fctl, sm);
   const int cap = 100000;
   // the first batch covers the previous call's pass count (consecutive
-  // projections need similar counts), later ones double
-  int j = 0, batch = std::min(std::max(*hint + 2, 8), 1024);
+  // projections need similar counts) and the kernel that decides on its
+  // last check; later batches double
+  int j = 0, batch = std::min(std::max(*hint + 3, 8), 1024);
   for (;;) {
     for (int b = 0; b < batch; b++) {
       j++;
       const double* in = (j & 1) ? x0 : x1;
       double* out = (j & 1) ? x1 : x0;
-      const int slot = (j - 1) % FSLOTS, nslot = j % FSLOTS;
-      hipLaunchKernelGGL(k_smooth_check, dim3(SC_BLOCKS), dim3(SC_THREADS), 0, c.stream, in, c.T.nx, c.T.nz, check_h,
-                         check_p, c.d_fctl, sm, slot);
-      hipLaunchKernelGGL(k_filter_step, gflt, b16, 0, c.stream, in, out, fa, c.d_fctl, sm, slot, nslot, j,
-                         cdts_of(c, j), delta);
+      hipLaunchKernelGGL(k_filter_pass, gflt, b16, 0, c.stream, in, out, fa, c.d_fctl, sm, (j - 1) % FSLOTS,
+                         j % FSLOTS, (j + 1) % FSLOTS, j, cdts_of(c, j), delta);
     }
     PT_HIP(hipMemcpyAsync(c.h_fctl, c.d_fctl, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
     PT_HIP(hipStreamSynchronize(c.stream));
@@ -550,7 +554,7 @@ static int filter_loop(Context& c, double* x0, double* x1, const FilterArgs& fa,
   *passes = c.h_fctl[1];
   *hint = *passes;
   if (*passes & 1) {
-    // the last pass wrote the second buffer
+    // the loop ended on the second buffer
     const size_t bytes = sizeof(double) * (size_t)(c.T.nx + 1 + 2 * fa.h) * (c.T.nz + 1 + 2 * fa.h);
     PT_HIP(hipMemcpyAsync(x0, x1, bytes, hipMemcpyDeviceToDevice, c.stream));
   }
@@ -565,6 +569,7 @@ static FilterArgs filter_args(const Context& c, int h, int clamp01) {
   a.clamp01 = clamp01;
   a.Tbot = c.P.Tbot; a.Tb = c.Tbot_iso; a.Tt = c.Ttop_iso; a.two_over_d20 = 2.0 / c.T.D2[3];
   a.H = c.P.H; a.Htype = c.P.Htype; a.Cvis = c.Cvis;
+  a.check_h = 3; a.check_p = c.T.nx + 7;
   return a;
 }
 
@@ -593,7 +598,7 @@ static int smoother_time(Context& c) {
   // the halo-3 shape (PS/smoother.f90:79, :602): sequence association makes
   // it read the C storage with the halo-3 row length
   int P = 0;
-  PT_TRY(filter_loop(c, c.Cvis, c.TxS, filter_args(c, 5, 1), 3, c.T.nx + 7, c.T.delta_C, cdts_comp, &P, &c.hint_comp));
+  PT_TRY(filter_loop(c, c.Cvis, c.TxS, filter_args(c, 5, 1), c.T.delta_C, cdts_comp, &P, &c.hint_comp));
   // n_comp counts the outer iterations after the first ten-pass one
   c.n_comp = P < 10 ? 0 : P - 9;
   return PT_OK;
@@ -646,7 +651,7 @@ static int smoother_time_T(Context& c) {
   if (c.P.RaT == 0.0 || c.P.ivis == 0) return PT_OK;
   if (c.filter_on_device) {
     int P = 0;
-    PT_TRY(filter_loop(c, c.Tvis, c.CxS, filter_args(c, 3, 0), 3, c.T.nx + 7, c.T.delta_T, cdts_temp, &P, &c.hint_temp));
+    PT_TRY(filter_loop(c, c.Tvis, c.CxS, filter_args(c, 3, 0), c.T.delta_T, cdts_temp, &P, &c.hint_temp));
     c.n_temp = P;
     return PT_OK;
   }
