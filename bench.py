@@ -85,7 +85,7 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=2,
+    ap.add_argument("--cpu-steps", type=int, default=5,
                     help="warm reference steps timed after its first loop iteration")
     ap.add_argument("--no-restart-io", action="store_true",
                     help="skip timing pt_write_restart / pt_read_restart after the headline run")

@@ -263,6 +263,12 @@ int pt_kernel_time(void* ctx, int32_t which, double* ms, int64_t* launches);
 /* Units of work of the timed launches of `which`: interior grid points for
  * the stage launches (PT_K_RKC_STAGE_*), 0 for the others. */
 int pt_kernel_work(void* ctx, int32_t which, int64_t* units);
+/* projections whose stable (cell, slot) sort was the windowed counting sort
+ * (tracers stored in an earlier projection's cell order, none more than one
+ * cell away) and those that took the radix sort (no such order yet, a
+ * farther move or one in the last 8 projections, or PT_PROJECT_RADIX=1 at
+ * pt_create); both give the same order, element for element */
+int pt_projection_counts(void* ctx, int64_t* windowed, int64_t* radix);
 
 /* ---- learned initial guess (no reference counterpart) -------------------
  * A four-level convolutional encoder-decoder (3x3 convolutions, channels

@@ -191,6 +191,7 @@ def load_library(path: os.PathLike | None = None):
         "pt_set_guess_net": (C.c_int, [vp, vp, C.c_int32, C.c_int32]),
         "pt_guess_counts": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "pt_guess_last": (C.c_int, [vp, dp, dp, dp]),
+        "pt_projection_counts": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -213,4 +214,5 @@ EXPORTED_SYMBOLS = [
     "pt_ensemble_workers", "pt_net_create", "pt_net_destroy", "pt_net_num_params", "pt_net_get_params", "pt_net_set_params",
     "pt_net_forward", "pt_net_loss_grad", "pt_net_adamw_step", "pt_net_dataset", "pt_net_dataset_size", "pt_net_dataset_get",
     "pt_net_train", "pt_net_out_scale", "pt_net_set_loss", "pt_set_guess_net", "pt_guess_counts", "pt_guess_last",
+    "pt_projection_counts",
 ]

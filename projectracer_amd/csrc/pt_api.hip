@@ -139,6 +139,8 @@ int pt_create(const pt_params* p, void** out) {
   if (gj_env) c->guess_judge = std::min(2, std::max(0, std::atoi(gj_env)));
   const char* filter_env = std::getenv("PT_FILTER_HOST");
   c->filter_on_device = !(filter_env && filter_env[0] == '1');
+  const char* radix_env = std::getenv("PT_PROJECT_RADIX");
+  c->project_radix_only = radix_env && radix_env[0] == '1';
   int rc = build_tables(c->P, c->T);
   if (rc != PT_OK) { delete c; return rc; }
   c->P.nx = c->T.nx; c->P.nz = c->T.nz; c->P.grids = c->T.grids;
@@ -443,6 +445,14 @@ int pt_set_guess_net(void* p, void* net, int32_t guess, int32_t capture) {
   c->guess_on = guess == PT_GUESS_LEARNED;
   c->capture = capture != 0;
   c->P.guess = guess;
+  return PT_OK;
+}
+
+int pt_projection_counts(void* p, int64_t* windowed, int64_t* radix) {
+  Context* c = ctx_of(p);
+  if (!c || !windowed || !radix) return PT_ERR_INVALID;
+  *windowed = c->proj_window;
+  *radix = c->proj_fallback;
   return PT_OK;
 }
 

@@ -115,9 +115,17 @@ struct Context {
   // projection work: cell keys, stable (cell, index) sort, per-cell partial sums
   int32_t *cellkey, *skey, *iota, *perm, *cstart, *cend, *nempty;
   void* sort_tmp; size_t sort_bytes; int sort_bits;
+  // the storage order's cells: okey per slot and each cell's slot range
+  // (ostart, oend), valid once the tracers have been reordered; ccount /
+  // bsum: per-cell counts and per-block totals of the windowed counting sort
+  int32_t *okey = nullptr, *ostart = nullptr, *oend = nullptr, *ccount = nullptr, *bsum = nullptr;
+  bool order_valid = false;
+  bool project_radix_only = false;              // PT_PROJECT_RADIX=1 at pt_create
+  int window_skip = 0;                          // projections left before the windowed sort is tried again
+  int64_t proj_window = 0, proj_fallback = 0;   // projections by windowed counting sort / by radix sort
   double* cellsum;          // 27 per cell: (T*w, C*w, w) for the 3x3 neighbour nodes
   double *TxS, *CxS;        // halo-5 scratch fields for the filters
-  int32_t* d_flags;         // [0]=tracer outside domain
+  int32_t* d_flags;         // [0]=tracer outside domain, [1]=tracer more than one cell from its slot's cell
   // device-driven filter loops: control (done flag, pass count), its pinned
   // host mirror, and the ring of per-check smoothness maxima (FP64 bit
   // patterns); filter_on_device is false when PT_FILTER_HOST=1 at pt_create

@@ -2,9 +2,12 @@
 //
 //   k_bin            nint binning of tracer positions to node dual cells + domain
 //                    check (PS/composition.f90:1002-1011)
-//   (rocPRIM)        stable radix sort of (cell, slot) pairs: each cell's tracers
-//                    become one run in a fixed order
-//   k_cell_bounds    run bounds per cell (nempty = run length)
+//   k_cs_count/scan/fill  stable counting sort of (cell, slot) pairs that reads
+//                    each cell's tracers from the storage order's neighbouring
+//                    cells (the tracers move less than a cell per stage);
+//                    (rocPRIM) stable radix sort when the storage order is not
+//                    that of a previous projection or a tracer moved farther
+//   k_cell_bounds    run bounds per cell after the radix sort (nempty = run length)
 //   k_cell_reduce    per-cell sums of T*w, C*w, w for the 3x3 neighbour nodes
 //                    (shape_func, PS/composition.f90:896-921), fixed order
 //   k_node_combine   node = sum of its 9 cells' partials in a fixed order;
@@ -41,9 +44,11 @@ __device__ __forceinline__ int nint(double x) { return (int)round(x); }
 // binning, stable sort by cell, per-cell sums, node combine
 // ---------------------------------------------------------------------------
 // key = dual cell of nint(r/dr), nint(s/ds); positions outside the domain
-// (or NaN) raise the flag and get the sentinel key ncell (sorted last)
+// (or NaN) raise flag 0 and get the sentinel key ncell (sorted last).  With
+// the storage order's cells (okey), a tracer whose cell is more than one
+// cell from its slot's in either direction raises flag 1.
 __global__ void k_bin(const double* r, const double* s, int64_t n, int nx, int nz, double dr, double ds,
-                      double aspect, int32_t* key, int32_t* flags) {
+                      double aspect, int32_t* key, int32_t* flags, const int32_t* okey) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   double rv = r[j], sv = s[j];
@@ -55,6 +60,192 @@ __global__ void k_bin(const double* r, const double* s, int64_t n, int nx, int n
   }
   int i = nint(rv / dr), k = nint(sv / ds);
   key[j] = k * (nx + 1) + i;
+  if (okey) {
+    const int o = okey[j], oi = o % (nx + 1), ok = o / (nx + 1);
+    if (abs(oi - i) > 1 || abs(ok - k) > 1) atomicOr(&flags[1], 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stable counting sort of the (cell, slot) pairs when the tracers are stored
+// in the cell order of an earlier projection and none moved more than one
+// cell from its slot's cell.  A block owns CS_CELLS consecutive cells of one
+// row; every tracer now in them is stored in the slot ranges of the cells of
+// the three rows around, one column beyond the block's on each side.  Those
+// three windows are read in ascending slot order, so each cell's tracers are
+// found in ascending slot order: the result (perm, skey, cstart, cend) is the
+// one the stable radix sort gives, element for element.
+// ---------------------------------------------------------------------------
+constexpr int CS_CELLS = 64;
+constexpr int CS_THREADS = 256;
+
+struct CsBlock {
+  int cfirst, ncb, nwin, lo[3], hi[3];
+};
+
+// The block's cells and its windows: for each of the rows around (up to
+// three), the slot range [lo, hi) of the cells ia..ib in the storage order,
+// from the first non-empty cell's start to the last one's end (empty cells
+// may hold any bounds).  One thread per (row, cell); ends with a barrier.
+__device__ void cs_block(const int32_t* ostart, const int32_t* oend, int nx, int nz, CsBlock& b) {
+  const int tid = threadIdx.x;
+  const int k = blockIdx.y, i0 = blockIdx.x * CS_CELLS;
+  const int ka = max(k - 1, 0), kb = min(k + 1, nz);
+  const int ia = max(i0 - 1, 0), ib = min(i0 + min(CS_CELLS, nx + 1 - i0), nx);
+  const int ncol = ib - ia + 1;   // at most CS_CELLS + 2
+  if (tid == 0) {
+    b.cfirst = k * (nx + 1) + i0;
+    b.ncb = min(CS_CELLS, nx + 1 - i0);
+    b.nwin = kb - ka + 1;
+    for (int w = 0; w < 3; w++) {
+      b.lo[w] = INT32_MAX;
+      b.hi[w] = 0;
+    }
+  }
+  __syncthreads();
+  if (tid < 3 * ncol) {
+    const int w = tid / ncol, i = ia + tid % ncol, kk = ka + w;
+    if (kk <= kb) {
+      const int c = kk * (nx + 1) + i;
+      const int s0 = ostart[c], s1 = oend[c];
+      if (s1 > s0) {
+        atomicMin(&b.lo[w], s0);
+        atomicMax(&b.hi[w], s1);
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < 3 && b.lo[tid] == INT32_MAX) {
+    b.lo[tid] = 0;
+    b.hi[tid] = 0;
+  }
+  __syncthreads();
+}
+
+// per-cell counts of the block's cells, and the block's total
+__global__ void __launch_bounds__(CS_THREADS) k_cs_count(const int32_t* __restrict__ key,
+                                                          const int32_t* __restrict__ ostart,
+                                                          const int32_t* __restrict__ oend, int nx, int nz,
+                                                          int32_t* __restrict__ ccount, int32_t* __restrict__ bsum) {
+  __shared__ CsBlock sb;
+  __shared__ int lcount[CS_CELLS];
+  const int tid = threadIdx.x;
+  if (tid < CS_CELLS) lcount[tid] = 0;
+  cs_block(ostart, oend, nx, nz, sb);
+  const int cfirst = sb.cfirst, ncb = sb.ncb;
+  for (int w = 0; w < sb.nwin; w++)
+    for (int q = sb.lo[w] + tid; q < sb.hi[w]; q += CS_THREADS) {
+      const int lc = key[q] - cfirst;
+      if (lc >= 0 && lc < ncb) atomicAdd(&lcount[lc], 1);
+    }
+  __syncthreads();
+  if (tid < CS_CELLS) {
+    int v = tid < ncb ? lcount[tid] : 0;
+    if (tid < ncb) ccount[cfirst + tid] = v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (tid == 0) bsum[blockIdx.y * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+// exclusive prefix sum of the block totals (one workgroup); flags[2] is
+// raised when the total is not n (a tracer outside every window)
+__global__ void __launch_bounds__(1024) k_cs_scan(int32_t* bsum, int nblk, int64_t n, int32_t* flags) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (nblk + 1023) / 1024;
+  const int a = min(t * per, nblk), b = min(a + per, nblk);
+  int loc = 0;
+  for (int u = a; u < b; u++) loc += bsum[u];
+  part[t] = loc;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - loc;   // exclusive
+  for (int u = a; u < b; u++) {
+    const int v = bsum[u];
+    bsum[u] = run;
+    run += v;
+  }
+  if (t == 1023 && (int64_t)part[1023] != n) atomicOr(&flags[2], 1);
+}
+
+// cell bounds, then the sorted (cell, slot) pairs: each window is read in
+// chunks of CS_THREADS slots; within a wave a tracer's rank among the lanes
+// before it with the same cell comes from ballots, across waves from the
+// per-wave counts, across chunks from the running count per cell
+__global__ void __launch_bounds__(CS_THREADS) k_cs_fill(const int32_t* __restrict__ key,
+                                                         const int32_t* __restrict__ ostart,
+                                                         const int32_t* __restrict__ oend, int nx, int nz,
+                                                         const int32_t* __restrict__ ccount,
+                                                         const int32_t* __restrict__ bsum, int32_t* __restrict__ cstart,
+                                                         int32_t* __restrict__ cend, int32_t* __restrict__ perm,
+                                                         int32_t* __restrict__ skey) {
+  __shared__ CsBlock sb;
+  __shared__ int lstart[CS_CELLS], running[CS_CELLS], cnt[CS_THREADS / 64][CS_CELLS];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  cs_block(ostart, oend, nx, nz, sb);
+  const int cfirst = sb.cfirst, ncb = sb.ncb;
+  if (tid < CS_CELLS) {
+    // the block's cell starts: its total's offset plus a wave prefix sum
+    const int v = tid < ncb ? ccount[cfirst + tid] : 0;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
+    }
+    const int st = bsum[blockIdx.y * gridDim.x + blockIdx.x] + incl - v;
+    lstart[tid] = st;
+    running[tid] = 0;
+    if (tid < ncb) {
+      cstart[cfirst + tid] = st;
+      cend[cfirst + tid] = st + v;
+    }
+  }
+  __syncthreads();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int w = 0; w < sb.nwin; w++) {
+    const int lo = sb.lo[w], hi = sb.hi[w];
+    for (int base = lo; base < hi; base += CS_THREADS) {
+      cnt[wv][lane] = 0;
+      const int q = base + tid;
+      int lc = -1;
+      if (q < hi) {
+        const int d = key[q] - cfirst;
+        if (d >= 0 && d < ncb) lc = d;
+      }
+      __syncthreads();
+      int rank = 0;
+      unsigned long long pending = __ballot(lc >= 0);
+      while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int lv = __shfl(lc, leader, 64);
+        const unsigned long long m = __ballot(lc == lv);
+        if (lc == lv) rank = __popcll(m & below);
+        if (lane == leader) cnt[wv][lv] = __popcll(m);
+        pending &= ~m;
+      }
+      __syncthreads();
+      if (lc >= 0) {
+        int pos = lstart[lc] + running[lc] + rank;
+        for (int u = 0; u < wv; u++) pos += cnt[u][lc];
+        perm[pos] = q;
+        skey[pos] = cfirst + lc;
+      }
+      __syncthreads();
+      if (tid < CS_CELLS) {
+        int add = 0;
+        for (int u = 0; u < CS_THREADS / 64; u++) add += cnt[u][tid];
+        running[tid] += add;
+      }
+      __syncthreads();
+    }
+  }
 }
 
 // segment bounds of each cell in the sorted key array (empty cells: 0,0)
@@ -709,25 +900,61 @@ int tracers_to_corners(Context& c, const double* r, const double* s, const doubl
   if (c.timing) timer_begin(c, PT_K_PROJECT);
   PT_HIP(hipMemcpyAsync(c.Tratio, c.T0, b3, hipMemcpyDeviceToDevice, c.stream));
   PT_HIP(hipMemsetAsync(c.d_flags, 0, sizeof(int32_t) * 4, c.stream));
-  PT_HIP(hipMemsetAsync(c.cstart, 0, sizeof(int32_t) * ncell, c.stream));
-  PT_HIP(hipMemsetAsync(c.cend, 0, sizeof(int32_t) * ncell, c.stream));
   unsigned nbt = (unsigned)((n + 255) / 256);
+  // The windowed sort needs the storage order of an earlier projection and
+  // no tracer more than one cell from its slot's cell.  Under-resolved flows
+  // move tracers several cells per stage; after such a projection the next
+  // WINDOW_BACKOFF projections take the radix sort without asking.
+  constexpr int WINDOW_BACKOFF = 8;
+  const bool try_window = c.order_valid && !c.project_radix_only && c.window_skip == 0;
+  if (c.window_skip > 0) c.window_skip--;
   hipLaunchKernelGGL(k_bin, dim3(nbt), dim3(256), 0, c.stream, r, s, n, nx, nz, c.T.dr, c.T.ds, c.T.aspect,
-                     c.cellkey, c.d_flags);
-  PT_TRY(sort_pairs(c.sort_tmp, c.sort_bytes, c.cellkey, c.skey, c.iota, c.perm, n, c.sort_bits, c.stream));
-  hipLaunchKernelGGL(k_cell_bounds, dim3(nbt), dim3(256), 0, c.stream, c.skey, n, ncell, c.cstart, c.cend);
+                     c.cellkey, c.d_flags, try_window ? c.okey : nullptr);
+  bool windowed = false;
+  if (try_window) {
+    // the domain check first, as the reference's binning loop does
+    // (PS/composition.f90:1002-1011); then the sort the order allows
+    PT_HIP(hipMemcpyAsync(c.h_fctl, c.d_flags, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+    PT_HIP(hipStreamSynchronize(c.stream));
+    if (c.h_fctl[0]) {
+      if (c.timing) timer_end(c, PT_K_PROJECT);
+      set_error("tracer position outside [0,aspect]x[0,1]");
+      return PT_ERR_TRACER_OUTSIDE;
+    }
+    windowed = c.h_fctl[1] == 0;
+    if (!windowed) c.window_skip = WINDOW_BACKOFF;
+  }
+  if (windowed) {
+    const dim3 gcs((unsigned)((nx + 1 + CS_CELLS - 1) / CS_CELLS), (unsigned)(nz + 1));
+    const int nblk = (int)(gcs.x * gcs.y);
+    hipLaunchKernelGGL(k_cs_count, gcs, dim3(CS_THREADS), 0, c.stream, c.cellkey, c.ostart, c.oend, nx, nz, c.ccount,
+                       c.bsum);
+    hipLaunchKernelGGL(k_cs_scan, dim3(1), dim3(1024), 0, c.stream, c.bsum, nblk, n, c.d_flags);
+    hipLaunchKernelGGL(k_cs_fill, gcs, dim3(CS_THREADS), 0, c.stream, c.cellkey, c.ostart, c.oend, nx, nz, c.ccount,
+                       c.bsum, c.cstart, c.cend, c.perm, c.skey);
+    c.proj_window++;
+  } else {
+    PT_HIP(hipMemsetAsync(c.cstart, 0, sizeof(int32_t) * ncell, c.stream));
+    PT_HIP(hipMemsetAsync(c.cend, 0, sizeof(int32_t) * ncell, c.stream));
+    PT_TRY(sort_pairs(c.sort_tmp, c.sort_bytes, c.cellkey, c.skey, c.iota, c.perm, n, c.sort_bits, c.stream));
+    hipLaunchKernelGGL(k_cell_bounds, dim3(nbt), dim3(256), 0, c.stream, c.skey, n, ncell, c.cstart, c.cend);
+    c.proj_fallback++;
+  }
   hipLaunchKernelGGL(k_cell_reduce, dim3((ncell + PJ_CELLS - 1) / PJ_CELLS), dim3(PJ_THREADS), 0, c.stream, r, s, Tt,
                      c.Ct, c.perm, c.skey, n, c.cstart, c.cend, c.cellsum, nx, nz, c.T.dr, c.T.ds);
   dim3 b16(64, 4);
   hipLaunchKernelGGL(k_node_combine, grid2(nx + 1, nz + 1, b16), b16, 0, c.stream, c.cellsum, c.cstart, c.cend,
                      c.Tratio, c.Cnew, c.nempty, nx, nz);
   if (c.timing) timer_end(c, PT_K_PROJECT);
-  int32_t flag = 0;
-  PT_HIP(hipMemcpyAsync(&flag, c.d_flags, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipMemcpyAsync(c.h_fctl, c.d_flags, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
   PT_HIP(hipStreamSynchronize(c.stream));
-  if (flag) {
+  if (c.h_fctl[0]) {
     set_error("tracer position outside [0,aspect]x[0,1]");
     return PT_ERR_TRACER_OUTSIDE;
+  }
+  if (c.h_fctl[2]) {
+    set_error("projection: the windowed counting sort did not place every tracer");
+    return PT_ERR_INVALID;
   }
   c_bcs(c, c.Cnew);
   PT_TRY(smoother_time(c));
@@ -1216,6 +1443,11 @@ static int reorder_tracers(Context& c) {
   std::swap(c.Tt, c.T1t);
   std::swap(c.Ct, c.r2);
   std::swap(c.tid, c.tid_tmp);
+  // the new storage order is the last projection's cell order
+  std::swap(c.okey, c.skey);
+  std::swap(c.ostart, c.cstart);
+  std::swap(c.oend, c.cend);
+  c.order_valid = true;
   return PT_OK;
 }
 __global__ void k_to_id_order(const int32_t* __restrict__ id, int64_t n, const double* __restrict__ a,
@@ -1237,6 +1469,8 @@ int tracers_in_id_order(Context& c, double** r, double** s, double** Tt, double*
 // tracers (reference ID order) onto the device, projected onto the nodes, in cell order
 static int place_tracers(Context& c, const double* rtr, const double* str, const double* Ttr, const double* Ctr) {
   size_t tb = sizeof(double) * (size_t)c.T.ntr;
+  c.order_valid = false;
+  c.window_skip = 0;
   PT_HIP(hipMemcpyAsync(c.r, rtr, tb, hipMemcpyHostToDevice, c.stream));
   PT_HIP(hipMemcpyAsync(c.s, str, tb, hipMemcpyHostToDevice, c.stream));
   PT_HIP(hipMemcpyAsync(c.Tt, Ttr, tb, hipMemcpyHostToDevice, c.stream));
@@ -1337,6 +1571,9 @@ int tracers_alloc(Context& c) {
   DALLOC(c.cellkey, ib); DALLOC(c.skey, ib); DALLOC(c.iota, ib); DALLOC(c.perm, ib);
   DALLOC(c.tid, ib); DALLOC(c.tid_tmp, ib);
   DALLOC(c.cstart, sizeof(int32_t) * ncell); DALLOC(c.cend, sizeof(int32_t) * ncell);
+  DALLOC(c.okey, ib); DALLOC(c.ostart, sizeof(int32_t) * ncell); DALLOC(c.oend, sizeof(int32_t) * ncell);
+  DALLOC(c.ccount, sizeof(int32_t) * ncell);
+  DALLOC(c.bsum, sizeof(int32_t) * (size_t)((nx + 1 + CS_CELLS - 1) / CS_CELLS) * (nz + 1));
   DALLOC(c.nempty, sizeof(int32_t) * ncell);
   DALLOC(c.cellsum, sizeof(double) * 27 * (size_t)ncell);
   DALLOC(c.d_flags, sizeof(int32_t) * 4);
@@ -1356,7 +1593,8 @@ int tracers_free(Context& c) {
   void* ps[] = {c.SF, c.Tratio, c.T0, c.Tvis, c.Tbuoy, c.u, c.w, c.src, c.cond_r, c.cond_s, c.Cnew, c.C0, c.Cvis,
                 c.Cbuoy, c.cond, c.Textend, c.TxS, c.CxS, c.itab, c.ctab, c.dtab, c.r, c.s, c.Tt, c.Ct, c.r0, c.s0, c.T0t,
                 c.r1, c.s1, c.T1t, c.r2, c.s2, c.T2t, c.cellkey, c.skey, c.iota, c.perm, c.tid, c.tid_tmp, c.cstart, c.cend,
-                c.nempty, c.cellsum, c.d_flags, c.sort_tmp, c.d_fctl, c.d_fsm};
+                c.nempty, c.cellsum, c.d_flags, c.sort_tmp, c.d_fctl, c.d_fsm, c.okey, c.ostart, c.oend, c.ccount,
+                c.bsum};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c.h_fctl) (void)hipHostFree(c.h_fctl);

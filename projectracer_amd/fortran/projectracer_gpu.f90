@@ -217,6 +217,11 @@ module projectracer_gpu
    integer(c_int32_t), value :: which
    integer(c_int64_t), intent(out) :: units
   end function
+  integer(c_int) function pt_projection_counts(ctx, windowed, radix) bind(C, name="pt_projection_counts")
+   import :: c_int, c_int64_t, c_ptr
+   type(c_ptr), value :: ctx
+   integer(c_int64_t), intent(out) :: windowed, radix
+  end function
   !! learned initial guess (include/projectracer_gpu.h, "learned initial guess")
   integer(c_int) function pt_net_create(seed, net) bind(C, name="pt_net_create")
    import :: c_int, c_int32_t, c_ptr

@@ -147,6 +147,12 @@ class Simulation:
                     "get_nempty")
         return out.reshape(self.nz + 1, self.nx + 1)
 
+    def projection_counts(self) -> tuple:
+        """(projections sorted by the windowed counting sort, by the radix sort)."""
+        a, b = C.c_int64(), C.c_int64()
+        self._check(self.lib.pt_projection_counts(self.ctx, C.byref(a), C.byref(b)), "projection_counts")
+        return a.value, b.value
+
     def info(self) -> dict:
         inf = _abi.PtSolveInfo()
         self._check(self.lib.pt_get_info(self.ctx, C.byref(inf)), "get_info")
