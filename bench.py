@@ -387,6 +387,24 @@ def cpu_baseline(args):
                 seconds_per_step=sec, tracer_updates_per_s=case.ntr / sec)
 
 
+# HBM bytes per launch of the mixed path's FP32 stage kernel, measured with
+# rocprofv3 FETCH_SIZE / WRITE_SIZE on the 1024x256 "law" workload and
+# calibrated on a known copy (the summary and its commands sit beside it)
+STAGE_TRAFFIC = ROOT / "profiles" / "k_stage_f32_general_1024x256_traffic.json"
+
+
+def stage_traffic(args, ev):
+    """Measured HBM bytes per stage launch, weighted by this run's launch
+    mix (level 1 against levels 2-4); None off the profiled workload."""
+    if args.workload != "law" or (args.nx, args.nz, args.tpc) != (1024, 256, 50) or not STAGE_TRAFFIC.exists():
+        return None
+    t = json.loads(STAGE_TRAFFIC.read_text())
+    l1, co = ev["stage_l1"]["launches"], ev["stage_coarse"]["launches"]
+    if l1 + co == 0:
+        return None
+    return (l1 * t["level1_bytes_per_launch"] + co * t["coarse_bytes_per_launch"]) / (l1 + co)
+
+
 def stage_roofline(ev, words, bytes_per_word, label):
     """Roofline of the RKC1 stage kernel from the event steps: all its
     launches (level 1 and the coarse levels swept a launch per stage), the
@@ -479,6 +497,10 @@ def main():
     roof = stage_roofline(ev, words, 4, "k_stage<float, %s> (RKC1 stage of the mixed path's FP32 V-cycle: "
                           "level 1 and the coarse levels swept one launch per stage)"
                           % ("general operator" if general else "isoviscous"))
+    if roof is not None:
+        roof["traffic"] = stage_traffic(args, ev)
+        if roof["traffic"] is not None:
+            roof["traffic_source"] = str(STAGE_TRAFFIC.relative_to(ROOT))
     sb, sb_parts = step_bytes(ev, mixed, words, 4)
     ntr = mixed["ntr"]
     rk_ms_step = ev["tracer_stages"]["ms"] / ev["steps"]
