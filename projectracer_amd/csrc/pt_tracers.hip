@@ -49,20 +49,31 @@ __device__ __forceinline__ int nint(double x) { return (int)round(x); }
 // cell from its slot's in either direction raises flag 1.
 __global__ void k_bin(const double* r, const double* s, int64_t n, int nx, int nz, double dr, double ds,
                       double aspect, int32_t* key, int32_t* flags, const int32_t* okey) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  double rv = r[j], sv = s[j];
-  int ncell = (nx + 1) * (nz + 1);
-  if (!(rv >= 0.0 && rv <= aspect && sv >= 0.0 && sv <= 1.0)) {
-    atomicOr(&flags[0], 1);
-    key[j] = ncell;
-    return;
+  // one flag store per wavefront that raises it: an under-resolved flow can
+  // move half the tracers farther than a cell, and same-address atomics
+  // from every one of them serialize
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = j < n;
+  const int ncell = (nx + 1) * (nz + 1);
+  bool outside = false, far = false;
+  if (live) {
+    const double rv = r[j], sv = s[j];
+    if (!(rv >= 0.0 && rv <= aspect && sv >= 0.0 && sv <= 1.0)) {
+      outside = true;
+      key[j] = ncell;
+    } else {
+      const int i = nint(rv / dr), k = nint(sv / ds);
+      key[j] = k * (nx + 1) + i;
+      if (okey) {
+        const int o = okey[j], oi = o % (nx + 1), ok = o / (nx + 1);
+        far = abs(oi - i) > 1 || abs(ok - k) > 1;
+      }
+    }
   }
-  int i = nint(rv / dr), k = nint(sv / ds);
-  key[j] = k * (nx + 1) + i;
-  if (okey) {
-    const int o = okey[j], oi = o % (nx + 1), ok = o / (nx + 1);
-    if (abs(oi - i) > 1 || abs(ok - k) > 1) atomicOr(&flags[1], 1);
+  const unsigned long long bo = __ballot(outside), bf = __ballot(far);
+  if ((threadIdx.x & 63) == 0) {
+    if (bo) atomicOr(&flags[0], 1);
+    if (bf) atomicOr(&flags[1], 1);
   }
 }
 
@@ -576,24 +587,28 @@ static void filter_pass(Context& c, const double* T0, double* out, int h, double
 // Filter loops driven on the device.  The reference checks the field's
 // smoothness before every pass and stops at the first check below the
 // threshold (PS/smoother.f90:582-672).  Here one kernel per pass does both
-// halves of a loop iteration on its input field: the check (its block maxima
-// folded into one slot with an integer atomicMax on the bit patterns,
-// order-free and exact since every |jump| is non-negative) and the pass
-// itself, written to the other buffer.  The pass is speculative: the kernel
-// of the next pass reads the check and, when it is below the threshold,
-// records the pass count and raises the `done` flag instead of going on, so
-// the field the loop returns is the input of the pass whose check succeeded,
-// as in the reference.  The host enqueues pass kernels in batches without
+// halves of a loop iteration on its input field: the check (a partial
+// maximum per check workgroup, stored in the pass's slot of a ring) and the
+// pass itself, written to the other buffer.  The pass is speculative: every
+// workgroup of the next pass kernel reduces the partials (max is order-free
+// and exact) and, when the check is below the threshold, the kernel records
+// the pass count and raises the `done` flag instead of going on, so the
+// field the loop returns is the input of the pass whose check succeeded, as
+// in the reference.  No atomics: 256 same-address atomics per pass cost more
+// than the pass itself.  The host enqueues pass kernels in batches without
 // waiting; once `done` is set every later kernel returns at once.
 // Passes alternate between two buffers; each writes its field's ghost
 // images itself (the reference's enforceBCs / enforceBCs_comp after the pass).
 // ---------------------------------------------------------------------------
-constexpr int FSLOTS = 64;   // ring of smoothness slots (a pass zeroes the slot after its own)
+// At most this many workgroups of a pass kernel take a share of the check,
+// striding over its points, each leaving one partial maximum.
+constexpr int SC_BLOCKS = 256;
+constexpr int FSLOTS = 64;   // ring of check slots, SC_BLOCKS partial maxima (FP64 bit patterns) each
 
 __global__ void k_filter_init(int32_t* ctl, unsigned long long* sm) {
   const int t = threadIdx.x;
   if (t < 4) ctl[t] = 0;
-  if (t < FSLOTS) sm[t] = 0ull;
+  for (int u = t; u < FSLOTS * SC_BLOCKS; u += blockDim.x) sm[u] = 0ull;
 }
 
 struct FilterArgs {
@@ -608,10 +623,6 @@ struct FilterArgs {
   int check_h, check_p; // the check's view of the field: halo and row pitch
 };
 
-// At most this many workgroups of a pass kernel fold a share of the check,
-// striding over its points, so one launch makes at most this many atomics on
-// the slot (one per 256 points made the atomics the check's cost).
-constexpr int SC_BLOCKS = 256;
 
 // Pass j of the loop, in -> out, on a (64, 4) block grid over the nodes
 // 0..nx x 0..nz.  First the decision on the previous kernel's check (the
@@ -622,56 +633,74 @@ constexpr int SC_BLOCKS = 256;
 // slot cur, and the pass (one Richardson pass, smoother_space at every node,
 // then the ghost images) into out.
 __global__ void __launch_bounds__(256) k_filter_pass(const double* in, double* out, FilterArgs a, int32_t* ctl,
-                                                     unsigned long long* sm, int prev, int cur, int next, int pass,
+                                                     unsigned long long* sm, int prev, int cur, int pass,
                                                      double cdts, double delta) {
-  if (ctl[0]) return;
   const int bid = blockIdx.y * gridDim.x + blockIdx.x;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-  if (pass > 1 && __longlong_as_double((long long)sm[prev]) < delta) {
+  const int nx = a.nx, nz = a.nz, h = a.h, p = a.p;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  const bool node = i <= nx && k <= nz;
+  const int ncheck = min((int)(gridDim.x * gridDim.y), SC_BLOCKS);
+  // The control word, the previous check and every load of `in` this
+  // thread needs are issued together, before the decision: the loads do
+  // not depend on it, and a kernel after the loop's end only reads.
+  const int32_t done = ctl[0];
+  // the previous check: one partial maximum per check workgroup
+  const double prev_part = pass > 1 && tid < ncheck ? __longlong_as_double((long long)sm[(long)prev * SC_BLOCKS + tid])
+                                                      : 0.0;
+  double y = 0.0;
+  if (node) {
+    auto F = [&](int u, int v) { return in[(long)(v + h) * p + (u + h)]; };
+    double sr = 0.0, ss = 0.0, srr = 0.0, sss = 0.0;
+    for (int q = -2; q <= 2; q++) sr = sr + a.D1[q + 2] * F(i + q, k);
+    for (int q = -2; q <= 2; q++) ss = ss + a.D1[q + 2] * F(i, k + q);
+    for (int q = -2; q <= 2; q++) srr = srr + a.D2[q + 2] * F(i + q, k);
+    for (int q = -2; q <= 2; q++) sss = sss + a.D2[q + 2] * F(i, k + q);
+    double Tr = sr / a.dr, Ts = ss / a.ds, Trr = srr / a.dr2, Tss = sss / a.ds2;
+    double Tx = Tr / 1.0, Tz = Ts / 1.0;
+    double Txx = (Trr - Tx * 0.0) / 1.0, Tzz = (Tss - Tz * 0.0) / 1.0;
+    double temp = Txx + Tzz;
+    y = F(i, k) + cdts * temp;
+  }
+  double d = 0.0;
+  if (bid < ncheck) {
+    const long npts = (long)nx * (nz - 1);
+    auto G = [&](int u, int v) { return in[(long)(v + a.check_h) * a.check_p + (u + a.check_h)]; };
+    for (long e = (long)bid * 256 + tid; e < npts; e += (long)ncheck * 256) {
+      const int kk = 1 + (int)(e / nx), ii = (int)(e - (long)(kk - 1) * nx);
+      const double t = G(ii, kk);
+      const double a1 = fabs(G(ii, kk - 1) - t), a2 = fabs(G(ii + 1, kk - 1) - t), a3 = fabs(G(ii + 1, kk) - t),
+                   a4 = fabs(G(ii, kk + 1) - t), a5 = fabs(G(ii + 1, kk + 1) - t);
+      d = fmax(d, fmax(fmax(fmax(fmax(a1, a2), a3), a4), a5));
+    }
+  }
+  if (done) return;
+  // both block maxima at once: the previous check's (the decision) and this
+  // pass's check (this block's partial)
+  __shared__ double red[2][4];
+  double pm = prev_part;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    pm = fmax(pm, __shfl_xor(pm, o, 64));
+    d = fmax(d, __shfl_xor(d, o, 64));
+  }
+  if ((tid & 63) == 0) {
+    red[0][tid >> 6] = pm;
+    red[1][tid >> 6] = d;
+  }
+  __syncthreads();
+  if (pass > 1 && fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3])) < delta) {
     if (bid == 0 && tid == 0) {
       ctl[1] = pass - 2;   // passes made before the successful check
       ctl[0] = 1;
     }
     return;
   }
-  if (bid == 0 && tid == 0) sm[next] = 0ull;
-  const int nx = a.nx, nz = a.nz, h = a.h, p = a.p;
-  const int ncheck = min((int)(gridDim.x * gridDim.y), SC_BLOCKS);
-  if (bid < ncheck) {
-    __shared__ double red[4];
-    const long npts = (long)nx * (nz - 1);
-    auto G = [&](int u, int v) { return in[(long)(v + a.check_h) * a.check_p + (u + a.check_h)]; };
-    double d = 0.0;
-    for (long e = (long)bid * 256 + tid; e < npts; e += (long)ncheck * 256) {
-      const int k = 1 + (int)(e / nx), i = (int)(e - (long)(k - 1) * nx);
-      const double t = G(i, k);
-      const double a1 = fabs(G(i, k - 1) - t), a2 = fabs(G(i + 1, k - 1) - t), a3 = fabs(G(i + 1, k) - t),
-                   a4 = fabs(G(i, k + 1) - t), a5 = fabs(G(i + 1, k + 1) - t);
-      d = fmax(d, fmax(fmax(fmax(fmax(a1, a2), a3), a4), a5));
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) d = fmax(d, __shfl_xor(d, o, 64));
-    if ((tid & 63) == 0) red[tid >> 6] = d;
-    __syncthreads();
-    if (tid == 0) {
-      const double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-      atomicMax(&sm[cur], (unsigned long long)__double_as_longlong(m));
-    }
-  }
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i > nx || k > nz) return;
-  auto F = [&](int u, int v) { return in[(long)(v + h) * p + (u + h)]; };
+  if (bid < ncheck && tid == 0)
+    sm[(long)cur * SC_BLOCKS + bid] =
+        (unsigned long long)__double_as_longlong(fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3])));
+  if (!node) return;
   auto W = [&](int u, int v, double val) { out[(long)(v + h) * p + (u + h)] = val; };
-  double sr = 0.0, ss = 0.0, srr = 0.0, sss = 0.0;
-  for (int q = -2; q <= 2; q++) sr = sr + a.D1[q + 2] * F(i + q, k);
-  for (int q = -2; q <= 2; q++) ss = ss + a.D1[q + 2] * F(i, k + q);
-  for (int q = -2; q <= 2; q++) srr = srr + a.D2[q + 2] * F(i + q, k);
-  for (int q = -2; q <= 2; q++) sss = sss + a.D2[q + 2] * F(i, k + q);
-  double Tr = sr / a.dr, Ts = ss / a.ds, Trr = srr / a.dr2, Tss = sss / a.ds2;
-  double Tx = Tr / 1.0, Tz = Ts / 1.0;
-  double Txx = (Trr - Tx * 0.0) / 1.0, Tzz = (Tss - Tz * 0.0) / 1.0;
-  double temp = Txx + Tzz;
-  double y = F(i, k) + cdts * temp;
   // the columns whose ghost images equal this node's column: itself, and
   // its mirror beyond a side wall when it lies within h of that wall
   int cols[2] = {i, i};
@@ -718,7 +747,7 @@ static int filter_loop(Context& c, double* x0, double* x1, const FilterArgs& fa,
   dim3 gflt = grid2(c.T.nx + 1, c.T.nz + 1, b16);
   unsigned long long* sm = c.d_fsm;
   if (c.timing) timer_begin(c, PT_K_FILTER);
-  hipLaunchKernelGGL(k_filter_init, dim3(1), dim3(64), 0, c.stream, c.d_fctl, sm);
+  hipLaunchKernelGGL(k_filter_init, dim3(1), dim3(256), 0, c.stream, c.d_fctl, sm);
   const int cap = 100000;
   // the first batch covers the previous call's pass count (consecutive
   // projections need similar counts) and the kernel that decides on its
@@ -730,7 +759,7 @@ static int filter_loop(Context& c, double* x0, double* x1, const FilterArgs& fa,
       const double* in = (j & 1) ? x0 : x1;
       double* out = (j & 1) ? x1 : x0;
       hipLaunchKernelGGL(k_filter_pass, gflt, b16, 0, c.stream, in, out, fa, c.d_fctl, sm, (j - 1) % FSLOTS,
-                         j % FSLOTS, (j + 1) % FSLOTS, j, cdts_of(c, j), delta);
+                         j % FSLOTS, j, cdts_of(c, j), delta);
     }
     PT_HIP(hipMemcpyAsync(c.h_fctl, c.d_fctl, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
     PT_HIP(hipStreamSynchronize(c.stream));
@@ -1578,7 +1607,7 @@ int tracers_alloc(Context& c) {
   DALLOC(c.cellsum, sizeof(double) * 27 * (size_t)ncell);
   DALLOC(c.d_flags, sizeof(int32_t) * 4);
   DALLOC(c.d_fctl, sizeof(int32_t) * 4);
-  DALLOC(c.d_fsm, sizeof(unsigned long long) * FSLOTS);
+  DALLOC(c.d_fsm, sizeof(unsigned long long) * FSLOTS * SC_BLOCKS);
   PT_HIP(hipHostMalloc((void**)&c.h_fctl, sizeof(int32_t) * 4, hipHostMallocDefault));
   hipLaunchKernelGGL(k_iota, dim3((unsigned)((c.T.ntr + 255) / 256)), dim3(256), 0, 0, c.iota, c.T.ntr);
   c.sort_bits = 1;
