@@ -638,9 +638,16 @@ __global__ void __launch_bounds__(256) k_filter_pass(const double* in, double* o
   const int bid = blockIdx.y * gridDim.x + blockIdx.x;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   const int nx = a.nx, nz = a.nz, h = a.h, p = a.p;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  // XCD-aware tiles: workgroups are dispatched round-robin over the 8 XCDs
+  // (bid % 8); each XCD gets a contiguous band of tiles, the same every
+  // pass, so a tile's field and most of its halo stay in that XCD's L2
+  const int nblk = (int)(gridDim.x * gridDim.y);
+  const int q8 = nblk / 8, r8 = nblk % 8, xcd = bid % 8;
+  const int lb = xcd * q8 + min(xcd, r8) + bid / 8;
+  const int tx = lb % (int)gridDim.x, ty = lb / (int)gridDim.x;
+  const int i = tx * blockDim.x + threadIdx.x, k = ty * blockDim.y + threadIdx.y;
   const bool node = i <= nx && k <= nz;
-  const int ncheck = min((int)(gridDim.x * gridDim.y), SC_BLOCKS);
+  const int ncheck = min(nblk, SC_BLOCKS);
   // The control word, the previous check and every load of `in` this
   // thread needs are issued together, before the decision: the loads do
   // not depend on it, and a kernel after the loop's end only reads.
@@ -666,7 +673,16 @@ __global__ void __launch_bounds__(256) k_filter_pass(const double* in, double* o
   if (bid < ncheck) {
     const long npts = (long)nx * (nz - 1);
     auto G = [&](int u, int v) { return in[(long)(v + a.check_h) * a.check_p + (u + a.check_h)]; };
-    for (long e = (long)bid * 256 + tid; e < npts; e += (long)ncheck * 256) {
+    // with a full set of check workgroups, the 32 on each XCD stride over
+    // that XCD's eighth of the points, near its band of tiles
+    long e0 = (long)bid * 256 + tid, e_end = npts, estride = (long)ncheck * 256;
+    if (ncheck == SC_BLOCKS) {
+      const int x = bid % 8, jx = bid / 8;
+      e0 = npts * x / 8 + (long)jx * 256 + tid;
+      e_end = npts * (x + 1) / 8;
+      estride = (long)(SC_BLOCKS / 8) * 256;
+    }
+    for (long e = e0; e < e_end; e += estride) {
       const int kk = 1 + (int)(e / nx), ii = (int)(e - (long)(kk - 1) * nx);
       const double t = G(ii, kk);
       const double a1 = fabs(G(ii, kk - 1) - t), a2 = fabs(G(ii + 1, kk - 1) - t), a3 = fabs(G(ii + 1, kk) - t),
