@@ -20,9 +20,10 @@
  * Supported subset (what the hot path and its configurations use):
  * uniform grid (gridX=gridZ=0), order=4, Database=2, iterate=1, MG=1,
  * RKC1=1, splitting=0, non_Newtonian=0, plastic=0, comp=1, ntype=1,
- * iC_continuous=1, shape_function=1, tracer=1, torder=4, RK_update=1,
- * vel_update=1, local_time=0, Chebyshev=0, CFL=0, equalize=0,
- * iheattype=0.  pt_create rejects anything else with PT_ERR_UNSUPPORTED.
+ * iC_continuous=1 or 0, shape_function=1, tracer=1, torder=4, RK_update=1,
+ * vel_update=1, local_time=0, Chebyshev=0, CFL=0, equalize=0 (or 1 with
+ * original=1 and iC_continuous=0), iheattype=0.  pt_create rejects anything
+ * else with PT_ERR_UNSUPPORTED.
  */
 #ifndef PROJECTRACER_GPU_H
 #define PROJECTRACER_GPU_H
@@ -92,6 +93,16 @@ typedef struct pt_params {
   int32_t iterate;        /* 0 direct banded LU (PS/momentum.f90:172-653), 1 multigrid */
   int32_t Database;       /* direct solver unknown order (PS/momentum.f90:238-253):
                              1 p = (nx+1) k + i + 1, 2 p = (nz+1) i + k + 1 */
+  /* &inputs: tracer composition and redistribution */
+  int32_t iC_continuous;  /* 1: continuous C carried by the tracers (Ctr); 0: discrete
+                             types Ttype in {0,1} (PS/composition.f90:1090-1276),
+                             exchanged as Ctr = 0.0 / 1.0 */
+  int32_t equalize;       /* 1: redistribute tracers at the start of every energy_time
+                             (PS/energy.f90:198-204); needs iC_continuous = 0 */
+  int32_t original;       /* 1: equalize_tracers_original (PS/composition.f90:84-263),
+                             the only redistribution built */
+  int32_t buffer;         /* half-width of the single-type neighbourhood test (10) */
+  double  dist_factor;    /* wall distance of redistributed tracers, in cells (1e-4) */
 } pt_params;
 
 /* Fill *p with the reference defaults used by this build (inputs.nml values
@@ -163,7 +174,9 @@ int pt_get_dims(void* ctx, int32_t* nx, int32_t* nz, int32_t* grids, int64_t* nt
  * (PS/basics.f90:352-359, PS/composition.f90:848) and PS/main.f90:228-229:
  * T node values (0:nx,0:nz), the isothermal boundary values, and tracer
  * positions/values in ID order (ntr each); projects the tracers
- * (tracers_to_corners) and zeroes psi and the coarse-level corrections. */
+ * (tracers_to_corners) and zeroes psi and the coarse-level corrections.
+ * With discrete types (iC_continuous=0) Ctr carries Ttype: every value must
+ * be 0.0 or 1.0 (PT_ERR_INVALID otherwise). */
 int pt_load_state(void* ctx, const double* T_nodes, double Tbot_iso, double Ttop_iso,
                   const double* rtr, const double* str, const double* Ttr,
                   const double* Ctr);
@@ -219,9 +232,10 @@ int pt_stats_noalloc(void* ctx, pt_stats_row* row);
  * pt_write_restart: print_restart (PS/main.f90:457-496) into directory dir:
  *   Trestart        xg(-6:nx+6), zg(-6:nz+6), T(-3:nx+3,-3:nz+3),
  *                   Cnew(1,-5:nx+5,-5:nz+5), SF(-3:nx+3,-3:nz+3)
- *   restart_tracers rtr, str, Ctr, Ttr in tracer ID order
+ *   restart_tracers rtr, str, Ctr, Ttr in tracer ID order (discrete types,
+ *                   iC_continuous=0: rtr, str, Ttype as INTEGER*4, Ttr)
  * each one Fortran unformatted sequential record (4-byte length before and
- * after), the files of a RaT /= 0, comp=1, tracer=1, iC_continuous=1 run.
+ * after), the files of a RaT /= 0, comp=1, tracer=1 run.
  * T is the module T (the initial field before the first energy_time).
  * pt_read_restart: the restart=1 start (PS/grid.f90:55-103, PS/main.f90:
  *   204-209): reads both files, re-projects the tracers onto T and Cnew
@@ -269,6 +283,20 @@ int pt_kernel_work(void* ctx, int32_t which, int64_t* units);
  * farther move or one in the last 8 projections, or PT_PROJECT_RADIX=1 at
  * pt_create); both give the same order, element for element */
 int pt_projection_counts(void* ctx, int64_t* windowed, int64_t* radix);
+
+/* ---- tracer redistribution (equalize=1, original=1, iC_continuous=0) -----
+ * equalize_tracers_original (PS/composition.f90:84-263) runs on the device
+ * at the start of every energy_time.  nequalize: calls that redistributed
+ * (the reference's counter, :151-155); moved_last / moved_total: tracers
+ * moved by the last such call and by all of them. */
+int pt_equalize_counts(void* ctx, int64_t* nequalize, int64_t* moved_last, int64_t* moved_total);
+/* mzran stream 0 (PS/composition.f90:7-20), the stream the redistribution's
+ * lottery draws from: state = seed_i(0), seed_j(0), seed_k(0), seed_n(0).
+ * pt_create starts it at PS/basics.f90:321-324's values; a host that drew
+ * the initial cloud from stream 0 (one OpenMP thread) sets the state it
+ * left. */
+int pt_get_rng_state(void* ctx, int32_t* state);
+int pt_set_rng_state(void* ctx, const int32_t* state);
 
 /* ---- learned initial guess (no reference counterpart) -------------------
  * A four-level convolutional encoder-decoder (3x3 convolutions, channels

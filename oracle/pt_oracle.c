@@ -123,7 +123,18 @@ struct pto_case {
   int err;
   /* stats: entrainment normalisation (set by the first call after a load) */
   double mass; int mass_set; int kent;
+  /* discrete tracer types (iC_continuous=0): count_type(0:1,-buffer:nx+buffer,
+     -buffer:nz+buffer) and nempty over the same range; tracer redistribution:
+     mzran stream 0 and counters */
+  int32_t *ct0, *ct1, *ne_h; int cb;
+  int32_t seed[4];
+  int64_t nequalize, moved_last, moved_total;
 };
+#define CTH(a, c, i, k) ((a)[(long)((k) + (c)->cb) * ((c)->nx + 1 + 2 * (c)->cb) + ((i) + (c)->cb)])
+
+/* mzran: PS/composition.f90:7-20, thread-0 seeds PS/basics.f90:321-324 */
+typedef struct { int32_t i, j, k, n; } mz_state;
+static double mzran(mz_state* st);
 
 #define DER(a, c, jj, i, k) ((a)[((long)((k) + 3) * ((c)->nx + 1) + (i)) * 3 + ((jj) - 1)])
 
@@ -1453,10 +1464,79 @@ static inline double shape_func(pto_case* c, int i, int k, double r, double s) {
   return a * b;
 }
 
-/* tracers_to_corners_iC_continuous: PS/composition.f90:942-1087 (one thread) */
+/* tracers_to_corners_OG with shape_function=1: PS/composition.f90:1090-1276
+ * (one thread, so each sum runs in ID order): per-type counts of the
+ * nearest-node tracers and per-type shape-function sums; nempty_real =
+ * sum over types, C = count_type_real(1)/nempty_real, T = T1/nempty_real at
+ * nodes with nempty_real > 0.  count_type is kept with its halos mirrored
+ * over `buffer` nodes as the reference does (:1177-1194); corners stay 0. */
+static int tracers_to_corners_og(pto_case* c, const double* rtr_, const double* str_, const double* Ttr_) {
+  int nx = c->nx, nz = c->nz, b = c->cb;
+  long nw = (long)(nx + 7) * (nz + 7), nh = (long)(nx + 1 + 2 * b) * (nz + 1 + 2 * b);
+  /* TxS: T1, CxS: count_type_real(1), shape_sum: count_type_real(0) */
+  memset(c->TxS, 0, sizeof(double) * nw); memset(c->CxS, 0, sizeof(double) * nw);
+  memset(c->shape_sum, 0, sizeof(double) * nw);
+  memset(c->ct0, 0, sizeof(int32_t) * nh); memset(c->ct1, 0, sizeof(int32_t) * nh);
+#define WK(i, k) ((long)((k) + 3) * (nx + 7) + ((i) + 3))
+  for (int64_t id = 0; id < c->ntr; id++) {
+    double r = rtr_[id], s = str_[id];
+    int i = (int)lround(r / c->dr), k = (int)lround(s / c->ds);
+    if (r < 0.0 || r > c->aspect || s < 0.0 || s > 1.0) return PT_ERR_TRACER_OUTSIDE;
+    int type = c->Ctr[id] != 0.0;
+    if (type) CTH(c->ct1, c, i, k) += 1; else CTH(c->ct0, c, i, k) += 1;
+    double* cr = type ? c->CxS : c->shape_sum;
+    for (int a = -1; a <= 1; a++)
+      for (int bb = -1; bb <= 1; bb++)
+        c->TxS[WK(i + a, k + bb)] = c->TxS[WK(i + a, k + bb)] + Ttr_[id] * shape_func(c, i + a, k + bb, r, s);
+    for (int a = -1; a <= 1; a++)
+      for (int bb = -1; bb <= 1; bb++)
+        cr[WK(i + a, k + bb)] = cr[WK(i + a, k + bb)] + shape_func(c, i + a, k + bb, r, s);
+  }
+  int32_t* cts[2] = {c->ct0, c->ct1};
+  for (int tt = 0; tt <= 1; tt++) {
+    int32_t* ct = cts[tt];
+    for (int i = 1; i <= b; i++)
+      for (int k = 0; k <= nz; k++) {
+        CTH(ct, c, -i, k) = CTH(ct, c, i, k);
+        CTH(ct, c, nx + i, k) = CTH(ct, c, nx - i, k);
+      }
+    for (int k = 1; k <= b; k++)
+      for (int i = 0; i <= nx; i++) {
+        CTH(ct, c, i, -k) = CTH(ct, c, i, k);
+        CTH(ct, c, i, nz + k) = CTH(ct, c, i, nz - k);
+      }
+  }
+  for (long q = 0; q < nh; q++) c->ne_h[q] = c->ct0[q] + c->ct1[q];
+  for (int k = 0; k <= nz; k++)
+    for (int i = 0; i <= nx; i++) {
+      c->nempty[(long)k * (nx + 1) + i] = CTH(c->ne_h, c, i, k);
+      double ner = c->shape_sum[WK(i, k)] + c->CxS[WK(i, k)];
+      if (ner > 0.0) {
+        AT(c->Cnew, i, k) = c->CxS[WK(i, k)] / ner;
+        AT(c->Tratio, i, k) = c->TxS[WK(i, k)] / ner;
+      }
+    }
+#undef WK
+  return PT_OK;
+}
+
+/* tracers_to_corners: PS/composition.f90:923-940 (the wrapper), the
+ * iC_continuous=1 body :942-1087 (one thread) or tracers_to_corners_OG;
+ * smooth = 0 is the reference's smooth_vis=0 (no smoother_time calls) */
+static int tracers_to_corners_smooth(pto_case* c, const double* rtr_, const double* str_, const double* Ttr_,
+                                     int smooth);
 static int tracers_to_corners(pto_case* c, const double* rtr_, const double* str_, const double* Ttr_) {
+  return tracers_to_corners_smooth(c, rtr_, str_, Ttr_, 1);
+}
+static int tracers_to_corners_smooth(pto_case* c, const double* rtr_, const double* str_, const double* Ttr_,
+                                     int smooth) {
   int nx = c->nx, nz = c->nz;
   a2_copy(&c->Tratio, &c->T0);
+  if (c->P.iC_continuous == 0) {
+    int rc = tracers_to_corners_og(c, rtr_, str_, Ttr_);
+    if (rc) return rc;
+    goto post;
+  }
   long nw = (long)(nx + 7) * (nz + 7);
   memset(c->TxS, 0, sizeof(double) * nw); memset(c->CxS, 0, sizeof(double) * nw);
   memset(c->shape_sum, 0, sizeof(double) * nw);
@@ -1490,10 +1570,11 @@ static int tracers_to_corners(pto_case* c, const double* rtr_, const double* str
       }
     }
 #undef WK
+post:
   enforce_bcs_comp(c, &c->Cnew);
-  smoother_time(c);
+  if (smooth) smoother_time(c);
   enforce_bcs_T(c, &c->Tratio);
-  smoother_time_T(c, &c->Tratio);
+  if (smooth) smoother_time_T(c, &c->Tratio);
   load_conductivity(c);
   return PT_OK;
 }
@@ -1571,11 +1652,125 @@ static void stability_rk4(pto_case* c) {
   c->dt = c->P.courant * dt;
 }
 
+/* equalize_tracers_original: PS/composition.f90:84-263 with comp=1,
+ * shape_function=1, ntype=1, tracer=1, tpc_max = tpc_min = tpc (:97-98);
+ * count_type and nempty as the last tracers_to_corners_OG left them.  The
+ * types are Ctr (0.0 / 1.0). */
+static int equalize_original(pto_case* c) {
+  const int nx = c->nx, nz = c->nz, b = c->cb, tpc = c->P.tpc;
+  const long nn = (long)(nx + 1) * (nz + 1);
+  int32_t* nexcess = (int32_t*)calloc((size_t)nn, sizeof(int32_t));
+#define NEX(i, k) nexcess[(long)(k) * (nx + 1) + (i)]
+  /* which nodes have excess tracers (:124-137) */
+  int32_t* cts[2] = {c->ct0, c->ct1};
+  for (int i = 0; i <= nx; i++)
+    for (int k = 0; k <= nz; k++) {
+      NEX(i, k) = 0;
+      for (int tt = 0; tt <= 1; tt++) {
+        int all = 1;
+        for (int kk = k - b; kk <= k + b && all; kk++)
+          for (int ii = i - b; ii <= i + b; ii++)
+            if (CTH(cts[tt], c, ii, kk) != CTH(c->ne_h, c, ii, kk)) { all = 0; break; }
+        if (all && CTH(c->ne_h, c, i, k) > 0) {
+          NEX(i, k) = CTH(c->ne_h, c, i, k) - tpc;
+          break;
+        }
+      }
+    }
+  long nfree = 0;
+  for (long q = 0; q < nn; q++) if (nexcess[q] > 0) nfree += nexcess[q];
+  /* is equalize worth doing? (:150-155) */
+  int all_within = 1;
+  for (long q = 0; q < nn; q++) if (!(nexcess[q] < 0 && nexcess[q] > 0)) { all_within = 0; break; }
+  if (all_within || nfree == 0) { free(nexcess); return PT_OK; }
+  c->nequalize++;
+  /* T interpolation table (:157-160) */
+  extend_T(c, &c->Tratio);
+  compute_derivatives(c, &c->Textend, c->DERr_tracer);
+  /* donors: near the walls first, then the others, each in ID order (:162-190) */
+  int64_t* IDsave = (int64_t*)calloc((size_t)nfree + 1, sizeof(int64_t));
+  const double dist_r = c->P.dist_factor * c->dr, dist_s = c->P.dist_factor * c->ds;   /* PS/basics.f90:276-277 */
+  long p = 1;
+  for (int64_t id = 0; id < c->ntr; id++) {
+    int i = (int)lround(c->rtr[id] / c->dr), k = (int)lround(c->str[id] / c->ds);
+    if (NEX(i, k) > 0) {
+      if (c->str[id] < dist_s || c->str[id] > 1.0 - dist_s || c->rtr[id] < dist_r || c->rtr[id] > c->aspect - dist_r) {
+        IDsave[p] = id; NEX(i, k) -= 1; p++;
+      }
+    }
+  }
+  for (int64_t id = 0; id < c->ntr; id++) {
+    int i = (int)lround(c->rtr[id] / c->dr), k = (int)lround(c->str[id] / c->ds);
+    if (NEX(i, k) > 0) {
+      if (c->str[id] >= dist_s && c->str[id] <= 1.0 - dist_s && c->rtr[id] >= dist_r &&
+          c->rtr[id] <= c->aspect - dist_r) {
+        IDsave[p] = id; NEX(i, k) -= 1; p++;
+        if (p > nfree) break;
+      }
+    }
+  }
+  /* lottery (:192-238) */
+  int32_t *istore = (int32_t*)calloc((size_t)nn + 1, sizeof(int32_t)),
+          *kstore = (int32_t*)calloc((size_t)nn + 1, sizeof(int32_t));
+  mz_state st = {c->seed[0], c->seed[1], c->seed[2], c->seed[3]};
+  p = 1;
+  for (;;) {
+    int minexcess = nexcess[0];
+    for (long q = 1; q < nn; q++) if (nexcess[q] < minexcess) minexcess = nexcess[q];
+    if (minexcess >= 0) { nfree = p - 1; break; }
+    long ncells = 0;
+    for (long q = 0; q < nn; q++) ncells += nexcess[q] == minexcess;
+    long q = 1;
+    for (int i = 0; i <= nx && q <= ncells; i++)
+      for (int k = 0; k <= nz; k++)
+        if (NEX(i, k) == minexcess) {
+          istore[q] = i; kstore[q] = k; q++;
+          if (q > ncells) break;
+        }
+    for (long pp = 1; pp <= ncells; pp++) {
+      double ran_q = mzran(&st);
+      q = (long)floor((double)ncells * ran_q) + 1;
+      int i = istore[q], k = kstore[q];
+      double r = c->dr * (double)i, s = c->ds * (double)k;
+      double perturb_r = mzran(&st), perturb_s = mzran(&st);
+      perturb_r = (perturb_r - 0.5) * c->dr;
+      perturb_s = (perturb_s - 0.5) * c->ds;
+      int64_t id = IDsave[p];
+      c->rtr[id] = r + perturb_r;
+      c->str[id] = s + perturb_s;
+      if (c->rtr[id] <= 0.0) c->rtr[id] = dist_r;
+      if (c->rtr[id] >= c->aspect) c->rtr[id] = c->aspect - dist_r;
+      if (c->str[id] <= 0.0) c->str[id] = dist_s;
+      if (c->str[id] >= 1.0) c->str[id] = 1.0 - dist_s;
+      /* Ttype = maxloc(count_type(0:ntype,i,k)) - 1: the first maximum */
+      c->Ctr[id] = CTH(c->ct1, c, i, k) > CTH(c->ct0, c, i, k) ? 1.0 : 0.0;
+      NEX(i, k) += 1;
+      p++;
+      if (p > nfree) break;
+    }
+    if (p > nfree) break;
+  }
+  c->seed[0] = st.i; c->seed[1] = st.j; c->seed[2] = st.k; c->seed[3] = st.n;
+  /* T of the moved tracers (:241-250) */
+  for (long pp = 1; pp <= nfree; pp++) {
+    int64_t id = IDsave[pp];
+    c->Ttr[id] = interpolate(c, c->rtr[id], c->str[id], c->DERr_tracer, &c->Textend);
+  }
+  c->moved_last = nfree;
+  c->moved_total += nfree;
+  free(istore); free(kstore); free(IDsave); free(nexcess);
+#undef NEX
+  /* tracers_to_corners(rtr,str,Ttr,T) with smooth_vis=0 (:256-259) */
+  a2_copy(&c->T0, &c->T);
+  return tracers_to_corners_smooth(c, c->rtr, c->str, c->Ttr, 0);
+}
+
 /* energy_time: PS/energy.f90:181-556 (tracer=1, comp=1, torder=4, RK_update=1) */
 int pto_energy_time(pto_case* c) {
   int rc;
   int64_t n = c->ntr;
   stability_rk4(c);
+  if (c->P.equalize == 1 && (rc = equalize_original(c))) return rc;   /* PS/energy.f90:198-204 */
   double dt = c->dt;
   a2_copy(&c->C0, &c->Cnew);
   a2_copy(&c->T0, &c->Tratio);
@@ -1673,6 +1868,8 @@ static int check_supported(const pt_params* p) {
   if (p->RaT == 0.0) return PT_ERR_UNSUPPORTED;
   if (p->iterate != 0 && p->iterate != 1) return PT_ERR_INVALID;
   if (p->iterate == 0 && (p->ivis != 0 || (p->Database != 1 && p->Database != 2))) return PT_ERR_UNSUPPORTED;
+  if ((p->iC_continuous != 0 && p->iC_continuous != 1) || (p->equalize != 0 && p->equalize != 1)) return PT_ERR_INVALID;
+  if (p->equalize == 1 && (p->iC_continuous != 0 || p->original != 1 || p->buffer < 1)) return PT_ERR_UNSUPPORTED;
   return PT_OK;
 }
 
@@ -1698,6 +1895,15 @@ int pto_create(const pt_params* p, pto_case** out) {
     c->dr_power[j] = pow(c->dr, (double)j); c->ds_power[j] = pow(c->ds, (double)j);
   }
   c->ntr = (int64_t)(c->nx + 1) * (c->nz + 1) * p->tpc;
+  c->seed[0] = 521288629; c->seed[1] = 362436069; c->seed[2] = 16163801; c->seed[3] = 1131199299;
+  if (p->iC_continuous == 0) {
+    /* count_type(0:1,-buffer:nx+buffer,-buffer:nz+buffer) (PS/basics.f90:115-117) */
+    c->cb = p->buffer > 0 ? p->buffer : 1;
+    size_t nh = (size_t)(c->nx + 1 + 2 * c->cb) * (c->nz + 1 + 2 * c->cb);
+    c->ct0 = (int32_t*)calloc(nh, sizeof(int32_t));
+    c->ct1 = (int32_t*)calloc(nh, sizeof(int32_t));
+    c->ne_h = (int32_t*)calloc(nh, sizeof(int32_t));
+  }
   for (int j = 1; j <= c->grids; j++) {
     c->nxg[j] = c->nx / (1 << (j - 1)); c->nzg[j] = c->nz / (1 << (j - 1));
     c->drg[j] = c->aspect / (double)c->nxg[j]; c->dsg[j] = 1.0 / (double)c->nzg[j];
@@ -1816,7 +2022,18 @@ void pto_destroy(pto_case* c) {
   double* tarr[] = {c->rtr, c->str, c->Ttr, c->Ctr, c->rtr0, c->rtr1, c->rtr2, c->str0, c->str1, c->str2, c->Ttr0, c->Ttr1, c->Ttr2};
   for (int j = 0; j < 13; j++) free(tarr[j]);
   free(c->nempty); free(c->TxS); free(c->CxS); free(c->shape_sum);
+  free(c->ct0); free(c->ct1); free(c->ne_h);
   free(c);
+}
+
+void pto_equalize_counts(pto_case* c, int64_t* nequalize, int64_t* moved_last, int64_t* moved_total) {
+  *nequalize = c->nequalize; *moved_last = c->moved_last; *moved_total = c->moved_total;
+}
+void pto_rng_state(pto_case* c, int32_t* get, const int32_t* set) {
+  for (int q = 0; q < 4; q++) {
+    if (set) c->seed[q] = set[q];
+    if (get) get[q] = c->seed[q];
+  }
 }
 
 void pto_dims(pto_case* c, int32_t* nx, int32_t* nz, int32_t* grids, int64_t* ntr) {
@@ -2050,8 +2267,6 @@ int pto_interpolate(pto_case* c, int32_t field_id, int64_t n, const double* r, c
   free(der);
   return PT_OK;
 }
-/* mzran: PS/composition.f90:7-20, thread-0 seeds PS/basics.f90:321-324 */
-typedef struct { int32_t i, j, k, n; } mz_state;
 static double mzran(mz_state* st) {
   int32_t ran_int = st->i - st->k;
   if (ran_int < 0) ran_int = ran_int + 2147483579;

@@ -46,6 +46,11 @@ void pto_get_info(pto_case* c, pt_solve_info* info);
  * the entrainment normalisation as the reference's tstep=0 call does */
 int  pto_stats(pto_case* c, pt_stats_row* row);
 
+/* tracer redistribution (equalize=1): the counters of pt_equalize_counts,
+ * and mzran stream 0's state (get and/or set, either may be null) */
+void pto_equalize_counts(pto_case* c, int64_t* nequalize, int64_t* moved_last, int64_t* moved_total);
+void pto_rng_state(pto_case* c, int32_t* get, const int32_t* set);
+
 /* unit-level hooks for closed-form tests */
 /* stencil tables: D1..D4 (-3:3) and the interpolation table D(1:4,-3:3) */
 void pto_fd_tables(pto_case* c, double* D1, double* D2, double* D3, double* D4, double* Dtab);

@@ -13,7 +13,10 @@
 !!     instead of the reference's snapshots (dump_every<0: no state files;
 !!     each loop iteration's wall time is printed for timing runs).
 !! With iterate=0 the set-up and the velocity solve follow main.f90's direct
-!! branches (PS/main.f90:189-203, 224-226, 247-251).
+!! branches (PS/main.f90:189-203, 224-226, 247-251).  With discrete tracer
+!! types (iC_continuous=0) the injected and written C values are Ttype as
+!! 0.0 / 1.0; with equalize=1 every step also prints nequalize and the
+!! state of mzran stream 0.
 !! Nothing in the reference sources is modified.
 program ref_driver
  use basics
@@ -21,7 +24,7 @@ program ref_driver
  implicit none
  integer*4 :: nrun, inject, dump_every, level, i, k, step_dump
  real*8 :: t1, t2, t_solve0, t_it0, t_it1
- real*8, allocatable :: Tn(:,:)
+ real*8, allocatable :: Tn(:,:), Cin(:)
  character(len=32) :: fname
  namelist /driver/ nrun, inject, dump_every
 
@@ -69,7 +72,14 @@ program ref_driver
   open(unit=601,file="inject.bin",access='stream',form='unformatted',status='old')
   read(601) Tbot_iso,Ttop_iso
   read(601) Tn
-  read(601) rtr,str,Ttr,Ctr
+  if (iC_continuous.eq.1) then
+   read(601) rtr,str,Ttr,Ctr
+  else
+   allocate(Cin(ntr))
+   read(601) rtr,str,Ttr,Cin
+   Ttype=nint(Cin)
+   deallocate(Cin)
+  end if
   close(601)
   T(0:nx,0:nz)=Tn
   T(:,0)=Tbot_iso
@@ -106,6 +116,8 @@ program ref_driver
   if (step_dump.eq.1) call write_state(tstep)
   if (tstep.eq.nrun) exit
   call energy_time
+  if (equalize.eq.1) write(*,'(a,i8,4(i12))') " driver: nequalize,seeds=",nequalize,seed_i(0),seed_j(0),&
+   &seed_k(0),seed_n(0)
   time=time+dt
   t_it1=omp_get_wtime_d()
   write(*,'(a,i6,3(g25.16))') " driver: step done",tstep,time,dt,t_it1-t_it0
@@ -147,7 +159,11 @@ contains
    write(603) vis_grid(:,:,1)
   end if
   write(603) nempty(0:nx,0:nz)
-  write(603) rtr,str,Ttr,Ctr
+  if (iC_continuous.eq.1) then
+   write(603) rtr,str,Ttr,Ctr
+  else
+   write(603) rtr,str,Ttr,real(Ttype,8)
+  end if
   close(603)
  end subroutine
 

@@ -72,6 +72,8 @@ class PtParams(C.Structure):
         ("eig_ratio_min", C.c_double), ("visC", C.c_double), ("delta_visC", C.c_double),
         ("precision", C.c_int32), ("guess", C.c_int32),
         ("dent", C.c_double), ("iterate", C.c_int32), ("Database", C.c_int32),
+        ("iC_continuous", C.c_int32), ("equalize", C.c_int32), ("original", C.c_int32),
+        ("buffer", C.c_int32), ("dist_factor", C.c_double),
     ]
 
 
@@ -109,7 +111,7 @@ def default_params(**overrides) -> PtParams:
         tpc=20, Tbot=0, Vbc=0, RaT=1e5, RaC=0.0, H=0.0, conduct_factor=1.0, Htype=0.0,
         ivis=0, visP=1.0, visT=1e3, delta_visT=2.0, eig_ratio_min=2.0, visC=10.0,
         delta_visC=2.0, precision=PT_PREC_FP64, guess=PT_GUESS_WARM, dent=0.5,
-        iterate=1, Database=2)
+        iterate=1, Database=2, iC_continuous=1, equalize=0, original=1, buffer=10, dist_factor=1e-4)
     for k, v in overrides.items():
         setattr(p, k, v)
     return p
@@ -192,6 +194,9 @@ def load_library(path: os.PathLike | None = None):
         "pt_guess_counts": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "pt_guess_last": (C.c_int, [vp, dp, dp, dp]),
         "pt_projection_counts": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "pt_equalize_counts": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "pt_get_rng_state": (C.c_int, [vp, C.POINTER(C.c_int32)]),
+        "pt_set_rng_state": (C.c_int, [vp, C.POINTER(C.c_int32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -214,5 +219,5 @@ EXPORTED_SYMBOLS = [
     "pt_ensemble_workers", "pt_net_create", "pt_net_destroy", "pt_net_num_params", "pt_net_get_params", "pt_net_set_params",
     "pt_net_forward", "pt_net_loss_grad", "pt_net_adamw_step", "pt_net_dataset", "pt_net_dataset_size", "pt_net_dataset_get",
     "pt_net_train", "pt_net_out_scale", "pt_net_set_loss", "pt_set_guess_net", "pt_guess_counts", "pt_guess_last",
-    "pt_projection_counts",
+    "pt_projection_counts", "pt_equalize_counts", "pt_get_rng_state", "pt_set_rng_state",
 ]

@@ -14,8 +14,8 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libprojectracer_gpu.so"
-SOURCES = ["pt_setup.cpp", "pt_solver.hip", "pt_direct.hip", "pt_tracers.hip", "pt_sort.hip", "pt_stats.hip",
-           "pt_files.cpp", "pt_api.hip", "pt_unet.hip", "pt_ensemble.cpp"]
+SOURCES = ["pt_setup.cpp", "pt_solver.hip", "pt_direct.hip", "pt_tracers.hip", "pt_equalize.hip", "pt_sort.hip",
+           "pt_stats.hip", "pt_files.cpp", "pt_api.hip", "pt_unet.hip", "pt_ensemble.cpp"]
 
 # gfx950 only.  -ffp-contract=off keeps every FP64 expression in the
 # reference's operation order (no fused multiply-add contraction); packed

@@ -64,6 +64,16 @@ static int check_supported(const pt_params& p) {
     set_error("the direct solver (iterate=0) is built for ivis=0 and Database 1 or 2");
     return PT_ERR_UNSUPPORTED;
   }
+  if ((p.iC_continuous != 0 && p.iC_continuous != 1) || (p.equalize != 0 && p.equalize != 1)) {
+    set_error("iC_continuous and equalize must be 0 or 1");
+    return PT_ERR_INVALID;
+  }
+  if (p.equalize == 1 && (p.iC_continuous != 0 || p.original != 1 || p.buffer < 1 || !(p.dist_factor >= 0.0))) {
+    // the reference requires discrete types for equalize (PS/main.f90:92);
+    // the threaded equalize_tracers (original=0) is not built
+    set_error("equalize=1 is built for iC_continuous=0, original=1, buffer >= 1, dist_factor >= 0");
+    return PT_ERR_UNSUPPORTED;
+  }
   return PT_OK;
 }
 
@@ -108,6 +118,7 @@ void pt_default_params(pt_params* p) {
   p->ivis = 0; p->visP = 1.0; p->visT = 1e3; p->delta_visT = 2.0; p->eig_ratio_min = 2.0;
   p->visC = 10.0; p->delta_visC = 2.0; p->precision = PT_PREC_FP64; p->guess = PT_GUESS_WARM;
   p->dent = 0.5; p->iterate = 1; p->Database = 2;
+  p->iC_continuous = 1; p->equalize = 0; p->original = 1; p->buffer = 10; p->dist_factor = 1e-4;
 }
 
 int pt_create(const pt_params* p, void** out) {
@@ -213,6 +224,13 @@ int pt_load_state(void* p, const double* T_nodes, double Tbot_iso, double Ttop_i
                   const double* str, const double* Ttr, const double* Ctr) {
   Context* c = ctx_of(p);
   if (!c || !T_nodes || !rtr || !str || !Ttr || !Ctr) { set_error("null argument"); return PT_ERR_INVALID; }
+  if (c->P.iC_continuous == 0)
+    for (int64_t j = 0; j < c->T.ntr; j++)
+      if (Ctr[j] != 0.0 && Ctr[j] != 1.0) {
+        set_error("discrete tracer types (iC_continuous=0): Ctr must be 0 or 1, tracer " + std::to_string(j + 1) +
+                  " has " + std::to_string(Ctr[j]));
+        return PT_ERR_INVALID;
+      }
   return load_state(*c, T_nodes, Tbot_iso, Ttop_iso, rtr, str, Ttr, Ctr);
 }
 
@@ -453,6 +471,29 @@ int pt_projection_counts(void* p, int64_t* windowed, int64_t* radix) {
   if (!c || !windowed || !radix) return PT_ERR_INVALID;
   *windowed = c->proj_window;
   *radix = c->proj_fallback;
+  return PT_OK;
+}
+
+int pt_equalize_counts(void* p, int64_t* nequalize, int64_t* moved_last, int64_t* moved_total) {
+  Context* c = ctx_of(p);
+  if (!c || !nequalize || !moved_last || !moved_total) return PT_ERR_INVALID;
+  *nequalize = c->nequalize;
+  *moved_last = c->eq_moved_last;
+  *moved_total = c->eq_moved_total;
+  return PT_OK;
+}
+
+int pt_get_rng_state(void* p, int32_t* state) {
+  Context* c = ctx_of(p);
+  if (!c || !state) return PT_ERR_INVALID;
+  for (int q = 0; q < 4; q++) state[q] = c->rng[q];
+  return PT_OK;
+}
+
+int pt_set_rng_state(void* p, const int32_t* state) {
+  Context* c = ctx_of(p);
+  if (!c || !state) return PT_ERR_INVALID;
+  for (int q = 0; q < 4; q++) c->rng[q] = state[q];
   return PT_OK;
 }
 

@@ -5,7 +5,9 @@
 // tracer = 1 and iC_continuous = 1:
 //   Trestart         one record: xg(-6:nx+6), zg(-6:nz+6), T(-3:nx+3,-3:nz+3),
 //                    Cnew(1,-5:nx+5,-5:nz+5), SF(-3:nx+3,-3:nz+3)
-//   restart_tracers  one record: rtr, str, Ctr, Ttr (ntr each, tracer ID order)
+//   restart_tracers  one record: rtr, str, Ctr, Ttr (ntr each, tracer ID order);
+//                    with discrete types (iC_continuous = 0) rtr, str, Ttype
+//                    (INTEGER*4), Ttr
 // as Fortran unformatted sequential files: each record is framed by its
 // length in bytes as a 4-byte little-endian integer before and after.  The
 // restart=1 branch of build_grid (PS/grid.f90:55-103) reads them back.
@@ -44,9 +46,10 @@ struct File {
 };
 
 // one record from the parts, framed by its length
-int write_record(const std::string& path, const std::vector<std::pair<const double*, size_t>>& parts) {
+// one record from byte spans (pointer, bytes)
+int write_record(const std::string& path, const std::vector<std::pair<const void*, size_t>>& parts) {
   size_t bytes = 0;
-  for (auto& p : parts) bytes += sizeof(double) * p.second;
+  for (auto& p : parts) bytes += p.second;
   if (bytes > 0x7fffffffu) {
     set_error(path + ": record of " + std::to_string(bytes) + " bytes exceeds the 4-byte record length");
     return PT_ERR_UNSUPPORTED;
@@ -59,7 +62,7 @@ int write_record(const std::string& path, const std::vector<std::pair<const doub
   }
   const uint32_t len = (uint32_t)bytes;
   bool ok = std::fwrite(&len, 4, 1, fh.f) == 1;
-  for (auto& p : parts) ok = ok && std::fwrite(p.first, sizeof(double), p.second, fh.f) == p.second;
+  for (auto& p : parts) ok = ok && std::fwrite(p.first, 1, p.second, fh.f) == p.second;
   ok = ok && std::fwrite(&len, 4, 1, fh.f) == 1;
   ok = (std::fclose(fh.f) == 0) && ok;
   fh.f = nullptr;
@@ -70,8 +73,8 @@ int write_record(const std::string& path, const std::vector<std::pair<const doub
   return PT_OK;
 }
 
-// the single record of `path`, which must hold exactly `count` doubles
-int read_record(const std::string& path, size_t count, std::vector<double>& out) {
+// the single record of `path`, which must hold exactly `bytes` bytes
+int read_record_bytes(const std::string& path, size_t bytes, void* out) {
   File fh;
   fh.f = std::fopen(path.c_str(), "rb");
   if (!fh.f) {
@@ -79,18 +82,20 @@ int read_record(const std::string& path, size_t count, std::vector<double>& out)
     return PT_ERR_INVALID;
   }
   uint32_t head = 0, tail = 0;
-  out.resize(count);
-  const size_t bytes = sizeof(double) * count;
   if (std::fread(&head, 4, 1, fh.f) != 1 || head != bytes) {
     set_error(path + ": record of " + std::to_string(head) + " bytes, this context expects " + std::to_string(bytes));
     return PT_ERR_INVALID;
   }
-  if (std::fread(out.data(), sizeof(double), count, fh.f) != count || std::fread(&tail, 4, 1, fh.f) != 1 ||
-      tail != head) {
+  if (std::fread(out, 1, bytes, fh.f) != bytes || std::fread(&tail, 4, 1, fh.f) != 1 || tail != head) {
     set_error(path + ": truncated or malformed record");
     return PT_ERR_INVALID;
   }
   return PT_OK;
+}
+// the single record of `path`, which must hold exactly `count` doubles
+int read_record(const std::string& path, size_t count, std::vector<double>& out) {
+  out.resize(count);
+  return read_record_bytes(path, sizeof(double) * count, out.data());
 }
 
 // xg, zg of a uniform grid as build_grid forms them (PS/grid.f90:46-116):
@@ -223,9 +228,19 @@ int write_restart(Context& c, const char* dir) {
   PT_HIP(hipMemcpyAsync(tr.data() + 2 * ntr, dC, tb, hipMemcpyDeviceToHost, c.stream));
   PT_HIP(hipMemcpyAsync(tr.data() + 3 * ntr, dT, tb, hipMemcpyDeviceToHost, c.stream));
   PT_HIP(hipStreamSynchronize(c.stream));
-  PT_TRY(write_record(join(dir, "Trestart"),
-                      {{xg.data(), xg.size()}, {zg.data(), zg.size()}, {T.data(), n3}, {C.data(), n5}, {SF.data(), n3}}));
-  return write_record(join(dir, "restart_tracers"), {{tr.data(), tr.size()}});
+  const size_t d = sizeof(double);
+  PT_TRY(write_record(join(dir, "Trestart"), {{xg.data(), d * xg.size()},
+                                              {zg.data(), d * zg.size()},
+                                              {T.data(), d * n3},
+                                              {C.data(), d * n5},
+                                              {SF.data(), d * n3}}));
+  if (c.P.iC_continuous == 1)
+    return write_record(join(dir, "restart_tracers"), {{tr.data(), d * tr.size()}});
+  // discrete types: rtr, str, Ttype (INTEGER*4), Ttr (PS/main.f90:488-489)
+  std::vector<int32_t> type(ntr);
+  for (size_t j = 0; j < ntr; j++) type[j] = (int32_t)tr[2 * ntr + j];
+  return write_record(join(dir, "restart_tracers"),
+                      {{tr.data(), 2 * tb}, {type.data(), sizeof(int32_t) * ntr}, {tr.data() + 3 * ntr, tb}});
 }
 
 int read_restart(Context& c, const char* dir, double Tbot, double Ttop) {
@@ -243,7 +258,25 @@ int read_restart(Context& c, const char* dir, double Tbot, double Ttop) {
     set_error("Trestart grid coordinates differ from this context's grid by " + std::to_string(dev));
     return PT_ERR_INVALID;
   }
-  PT_TRY(read_record(join(dir, "restart_tracers"), 4 * ntr, tr));
+  if (c.P.iC_continuous == 1) {
+    PT_TRY(read_record(join(dir, "restart_tracers"), 4 * ntr, tr));
+  } else {
+    // rtr, str, Ttype (INTEGER*4), Ttr (PS/grid.f90:86-87)
+    std::vector<char> raw(28 * ntr);
+    PT_TRY(read_record_bytes(join(dir, "restart_tracers"), raw.size(), raw.data()));
+    tr.resize(4 * ntr);
+    std::memcpy(tr.data(), raw.data(), 16 * ntr);
+    for (size_t j = 0; j < ntr; j++) {
+      int32_t ty;
+      std::memcpy(&ty, raw.data() + 16 * ntr + 4 * j, 4);
+      if (ty != 0 && ty != 1) {
+        set_error("restart_tracers: tracer type " + std::to_string(ty) + " outside {0,1}");
+        return PT_ERR_INVALID;
+      }
+      tr[2 * ntr + j] = (double)ty;
+    }
+    std::memcpy(tr.data() + 3 * ntr, raw.data() + 20 * ntr, 8 * ntr);
+  }
   const double* T3 = rec.data() + xg.size() + zg.size();
   const double* C5 = T3 + n3;
   const double* SF3 = C5 + n5;

@@ -123,6 +123,14 @@ struct Context {
   bool project_radix_only = false;              // PT_PROJECT_RADIX=1 at pt_create
   int window_skip = 0;                          // projections left before the windowed sort is tried again
   int64_t proj_window = 0, proj_fallback = 0;   // projections by windowed counting sort / by radix sort
+  // tracer redistribution (equalize_tracers_original): mzran stream 0
+  // (seed_i, seed_j, seed_k, seed_n), counters, and the work buffers
+  // allocated on the first call (eq_node: per-node counts, excess and the
+  // lottery's cell list; eq_id: per-ID donor class and slot; eq_save: the
+  // donor slots in IDsave order; eq_dev: device scalars)
+  int32_t rng[4] = {521288629, 362436069, 16163801, 1131199299};
+  int64_t nequalize = 0, eq_moved_last = 0, eq_moved_total = 0;
+  int32_t *eq_node = nullptr, *eq_id = nullptr, *eq_save = nullptr, *eq_dev = nullptr, *eq_bsum = nullptr;
   double* cellsum;          // 27 per cell: (T*w, C*w, w) for the 3x3 neighbour nodes
   double *TxS, *CxS;        // halo-5 scratch fields for the filters
   int32_t* d_flags;         // [0]=tracer outside domain, [1]=tracer more than one cell from its slot's cell
@@ -226,8 +234,16 @@ inline int solve_stream(Context& c) { return c.P.iterate == 0 ? solve_direct(c) 
 // ---- tracers (pt_tracers.hip) ------------------------------------------------
 int tracers_alloc(Context& c);
 int tracers_free(Context& c);
-int tracers_to_corners(Context& c, const double* r, const double* s, const double* Tt);
+int tracers_to_corners(Context& c, const double* r, const double* s, const double* Tt, bool filters = true);
 int energy_time(Context& c);
+// equalize_tracers_original (pt_equalize.hip); frees its buffers with the context
+int equalize_tracers(Context& c);
+void equalize_free(Context& c);
+// its helpers in the tracer pipeline: storage in the current cell order, Ttr
+// at listed slots interpolated from Tratio, the closing projection
+int ensure_cell_order(Context& c);
+int interpolate_T_at(Context& c, const int32_t* slots, int64_t n);
+int equalize_projection(Context& c, bool moved);
 int load_state(Context& c, const double* T_nodes, double Tbot, double Ttop, const double* rtr,
                const double* str, const double* Ttr, const double* Ctr);
 // tracer state in reference ID order into device buffers (any may be null)

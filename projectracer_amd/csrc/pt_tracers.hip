@@ -937,8 +937,13 @@ static void load_conductivity(Context& c) {
                      c.T.dr, c.T.ds, D1[1], D1[2], D1[3], D1[4], D1[5]);
 }
 
-// tracers_to_corners: PS/composition.f90:923-1087
-int tracers_to_corners(Context& c, const double* r, const double* s, const double* Tt) {
+// tracers_to_corners: PS/composition.f90:923-1087.  filters = false is the
+// reference's smooth_vis=0 (no smoother_time / smoother_time_T).  With
+// discrete types (iC_continuous=0, tracers_to_corners_OG :1090-1276 with
+// shape_function=1) Ct holds Ttype as 0.0 / 1.0: the type-1 weight sum over
+// the weight sum is count_type_real(1)/nempty_real, the same weighted
+// average, and nempty counts the same nearest-node tracers.
+int tracers_to_corners(Context& c, const double* r, const double* s, const double* Tt, bool filters) {
   const int nx = c.T.nx, nz = c.T.nz, ncell = (nx + 1) * (nz + 1);
   const int64_t n = c.T.ntr;
   size_t b3 = sizeof(double) * (size_t)(nx + 7) * (nz + 7);
@@ -1002,9 +1007,9 @@ int tracers_to_corners(Context& c, const double* r, const double* s, const doubl
     return PT_ERR_INVALID;
   }
   c_bcs(c, c.Cnew);
-  PT_TRY(smoother_time(c));
+  if (filters) PT_TRY(smoother_time(c));
   t_bcs(c, c.Tratio);
-  PT_TRY(smoother_time_T(c));
+  if (filters) PT_TRY(smoother_time_T(c));
   load_conductivity(c);
   return PT_OK;
 }
@@ -1317,10 +1322,10 @@ static int source_and_tables(Context& c, bool velocity_too) {
   return PT_OK;
 }
 
-static int build_itab(Context& c) {
+static int build_tables(Context& c, const double* u, const double* w, const double* src) {
   int nx = c.T.nx, nz = c.T.nz;
   dim3 b(64, 4);
-  hipLaunchKernelGGL(k_itab, grid2(nz + 7, nx + 1, b), b, 0, c.stream, c.u, c.w, c.src, c.itab, nx, nz,
+  hipLaunchKernelGGL(k_itab, grid2(nz + 7, nx + 1, b), b, 0, c.stream, u, w, src, c.itab, nx, nz,
                      (const double*)c.dtab, c.T.dr_power[1], c.T.dr_power[2], c.T.dr_power[3]);
   InterpConstTab C;
   C.dsp1 = c.T.ds_power[1]; C.dsp2 = c.T.ds_power[2]; C.dsp3 = c.T.ds_power[3];
@@ -1330,6 +1335,7 @@ static int build_itab(Context& c) {
   hipLaunchKernelGGL(k_ctab, dim3((unsigned)((ncoef + 255) / 256)), dim3(256), 0, c.stream, c.itab, c.ctab, nx, nz, C);
   return PT_OK;
 }
+static int build_itab(Context& c) { return build_tables(c, c.u, c.w, c.src); }
 
 static RkArgs rk_args(Context& c) {
   RkArgs a;
@@ -1413,6 +1419,7 @@ static int reorder_tracers(Context& c);
 
 int energy_time(Context& c) {
   PT_TRY(stability_rk4(c));
+  if (c.P.equalize == 1) PT_TRY(equalize_tracers(c));   // PS/energy.f90:198-204
   const double dt = c.dt;
   const int nx = c.T.nx, nz = c.T.nz;
   size_t b3 = sizeof(double) * (size_t)(nx + 7) * (nz + 7), b5 = sizeof(double) * (size_t)(nx + 11) * (nz + 11);
@@ -1509,6 +1516,79 @@ int tracers_in_id_order(Context& c, double** r, double** s, double** Tt, double*
   if (Tt) { hipLaunchKernelGGL(k_to_id_order, g, b, 0, c.stream, c.tid, n, c.Tt, c.T1t); *Tt = c.T1t; }
   if (Ct) { hipLaunchKernelGGL(k_to_id_order, g, b, 0, c.stream, c.tid, n, c.Ct, c.r2); *Ct = c.r2; }
   return PT_OK;
+}
+
+// ---- helpers of the tracer redistribution (pt_equalize.hip) ----------------
+// The storage order as the cell order of the current positions (ostart /
+// oend per cell, okey per slot).  Every projection of a step's final
+// positions leaves it so; otherwise bin, sort and reorder here.
+int ensure_cell_order(Context& c) {
+  if (c.order_valid) return PT_OK;
+  const int nx = c.T.nx, nz = c.T.nz, ncell = (nx + 1) * (nz + 1);
+  const int64_t n = c.T.ntr;
+  const unsigned nbt = (unsigned)((n + 255) / 256);
+  PT_HIP(hipMemsetAsync(c.d_flags, 0, sizeof(int32_t) * 4, c.stream));
+  hipLaunchKernelGGL(k_bin, dim3(nbt), dim3(256), 0, c.stream, c.r, c.s, n, nx, nz, c.T.dr, c.T.ds, c.T.aspect,
+                     c.cellkey, c.d_flags, nullptr);
+  PT_HIP(hipMemsetAsync(c.cstart, 0, sizeof(int32_t) * ncell, c.stream));
+  PT_HIP(hipMemsetAsync(c.cend, 0, sizeof(int32_t) * ncell, c.stream));
+  PT_TRY(sort_pairs(c.sort_tmp, c.sort_bytes, c.cellkey, c.skey, c.iota, c.perm, n, c.sort_bits, c.stream));
+  hipLaunchKernelGGL(k_cell_bounds, dim3(nbt), dim3(256), 0, c.stream, c.skey, n, ncell, c.cstart, c.cend);
+  PT_HIP(hipMemcpyAsync(c.h_fctl, c.d_flags, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  PT_HIP(hipStreamSynchronize(c.stream));
+  if (c.h_fctl[0]) {
+    set_error("tracer position outside [0,aspect]x[0,1]");
+    return PT_ERR_TRACER_OUTSIDE;
+  }
+  return reorder_tracers(c);
+}
+
+// Textend(-3:nx+3,-3:nz+3) into a halo-3 field (the reference's Tdumb)
+__global__ void k_window3(const double* T5, double* T3, int nx, int nz) {
+  int i = -3 + (int)(blockIdx.x * blockDim.x + threadIdx.x), k = -3 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i > nx + 3 || k > nz + 3) return;
+  T3[I3(i, k, nx + 7)] = T5[I5(i, k, nx + 11)];
+}
+// T at the listed tracer slots from the coefficient table of one field
+__global__ void k_interp_slots(const double* __restrict__ ctab, InterpConst C, const int32_t* __restrict__ slots,
+                               int64_t n, const double* __restrict__ r, const double* __restrict__ s,
+                               double* __restrict__ T) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int32_t q = slots[j];
+  const Located p = locate(C, r[q], s[q]);
+  double o[3];
+  eval3(reinterpret_cast<const double2*>(ctab) + ((long)p.k * (C.nx + 1) + p.i) * 24, p, o);
+  T[q] = o[0];
+}
+// Ttr of redistributed tracers: extendT(Tratio), compute_derivatives of
+// Textend(-3:nx+3,-3:nz+3) and interpolate (PS/composition.f90:174-177,
+// 245-251).  src, itab and ctab are scratch here: stage 1 of the RK4
+// rebuilds them.
+int interpolate_T_at(Context& c, const int32_t* slots, int64_t n) {
+  const int nx = c.T.nx, nz = c.T.nz;
+  extend_t(c);
+  dim3 b16(64, 4);
+  hipLaunchKernelGGL(k_window3, grid2(nx + 7, nz + 7, b16), b16, 0, c.stream, c.Textend, c.src, nx, nz);
+  PT_TRY(build_tables(c, c.src, c.src, c.src));
+  InterpConst C;
+  C.nx = nx; C.nz = nz; C.dr = c.T.dr; C.ds = c.T.ds; C.f3 = c.T.factorial[3];
+  if (n > 0)
+    hipLaunchKernelGGL(k_interp_slots, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, c.ctab, C, slots, n,
+                       c.r, c.s, c.Tt);
+  return PT_OK;
+}
+// The projection that ends the redistribution: tracers_to_corners(rtr, str,
+// Ttr, T) with smooth_vis=0 (PS/composition.f90:257-259); nodes without
+// tracers keep the module T (the initial field before the first
+// energy_time, Tratio after it).  The storage order then follows the new
+// positions.
+int equalize_projection(Context& c, bool moved) {
+  const size_t b3 = sizeof(double) * (size_t)(c.T.nx + 7) * (c.T.nz + 7);
+  if (!c.t_initial) PT_HIP(hipMemcpyAsync(c.T0, c.Tratio, b3, hipMemcpyDeviceToDevice, c.stream));
+  if (moved) c.order_valid = false;
+  PT_TRY(tracers_to_corners(c, c.r, c.s, c.Tt, false));
+  return reorder_tracers(c);
 }
 
 // tracers (reference ID order) onto the device, projected onto the nodes, in cell order
@@ -1643,6 +1723,7 @@ int tracers_free(Context& c) {
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c.h_fctl) (void)hipHostFree(c.h_fctl);
+  equalize_free(c);
   return PT_OK;
 }
 

@@ -47,6 +47,8 @@ module projectracer_gpu
   integer(c_int32_t) :: precision, guess
   real(c_double) :: dent
   integer(c_int32_t) :: iterate, Database
+  integer(c_int32_t) :: iC_continuous, equalize, original, buffer
+  real(c_double) :: dist_factor
  end type pt_params
 
  !! stats.dat / viscosity_smoother.dat values (PS/basics.f90:363-462)
@@ -221,6 +223,23 @@ module projectracer_gpu
    import :: c_int, c_int64_t, c_ptr
    type(c_ptr), value :: ctx
    integer(c_int64_t), intent(out) :: windowed, radix
+  end function
+  !! tracer redistribution (equalize_tracers_original, PS/composition.f90:84-263)
+  integer(c_int) function pt_equalize_counts(ctx, nequalize, moved_last, moved_total) &
+    bind(C, name="pt_equalize_counts")
+   import :: c_int, c_int64_t, c_ptr
+   type(c_ptr), value :: ctx
+   integer(c_int64_t), intent(out) :: nequalize, moved_last, moved_total
+  end function
+  integer(c_int) function pt_get_rng_state(ctx, state) bind(C, name="pt_get_rng_state")
+   import :: c_int, c_int32_t, c_ptr
+   type(c_ptr), value :: ctx
+   integer(c_int32_t), intent(out) :: state(4)
+  end function
+  integer(c_int) function pt_set_rng_state(ctx, state) bind(C, name="pt_set_rng_state")
+   import :: c_int, c_int32_t, c_ptr
+   type(c_ptr), value :: ctx
+   integer(c_int32_t), intent(in) :: state(4)
   end function
   !! learned initial guess (include/projectracer_gpu.h, "learned initial guess")
   integer(c_int) function pt_net_create(seed, net) bind(C, name="pt_net_create")

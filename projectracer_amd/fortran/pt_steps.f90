@@ -90,6 +90,8 @@ program pt_steps
  p%visC = visC(1); p%delta_visC = delta_visC(1)
  p%precision = precision; p%guess = PT_GUESS_WARM
  p%dent = dent; p%iterate = iterate; p%Database = Database
+ p%iC_continuous = iC_continuous; p%equalize = equalize; p%original = original; p%buffer = buffer
+ p%dist_factor = dist_factor
 
  call pt_check(pt_create(p, ctx), "pt_create")
  call pt_check(pt_get_dims(ctx, gnx, gnz, ggrids, ntr), "pt_get_dims")

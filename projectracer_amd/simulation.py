@@ -153,6 +153,24 @@ class Simulation:
         self._check(self.lib.pt_projection_counts(self.ctx, C.byref(a), C.byref(b)), "projection_counts")
         return a.value, b.value
 
+    def equalize_counts(self) -> tuple:
+        """(redistributions made, tracers moved by the last one, tracers moved
+        in total) of equalize_tracers_original (PS/composition.f90:84-263)."""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self.lib.pt_equalize_counts(self.ctx, C.byref(a), C.byref(b), C.byref(c)), "equalize_counts")
+        return a.value, b.value, c.value
+
+    def rng_state(self) -> tuple:
+        """mzran stream 0 (seed_i, seed_j, seed_k, seed_n), the stream the
+        redistribution draws from."""
+        st = (C.c_int32 * 4)()
+        self._check(self.lib.pt_get_rng_state(self.ctx, st), "get_rng_state")
+        return tuple(int(v) for v in st)
+
+    def set_rng_state(self, state) -> None:
+        st = (C.c_int32 * 4)(*[int(v) for v in state])
+        self._check(self.lib.pt_set_rng_state(self.ctx, st), "set_rng_state")
+
     def info(self) -> dict:
         inf = _abi.PtSolveInfo()
         self._check(self.lib.pt_get_info(self.ctx, C.byref(inf)), "get_info")

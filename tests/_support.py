@@ -65,6 +65,8 @@ def oracle_lib():
             "pto_mzran": (None, [C.c_int64, dp]),
             "pto_initial_positions": (None, [vp, C.c_int, dp, dp]),
             "pto_grid_coords": (None, [C.c_int, C.c_int, C.c_double, dp, dp]),
+            "pto_equalize_counts": (None, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+            "pto_rng_state": (None, [vp, ip, ip]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -191,6 +193,20 @@ class OracleSim:
         self._chk(self.lib.pto_stats(self.ctx, C.byref(row)), "stats")
         return {name: getattr(row, name) for name, _ in _abi.PtStatsRow._fields_}
 
+    def equalize_counts(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self.lib.pto_equalize_counts(self.ctx, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+    def rng_state(self):
+        st = (C.c_int32 * 4)()
+        self.lib.pto_rng_state(self.ctx, st, None)
+        return tuple(int(v) for v in st)
+
+    def set_rng_state(self, state):
+        st = (C.c_int32 * 4)(*[int(v) for v in state])
+        self.lib.pto_rng_state(self.ctx, None, st)
+
 
 # ---------------------------------------------------------------------------
 # flang-built reference
@@ -215,8 +231,9 @@ tend=1.d0, tsnap=1.d0, Chebyshev=0, nstage=3, damping=100.d0, aRK2=0.5d0, torder
 tmovie_snap=1.d0, Nlocal=1000, Nsnap=100, local_time=0, dt_manual=0.125d0,
 courant={f(d['courant'])}, courantT={f(d['courantT'])}, courantC={f(d['courantC'])},
 courant_stream={f(d['courant_stream'])}, RK_update=1, vel_update=1, CFL=0,
-comp=1, ntype=1, iC_continuous=1, boundary_tracers=1, original=1, shape_function=1, buffer=10,
-equalize=0, tracer=1, tpc={d['tpc']}, dist_factor=1.d-4,
+comp=1, ntype=1, iC_continuous={d['iC_continuous']}, boundary_tracers=1, original={d['original']},
+shape_function=1, buffer={d['buffer']}, equalize={d['equalize']}, tracer=1, tpc={d['tpc']},
+dist_factor={f(d['dist_factor'])},
 restart=0, initial_temp=2, initial_comp=0, initial_temp_smooth=0, delta_visT_init=2.0d0,
 add_layer=0, dlayer=-1.d0, dent=0.5d0, klayer=35.d0, Tbot={d['Tbot']}, Vbc={d['Vbc']}
 /
@@ -351,6 +368,39 @@ VARIANTS = {
     "rigid_insul_direct": dict(kind="iso", Vbc=1, Tbot=1, iterate=0),
     "ivis1_thermo": dict(kind="thermo", RaT=1e5, RaC=5e4, ivis=1, visT=1e2, courant=0.1, tolerance=1e-10),
 }
+
+
+# Tracer redistribution (equalize=1, original=1, discrete types
+# iC_continuous=0), run through the reference program for three steps
+# (tests/golden/ref_equalize.npz): a 48x32 isoviscous state whose tracers
+# are all of type 0 (every node's neighbourhood is single-type), and a 96x32
+# state with a dense layer (type 1 below z ~ 0.2) with the reference's
+# neighbourhood half-width buffer=10 and with buffer=2.
+EQUALIZE_RUNS = {
+    "iso48": dict(kind="iso", buffer=10),
+    "thermo96": dict(kind="thermo", buffer=10),
+    "thermo96_b2": dict(kind="thermo", buffer=2),
+}
+
+
+def equalize_case(name, **over):
+    from projectracer_amd import cases
+    kw = dict(EQUALIZE_RUNS[name])
+    kind = kw.pop("kind")
+    kw.update(iC_continuous=0, equalize=1, original=1, **over)
+    if kind == "iso":
+        return cases.isoviscous(nx=48, nz=32, tpc=4, aspect=1.5, **kw)
+    return cases.thermochem(nx=96, nz=32, tpc=4, RaT=1e5, RaC=5e4, courant=0.1, **kw)
+
+
+def parse_equalize(stdout):
+    """(nequalize, seed_i, seed_j, seed_k, seed_n) after each energy_time,
+    as ref_driver prints them with equalize=1."""
+    rows = []
+    for line in stdout.splitlines():
+        if "driver: nequalize,seeds=" in line:
+            rows.append([int(v) for v in line.split("=", 1)[1].split()])
+    return np.array(rows, dtype=np.int64).reshape(-1, 5)
 
 
 def variant_case(name):

@@ -41,6 +41,12 @@ Two sources:
                            and through the direct solver, and the T- and
                            depth-dependent viscosity law ivis=1; q of every
                            solve, states after the solves of steps 0 and 2
+      ref_equalize.npz  -- tracer redistribution (equalize=1, original=1,
+                           discrete types) on the EQUALIZE_RUNS states, three
+                           steps, 1 thread: after each energy_time nequalize
+                           and mzran stream 0's state; after each step's solve
+                           the node counts, every tracer's type, every 7th
+                           tracer's r, s, T, and (steps 1 and 3) psi, T, C
 Usage: python tests/golden/make_golden.py
 """
 from __future__ import annotations
@@ -182,6 +188,30 @@ def reference_runs():
     reference_config1(tmp)
     reference_direct(tmp)
     reference_variants(tmp)
+    reference_equalize(tmp)
+
+
+def reference_equalize(tmp, nrun=3, stride=7):
+    from _support import run_reference, EQUALIZE_RUNS, equalize_case, parse_equalize
+    out = {}
+    for name in EQUALIZE_RUNS:
+        states, _, stdout, _ = run_reference(equalize_case(name), nrun=nrun, dump_every=1, threads=1,
+                                             workdir=tmp / f"equalize_{name}")
+        out[f"{name}_eq"] = parse_equalize(stdout)
+        for step in range(nrun + 1):
+            st = states[step]
+            out[f"{name}_s{step}_nempty"] = st["nempty"].astype(np.int16)
+            out[f"{name}_s{step}_type"] = st["Ctr"].astype(np.int8)
+            sel = np.arange(0, st["ntr"], stride)
+            for n in ("rtr", "str", "Ttr"):
+                out[f"{name}_s{step}_{n}"] = st[n][sel].copy()
+            if step in (1, nrun):
+                for n in ("SF", "T", "CNEW"):
+                    out[f"{name}_s{step}_{n}"] = _node(st, n)
+        print(name, "equalize rows", out[f"{name}_eq"].tolist())
+    out["stride"] = np.int64(stride)
+    np.savez_compressed(HERE / "ref_equalize.npz", **out)
+    print("wrote ref_equalize.npz")
 
 
 def reference_direct(tmp):
@@ -376,6 +406,9 @@ if __name__ == "__main__":
         raise SystemExit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "variants":
         reference_variants(Path(tempfile.mkdtemp(prefix="golden_")))
+        raise SystemExit(0)
+    if len(sys.argv) > 1 and sys.argv[1] == "equalize":
+        reference_equalize(Path(tempfile.mkdtemp(prefix="golden_")))
         raise SystemExit(0)
     if REF_DIR.exists():
         shipped_snapshot()
