@@ -93,6 +93,8 @@ def parse():
                     help="GPU paths to time (profiling runs may select one)")
     ap.add_argument("--direct-only", action="store_true",
                     help="time only the direct solver (100x100) and print its object")
+    ap.add_argument("--equalize-only", action="store_true",
+                    help="time only the tracer redistribution (equalize=1, discrete types) at nx x nz")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB read + 256 MiB write elementwise kernel, the "
                          "reference for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -208,6 +210,43 @@ def time_direct(nx=100, nz=100, solves=20):
         }
     sim.close()
     return out
+
+
+def time_equalize(args, steps=4):
+    """Tracer redistribution (equalize_tracers_original, discrete types) on
+    the 1024x256 state, isoviscous, mixed path: the first step finds tpc
+    tracers at every node (nothing to move); then `steps` steps, each
+    redistributing.  Reports the redistribution's own time per call (HIP
+    events around it, host waits included), the tracers moved per call, and
+    the step time beside it."""
+    from projectracer_amd import Simulation, _abi
+    case = cases_large(args, iC_continuous=0, equalize=1, ivis=0, tolerance=1e-7)
+    case.params.precision = _abi.PT_PREC_MIXED
+    sim = Simulation(case.params)
+    sim.load_state(case)
+    sim.time_step()
+    sim.synchronize()
+    sim.enable_kernel_timing(True)
+    n0, _, m0 = sim.equalize_counts()
+    t0 = time.perf_counter()
+    sim.run_steps(steps)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    ms, calls = sim.kernel_time(_abi.PT_K_EQUALIZE)
+    n1, last, m1 = sim.equalize_counts()
+    out = {
+        "grid": f"{sim.nx}x{sim.nz}, {sim.ntr} tracers, buffer {case.params.buffer}, isoviscous, mixed path",
+        "calls": calls, "redistributions": n1 - n0, "moved_per_call": (m1 - m0) / max(1, n1 - n0),
+        "equalize_ms_per_call": ms / max(1, calls), "ms_per_step_with_timers": 1e3 * (t1 - t0) / steps,
+        "dtype": "int32 counts, f64 positions",
+    }
+    sim.close()
+    return out
+
+
+def cases_large(args, **over):
+    from projectracer_amd import cases
+    return cases.thermochem_large(nx=args.nx, nz=args.nz, tpc=args.tpc, **over)
 
 
 def time_learned(args, capture_steps=3, train_iters=100, batch=4, timed_steps=4):
@@ -470,6 +509,9 @@ def main():
         del x, y
     if args.direct_only:
         print(json.dumps({"direct_solver": time_direct()}), flush=True)
+        return
+    if args.equalize_only:
+        print(json.dumps({"equalize": time_equalize(args)}), flush=True)
         return
     paths = set(args.paths.split(","))
     if paths != {"mixed", "fp64"}:

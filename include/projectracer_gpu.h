@@ -269,9 +269,13 @@ int pt_synchronize(void* ctx);
  *                         swept one launch per stage)
  *   PT_K_FILTER           one filter loop of a projection (smoother_time or
  *                         smoother_time_T, its batches and waits included)
- *   PT_K_SOLVE            one multigrid call (set-up and cycles) */
+ *   PT_K_SOLVE            one multigrid call (set-up and cycles)
+ *   PT_K_EQUALIZE         one call of the tracer redistribution (equalize=1):
+ *                         the counts, and when it redistributes the donors,
+ *                         lottery, T interpolation and closing projection,
+ *                         host waits included */
 enum { PT_K_RKC_STAGE_L1 = 0, PT_K_TRACER_STAGE = 1, PT_K_PROJECT = 2, PT_K_TRACER_RK2 = 3,
-       PT_K_RKC_STAGE_COARSE = 4, PT_K_FILTER = 5, PT_K_SOLVE = 6, PT_K_COUNT = 7 };
+       PT_K_RKC_STAGE_COARSE = 4, PT_K_FILTER = 5, PT_K_SOLVE = 6, PT_K_EQUALIZE = 7, PT_K_COUNT = 8 };
 int pt_enable_kernel_timing(void* ctx, int32_t on);
 int pt_kernel_time(void* ctx, int32_t which, double* ms, int64_t* launches);
 /* Units of work of the timed launches of `which`: interior grid points for
@@ -294,7 +298,8 @@ int pt_equalize_counts(void* ctx, int64_t* nequalize, int64_t* moved_last, int64
  * lottery draws from: state = seed_i(0), seed_j(0), seed_k(0), seed_n(0).
  * pt_create starts it at PS/basics.f90:321-324's values; a host that drew
  * the initial cloud from stream 0 (one OpenMP thread) sets the state it
- * left. */
+ * left.  seed_i, seed_j and seed_k lie in [0, 2147483579) in every state the
+ * reference's seeds lead to; pt_set_rng_state rejects others (PT_ERR_INVALID). */
 int pt_get_rng_state(void* ctx, int32_t* state);
 int pt_set_rng_state(void* ctx, const int32_t* state);
 

@@ -28,6 +28,11 @@
  *    exactness of interpolate, projection partition of unity, restriction
  *    and prolongation of constants/linears).
  *  - band LU: scipy.linalg.lapack dgbtrf/dgbtrs golden vectors.
+ *  - equalize_tracers_original and tracers_to_corners_OG (discrete types):
+ *    flang-built reference runs with equalize=1 on three injected states
+ *    (tests/golden/ref_equalize.npz, tests/test_oracle_equalize.py):
+ *    nequalize and the mzran stream-0 state after every step, node counts
+ *    and types exact, positions to rounding.
  *  - mzran / initialize_tracers positions: the reference's own initial
  *    clouds with 2 and 3 OpenMP threads (flang build), bit for bit, and the
  *    shipped T10000 snapshot (2-thread cloud projected and filtered).

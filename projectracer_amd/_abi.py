@@ -41,6 +41,7 @@ PT_K_TRACER_RK2 = 3
 PT_K_RKC_STAGE_COARSE = 4
 PT_K_FILTER = 5
 PT_K_SOLVE = 6
+PT_K_EQUALIZE = 7
 
 ERROR_TEXT = {
     PT_ERR_TRACER_OUTSIDE: "Tracer advected outside of domain - stopping",

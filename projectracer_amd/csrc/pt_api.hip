@@ -493,6 +493,13 @@ int pt_get_rng_state(void* p, int32_t* state) {
 int pt_set_rng_state(void* p, const int32_t* state) {
   Context* c = ctx_of(p);
   if (!c || !state) return PT_ERR_INVALID;
+  for (int q = 0; q < 3; q++)
+    if (state[q] < 0 || state[q] >= 2147483579) {
+      // every state the reference's seeds lead to keeps seed_i, seed_j,
+      // seed_k in [0, 2147483579) (PS/basics.f90:321-331, composition.f90:12-16)
+      set_error("mzran state: seed_i, seed_j, seed_k must lie in [0, 2147483579)");
+      return PT_ERR_INVALID;
+    }
   for (int q = 0; q < 4; q++) c->rng[q] = state[q];
   return PT_OK;
 }

@@ -28,9 +28,11 @@
 // (the storage order is the cell order, so a node's tracers are one slot
 // range), then an ID-order compaction writes IDsave; (3) one workgroup runs
 // the lottery: per round the minimum and the node list by a workgroup
-// reduction and scan, the draws by one lane (the stream is sequential, three
-// draws per move, handed over through LDS in chunks), the moves by all
-// lanes; (4) the tracer pipeline's interpolation table and projection.
+// reduction and ordered compactions over the nodes that lack tracers, the
+// draws and moves by all lanes, each lane jumping the mzran stream to its
+// own run of draws (the stream's two parts are a linear recurrence mod M
+// and an affine map mod 2^32, so a jump is a product of precomputed
+// powers); (4) the tracer pipeline's interpolation table and projection.
 #include <climits>
 #include <string>
 #include "pt_internal.hpp"
@@ -39,7 +41,6 @@ namespace pt {
 
 namespace {
 
-constexpr int EQ_CH = 2048;       // moves drawn per chunk of the lottery
 constexpr int EQ_BLOCK = 1024;    // IDs per block of the IDsave compaction
 constexpr int EQ_LOT = 1024;      // lanes of the lottery workgroup
 
@@ -95,9 +96,11 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 // nexcess (PS/composition.f90:124-137, shape_function=1): a node with
 // tracers whose (2 buffer + 1)^2 neighbourhood holds one type only.  Halo
 // nodes outside both index ranges hold no counts (count_type there stays 0,
-// equal to nempty for every type).  nfree accumulates the positive parts.
+// equal to nempty for every type).  Written node-major (nex) and i-major
+// (nexT, the lottery's order); nfree accumulates the positive parts.
 __global__ void __launch_bounds__(256) k_eq_excess(const int32_t* __restrict__ cnt, const int32_t* __restrict__ cnt1,
-                                                   int nx, int nz, int b, int tpc, int32_t* nex, int32_t* nfree) {
+                                                   int nx, int nz, int b, int tpc, int32_t* nex, int32_t* nexT,
+                                                   int32_t* nfree) {
   const int node = blockIdx.x * blockDim.x + threadIdx.x;
   const int ncell = (nx + 1) * (nz + 1);
   int e = 0;
@@ -122,6 +125,7 @@ __global__ void __launch_bounds__(256) k_eq_excess(const int32_t* __restrict__ c
       if (!mixed0 || !mixed1) e = n - tpc;
     }
     nex[node] = e;
+    nexT[i * (nz + 1) + k] = e;
   }
   const int w = wave_sum(e > 0 ? e : 0);
   if ((threadIdx.x & 63) == 0 && w) atomicAdd(nfree, w);
@@ -239,23 +243,45 @@ __global__ void __launch_bounds__(EQ_BLOCK) k_eq_place(const int32_t* cls, const
 }
 
 // mzran (PS/composition.f90:7-20) on one stream's state
+constexpr uint32_t MZ_M = 2147483579u;
 struct Mz {
   int32_t i, j, k, n;
   __device__ double next() {
     int32_t ran = (int32_t)((uint32_t)i - (uint32_t)k);
-    if (ran < 0) ran = ran + 2147483579;
+    if (ran < 0) ran = ran + (int32_t)MZ_M;
     i = j; j = k; k = ran;
     n = (int32_t)(69069u * (uint32_t)n + 1013904243u);
     ran = (int32_t)((uint32_t)ran + (uint32_t)n);
     return 0.5 + 0.2328306e-9 * (double)ran;
   }
+  // Advance by s draws at once.  With i, j, k in [0, M) the first part of
+  // the stream is the linear recurrence x(d) = x(d-3) - x(d-1) mod M on the
+  // state (i, j, k) = (x(d-3), x(d-2), x(d-1)), matrix [[0,1,0],[0,0,1],
+  // [1,0,M-1]]; the second part is the affine map n -> 69069 n + 1013904243
+  // mod 2^32.  jump holds both maps raised to 2^b, b = 0..31: 9 entries per
+  // matrix (row-major), then (multiplier, increment) pairs.
+  __device__ void advance(uint64_t s, const uint32_t* __restrict__ jump) {
+    uint32_t v0 = (uint32_t)i, v1 = (uint32_t)j, v2 = (uint32_t)k, w = (uint32_t)n;
+    for (int b = 0; s; b++, s >>= 1) {
+      if (!(s & 1)) continue;
+      const uint32_t* P = jump + 9 * b;
+      const uint64_t a0 = ((uint64_t)P[0] * v0 + (uint64_t)P[1] * v1 + (uint64_t)P[2] * v2) % MZ_M;
+      const uint64_t a1 = ((uint64_t)P[3] * v0 + (uint64_t)P[4] * v1 + (uint64_t)P[5] * v2) % MZ_M;
+      const uint64_t a2 = ((uint64_t)P[6] * v0 + (uint64_t)P[7] * v1 + (uint64_t)P[8] * v2) % MZ_M;
+      v0 = (uint32_t)a0; v1 = (uint32_t)a1; v2 = (uint32_t)a2;
+      const uint32_t* L = jump + 9 * 32 + 2 * b;
+      w = L[0] * w + L[1];
+    }
+    i = (int32_t)v0; j = (int32_t)v1; k = (int32_t)v2; n = (int32_t)w;
+  }
 };
 
 struct Lottery {
-  int32_t *nex, *list;
+  int32_t *nexT, *actA, *actB, *list;   // nexcess in i-major order (o = i (nz+1) + k) and three o lists
   const int32_t *save, *cnt, *cnt1;
   double *r, *s, *Ct;
   int32_t* dev;   // [0] donors, [1] moves made (out), [4..7] mzran state (in/out)
+  const uint32_t* jump;
   int nx, nz;
   double dr, ds, aspect, dist_r, dist_s;
 };
@@ -265,23 +291,58 @@ __device__ __forceinline__ int load_nex(int32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// The lottery (PS/composition.f90:193-238) in one workgroup.  Every loop
-// bound below is the same in all lanes; each round moves at least one
-// tracer, so it ends after at most `donors` rounds.
+// the flagged lanes' order within the workgroup, and the workgroup's count
+__device__ __forceinline__ int block_rank(bool f, int* s_w, int* total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(f);
+  if (lane == 0) s_w[wv] = __popcll(m);
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int q = 0; q < EQ_LOT / 64; q++) {
+    const int v = s_w[q];
+    if (q < wv) off += v;
+    tot += v;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + __popcll(m & ((1ull << lane) - 1ull));
+}
+
+// The lottery (PS/composition.f90:193-238) in one workgroup.  Only nodes
+// that lack tracers at the start can be at the minimum, and a node leaves
+// for good once it reaches 0, so the rounds scan a shrinking list of those
+// nodes (kept in the reference's i-outer order).  Per round: the minimum,
+// the nodes at it (istore) and the list of nodes still lacking, by
+// workgroup reductions and ordered compactions; then the round's moves,
+// lane t taking a contiguous run of them: it advances a copy of the round's
+// mzran state to its first draw (three per move) and draws its moves in the
+// reference's order.  Every loop bound is the same in all lanes; each round
+// moves at least one tracer, so the rounds end after at most `donors`.
 __global__ void __launch_bounds__(EQ_LOT) k_eq_lottery(Lottery a) {
-  __shared__ int s_w[EQ_LOT / 64];
+  __shared__ int s_w[EQ_LOT / 64], s_w2[EQ_LOT / 64];
   __shared__ int s_min;
-  __shared__ int s_q[EQ_CH];
-  __shared__ double s_pr[EQ_CH], s_ps[EQ_CH];
+  __shared__ int32_t s_state[4], s_next[4];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int nnode = (a.nx + 1) * (a.nz + 1);
-  const int donors = a.dev[0];
-  Mz mz{a.dev[4], a.dev[5], a.dev[6], a.dev[7]};
-  int p = 0;   // moves made
-  for (;;) {
-    // the lowest excess
+  const int64_t donors = a.dev[0];
+  if (t < 4) s_state[t] = a.dev[4 + t];
+  // the nodes that lack tracers, i outer and k inner
+  int32_t *act = a.actA, *act2 = a.actB;
+  int nact = 0;
+  for (int base = 0; base < nnode; base += EQ_LOT) {
+    const int o = base + t;
+    const bool f = o < nnode && a.nexT[o] < 0;
+    int tot;
+    const int pos = block_rank(f, s_w, &tot);
+    if (f) act[nact + pos] = o;
+    nact += tot;
+  }
+  __syncthreads();
+  int64_t p = 0;   // moves made
+  while (p < donors) {
+    // the lowest nexcess
     int mn = INT_MAX;
-    for (int o = t; o < nnode; o += EQ_LOT) mn = min(mn, load_nex(a.nex + o));
+    for (int j = t; j < nact; j += EQ_LOT) mn = min(mn, load_nex(a.nexT + act[j]));
 #pragma unroll
     for (int o = 32; o; o >>= 1) mn = min(mn, __shfl_xor(mn, o));
     if (lane == 0) s_w[wv] = mn;
@@ -294,49 +355,38 @@ __global__ void __launch_bounds__(EQ_LOT) k_eq_lottery(Lottery a) {
     __syncthreads();
     mn = s_min;
     if (mn >= 0) break;
-    // the nodes at the minimum, i outer and k inner (istore, :204-215)
-    const int per = (nnode + EQ_LOT - 1) / EQ_LOT;
-    const int o0 = t * per < nnode ? t * per : nnode, o1 = o0 + per < nnode ? o0 + per : nnode;
-    int own = 0;
-    for (int o = o0; o < o1; o++) {
-      const int i = o / (a.nz + 1), k = o - i * (a.nz + 1);
-      own += load_nex(a.nex + k * (a.nx + 1) + i) == mn;
+    // istore (the nodes at the minimum) and the nodes still lacking
+    int ncells = 0, nkeep = 0;
+    for (int base = 0; base < nact; base += EQ_LOT) {
+      const int j = base + t;
+      const int o = j < nact ? act[j] : 0;
+      const int v = j < nact ? load_nex(a.nexT + o) : 0;
+      int tot1, tot2;
+      const int pos1 = block_rank(j < nact && v == mn, s_w, &tot1);
+      const int pos2 = block_rank(j < nact && v < 0, s_w2, &tot2);
+      if (j < nact && v == mn) a.list[ncells + pos1] = o;
+      if (j < nact && v < 0) act2[nkeep + pos2] = o;
+      ncells += tot1;
+      nkeep += tot2;
     }
-    const int incl = wave_incl_scan(own);
-    __syncthreads();   // s_w is reused
-    if (lane == 63) s_w[wv] = incl;
+    int32_t* sw = act; act = act2; act2 = sw;
+    nact = nkeep;
+    if (t < 4) s_next[t] = s_state[t];
     __syncthreads();
-    int off = 0, ncells = 0;
-    for (int q = 0; q < EQ_LOT / 64; q++) {
-      if (q < wv) off += s_w[q];
-      ncells += s_w[q];
-    }
-    off += incl - own;
-    for (int o = o0; o < o1; o++) {
-      const int i = o / (a.nz + 1), k = o - i * (a.nz + 1);
-      const int node = k * (a.nx + 1) + i;
-      if (load_nex(a.nex + node) == mn) a.list[off++] = node;
-    }
-    __syncthreads();
-    // one round: ncells draws, or fewer when the donors run out
-    int pp = 0;
-    while (pp < ncells && p < donors) {
-      int m = ncells - pp < EQ_CH ? ncells - pp : EQ_CH;
-      if (donors - p < m) m = donors - p;
-      if (t == 0) {
-        for (int j = 0; j < m; j++) {
-          const double u = mz.next();
-          s_q[j] = (int)floor((double)ncells * u);
-          s_pr[j] = mz.next();
-          s_ps[j] = mz.next();
-        }
-      }
-      __syncthreads();
-      for (int j = t; j < m; j += EQ_LOT) {
-        const int node = a.list[s_q[j]];
-        const int k = node / (a.nx + 1), i = node - k * (a.nx + 1);
+    // this round's moves: ncells draws, or fewer when the donors run out
+    const int m = (int)(donors - p < (int64_t)ncells ? donors - p : (int64_t)ncells);
+    const int per = (m + EQ_LOT - 1) / EQ_LOT;
+    const int j0 = t * per < m ? t * per : m, j1 = j0 + per < m ? j0 + per : m;
+    if (j0 < j1) {
+      Mz mz{s_state[0], s_state[1], s_state[2], s_state[3]};
+      mz.advance(3ull * (uint64_t)j0, a.jump);
+      for (int j = j0; j < j1; j++) {
+        const double u = mz.next();
+        const int o = a.list[(int)floor((double)ncells * u)];
+        const double ur = mz.next(), us = mz.next();
+        const int i = o / (a.nz + 1), k = o - i * (a.nz + 1);
         const double rc = a.dr * (double)i, sc = a.ds * (double)k;
-        const double pr = (s_pr[j] - 0.5) * a.dr, ps = (s_ps[j] - 0.5) * a.ds;
+        const double pr = (ur - 0.5) * a.dr, ps = (us - 0.5) * a.ds;
         double rn = rc + pr, sn = sc + ps;
         if (rn <= 0.0) rn = a.dist_r;
         if (rn >= a.aspect) rn = a.aspect - a.dist_r;
@@ -346,23 +396,50 @@ __global__ void __launch_bounds__(EQ_LOT) k_eq_lottery(Lottery a) {
         a.r[q] = rn;
         a.s[q] = sn;
         // maxloc(count_type(0:1,i,k)) - 1: type 0 on a tie
+        const int node = k * (a.nx + 1) + i;
         const int n1 = a.cnt1[node];
         a.Ct[q] = n1 > a.cnt[node] - n1 ? 1.0 : 0.0;
-        atomicAdd(a.nex + node, 1);
+        atomicAdd(a.nexT + o, 1);
       }
-      pp += m;
-      p += m;
-      __syncthreads();
+      if (j1 == m) {
+        s_next[0] = mz.i; s_next[1] = mz.j; s_next[2] = mz.k; s_next[3] = mz.n;
+      }
     }
-    if (p >= donors) break;
+    __syncthreads();
+    if (t < 4) s_state[t] = s_next[t];
+    __syncthreads();
+    p += m;
   }
   if (t == 0) {
-    a.dev[1] = p;
-    a.dev[4] = mz.i; a.dev[5] = mz.j; a.dev[6] = mz.k; a.dev[7] = mz.n;
+    a.dev[1] = (int32_t)p;
+    a.dev[4] = s_state[0]; a.dev[5] = s_state[1]; a.dev[6] = s_state[2]; a.dev[7] = s_state[3];
   }
 }
 
 unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// mzran's two maps raised to 2^b, b = 0..31 (the layout Mz::advance reads)
+constexpr int EQ_JUMP = 32 * 9 + 32 * 2;
+void mz_jump_table(uint32_t* tab) {
+  const uint64_t M = MZ_M;
+  uint64_t P[9] = {0, 1, 0, 0, 0, 1, 1, 0, M - 1};
+  uint32_t A = 69069u, C = 1013904243u;
+  for (int b = 0; b < 32; b++) {
+    for (int q = 0; q < 9; q++) tab[9 * b + q] = (uint32_t)P[q];
+    tab[9 * 32 + 2 * b] = A;
+    tab[9 * 32 + 2 * b + 1] = C;
+    uint64_t Q[9];
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) {
+        uint64_t acc = 0;
+        for (int k = 0; k < 3; k++) acc = (acc + P[3 * r + k] * P[3 * k + c] % M) % M;
+        Q[3 * r + c] = acc;
+      }
+    for (int q = 0; q < 9; q++) P[q] = Q[q];
+    C = A * C + C;
+    A = A * A;
+  }
+}
 
 }  // namespace
 
@@ -372,19 +449,36 @@ void equalize_free(Context& c) {
   c.eq_node = c.eq_id = c.eq_save = c.eq_dev = c.eq_bsum = nullptr;
 }
 
+static int equalize_body(Context& c, int32_t* cnt, int32_t* cnt1, int32_t* nex, int32_t* list, int32_t* cls,
+                         int32_t* slot, int nb);
+
 int equalize_tracers(Context& c) {
   const int nx = c.T.nx, nz = c.T.nz, ncell = (nx + 1) * (nz + 1);
   const int64_t n = c.T.ntr;
   const int nb = (int)blocks(n, EQ_BLOCK);
   if (!c.eq_node) {
-    PT_HIP(hipMalloc((void**)&c.eq_node, sizeof(int32_t) * 4 * (size_t)ncell));
+    PT_HIP(hipMalloc((void**)&c.eq_node, sizeof(int32_t) * 7 * (size_t)ncell));
     PT_HIP(hipMalloc((void**)&c.eq_id, sizeof(int32_t) * 2 * (size_t)n));
     PT_HIP(hipMalloc((void**)&c.eq_save, sizeof(int32_t) * (size_t)n));
-    PT_HIP(hipMalloc((void**)&c.eq_dev, sizeof(int32_t) * 8));
+    PT_HIP(hipMalloc((void**)&c.eq_dev, sizeof(int32_t) * (8 + EQ_JUMP)));
     PT_HIP(hipMalloc((void**)&c.eq_bsum, sizeof(int32_t) * 2 * (size_t)nb));
+    static uint32_t jump[EQ_JUMP];
+    static bool jump_ready = false;
+    if (!jump_ready) { mz_jump_table(jump); jump_ready = true; }
+    PT_HIP(hipMemcpy(c.eq_dev + 8, jump, sizeof jump, hipMemcpyHostToDevice));
   }
   int32_t *cnt = c.eq_node, *cnt1 = cnt + ncell, *nex = cnt1 + ncell, *list = nex + ncell;
   int32_t *cls = c.eq_id, *slot = cls + n;
+  if (c.timing) timer_begin(c, PT_K_EQUALIZE);
+  const int rc = equalize_body(c, cnt, cnt1, nex, list, cls, slot, nb);
+  if (c.timing) timer_end(c, PT_K_EQUALIZE);
+  return rc;
+}
+
+static int equalize_body(Context& c, int32_t* cnt, int32_t* cnt1, int32_t* nex, int32_t* list, int32_t* cls,
+                         int32_t* slot, int nb) {
+  const int nx = c.T.nx, nz = c.T.nz, ncell = (nx + 1) * (nz + 1);
+  const int64_t n = c.T.ntr;
   PT_TRY(ensure_cell_order(c));
   // per-node counts from the cell-ordered storage (c.cstart / c.cend are
   // free between projections)
@@ -394,8 +488,9 @@ int equalize_tracers(Context& c) {
   hipLaunchKernelGGL(k_eq_cells, dim3(blocks(ncell, 256)), dim3(256), 0, c.stream, c.cstart, c.cend, c.Ct, ncell, cnt,
                      cnt1);
   hipLaunchKernelGGL(k_eq_fill, dim3(1), dim3(64), 0, c.stream, c.eq_dev, (int64_t)8, 0);
+  int32_t* nexT = list + ncell;
   hipLaunchKernelGGL(k_eq_excess, dim3(blocks(ncell, 256)), dim3(256), 0, c.stream, cnt, cnt1, nx, nz, c.P.buffer,
-                     c.P.tpc, nex, c.eq_dev);
+                     c.P.tpc, nex, nexT, c.eq_dev);
   int32_t h[8];
   PT_HIP(hipMemcpyAsync(h, c.eq_dev, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
   PT_HIP(hipStreamSynchronize(c.stream));
@@ -414,7 +509,9 @@ int equalize_tracers(Context& c) {
   // the lottery, from the context's mzran stream 0
   PT_HIP(hipMemcpyAsync(c.eq_dev + 4, c.rng, sizeof(int32_t) * 4, hipMemcpyHostToDevice, c.stream));
   Lottery L;
-  L.nex = nex; L.list = list; L.save = c.eq_save; L.cnt = cnt; L.cnt1 = cnt1;
+  L.nexT = nexT; L.actA = nexT + ncell; L.actB = nexT + 2 * ncell; L.list = list;
+  L.save = c.eq_save; L.cnt = cnt; L.cnt1 = cnt1;
+  L.jump = reinterpret_cast<const uint32_t*>(c.eq_dev + 8);
   L.r = c.r; L.s = c.s; L.Ct = c.Ct; L.dev = c.eq_dev;
   L.nx = nx; L.nz = nz; L.dr = c.T.dr; L.ds = c.T.ds; L.aspect = c.T.aspect; L.dist_r = dist_r; L.dist_s = dist_s;
   hipLaunchKernelGGL(k_eq_lottery, dim3(1), dim3(EQ_LOT), 0, c.stream, L);

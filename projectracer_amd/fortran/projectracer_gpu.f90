@@ -27,7 +27,7 @@ module projectracer_gpu
   PT_FIELD_CBUOY = 8, PT_FIELD_CONDUCT = 9, PT_FIELD_VIS = 10, PT_FIELD_RESIDUAL = 11, &
   PT_FIELD_SOURCE = 12
  integer(c_int32_t), parameter :: PT_K_RKC_STAGE_L1 = 0, PT_K_TRACER_STAGE = 1, PT_K_PROJECT = 2, &
-  PT_K_TRACER_RK2 = 3, PT_K_RKC_STAGE_COARSE = 4, PT_K_FILTER = 5, PT_K_SOLVE = 6
+  PT_K_TRACER_RK2 = 3, PT_K_RKC_STAGE_COARSE = 4, PT_K_FILTER = 5, PT_K_SOLVE = 6, PT_K_EQUALIZE = 7
 
  !! namelist values (PS/basics.f90:54-66); layout of struct pt_params
  type, bind(C) :: pt_params
