@@ -243,7 +243,14 @@ __global__ void __launch_bounds__(EQ_BLOCK) k_eq_place(const int32_t* cls, const
 }
 
 // mzran (PS/composition.f90:7-20) on one stream's state
-constexpr uint32_t MZ_M = 2147483579u;
+constexpr uint32_t MZ_M = 2147483579u;   // 2^31 - 69
+// x mod M for x < 2^64: 2^31 = 69 (mod M), so two folds hi*69 + lo bring x
+// below 2M
+__device__ __forceinline__ uint32_t mz_reduce(uint64_t x) {
+  x = (x >> 31) * 69u + (x & 0x7FFFFFFFu);
+  x = (x >> 31) * 69u + (x & 0x7FFFFFFFu);
+  return (uint32_t)(x >= MZ_M ? x - MZ_M : x);
+}
 struct Mz {
   int32_t i, j, k, n;
   __device__ double next() {
@@ -265,10 +272,10 @@ struct Mz {
     for (int b = 0; s; b++, s >>= 1) {
       if (!(s & 1)) continue;
       const uint32_t* P = jump + 9 * b;
-      const uint64_t a0 = ((uint64_t)P[0] * v0 + (uint64_t)P[1] * v1 + (uint64_t)P[2] * v2) % MZ_M;
-      const uint64_t a1 = ((uint64_t)P[3] * v0 + (uint64_t)P[4] * v1 + (uint64_t)P[5] * v2) % MZ_M;
-      const uint64_t a2 = ((uint64_t)P[6] * v0 + (uint64_t)P[7] * v1 + (uint64_t)P[8] * v2) % MZ_M;
-      v0 = (uint32_t)a0; v1 = (uint32_t)a1; v2 = (uint32_t)a2;
+      const uint32_t a0 = mz_reduce((uint64_t)P[0] * v0 + (uint64_t)P[1] * v1 + (uint64_t)P[2] * v2);
+      const uint32_t a1 = mz_reduce((uint64_t)P[3] * v0 + (uint64_t)P[4] * v1 + (uint64_t)P[5] * v2);
+      const uint32_t a2 = mz_reduce((uint64_t)P[6] * v0 + (uint64_t)P[7] * v1 + (uint64_t)P[8] * v2);
+      v0 = a0; v1 = a1; v2 = a2;
       const uint32_t* L = jump + 9 * 32 + 2 * b;
       w = L[0] * w + L[1];
     }
@@ -277,7 +284,7 @@ struct Mz {
 };
 
 struct Lottery {
-  int32_t *nexT, *actA, *actB, *list;   // nexcess in i-major order (o = i (nz+1) + k) and three o lists
+  int32_t *nexT, *actA, *actB, *listA, *listB;   // nexcess in i-major order (o = i (nz+1) + k), o lists
   const int32_t *save, *cnt, *cnt1;
   double *r, *s, *Ct;
   int32_t* dev;   // [0] donors, [1] moves made (out), [4..7] mzran state (in/out)
@@ -291,31 +298,31 @@ __device__ __forceinline__ int load_nex(int32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// the flagged lanes' order within the workgroup, and the workgroup's count
-__device__ __forceinline__ int block_rank(bool f, int* s_w, int* total) {
+constexpr int EQ_PT = 8;   // consecutive list entries per lane in one compaction pass
+
+// exclusive prefix sums over the workgroup's lanes of two counts, and their totals
+__device__ __forceinline__ void block_scan2(int a, int b, int* s_a, int* s_b, int* ea, int* eb, int* ta, int* tb) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const unsigned long long m = __ballot(f);
-  if (lane == 0) s_w[wv] = __popcll(m);
+  const int ia = wave_incl_scan(a), ib = wave_incl_scan(b);
+  if (lane == 63) { s_a[wv] = ia; s_b[wv] = ib; }
   __syncthreads();
-  int off = 0, tot = 0;
+  int oa = 0, ob = 0, sa = 0, sb = 0;
   for (int q = 0; q < EQ_LOT / 64; q++) {
-    const int v = s_w[q];
-    if (q < wv) off += v;
-    tot += v;
+    if (q < wv) { oa += s_a[q]; ob += s_b[q]; }
+    sa += s_a[q]; sb += s_b[q];
   }
   __syncthreads();
-  *total = tot;
-  return off + __popcll(m & ((1ull << lane) - 1ull));
+  *ea = oa + ia - a; *eb = ob + ib - b; *ta = sa; *tb = sb;
 }
 
 // The lottery (PS/composition.f90:193-238) in one workgroup.  Only nodes
 // that lack tracers at the start can be at the minimum, and a node leaves
-// for good once it reaches 0, so the rounds scan a shrinking list of those
-// nodes (kept in the reference's i-outer order).  Per round: the minimum,
-// the nodes at it (istore) and the list of nodes still lacking, by
-// workgroup reductions and ordered compactions; then the round's moves,
-// lane t taking a contiguous run of them: it advances a copy of the round's
-// mzran state to its first draw (three per move) and draws its moves in the
+// for good once it reaches 0, so a new minimum is found on a shrinking list
+// of those nodes (kept in the reference's i-outer order), which also gives
+// istore, the nodes at it; while the minimum stays, the next round's istore
+// is the last one's nodes still at it.  Then the round's moves, lane t
+// taking a contiguous run of them: it advances a copy of the round's mzran
+// state to its first draw (three per move) and draws its moves in the
 // reference's order.  Every loop bound is the same in all lanes; each round
 // moves at least one tracer, so the rounds end after at most `donors`.
 __global__ void __launch_bounds__(EQ_LOT) k_eq_lottery(Lottery a) {
@@ -326,51 +333,92 @@ __global__ void __launch_bounds__(EQ_LOT) k_eq_lottery(Lottery a) {
   const int nnode = (a.nx + 1) * (a.nz + 1);
   const int64_t donors = a.dev[0];
   if (t < 4) s_state[t] = a.dev[4 + t];
-  // the nodes that lack tracers, i outer and k inner
+  // the nodes that lack tracers, i outer and k inner; each pass, lane t
+  // takes EQ_PT consecutive entries
   int32_t *act = a.actA, *act2 = a.actB;
   int nact = 0;
-  for (int base = 0; base < nnode; base += EQ_LOT) {
-    const int o = base + t;
-    const bool f = o < nnode && a.nexT[o] < 0;
-    int tot;
-    const int pos = block_rank(f, s_w, &tot);
-    if (f) act[nact + pos] = o;
+  for (int base = 0; base < nnode; base += EQ_LOT * EQ_PT) {
+    const int o0 = base + t * EQ_PT;
+    int c = 0;
+    for (int u = 0; u < EQ_PT; u++) c += o0 + u < nnode && a.nexT[o0 + u] < 0;
+    int e, e2, tot, tot2;
+    block_scan2(c, 0, s_w, s_w2, &e, &e2, &tot, &tot2);
+    for (int u = 0; u < EQ_PT; u++)
+      if (o0 + u < nnode && a.nexT[o0 + u] < 0) act[nact + e++] = o0 + u;
     nact += tot;
   }
   __syncthreads();
   int64_t p = 0;   // moves made
+  int32_t *list = a.listA, *list2 = a.listB;
+  int level = 0, ncells = 0;   // the minimum and its nodes (istore), once found
   while (p < donors) {
-    // the lowest nexcess
-    int mn = INT_MAX;
-    for (int j = t; j < nact; j += EQ_LOT) mn = min(mn, load_nex(a.nexT + act[j]));
+    // While the minimum stays at `level` no node can join it (nexcess only
+    // grows): the nodes still at it are the last list's, in the same order.
+    if (ncells > 0) {
+      int nk = 0;
+      for (int base = 0; base < ncells; base += EQ_LOT * EQ_PT) {
+        const int j0 = base + t * EQ_PT;
+        int o[EQ_PT], c1 = 0;
 #pragma unroll
-    for (int o = 32; o; o >>= 1) mn = min(mn, __shfl_xor(mn, o));
-    if (lane == 0) s_w[wv] = mn;
-    __syncthreads();
-    if (t == 0) {
-      int m = s_w[0];
-      for (int q = 1; q < EQ_LOT / 64; q++) m = min(m, s_w[q]);
-      s_min = m;
+        for (int u = 0; u < EQ_PT; u++) {
+          o[u] = j0 + u < ncells ? list[j0 + u] : -1;
+          if (o[u] >= 0 && load_nex(a.nexT + o[u]) != level) o[u] = -1;
+          c1 += o[u] >= 0;
+        }
+        int e1, e2, tot1, tot2;
+        block_scan2(c1, 0, s_w, s_w2, &e1, &e2, &tot1, &tot2);
+#pragma unroll
+        for (int u = 0; u < EQ_PT; u++)
+          if (o[u] >= 0) list2[nk + e1++] = o[u];
+        nk += tot1;
+      }
+      int32_t* sw = list; list = list2; list2 = sw;
+      ncells = nk;
     }
-    __syncthreads();
-    mn = s_min;
-    if (mn >= 0) break;
-    // istore (the nodes at the minimum) and the nodes still lacking
-    int ncells = 0, nkeep = 0;
-    for (int base = 0; base < nact; base += EQ_LOT) {
-      const int j = base + t;
-      const int o = j < nact ? act[j] : 0;
-      const int v = j < nact ? load_nex(a.nexT + o) : 0;
-      int tot1, tot2;
-      const int pos1 = block_rank(j < nact && v == mn, s_w, &tot1);
-      const int pos2 = block_rank(j < nact && v < 0, s_w2, &tot2);
-      if (j < nact && v == mn) a.list[ncells + pos1] = o;
-      if (j < nact && v < 0) act2[nkeep + pos2] = o;
-      ncells += tot1;
-      nkeep += tot2;
+    if (ncells == 0) {
+      // a new minimum: scan the nodes still lacking
+      int mn = INT_MAX;
+      for (int j = t; j < nact; j += EQ_LOT) mn = min(mn, load_nex(a.nexT + act[j]));
+#pragma unroll
+      for (int o = 32; o; o >>= 1) mn = min(mn, __shfl_xor(mn, o));
+      if (lane == 0) s_w[wv] = mn;
+      __syncthreads();
+      if (t == 0) {
+        int m = s_w[0];
+        for (int q = 1; q < EQ_LOT / 64; q++) m = min(m, s_w[q]);
+        s_min = m;
+      }
+      __syncthreads();
+      mn = s_min;
+      if (mn >= 0) break;
+      level = mn;
+      // istore (the nodes at the minimum) and the nodes still lacking
+      int nkeep = 0;
+      for (int base = 0; base < nact; base += EQ_LOT * EQ_PT) {
+        const int j0 = base + t * EQ_PT;
+        int o[EQ_PT], v[EQ_PT], c1 = 0, c2 = 0;
+#pragma unroll
+        for (int u = 0; u < EQ_PT; u++) {
+          const bool live = j0 + u < nact;
+          o[u] = live ? act[j0 + u] : 0;
+          v[u] = live ? load_nex(a.nexT + o[u]) : 0;
+          c1 += live && v[u] == mn;
+          c2 += live && v[u] < 0;
+        }
+        int e1, e2, tot1, tot2;
+        block_scan2(c1, c2, s_w, s_w2, &e1, &e2, &tot1, &tot2);
+#pragma unroll
+        for (int u = 0; u < EQ_PT; u++) {
+          const bool live = j0 + u < nact;
+          if (live && v[u] == mn) list[ncells + e1++] = o[u];
+          if (live && v[u] < 0) act2[nkeep + e2++] = o[u];
+        }
+        ncells += tot1;
+        nkeep += tot2;
+      }
+      int32_t* sw = act; act = act2; act2 = sw;
+      nact = nkeep;
     }
-    int32_t* sw = act; act = act2; act2 = sw;
-    nact = nkeep;
     if (t < 4) s_next[t] = s_state[t];
     __syncthreads();
     // this round's moves: ncells draws, or fewer when the donors run out
@@ -382,7 +430,7 @@ __global__ void __launch_bounds__(EQ_LOT) k_eq_lottery(Lottery a) {
       mz.advance(3ull * (uint64_t)j0, a.jump);
       for (int j = j0; j < j1; j++) {
         const double u = mz.next();
-        const int o = a.list[(int)floor((double)ncells * u)];
+        const int o = list[(int)floor((double)ncells * u)];
         const double ur = mz.next(), us = mz.next();
         const int i = o / (a.nz + 1), k = o - i * (a.nz + 1);
         const double rc = a.dr * (double)i, sc = a.ds * (double)k;
@@ -457,7 +505,7 @@ int equalize_tracers(Context& c) {
   const int64_t n = c.T.ntr;
   const int nb = (int)blocks(n, EQ_BLOCK);
   if (!c.eq_node) {
-    PT_HIP(hipMalloc((void**)&c.eq_node, sizeof(int32_t) * 7 * (size_t)ncell));
+    PT_HIP(hipMalloc((void**)&c.eq_node, sizeof(int32_t) * 8 * (size_t)ncell));
     PT_HIP(hipMalloc((void**)&c.eq_id, sizeof(int32_t) * 2 * (size_t)n));
     PT_HIP(hipMalloc((void**)&c.eq_save, sizeof(int32_t) * (size_t)n));
     PT_HIP(hipMalloc((void**)&c.eq_dev, sizeof(int32_t) * (8 + EQ_JUMP)));
@@ -509,7 +557,7 @@ static int equalize_body(Context& c, int32_t* cnt, int32_t* cnt1, int32_t* nex, 
   // the lottery, from the context's mzran stream 0
   PT_HIP(hipMemcpyAsync(c.eq_dev + 4, c.rng, sizeof(int32_t) * 4, hipMemcpyHostToDevice, c.stream));
   Lottery L;
-  L.nexT = nexT; L.actA = nexT + ncell; L.actB = nexT + 2 * ncell; L.list = list;
+  L.nexT = nexT; L.actA = nexT + ncell; L.actB = nexT + 2 * ncell; L.listA = list; L.listB = nexT + 3 * ncell;
   L.save = c.eq_save; L.cnt = cnt; L.cnt1 = cnt1;
   L.jump = reinterpret_cast<const uint32_t*>(c.eq_dev + 8);
   L.r = c.r; L.s = c.s; L.Ct = c.Ct; L.dev = c.eq_dev;
