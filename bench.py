@@ -93,6 +93,7 @@ def parse():
                     help="GPU paths to time (profiling runs may select one)")
     ap.add_argument("--direct-only", action="store_true",
                     help="time only the direct solver (100x100) and print its object")
+    ap.add_argument("--no-equalize", action="store_true", help="skip the tracer redistribution measurement")
     ap.add_argument("--equalize-only", action="store_true",
                     help="time only the tracer redistribution (equalize=1, discrete types) at nx x nz")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -640,6 +641,9 @@ def main():
         result["learned_guess"] = time_learned(args)
     note("direct solver")
     result["direct_solver"] = time_direct()
+    if not args.no_equalize:
+        note("tracer redistribution")
+        result["equalize"] = time_equalize(args)
     if not args.no_cpu_baseline:
         note("CPU baseline (the reference program on the host cores)")
         result["cpu_baseline"] = cpu_baseline(args)

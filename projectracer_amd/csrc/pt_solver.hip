@@ -1918,7 +1918,6 @@ static void restore_roles(Context& c, const std::vector<void*>& v) {
 // eager; the results are the same either way.
 static constexpr size_t MAX_CYCLE_GRAPHS = 4;
 static int rest_and_next_level1(Context& c, bool mixed) {
-  const Tables& t = c.T;
   auto enqueue = [&]() -> int {
     for (int p = 2; p <= c.cyc_n; p++) PT_TRY(visit(c, p, mixed));
     return visit(c, 1, mixed);

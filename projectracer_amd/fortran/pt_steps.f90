@@ -60,6 +60,8 @@ program pt_steps
  real(c_double) :: Tbot_iso, Ttop_iso
  integer(c_int32_t), parameter :: out_fields(4) = [PT_FIELD_SF, PT_FIELD_T, PT_FIELD_U, PT_FIELD_W]
  integer :: step, j
+ integer(c_int64_t) :: neq, moved_last, moved_total
+ integer(c_int32_t) :: seeds(4)
 
  ntype = 1; comp = 1
  open(unit=501, file="inputs.nml", status="old")
@@ -133,6 +135,13 @@ program pt_steps
  call pt_check(pt_get_tracers(ctx, rtr, str, Ttr, Ctr), "pt_get_tracers")
  write(701) rtr, str, Ttr, Ctr
  close(701)
+ if (equalize .eq. 1) then
+  !! the reference's nequalize (PS/composition.f90:151-155) and mzran stream 0
+  call pt_check(pt_equalize_counts(ctx, neq, moved_last, moved_total), "pt_equalize_counts")
+  call pt_check(pt_get_rng_state(ctx, seeds), "pt_get_rng_state")
+  write(*,'(a,3(i12),4(i12))') "pt_steps: nequalize, moved, moved total, seeds ", neq, moved_last, &
+   moved_total, seeds
+ end if
  call pt_check(pt_destroy(ctx), "pt_destroy")
  write(*,'(a,i4,a,i8)') "pt_steps: ", nrun, " steps, last cycle count ", qs(nrun+1)
 end program pt_steps

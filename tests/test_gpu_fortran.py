@@ -65,7 +65,9 @@ def _run(tmp_path, case, nrun, precision):
                           env=dict(os.environ))
     print(proc.stdout[-400:], proc.stderr[-400:], flush=True)
     assert proc.returncode == 0, proc.stdout + proc.stderr
-    return _read_state(tmp_path / "state.bin", nrun)
+    st = _read_state(tmp_path / "state.bin", nrun)
+    st["stdout"] = proc.stdout
+    return st
 
 
 def test_fortran_time_loop_matches_reference_build(tmp_path):
@@ -85,6 +87,33 @@ def test_fortran_time_loop_matches_reference_build(tmp_path):
     for key in ("rtr", "str", "Ttr", "Ctr"):
         assert rel_err(st[key][idx], g[f"s2_{key}"]) <= 1e-10, key
     assert st["time"] == pytest.approx(float(g["s2_time"]), rel=1e-14)
+
+
+def test_fortran_time_loop_with_redistribution(tmp_path):
+    """equalize=1 with discrete tracer types from the namelist file: the
+    48x32 all-type-0 state of tests/golden/ref_equalize.npz, three steps
+    and one more solve through the Fortran module.  Against the reference
+    program: the counters and mzran state it prints after the last step,
+    node counts and every tracer's type exact, psi and T within 1e-10,
+    sampled tracers within 1e-12."""
+    from _support import equalize_case
+    g = np.load(GOLDEN / "ref_equalize.npz")
+    case = equalize_case("iso48")
+    st = _run(tmp_path, case, 3, _abi.PT_PREC_FP64)
+    assert np.array_equal(st["nempty"], g["iso48_s3_nempty"])
+    assert np.array_equal(st["Ctr"].astype(np.int8), g["iso48_s3_type"])
+    for name in ("SF", "T"):
+        err = rel_err(st[name], g[f"iso48_s3_{name}"])
+        print(f"[fortran equalize] {name} {err:.2e}", flush=True)
+        assert err <= 1e-10, name
+    sel = np.arange(0, case.ntr, int(g["stride"]))
+    for key in ("rtr", "str", "Ttr"):
+        assert np.max(np.abs(st[key][sel] - g[f"iso48_s3_{key}"])) < 1e-12, key
+    line = [ln for ln in st["stdout"].splitlines() if "nequalize" in ln]
+    assert line, st["stdout"]
+    vals = [int(v) for v in line[0].split()[-7:]]
+    row = [int(v) for v in g["iso48_eq"][-1]]
+    assert vals[0] == row[0] and vals[3:] == row[1:], (vals, row)
 
 
 def test_fortran_time_loop_mixed_precision(tmp_path):
