@@ -33,6 +33,7 @@
 // own run of draws (the stream's two parts are a linear recurrence mod M
 // and an affine map mod 2^32, so a jump is a product of precomputed
 // powers); (4) the tracer pipeline's interpolation table and projection.
+#include <array>
 #include <climits>
 #include <string>
 #include "pt_internal.hpp"
@@ -510,10 +511,14 @@ int equalize_tracers(Context& c) {
     PT_HIP(hipMalloc((void**)&c.eq_save, sizeof(int32_t) * (size_t)n));
     PT_HIP(hipMalloc((void**)&c.eq_dev, sizeof(int32_t) * (8 + EQ_JUMP)));
     PT_HIP(hipMalloc((void**)&c.eq_bsum, sizeof(int32_t) * 2 * (size_t)nb));
-    static uint32_t jump[EQ_JUMP];
-    static bool jump_ready = false;
-    if (!jump_ready) { mz_jump_table(jump); jump_ready = true; }
-    PT_HIP(hipMemcpy(c.eq_dev + 8, jump, sizeof jump, hipMemcpyHostToDevice));
+    // one table for the process, built once (thread-safe static
+    // initialisation: ensemble cases call this from several host threads)
+    static const std::array<uint32_t, EQ_JUMP> jump = [] {
+      std::array<uint32_t, EQ_JUMP> tab{};
+      mz_jump_table(tab.data());
+      return tab;
+    }();
+    PT_HIP(hipMemcpyAsync(c.eq_dev + 8, jump.data(), sizeof(uint32_t) * EQ_JUMP, hipMemcpyHostToDevice, c.stream));
   }
   int32_t *cnt = c.eq_node, *cnt1 = cnt + ncell, *nex = cnt1 + ncell, *list = nex + ncell;
   int32_t *cls = c.eq_id, *slot = cls + n;

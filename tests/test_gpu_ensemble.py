@@ -52,6 +52,39 @@ def test_ensemble_cases_match_single_runs():
     ens.close()
 
 
+def test_ensemble_with_tracer_redistribution():
+    """Four cases with discrete tracer types and equalize=1 stepped together
+    on four host threads: each redistributes tracers from its own mzran
+    stream, and every case, counters and stream state included, is
+    bit-identical to the same case run alone."""
+    print("[ensemble] 4 cases with equalize=1, 3 steps, 4 threads vs each case alone", flush=True)
+    cs = [case4(200 + i, nx, nz) for i, (nx, nz) in enumerate(GRIDS)]
+    for c in cs:
+        c.params.iC_continuous = 0
+        c.params.equalize = 1
+        c.params.buffer = 2
+    ens = Ensemble([c.params for c in cs])
+    for sim, c in zip(ens.cases, cs):
+        sim.load_state(c)
+    st = ens.run_steps(3, workers=4)
+    assert np.all(st == _abi.PT_OK), st.tolist()
+    for i, c in enumerate(cs):
+        alone = Simulation(c.params)
+        alone.load_state(c)
+        for _ in range(3):
+            alone.time_step()
+        e, s = ens.cases[i], alone
+        print(f"[ensemble] case {i}: counts {e.equalize_counts()} alone {s.equalize_counts()}", flush=True)
+        assert e.equalize_counts() == s.equalize_counts() and e.equalize_counts()[0] >= 1, i
+        assert e.rng_state() == s.rng_state(), i
+        a, b = _state(e), _state(s)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), i
+        for x, y in zip(a[3], b[3]):
+            assert np.array_equal(x, y), i
+        alone.close()
+    ens.close()
+
+
 def test_a_failing_case_leaves_the_others_running():
     cs = [case4(100, 128, 64), case4(101, 128, 64)]
     # the second case is set to stop at once: one cycle allowed, tolerance out of reach
