@@ -126,6 +126,7 @@ program ref_driver
 !$ t2=omp_get_wtime_d()
  write(*,'(a,g25.16)') " driver: loop seconds=",t2-t1
  write(*,'(a,4(g25.16))') " driver: timers multigrid,energy,advect,convert=",tmultigrid,tenergy,tadvect_tracer,tconvert
+ if (equalize.eq.1) write(*,'(a,g25.16,i8)') " driver: equalize seconds, calls=",tequalize,nequalize
 
 contains
 
