@@ -159,14 +159,18 @@ int pt_create(const pt_params* p, void** out) {
   c->cyc_n = c->T.nsteps;
   if (c->P.precision == PT_PREC_MIXED) {
     // the mixed path's cycle: a V-cycle whose coarse corrections start from
-    // zero at each descent (PT_MIXED_CYCLE=W|F|V and PT_MIXED_COARSE_ZERO=0|1
-    // select others, e.g. the reference's W-cycle with warm-started coarse
-    // levels)
+    // zero at each descent and whose levels >= 2 are smoothed on the way up
+    // only (V(0,1) below level 1).  PT_MIXED_CYCLE=W|F|V,
+    // PT_MIXED_COARSE_ZERO=0|1 and PT_MIXED_PRESMOOTH_FROM=l (smooth on the
+    // way down too below level l; 99: at every level) select others, e.g. the
+    // reference's W-cycle with warm-started coarse levels
     const char* ce = std::getenv("PT_MIXED_CYCLE");
     const char kind = (ce && (ce[0] == 'W' || ce[0] == 'F' || ce[0] == 'V')) ? ce[0] : 'V';
     c->cyc_n = cycle_schedule_kind(c->T.grids, kind, c->cyc);
     const char* cz = std::getenv("PT_MIXED_COARSE_ZERO");
     c->coarse_zero = !(cz && cz[0] == '0');
+    const char* ps = std::getenv("PT_MIXED_PRESMOOTH_FROM");
+    c->presmooth_from = ps ? std::max(2, std::atoi(ps)) : 2;
   }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");

@@ -173,6 +173,7 @@ struct Context {
   int cyc[52] = {0};
   int cyc_n = 0;
   bool coarse_zero = false;
+  int presmooth_from = 99;   // the mixed path's cycle smooths levels >= this on the way up only (2 by default)
   std::vector<CycleGraph> cycle_graphs;
   // mixed precision work
   double* dres64;           // FP64 residual for defect correction

@@ -1829,6 +1829,16 @@ static int visit(Context& c, int p, bool mixed) {
       launch_residual<double>(c, grd, blk, a, L.res, c.red);
       int nb = grd.x * grd.y;
       hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, nb, (int)RED_SUM_ABS, c.red_out);
+    } else if (l >= c.presmooth_from && c.coarse_zero && multi[p - 1] < multi[p] && multi[p + 1] > multi[p]) {
+      // no smoothing on the way down: a level entered from the finer one
+      // and left for the coarser one starts its correction from zero, so
+      // the residual restricted next is the level's right-hand side (the
+      // turning visits of F- and W-cycles, entered from a coarser level,
+      // carry a prolonged correction and are smoothed)
+      StageArgs<float> a = stage_base<float>(c, l, true);
+      a.Sx = L.ferr;
+      dim3 blk(TX, TZ), grd((L.g.n - 1 + TX - 1) / TX, (L.g.m - 1 + TZ * PZ - 1) / (TZ * PZ));
+      launch_residual<float>(c, grd, blk, a, L.fB, c.red);
     } else {
       PT_TRY(iterate_stream_t<float>(c, l, p, true, false));
     }
