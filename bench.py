@@ -567,7 +567,7 @@ def main():
             "nx": mixed["nx"], "nz": mixed["nz"], "aspect": 4.0, "tracers": ntr,
             "RaT": 1e7, "RaC": 3e6, "levels": 6, "rkc_stages": 35, **WORKLOADS[args.workload],
             "initial_guess": "warm start from the previous psi",
-            "mixed_cycle": "V-cycle, coarse corrections from zero at each descent, levels >= 2 smoothed on the way up only",
+            "mixed_cycle": "V-cycle, coarse corrections from zero at each descent, levels >= 2 smoothed on the way up only with 15 RKC1 stages",
         },
         "tracer_updates_per_s": steps_s * ntr,
         "vcycles_per_solve": mixed["cycles"] / max(1, mixed["solves"]),

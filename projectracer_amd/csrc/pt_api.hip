@@ -171,6 +171,12 @@ int pt_create(const pt_params* p, void** out) {
     c->coarse_zero = !(cz && cz[0] == '0');
     const char* ps = std::getenv("PT_MIXED_PRESMOOTH_FROM");
     c->presmooth_from = ps ? std::max(2, std::atoi(ps)) : 2;
+    // its coarse levels' sweeps: 15 RKC1 stages (the reference's recurrence
+    // and damping; dt_stream for that stage count); PT_MIXED_COARSE_STAGES=s
+    // selects another count, nstageS or more the reference's everywhere
+    const char* cs = std::getenv("PT_MIXED_COARSE_STAGES");
+    const int sc = cs ? std::atoi(cs) : 15;
+    if (sc >= 2 && sc < c->P.nstageS) rkc1_coarse_tables(c->T, sc, c->P.dampingS);
   }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");

@@ -40,6 +40,9 @@ struct Tables {
   double SI1[MAXL + 1], SI2[MAXL + 1], SI4[MAXL + 1], SI5[MAXL + 1], SI6[MAXL + 1];
   double SM1[7][7], SM2[7][7], SM4[7][7], SM5[7][7], SM6[7][7];   // [dk+3][di+3]
   double mewt[64], mew[64], vv[64], cc[64], RKC1_stability;
+  // the mixed path's coarse levels with s_coarse stages (0: nstageS, the tables above)
+  double mewt_c[64], mew_c[64], vv_c[64], cc_c[64], RKC1_stability_c;
+  int s_coarse = 0;
   double delta_T, delta_C;
   double dts;                    // dts_array value (uniform grid => constant)
   double dt_vis[MAXL + 1];       // dt_vis per level (constant per level)
@@ -48,6 +51,7 @@ struct Tables {
   int64_t ntr;
 };
 int build_tables(const pt_params& p, Tables& t);   // returns PT_* status
+void rkc1_coarse_tables(Tables& t, int s, double damping);
 int cycle_schedule_kind(int grids, char kind, int* multi);   // 'W', 'V' or 'F'; returns nsteps
 
 // ---- stencil constants passed to kernels by value ----------------------

@@ -146,8 +146,9 @@ static void fd_tables(Tables& t) {
     }
 }
 
-// Chebyshev_prep_stream: PS/Chebyshev.f90:20-34, 82-110
-static void rkc1_tables(Tables& t, int s, double damping) {
+// Chebyshev_prep_stream: PS/Chebyshev.f90:20-34, 82-110, into the given arrays
+static void rkc1_coefficients(int s, double damping, double* mewt, double* mew, double* vv, double* cc,
+                              double* stability) {
   double T[64], Tx[64], b[64];
   double w0 = 1.0 + damping / (double)(s * s);
   T[0] = 1.0; T[1] = w0; Tx[0] = 0.0; Tx[1] = 1.0;
@@ -157,16 +158,24 @@ static void rkc1_tables(Tables& t, int s, double damping) {
   }
   double w1 = T[s] / Tx[s];
   for (int j = 0; j <= s; j++) b[j] = 1.0 / T[j];
-  std::memset(t.mewt, 0, sizeof t.mewt); std::memset(t.mew, 0, sizeof t.mew);
-  std::memset(t.vv, 0, sizeof t.vv); std::memset(t.cc, 0, sizeof t.cc);
-  t.mewt[1] = b[1] * w1;
+  std::memset(mewt, 0, 64 * sizeof(double)); std::memset(mew, 0, 64 * sizeof(double));
+  std::memset(vv, 0, 64 * sizeof(double)); std::memset(cc, 0, 64 * sizeof(double));
+  mewt[1] = b[1] * w1;
   for (int j = 2; j <= s; j++) {
-    t.mew[j] = 2.0 * b[j] * w0 / b[j - 1];
-    t.vv[j] = -b[j] / b[j - 2];
-    t.cc[j] = 1.0 - t.mew[j] - t.vv[j];
+    mew[j] = 2.0 * b[j] * w0 / b[j - 1];
+    vv[j] = -b[j] / b[j - 2];
+    cc[j] = 1.0 - mew[j] - vv[j];
   }
-  for (int j = 2; j <= s; j++) t.mewt[j] = 2.0 * b[j] * w1 / b[j - 1];
-  t.RKC1_stability = -(2.0 * w0 / w1);
+  for (int j = 2; j <= s; j++) mewt[j] = 2.0 * b[j] * w1 / b[j - 1];
+  *stability = -(2.0 * w0 / w1);
+}
+static void rkc1_tables(Tables& t, int s, double damping) {
+  rkc1_coefficients(s, damping, t.mewt, t.mew, t.vv, t.cc, &t.RKC1_stability);
+}
+// the mixed path's coarse-level smoother: the same recurrence with s stages
+void rkc1_coarse_tables(Tables& t, int s, double damping) {
+  rkc1_coefficients(s, damping, t.mewt_c, t.mew_c, t.vv_c, t.cc_c, &t.RKC1_stability_c);
+  t.s_coarse = s;
 }
 
 // derivative_eigenvalues: PS/energy.f90:632-754.  The Fourier samples are

@@ -292,6 +292,24 @@ __global__ void __launch_bounds__(TX* TZ) k_residual(StageArgs<R> a, R* res, dou
 struct RkcCoef {
   double mewt[64], mew[64], vv[64], cc[64];
 };
+// the RKC1 stage count and coefficients of a level's sweeps: the reference's
+// nstageS everywhere, or s_coarse stages on the mixed path's levels >= 2
+struct LevelRkc {
+  int s;
+  const double *mewt, *mew, *vv, *cc;
+  double stability;
+};
+static LevelRkc level_rkc(const Context& c, int l, bool fp32) {
+  const Tables& t = c.T;
+  if (fp32 && l >= 2 && t.s_coarse >= 2) return {t.s_coarse, t.mewt_c, t.mew_c, t.vv_c, t.cc_c, t.RKC1_stability_c};
+  return {c.P.nstageS, t.mewt, t.mew, t.vv, t.cc, t.RKC1_stability};
+}
+static RkcCoef rkc_coef(const LevelRkc& rk) {
+  RkcCoef co;
+  std::memcpy(co.mewt, rk.mewt, sizeof co.mewt); std::memcpy(co.mew, rk.mew, sizeof co.mew);
+  std::memcpy(co.vv, rk.vv, sizeof co.vv); std::memcpy(co.cc, rk.cc, sizeof co.cc);
+  return co;
+}
 // the FP32 sweep's stage coefficients, rounded once on the host: c0 = cc,
 // c1 = mew, c2 = vv, cf = mewt * cpar (the product in FP64, as the device
 // would form it); read at a uniform stage index they are scalar loads of
@@ -1464,7 +1482,7 @@ static int strip_sweep(Context& c, int l, const StripCfg& sc, double cpar, const
   Level& L = c.L[l];
   float** f[5] = {&L.ferr, &L.fB, &L.fC, &L.fD, &L.fE};
   const LevelGeom& g = L.g;
-  const int s = c.P.nstageS;
+  const int s = level_rkc(c, l, true).s;
   const int nseg = (s + sc.ns - 1) / sc.ns, base = s / nseg, rem = s % nseg;
   if (base < 2) {
     set_error("strip sweep needs at least 2 stages per segment");
@@ -1515,7 +1533,8 @@ template <typename R>
 static int iterate_stream_t(Context& c, int l, int p, bool fp32, bool l1_sum) {
   Level& L = c.L[l];
   const LevelGeom& g = L.g;
-  const int s = c.P.nstageS;
+  const LevelRkc rk = level_rkc(c, l, fp32);
+  const int s = rk.s;
   dim3 blk(TX, TZ), grd((g.n - 1 + TX - 1) / TX, (g.m - 1 + TZ * PZ - 1) / (TZ * PZ));
   R* err = fp32 ? (R*)L.ferr : (R*)L.err;
   R* B = fp32 ? (R*)L.fB : (R*)L.bufB;
@@ -1528,9 +1547,7 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, bool l1_sum) {
     // FP32 isoviscous levels too large for one workgroup: strip sweeps
     const StripCfg sc = strip_cfg(g);
     if (l >= 2 && c.P.ivis == 0 && ppt == 0 && sc.cw > 0 && !l1_sum) {
-      RkcCoef co;
-      std::memcpy(co.mewt, c.T.mewt, sizeof co.mewt); std::memcpy(co.mew, c.T.mew, sizeof co.mew);
-      std::memcpy(co.vv, c.T.vv, sizeof co.vv); std::memcpy(co.cc, c.T.cc, sizeof co.cc);
+      const RkcCoef co = rkc_coef(rk);
       for (int it = 1; it <= c.P.Nmulti; it++) {
         int ntemp = c.P.Nmulti - it + 1;
         double temp = (c.P.Nmulti > 1) ? (c.P.c_2 - c.P.c_1) * (double)(ntemp - 1) / (double)(1 - c.P.Nmulti) : 0.0;
@@ -1545,9 +1562,7 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, bool l1_sum) {
   }
   if (l >= 2 && ppt > 0 && lds <= SMALL_LDS_BYTES && !l1_sum) {
     // whole sweep + residual in one workgroup, buffers in LDS
-    RkcCoef co;
-    std::memcpy(co.mewt, c.T.mewt, sizeof co.mewt); std::memcpy(co.mew, c.T.mew, sizeof co.mew);
-    std::memcpy(co.vv, c.T.vv, sizeof co.vv); std::memcpy(co.cc, c.T.cc, sizeof co.cc);
+    const RkcCoef co = rkc_coef(rk);
     R* res = fp32 ? (R*)L.fB : (R*)L.res;
     for (int it = 1; it <= c.P.Nmulti; it++) {
       int ntemp = c.P.Nmulti - it + 1;
@@ -1575,7 +1590,7 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, bool l1_sum) {
     double cc = temp + c.P.c_2;
     StageArgs<R> a = stage_base<R>(c, l, fp32);
     // stage 1: S1 = S0 + mewt1*c*dt*F(S0) -> B
-    a.Sx = err; a.S0 = err; a.S2 = err; a.Y = B; a.stage1 = 1; a.cf = (R)(c.T.mewt[1] * cc);
+    a.Sx = err; a.S0 = err; a.S2 = err; a.Y = B; a.stage1 = 1; a.cf = (R)(rk.mewt[1] * cc);
     if (timed) timer_begin(c, tk, (int64_t)(g.n - 1) * (g.m - 1));
     launch_stage<R>(c, grd, blk, a);
     if (timed) timer_end(c, tk);
@@ -1587,7 +1602,7 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, bool l1_sum) {
     for (int j = 2; j <= s; j++) {
       a.stage1 = 0;
       a.Sx = cur; a.S0 = err; a.S2 = prev;
-      a.c0 = (R)c.T.cc[j]; a.c1 = (R)c.T.mew[j]; a.c2 = (R)c.T.vv[j]; a.cf = (R)(c.T.mewt[j] * cc);
+      a.c0 = (R)rk.cc[j]; a.c1 = (R)rk.mew[j]; a.c2 = (R)rk.vv[j]; a.cf = (R)(rk.mewt[j] * cc);
       R* out;
       if (j == s) out = err;
       else if (prev == err) out = spare;
@@ -1642,8 +1657,11 @@ static int smoother_vis(Context& c, int l, VisSet& vs, int interpolate) {
   dim3 gnodes = grid2(g.n + 1, g.m + 1, b16);
   dim3 gint = grid2(g.n - 1, g.m - 1, b16);
   const int ivis0 = c.P.ivis == 0;
-  double ae = -(t.RKC1_stability / 2.0), be = (double)c.P.nstageS;
-  double ctemp = c.P.courant_stream * (t.RKC1_stability / (ae * ae));
+  // dt_stream for the level's own RKC1 stage count (the mixed path's coarse
+  // levels may use fewer stages than nstageS)
+  const LevelRkc rk = level_rkc(c, l, c.P.precision == PT_PREC_MIXED);
+  double ae = -(rk.stability / 2.0), be = (double)rk.s;
+  double ctemp = c.P.courant_stream * (rk.stability / (ae * ae));
   for (;;) {
     for (int nn = nn_store; nn <= Niter; nn++) {
       hipLaunchKernelGGL(k_vis_grad, gnodes, b16, 0, c.stream, vs, g.n, g.m, g.pitch, Nvis_store >= 1 ? 1 : 0, ivis0,
