@@ -73,7 +73,10 @@ __global__ void k_eq_cells(const int32_t* cstart, const int32_t* cend, const dou
 }
 
 // an index mirrored into 0..n as the halo copies of count_type do it
-// (count(-i) = count(i), count(n+i) = count(n-i), for i = 1..buffer in turn)
+// (count(-i) = count(i), count(n+i) = count(n-i), for i = 1..buffer in turn).
+// An index that mirrors past the other edge needs buffer > n, and then the
+// window already holds every index 0..n of that direction itself, so the
+// test's outcome does not depend on where such an index lands.
 __device__ __forceinline__ int reflect(int x, int n) {
   while (x < 0 || x > n) x = x < 0 ? -x : 2 * n - x;
   return x;
