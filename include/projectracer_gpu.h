@@ -19,7 +19,7 @@
  *
  * Supported subset (what the hot path and its configurations use):
  * uniform grid (gridX=gridZ=0), order=4, Database=2, iterate=1, MG=1,
- * RKC1=1, splitting=0, non_Newtonian=0, plastic=0, comp=1, ntype=1,
+ * RKC1=1 or 0 (FP64), splitting=0, non_Newtonian=0, plastic=0, comp=1, ntype=1,
  * iC_continuous=1 or 0, shape_function=1, tracer=1, torder=4, RK_update=1,
  * vel_update=1, local_time=0, Chebyshev=0, CFL=0, equalize=0 (or 1 with
  * original=1 and iC_continuous=0), iheattype=0.  pt_create rejects anything
@@ -103,6 +103,11 @@ typedef struct pt_params {
                              the only redistribution built */
   int32_t buffer;         /* half-width of the single-type neighbourhood test (10) */
   double  dist_factor;    /* wall distance of redistributed tracers, in cells (1e-4) */
+  /* &inputs: fine grid smoother */
+  int32_t RKC1;           /* 1: RKC1 sweeps of nstageS stages; 0: RK1, one explicit step
+                             per iteration with local Chebyshev-ordered steps
+                             (PS/iterative_solver.f90:1468-1485, :1503-1513); RK1 is
+                             built for precision = PT_PREC_FP64 */
 } pt_params;
 
 /* Fill *p with the reference defaults used by this build (inputs.nml values

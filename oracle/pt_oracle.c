@@ -770,8 +770,24 @@ static void viscosity_gradients(pto_case* c, int l, int n_full) {
     }
 }
 
-/* test_eigenvalues (RKC1 branch): PS/iterative_solver.f90:1187-1208 */
+/* test_eigenvalues: the RK1 branch (RKC1=0, PS/iterative_solver.f90:1173-1186)
+   and the RKC1 branch (:1187-1208) */
 static void test_eigenvalues(pto_case* c, int l) {
+  if (c->P.RKC1 == 0) {
+    double ctemp = -(c->P.courant_stream * 2.0);
+    for (int k = 1; k <= c->nzg[l] - 1; k++)
+      for (int i = 1; i <= c->nxg[l] - 1; i++) {
+        double eI = fabs(AT(c->vis_xxpzz[l], i, k) * c->SI1[l]) + AT(c->vis_grid[l], i, k) * c->SI2[l] +
+                    fabs(AT(c->vis_xz[l], i, k) * c->SI4[l]) + fabs(AT(c->vis_x[l], i, k) * c->SI5[l]) +
+                    fabs(AT(c->vis_z[l], i, k) * c->SI6[l]);
+        double eR = AT(c->vis_xxmzz[l], i, k) * c->SR1[l] + AT(c->vis_grid[l], i, k) * c->SR2[l] +
+                    AT(c->vis_xz[l], i, k) * c->SR4[l] + AT(c->vis_x[l], i, k) * c->SR5[l] +
+                    AT(c->vis_z[l], i, k) * c->SR6[l];
+        AT(c->ratio[l], i, k) = fabs(eR / eI);
+        AT(c->dt_stream[l], i, k) = ctemp * eR / (eR * eR + eI * eI);
+      }
+    return;
+  }
   double ae = -(c->RKC1_stability / 2.0);
   double be = (double)c->P.nstageS;
   double ctemp = c->P.courant_stream * (c->RKC1_stability / (ae * ae));
@@ -1257,6 +1273,30 @@ int pto_direct_matrix(pto_case* c, double* ab, double* row, int32_t* kl, int32_t
 /* PS/iterative_solver.f90:1418-1614 */
 static int iterate_stream(pto_case* c, int l, int Niter, int p) {
   int n = c->nxg[l], m = c->nzg[l], sp = c->span, s = c->P.nstageS;
+  if (c->P.RKC1 == 0) {
+    /* RK1 (:1468-1485, :1503-1513): local Chebyshev-ordered steps from the
+       eigenvalue ratio, c = 1 going up the cycle (dir = 1, :166-171) */
+    const int dir = c->multi[p + 1] > c->multi[p] ? 0 : 1;
+    const double la = -1.0, lb_user = -2.0;
+    for (int it = 1; it <= Niter; it++) {
+      a2_copy(&c->S0[l], &c->error[l]);
+      stream_derivatives(c, l, &c->S0[l]);
+      int ntemp = Niter - it + 1;
+      double temp = cos((double)(2 * ntemp - 1) * 3.1415926535897932 / (double)(2 * Niter));
+      for (int k = 1; k <= m - 1; k++)
+        for (int i = 1; i <= n - 1; i++) {
+          double r = AT(c->ratio[l], i, k), lb, cc;
+          if (r < 1.0e99) lb = lb_user * (r * r) / (1.0 + r * r);
+          else lb = lb_user;
+          cc = 2.0 / (-lb - la + (lb - la) * temp);
+          if (dir == 1) cc = 1.0;
+          AT(c->error[l], i, k) = AT(c->S0[l], i, k) + cc * AT(c->dt_stream[l], i, k) * stream_space(c, l, i, k);
+        }
+      enforce_bcs_stream(c, l, &c->error[l]);
+    }
+    stream_derivatives(c, l, &c->error[l]);
+    return compute_residual(c, l, p);
+  }
   for (int it = 1; it <= Niter; it++) {
     a2_copy(&c->S0[l], &c->error[l]);
     stream_derivatives(c, l, &c->S0[l]);
@@ -1875,6 +1915,7 @@ static int check_supported(const pt_params* p) {
   if (p->iterate == 0 && (p->ivis != 0 || (p->Database != 1 && p->Database != 2))) return PT_ERR_UNSUPPORTED;
   if ((p->iC_continuous != 0 && p->iC_continuous != 1) || (p->equalize != 0 && p->equalize != 1)) return PT_ERR_INVALID;
   if (p->equalize == 1 && (p->iC_continuous != 0 || p->original != 1 || p->buffer < 1)) return PT_ERR_UNSUPPORTED;
+  if (p->RKC1 != 0 && p->RKC1 != 1) return PT_ERR_INVALID;
   return PT_OK;
 }
 

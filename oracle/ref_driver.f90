@@ -137,11 +137,17 @@ contains
  end function
 
  subroutine write_init
+  integer*4 :: i_rk
   open(unit=602,file="init.bin",access='stream',form='unformatted',status='replace')
   write(602) nx,nz,grids,ntr,nstageS
   write(602) D1,D2,D3,D4,D
   write(602) Tr_base,Trr_base,Trrr_base,Trrrr_base,Trs_base,Trrs_base,Trrss_base,Tr_base_so,Trr_base_so
-  write(602) mewtChebS(1:nstageS),mewChebS(2:nstageS),vChebS(2:nstageS),cChebS(2:nstageS),RKC1_stability
+  if (RKC1.eq.1) then
+   write(602) mewtChebS(1:nstageS),mewChebS(2:nstageS),vChebS(2:nstageS),cChebS(2:nstageS),RKC1_stability
+  else
+   !! RK1 (RKC1=0): the reference allocates no RKC1 tables; zeros keep the layout
+   write(602) (0.d0, i_rk = 1, 4*nstageS-2)
+  end if
   write(602) dts_array(0,0),delta_T,delta_C(1)
   close(602)
  end subroutine

@@ -75,6 +75,7 @@ class PtParams(C.Structure):
         ("dent", C.c_double), ("iterate", C.c_int32), ("Database", C.c_int32),
         ("iC_continuous", C.c_int32), ("equalize", C.c_int32), ("original", C.c_int32),
         ("buffer", C.c_int32), ("dist_factor", C.c_double),
+        ("RKC1", C.c_int32),
     ]
 
 
@@ -112,7 +113,7 @@ def default_params(**overrides) -> PtParams:
         tpc=20, Tbot=0, Vbc=0, RaT=1e5, RaC=0.0, H=0.0, conduct_factor=1.0, Htype=0.0,
         ivis=0, visP=1.0, visT=1e3, delta_visT=2.0, eig_ratio_min=2.0, visC=10.0,
         delta_visC=2.0, precision=PT_PREC_FP64, guess=PT_GUESS_WARM, dent=0.5,
-        iterate=1, Database=2, iC_continuous=1, equalize=0, original=1, buffer=10, dist_factor=1e-4)
+        iterate=1, Database=2, iC_continuous=1, equalize=0, original=1, buffer=10, dist_factor=1e-4, RKC1=1)
     for k, v in overrides.items():
         setattr(p, k, v)
     return p

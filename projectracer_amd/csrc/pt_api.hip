@@ -74,6 +74,15 @@ static int check_supported(const pt_params& p) {
     set_error("equalize=1 is built for iC_continuous=0, original=1, buffer >= 1, dist_factor >= 0");
     return PT_ERR_UNSUPPORTED;
   }
+  if (p.RKC1 != 0 && p.RKC1 != 1) {
+    set_error("RKC1 must be 0 (RK1) or 1 (RKC1)");
+    return PT_ERR_INVALID;
+  }
+  if (p.RKC1 == 0 && p.iterate == 1 && p.precision != PT_PREC_FP64) {
+    // the mixed path's FP32 cycle is built on RKC1 sweeps
+    set_error("the RK1 smoother (RKC1=0) is built for precision = PT_PREC_FP64");
+    return PT_ERR_UNSUPPORTED;
+  }
   return PT_OK;
 }
 
@@ -119,6 +128,7 @@ void pt_default_params(pt_params* p) {
   p->visC = 10.0; p->delta_visC = 2.0; p->precision = PT_PREC_FP64; p->guess = PT_GUESS_WARM;
   p->dent = 0.5; p->iterate = 1; p->Database = 2;
   p->iC_continuous = 1; p->equalize = 0; p->original = 1; p->buffer = 10; p->dist_factor = 1e-4;
+  p->RKC1 = 1;
 }
 
 int pt_create(const pt_params* p, void** out) {

@@ -283,6 +283,28 @@ __global__ void __launch_bounds__(TX* TZ) k_residual(StageArgs<R> a, R* res, dou
   if (tid == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
+// one RK1 smoothing iteration (RKC1=0, PS/iterative_solver.f90:1468-1485,
+// :1503-1513) from F = stream_space(S0): psi = S0 + (c dt) F with the local
+// step c = 2/(-lb-la+(lb-la) temp), la = -1, lb = -2 r^2/(1+r^2) from the
+// point's eigenvalue ratio r (-2 where r is not below 1e99), c = 1 on the
+// way up the cycle; then the ghost images (enforceBCs_stream).  Each thread
+// reads and writes only its own interior point, so the update is in place.
+__global__ void k_rk1_update(double* psi, const double* F, const double* dts, const double* ratio, int n, int m,
+                             int pitch, double temp, int dir, double sz) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, k = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > n - 1 || k > m - 1) return;
+  const long g = IDX(i, k, pitch);
+  double cc = 1.0;
+  if (dir == 0) {
+    const double la = -1.0, lb_user = -2.0, r = ratio[g];
+    double lb = lb_user;
+    if (r < 1.0e99) lb = lb_user * (r * r) / (1.0 + r * r);
+    cc = 2.0 / (-lb - la + (lb - la) * temp);
+  }
+  const double y = psi[g] + (cc * dts[g]) * F[g];
+  store_with_ghosts<double>(psi, i, k, n, m, pitch, y, sz);
+}
+
 // ---------------------------------------------------------------------------
 // small levels: the whole RKC1 sweep (all stages) and compute_residual in one
 // workgroup with the three RKC buffers resident in LDS.  Each thread owns up
@@ -1529,6 +1551,23 @@ static int launch_coarse_apply_wg(Context& c, const float* rhs, float* err, int 
 // iterate_stream (RKC1, Nmulti sweeps) followed by compute_residual
 // (PS/iterative_solver.f90:1418-1614, 415-448).  `s0zero` runs the smoother
 // on a correction that starts from zero (mixed precision).
+// residual of the smoothed field
+template <typename R>
+static int smoothed_residual(Context& c, int l, dim3 grd, dim3 blk, bool fp32, bool l1_sum) {
+  Level& L = c.L[l];
+  StageArgs<R> a = stage_base<R>(c, l, fp32);
+  a.Sx = fp32 ? (R*)L.ferr : (R*)L.err;
+  R* res = fp32 ? (R*)L.fB : (R*)L.res;   // FP32 residual reuses the B buffer
+  int nb = grd.x * grd.y;
+  launch_residual<R>(c, grd, blk, a, res, c.red);
+  if (l1_sum) {
+    // the L1 norm of the residual into device memory; level1_residual copies
+    // it to the host after the cycle's launches (or graph replay)
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, nb, (int)RED_SUM_ABS, c.red_out);
+  }
+  return PT_OK;
+}
+
 template <typename R>
 static int iterate_stream_t(Context& c, int l, int p, bool fp32, bool l1_sum) {
   Level& L = c.L[l];
@@ -1543,6 +1582,25 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, bool l1_sum) {
   const int tk = l == 1 ? PT_K_RKC_STAGE_L1 : PT_K_RKC_STAGE_COARSE;
   const size_t lds = 3 * L.nelem * sizeof(R);
   const int ppt = small_ppt(g.n, g.m, c.P.ivis == 0);
+  if constexpr (sizeof(R) == 8) {
+    if (c.P.RKC1 == 0) {
+      // RK1: per iteration F = stream_space(S0) into B, then the update
+      const int dir = c.cyc[p + 1] > c.cyc[p] ? 0 : 1;
+      dim3 b16(64, 4), gi((g.n - 1 + 63) / 64, (g.m - 1 + 3) / 4);
+      for (int it = 1; it <= c.P.Nmulti; it++) {
+        const int ntemp = c.P.Nmulti - it + 1;
+        const double temp = std::cos((double)(2 * ntemp - 1) * 3.1415926535897932 / (double)(2 * c.P.Nmulti));
+        StageArgs<R> a = stage_base<R>(c, l, false);
+        a.Sx = err;
+        if (timed) timer_begin(c, tk, (int64_t)(g.n - 1) * (g.m - 1));
+        launch_residual<R>(c, grd, blk, a, B, c.red);
+        hipLaunchKernelGGL(k_rk1_update, gi, b16, 0, c.stream, (double*)err, (const double*)B, L.dts, L.ratio, g.n,
+                           g.m, g.pitch, temp, dir, (double)a.sz);
+        if (timed) timer_end(c, tk);
+      }
+      return smoothed_residual<R>(c, l, grd, blk, fp32, l1_sum);
+    }
+  }
   if constexpr (sizeof(R) == 4) {
     // FP32 isoviscous levels too large for one workgroup: strip sweeps
     const StripCfg sc = strip_cfg(g);
@@ -1619,19 +1677,8 @@ static int iterate_stream_t(Context& c, int l, int p, bool fp32, bool l1_sum) {
       }
     }
   }
-  // residual of the smoothed field
-  StageArgs<R> a = stage_base<R>(c, l, fp32);
-  a.Sx = err;
-  R* res = fp32 ? (R*)L.fB : (R*)L.res;   // FP32 residual reuses the B buffer
-  int nb = grd.x * grd.y;
-  launch_residual<R>(c, grd, blk, a, res, c.red);
-  if (l1_sum) {
-    // the L1 norm of the residual into device memory; level1_residual copies
-    // it to the host after the cycle's launches (or graph replay)
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, c.red, nb, (int)RED_SUM_ABS, c.red_out);
-  }
   (void)p;
-  return PT_OK;
+  return smoothed_residual<R>(c, l, grd, blk, fp32, l1_sum);
 }
 
 // smoother_vis (PS/smoother.f90:454-580) on the field set `vs` of level l
@@ -1662,6 +1709,12 @@ static int smoother_vis(Context& c, int l, VisSet& vs, int interpolate) {
   const LevelRkc rk = level_rkc(c, l, c.P.precision == PT_PREC_MIXED);
   double ae = -(rk.stability / 2.0), be = (double)rk.s;
   double ctemp = c.P.courant_stream * (rk.stability / (ae * ae));
+  if (c.P.RKC1 == 0) {
+    // RK1 (PS/iterative_solver.f90:1173-1186): dt = -2 courant eR/(eR^2+eI^2),
+    // k_test_eig's RKC1 form with unit ellipse axes (eR/1 and eI/1 are exact)
+    ae = 1.0; be = 1.0;
+    ctemp = -(c.P.courant_stream * 2.0);
+  }
   for (;;) {
     for (int nn = nn_store; nn <= Niter; nn++) {
       hipLaunchKernelGGL(k_vis_grad, gnodes, b16, 0, c.stream, vs, g.n, g.m, g.pitch, Nvis_store >= 1 ? 1 : 0, ivis0,

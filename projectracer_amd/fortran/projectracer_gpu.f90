@@ -49,6 +49,7 @@ module projectracer_gpu
   integer(c_int32_t) :: iterate, Database
   integer(c_int32_t) :: iC_continuous, equalize, original, buffer
   real(c_double) :: dist_factor
+  integer(c_int32_t) :: RKC1
  end type pt_params
 
  !! stats.dat / viscosity_smoother.dat values (PS/basics.f90:363-462)

@@ -94,6 +94,7 @@ program pt_steps
  p%dent = dent; p%iterate = iterate; p%Database = Database
  p%iC_continuous = iC_continuous; p%equalize = equalize; p%original = original; p%buffer = buffer
  p%dist_factor = dist_factor
+ p%RKC1 = RKC1
 
  call pt_check(pt_create(p, ctx), "pt_create")
  call pt_check(pt_get_dims(ctx, gnx, gnz, ggrids, ntr), "pt_get_dims")

@@ -225,7 +225,7 @@ aspect={f(d['aspect'])}, nx={d['nx']}, nz={d['nz']}, Database={d['Database']}, o
 thickness=0.1d0, nbound=8, spaceX=1.d0, spaceZ=1.d0,
 iterate={d['iterate']}, MG=1, non_Newtonian=0, tolerance={f(d['tolerance'])}, Nmulti={d['Nmulti']},
 mingrid={d['mingrid']}, grids={d['grids']}, eig_ratio={f(d['eig_ratio'])}, vis_iter={d['vis_iter']},
-max_cycles={d['max_cycles']}, tol_coarse=1.d-3, fp_max=100, splitting=0, RKC1=1,
+max_cycles={d['max_cycles']}, tol_coarse=1.d-3, fp_max=100, splitting=0, RKC1={d['RKC1']},
 nstageS={d['nstageS']}, dampingS={f(d['dampingS'])}, c_2={f(d['c_2'])}, c_1={f(d['c_1'])},
 tend=1.d0, tsnap=1.d0, Chebyshev=0, nstage=3, damping=100.d0, aRK2=0.5d0, torder=4, movie=0,
 tmovie_snap=1.d0, Nlocal=1000, Nsnap=100, local_time=0, dt_manual=0.125d0,
@@ -369,6 +369,16 @@ VARIANTS = {
     "ivis1_thermo": dict(kind="thermo", RaT=1e5, RaC=5e4, ivis=1, visT=1e2, courant=0.1, tolerance=1e-10),
 }
 
+# The RK1 fine grid smoother (RKC1=0, PS/iterative_solver.f90:1173-1186,
+# :1468-1485, :1503-1513) with the 70 iterations per level that the
+# reference's inputs file suggests for it, run through the reference program
+# for two steps (tests/golden/ref_rk1.npz): isoviscous, and the T- and
+# C-dependent viscosity (ivis=2, where the eigenvalue ratio varies per point).
+RK1_VARIANTS = {
+    "rk1_iso": dict(kind="iso", RKC1=0, Nmulti=70, tolerance=1e-10),
+    "rk1_thermo": dict(kind="thermo", RaT=1e5, RaC=5e4, ivis=2, RKC1=0, Nmulti=70, courant=0.1, tolerance=1e-10),
+}
+
 
 # Tracer redistribution (equalize=1, original=1, discrete types
 # iC_continuous=0), run through the reference program for three steps
@@ -405,7 +415,7 @@ def parse_equalize(stdout):
 
 def variant_case(name):
     from projectracer_amd import cases
-    kw = dict(VARIANTS[name])
+    kw = dict(VARIANTS[name] if name in VARIANTS else RK1_VARIANTS[name])
     if kw.pop("kind") == "iso":
         return cases.isoviscous(nx=48, nz=32, tpc=4, aspect=1.5, **kw)
     return cases.thermochem(nx=48, nz=32, tpc=4, aspect=1.5, **kw)

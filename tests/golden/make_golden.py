@@ -47,6 +47,10 @@ Two sources:
                            and mzran stream 0's state; after each step's solve
                            the node counts, every tracer's type, every 7th
                            tracer's r, s, T, and (steps 1 and 3) psi, T, C
+      ref_rk1.npz       -- the RK1 fine grid smoother (RKC1=0, Nmulti=70) on the
+                           48x32 isoviscous and ivis=2 states, tpc=4, two
+                           steps: q of every solve, node counts, psi, u, w, T
+                           after the solves of steps 0 and 2
 Usage: python tests/golden/make_golden.py
 """
 from __future__ import annotations
@@ -254,6 +258,22 @@ def reference_variants(tmp):
     print("wrote ref_variants.npz")
 
 
+def reference_rk1(tmp):
+    from _support import run_reference, RK1_VARIANTS, variant_case
+    out = {}
+    for name in RK1_VARIANTS:
+        states, solves, _, _ = run_reference(variant_case(name), nrun=2, dump_every=1, threads=1,
+                                             workdir=tmp / f"variant_{name}")
+        for step in (0, 1, 2):
+            out[f"{name}_s{step}_nempty"] = states[step]["nempty"].astype(np.int32)
+        for step in (0, 2):
+            for n in ("SF", "U", "W", "T"):
+                out[f"{name}_s{step}_{n}"] = _node(states[step], n)
+        out[f"{name}_solves_tq"], _ = _solves(solves)
+    np.savez_compressed(HERE / "ref_rk1.npz", **out)
+    print("wrote ref_rk1.npz")
+
+
 def reference_config1(tmp, tolerance=1e-8, name="ref_config1_100.npz"):
     from projectracer_amd import cases
     from _support import run_reference
@@ -406,6 +426,9 @@ if __name__ == "__main__":
         raise SystemExit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "variants":
         reference_variants(Path(tempfile.mkdtemp(prefix="golden_")))
+        raise SystemExit(0)
+    if len(sys.argv) > 1 and sys.argv[1] == "rk1":
+        reference_rk1(Path(tempfile.mkdtemp(prefix="golden_")))
         raise SystemExit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "equalize":
         reference_equalize(Path(tempfile.mkdtemp(prefix="golden_")))

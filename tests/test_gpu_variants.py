@@ -3,7 +3,10 @@ the reference program compiled from its Fortran sources
 (tests/golden/ref_variants.npz): rigid top/bottom walls with an insulating
 bottom (Vbc=1, Tbot=1; PS/iterative_solver.f90:1616-1664, PS/energy.f90:558-592)
 through multigrid and through the direct solver (PS/momentum.f90:362-606), and
-the T- and depth-dependent viscosity law ivis=1 (PS/rheology.f90:2-23).
+the T- and depth-dependent viscosity law ivis=1 (PS/rheology.f90:2-23); and
+the RK1 fine grid smoother (RKC1=0, PS/iterative_solver.f90:1173-1186,
+:1468-1485, :1503-1513) on an isoviscous and an ivis=2 state
+(tests/golden/ref_rk1.npz), which pt_create refuses on the mixed path.
 Two steps: the cycle count of every solve group exact, node counts exact,
 psi, u, w, T within 1e-10.  The mixed path
 reaches the same residual target on the multigrid variants and stays within
@@ -12,15 +15,15 @@ import numpy as np
 import pytest
 
 from projectracer_amd import Simulation, _abi
-from _support import GOLDEN, VARIANTS, rel_err, run_variant, variant_case
+from _support import GOLDEN, RK1_VARIANTS, VARIANTS, rel_err, run_variant, variant_case
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", list(VARIANTS))
+@pytest.mark.parametrize("name", list(VARIANTS) + list(RK1_VARIANTS))
 def test_variant_against_the_reference_program(name):
     print(f"[variant] {name}: GPU FP64 vs reference program", flush=True)
-    g = np.load(GOLDEN / "ref_variants.npz")
+    g = np.load(GOLDEN / ("ref_rk1.npz" if name in RK1_VARIANTS else "ref_variants.npz"))
     case = variant_case(name)
     sim = Simulation(case.params)
     sim.load_state(case)
@@ -48,3 +51,10 @@ def test_variant_mixed_precision(name):
             for n in ("SF", "U", "W")}
     print(f"[variant] {name} mixed: {errs}", flush=True)
     assert max(errs.values()) <= 1e-6
+
+
+def test_rk1_refused_on_the_mixed_path():
+    case = variant_case("rk1_iso")
+    case.params.precision = _abi.PT_PREC_MIXED
+    with pytest.raises(Exception, match="RKC1=0"):
+        Simulation(case.params)
