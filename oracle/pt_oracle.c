@@ -42,6 +42,10 @@
  *    branches, through multigrid and the direct solver, and the ivis=1
  *    viscosity law: reference program runs on injected 48x32 states
  *    (tests/golden/ref_variants.npz), cycle counts of every solve exact.
+ *  - the RK1 fine grid smoother (RKC1=0: its dt_stream in test_eigenvalues,
+ *    its local Chebyshev-ordered steps in iterate_stream): reference program
+ *    runs on injected 48x32 isoviscous and ivis=2 states with Nmulti=70
+ *    (tests/golden/ref_rk1.npz), cycle counts of every solve exact.
  *  - direct solver (iterate=0, build_SF_matrix / solve_SF_equation): the
  *    shipped stdout band widths and stats.dat row 1, reference program runs
  *    with Database 1 and 2, the operator's symbol on sine modes, LAPACK's
