@@ -26,6 +26,7 @@ program binding_probe
  write(*,'(a,4i8)') "ints ", p%nx, p%nz, p%grids, p%nstageS
  write(*,'(a,4es25.17)') "reals ", p%tolerance, p%dampingS, p%c_2, p%courant_stream
  write(*,'(a,3i6)') "sizes ", int(c_sizeof(p)), int(c_sizeof(info)), int(c_sizeof(row))
+ write(*,'(a,3i8,es25.17)') "tail ", p%buffer, p%equalize, p%RKC1, p%dist_factor
  p%nx = 2
  ierr = pt_create(p, ctx)
  write(*,'(a,i4)') "invalid ", ierr
@@ -51,5 +52,9 @@ def test_fortran_interface_compiles_links_and_matches_header(tmp_path):
     assert reals == [p.tolerance, p.dampingS, p.c_2, p.courant_stream]
     assert [int(v) for v in rows["sizes"].split()] == [C.sizeof(_abi.PtParams), C.sizeof(_abi.PtSolveInfo),
                                                         C.sizeof(_abi.PtStatsRow)]
+    # the struct's last fields, where an added field would shift a mismatched layout
+    tail = rows["tail"].split()
+    assert [int(v) for v in tail[:3]] == [p.buffer, p.equalize, p.RKC1] == [10, 0, 1]
+    assert float(tail[3]) == p.dist_factor
     assert int(rows["invalid"]) == _abi.PT_ERR_INVALID
     assert "nx" in rows["message"]
