@@ -64,6 +64,10 @@ program pt_steps
  integer(c_int32_t) :: seeds(4)
 
  ntype = 1; comp = 1
+ !! the supported values of the options pt_params does not carry, for a file that omits them
+ order = 4; gridX = 0; gridZ = 0; MG = 1; non_Newtonian = 0; splitting = 0; RKC1 = 1; Chebyshev = 0
+ torder = 4; local_time = 0; CFL = 0; RK_update = 1; vel_update = 1; tracer = 1; shape_function = 1
+ plastic = 0; iheattype = 0
  open(unit=501, file="inputs.nml", status="old")
  read(nml=inputs, unit=501)
  read(nml=Rayleigh_numbers, unit=501)
@@ -72,6 +76,17 @@ program pt_steps
  close(501)
  if (comp .ne. 1 .or. ntype .ne. 1) then
   write(*,*) "pt_steps: one composition type (comp=1, ntype=1) is supported"
+  stop 2
+ end if
+ !! options outside the subset the GPU path is built for (include/projectracer_gpu.h); pt_params
+ !! has no field for them, so they are refused here rather than ignored
+ if (order .ne. 4 .or. gridX .ne. 0 .or. gridZ .ne. 0 .or. MG .ne. 1 .or. non_Newtonian .ne. 0 .or. &
+     & (RKC1 .eq. 1 .and. splitting .ne. 0) .or. Chebyshev .ne. 0 .or. torder .ne. 4 .or. &
+     & local_time .ne. 0 .or. CFL .ne. 0 .or. RK_update .ne. 1 .or. vel_update .ne. 1 .or. &
+     & tracer .ne. 1 .or. shape_function .ne. 1 .or. plastic .ne. 0 .or. iheattype .ne. 0) then
+  write(*,*) "pt_steps: a namelist option outside the supported subset: order=4, gridX=gridZ=0, MG=1, ", &
+     & "non_Newtonian=0, splitting=0, Chebyshev=0, torder=4, local_time=0, CFL=0, RK_update=1, ", &
+     & "vel_update=1, tracer=1, shape_function=1, plastic=0, iheattype=0"
   stop 2
  end if
  nrun = 1; precision = PT_PREC_FP64

@@ -58,3 +58,26 @@ def test_fortran_interface_compiles_links_and_matches_header(tmp_path):
     assert float(tail[3]) == p.dist_factor
     assert int(rows["invalid"]) == _abi.PT_ERR_INVALID
     assert "nx" in rows["message"]
+
+
+@pytest.mark.parametrize("option", ["splitting=1", "Chebyshev=1", "non_Newtonian=1", "plastic=1", "iheattype=1"])
+def test_fortran_program_refuses_options_outside_the_subset(tmp_path, option):
+    """projectracer_amd/fortran/pt_steps.f90 reads the reference's namelist
+    file; an option that pt_params cannot carry and the GPU path does not
+    build stops the program (code 2) before pt_create, with the supported
+    values in the message.  No device is needed."""
+    from projectracer_amd import cases
+    from projectracer_amd.build import FORTRAN_PROG
+    from _support import write_inputs_nml
+    if not FORTRAN_PROG.exists():
+        pytest.skip(f"{FORTRAN_PROG} not built")
+    write_inputs_nml(tmp_path / "inputs.nml", cases.isoviscous(nx=64, nz=64, tpc=20).params)
+    txt = (tmp_path / "inputs.nml").read_text()
+    name = option.split("=")[0]
+    default = f"{name}=0"
+    assert default in txt
+    (tmp_path / "inputs.nml").write_text(txt.replace(default, option))
+    (tmp_path / "steps.nml").write_text("&steps\nnrun=1, precision=0\n/\n")
+    proc = subprocess.run([str(FORTRAN_PROG)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert proc.returncode == 2, proc.stdout + proc.stderr
+    assert "outside the supported subset" in proc.stdout
